@@ -1,0 +1,159 @@
+"""ctypes mirror of include/dfn.h.
+
+The same declarations are bound against two libraries: the device library
+(``libdfn.so``, prefix ``dfn_``) and, in the tests only, the CPU oracle
+(prefix ``dfno_``).  Nothing here computes anything.
+"""
+import ctypes as C
+
+import numpy as np
+
+DFN_OK = 0
+DFN_ERR_ARG = 1
+DFN_ERR_HIP = 2
+DFN_ERR_CG = 3
+DFN_ERR_UNDERFLOW = 4
+DFN_ERR_TENSILE = 5
+DFN_ERR_STATE = 6
+DFN_ERR_NOMEM = 7
+DFN_ERR_UNSUPPORTED = 8
+
+ERROR_NAMES = {
+    DFN_OK: "ok",
+    DFN_ERR_ARG: "bad argument",
+    DFN_ERR_HIP: "HIP runtime error",
+    DFN_ERR_CG: "compute_P did not converge",
+    DFN_ERR_UNDERFLOW: "stepsize underflow in rkqs",
+    DFN_ERR_TENSILE: "normal traction became extensional",
+    DFN_ERR_STATE: "call out of order",
+    DFN_ERR_NOMEM: "allocation failed",
+    DFN_ERR_UNSUPPORTED: "unsupported option",
+}
+
+MODE_II, MODE_III = 2, 3
+KERNEL_DIRECT, KERNEL_HMATRIX = 0, 1
+OP_T, OP_N = 0, 1
+PREC_FP64, PREC_FP32 = 0, 1
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+
+
+class DfnProblem(C.Structure):
+    _fields_ = [
+        ("fracture_mode", C.c_int32),
+        ("static_kernel", C.c_int32),
+        ("nb_fault", C.c_int32),
+        ("nb_element", C.c_int32),
+        ("fault_nb_element", _i64p),
+        ("node", _dp),
+        ("a", _dp), ("b", _dp), ("Dc", _dp), ("f0", _dp), ("V0", _dp),
+        ("normal_loading_dot", _dp), ("shear_loading_dot", _dp),
+        ("V", _dp), ("theta", _dp), ("sigmaN", _dp), ("P", _dp),
+        ("permeability", _dp), ("kmin", _dp), ("kmax", _dp), ("Snk", _dp), ("Lk", _dp), ("Tk", _dp),
+        ("mu", C.c_double), ("cs", C.c_double), ("cp", C.c_double),
+        ("Sigma11_dot", C.c_double), ("Sigma12_dot", C.c_double), ("Sigma22_dot", C.c_double),
+        ("Sigma31_dot", C.c_double), ("Sigma32_dot", C.c_double),
+        ("porosity", C.c_double), ("fluid_comp", C.c_double), ("rock_comp", C.c_double),
+        ("dyn_viscosity", C.c_double), ("fluid_density", C.c_double),
+        ("tol_solver", C.c_double),
+        ("permeability_coupling", C.c_int32),
+        ("nb_source", C.c_int32),
+        ("source_fault", _i64p), ("source_index", _i64p),
+        ("source_Q", _dp), ("source_t_beg", _dp), ("source_t_end", _dp),
+        ("eta", C.c_double), ("eps_aca", C.c_double),
+        ("nleaf", C.c_int32), ("max_rank", C.c_int32),
+    ]
+
+
+class DfnRunStats(C.Structure):
+    _fields_ = [
+        ("accepted_steps", C.c_int64),
+        ("rejected_attempts", C.c_int64),
+        ("rhs_evaluations", C.c_int64),
+        ("cg_solves", C.c_int64),
+        ("cg_iterations", C.c_int64),
+        ("cg_halvings", C.c_int64),
+        ("time", C.c_double),
+        ("t_try", C.c_double),
+        ("max_V", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_H = C.c_void_p
+# name -> argtypes (restype is always c_int)
+SIGNATURES = {
+    "create": [C.POINTER(DfnProblem), C.POINTER(C.c_void_p)],
+    "destroy": [_H],
+    "build_hmat": [_H],
+    "set_precision": [_H, C.c_int32],
+    "hmat_stats": [_H, C.c_int32, _i64p],
+    "hmat_export": [_H, C.c_int32, _i32p, _i32p, _i32p, _i32p, _i32p, _i64p, _dp,
+                    _i32p, _i32p, _i32p, _i32p, _i32p, _i64p, _i64p, _dp, _dp],
+    "hmat_import": [_H, C.c_int32, C.c_int32, _i32p, C.c_int32, _i32p, _i32p, _i32p, _i32p, _i64p, _dp,
+                    C.c_int32, _i32p, _i32p, _i32p, _i32p, _i32p, _i64p, _i64p, _dp, _dp],
+    "set_state": [_H, _dp, _dp, C.c_double],
+    "get_state": [_H, _dp, _dp, _dp, C.POINTER(C.c_double)],
+    "get_stats": [_H, C.POINTER(DfnRunStats)],
+    "kernel": [_H, _dp, _dp, _dp],
+    "hgemv": [_H, C.c_int32, C.c_double, C.c_double, _dp, _dp],
+    "friction_law": [_H, C.c_double, _dp, _dp],
+    "compute_P": [_H, C.c_double, _dp, _dp, _i32p],
+    "rkqs": [_H, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
+    "solve": [_H, C.c_int64, C.c_double, C.POINTER(DfnRunStats)],
+    "bench_kernel": [_H, C.c_int32, _dp],
+    "device_info": [C.c_char_p, C.c_int32],
+}
+
+# entry points every build of the device library must export (checked by the CPU tests)
+REQUIRED = list(SIGNATURES)
+
+
+def bind(lib, prefix, names=None):
+    """Set argtypes/restype for `prefix + name` on lib; returns {name: fn}."""
+    out = {}
+    for name in names or SIGNATURES:
+        fn = getattr(lib, prefix + name, None)
+        if fn is None:
+            continue
+        fn.argtypes = SIGNATURES[name]
+        fn.restype = C.c_int
+        out[name] = fn
+    return out
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_dp)
+
+
+def i64ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_i64p)
+
+
+def i32ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_i32p)
+
+
+class DfnError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {ERROR_NAMES.get(code, 'error')} (status {code})")
+
+
+def check(code, where):
+    if code != DFN_OK:
+        raise DfnError(code, where)

@@ -1,0 +1,177 @@
+/*
+ * dfn.h -- C ABI of the MI355X (gfx950) per-Runge-Kutta-stage hot path of
+ * DemystiFicatioN (2D quasi-dynamic earthquake cycles on fault networks).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (DFN_OK == 0) and never aborts.  All arrays are FP64 unless noted,
+ * element-indexed in the reference's concatenated order: N = sum_i(nb_el_i + 1) - 1
+ * elements, i.e. the faults one after another with one fake element between
+ * consecutive faults (src/configuration.f90:126,136).  Y = [V(1:N) | psi(1:N) |
+ * sigmaN(1:N)] as one contiguous 3N array (src/configuration.f90:185-237), with
+ * psi = ln(theta V0 / Dc) the log state.  Indices that the reference keeps as
+ * Fortran integer(kind=8) are int64_t here, default integers are int32_t.
+ *
+ * One handle owns one problem: geometry, friction parameters, the stress-transfer
+ * operator(s) (device-resident block lists), the fluid-diffusion state and the
+ * ODE state.  Calls on one handle must not overlap; several handles may coexist.
+ * The same declarations, with the prefix dfno_ instead of dfn_, are implemented
+ * by the CPU oracle used in the tests.
+ */
+#ifndef DFN_H
+#define DFN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference: ier flags and STOPs, see the notes per call) -- */
+enum {
+  DFN_OK = 0,
+  DFN_ERR_ARG = 1,        /* bad argument / size (reference: assert_eq STOP, nrutil.f90:404-458) */
+  DFN_ERR_HIP = 2,        /* HIP runtime error, or no gfx950 device */
+  DFN_ERR_CG = 3,         /* compute_P did not converge in 200 iterations (fluid_diffusion.f90:120-125) */
+  DFN_ERR_UNDERFLOW = 4,  /* 'stepsize underflow in rkqs' (ode_bs.f90:163-164) */
+  DFN_ERR_TENSILE = 5,    /* sigmaN > 0 somewhere (solver.f90:412-420) */
+  DFN_ERR_STATE = 6,      /* call out of order (e.g. H-matrix kernel before dfn_build_hmat) */
+  DFN_ERR_NOMEM = 7,      /* host or device allocation failed */
+  DFN_ERR_UNSUPPORTED = 8 /* option the library does not implement */
+};
+
+enum { DFN_MODE_II = 2, DFN_MODE_III = 3 };               /* hyperparameter/fracture_mode */
+enum { DFN_KERNEL_DIRECT = 0, DFN_KERNEL_HMATRIX = 1 };   /* hyperparameter/static_kernel */
+enum { DFN_OP_T = 0, DFN_OP_N = 1 };                      /* shear (Mode II/III) and normal (Mode II) operator */
+enum { DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1 };            /* storage of the operator blocks */
+
+/* Problem description: the content of problems/<name>/config
+ * (src/configuration.f90:70-379) as flat arrays.  Per-element arrays have
+ * N entries and include the fake slots between faults; the library overwrites
+ * the fake slots with the reference's values (a=b=Dc=f0=V0=Snk=Lk=Tk=1,
+ * src/configuration.f90:199-203,:226-232; everything else 0). */
+typedef struct dfn_problem {
+  int32_t fracture_mode;             /* DFN_MODE_II or DFN_MODE_III */
+  int32_t static_kernel;             /* DFN_KERNEL_DIRECT or DFN_KERNEL_HMATRIX */
+  int32_t nb_fault;
+  int32_t nb_element;                /* N, fake elements included */
+  const int64_t *fault_nb_element;   /* [nb_fault] faults(i)%nb_element */
+  const double *node;                /* [2*(N+1)] node(2,N+1), (x,y) pairs */
+  const double *a, *b, *Dc, *f0, *V0;                    /* [N] */
+  const double *normal_loading_dot, *shear_loading_dot;  /* [N] */
+  const double *V, *theta, *sigmaN, *P;                  /* [N] initial state, theta = psi (log state) */
+  const double *permeability, *kmin, *kmax, *Snk, *Lk, *Tk; /* [N] */
+  double mu, cs, cp;
+  double Sigma11_dot, Sigma12_dot, Sigma22_dot, Sigma31_dot, Sigma32_dot;
+  double porosity, fluid_comp, rock_comp, dyn_viscosity, fluid_density;
+  double tol_solver;
+  int32_t permeability_coupling;     /* 0 or 1 */
+  int32_t nb_source;                 /* total number of fluid sources */
+  const int64_t *source_fault;       /* [nb_source] 1-based fault id */
+  const int64_t *source_index;       /* [nb_source] 1-based element index within that fault */
+  const double *source_Q, *source_t_beg, *source_t_end; /* [nb_source] */
+  /* H-matrix parameters (reference hard-codes nleaf=100, eta=3, eps_ACA=1e-8,
+   * src/variables.f90:122-124, and max rank 20, H_matrix_construction.f90:59) */
+  double eta, eps_aca;
+  int32_t nleaf, max_rank;
+} dfn_problem;
+
+typedef struct dfn_handle_s *dfn_handle;
+
+/* Statistics of a dfn_solve call (accumulated over the handle's life). */
+typedef struct dfn_run_stats {
+  int64_t accepted_steps;   /* it_count of src/solver.f90:60 */
+  int64_t rejected_attempts;
+  int64_t rhs_evaluations;  /* friction_law calls */
+  int64_t cg_solves;        /* compute_P calls */
+  int64_t cg_iterations;    /* CG iterations summed over all compute_P calls */
+  int64_t cg_halvings;      /* t_try halvings after CG non-convergence (solver.f90:88-92) */
+  double time;              /* simulated time reached (s) */
+  double t_try;             /* next trial step (s) */
+  double max_V;             /* max over elements of V at the end */
+} dfn_run_stats;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+/* Replaces config()+initialise() (src/configuration.f90:19, src/initialisation.f90:24):
+ * copies the problem, computes element midpoints/ds/tangent/normal and the loading
+ * projections (initialisation.f90:139-206), fault crossings (:279-344), the initial
+ * permeability_star/permeability_x (:252-266), and uploads the state to the device. */
+int dfn_create(const dfn_problem *problem, dfn_handle *out);
+int dfn_destroy(dfn_handle h);
+
+/* Replaces initiate_hmat (src/hmatrix/initialisation_hmat.f90:13-100) for the
+ * problem's mode: Mode III builds the shear operator, Mode II the normal then the
+ * shear operator (src/initialisation.f90:210-248).  No-op for the direct kernel. */
+int dfn_build_hmat(dfn_handle h);
+/* Operator storage precision for subsequent kernel calls (DFN_PREC_FP64 default). */
+int dfn_set_precision(dfn_handle h, int32_t precision);
+/* stats[0..9] = {nb_dense, nb_lowrank, S_dense, S_lowrank_U, S_lowrank_V, max_rank,
+ *                nbclus, nbblclus, n, sum_rank} of operator op. */
+int dfn_hmat_stats(dfn_handle h, int32_t op, int64_t *stats);
+/* Export an operator as flat block lists (H order, 0-based).  Buffers sized from
+ * dfn_hmat_stats.  perm[k] = original element index of H position k (mat_corr - 1).
+ * Dense block b: rows dense_row0[b]..+dense_m[b], cols dense_col0[b]..+dense_n[b],
+ * column-major at dense_data + dense_off[b].  Low-rank block b = U V^T with
+ * U (m x k) at lr_u + lr_uoff[b], V (n x k) at lr_v + lr_voff[b], column-major. */
+int dfn_hmat_export(dfn_handle h, int32_t op, int32_t *perm,
+                    int32_t *dense_row0, int32_t *dense_col0, int32_t *dense_m, int32_t *dense_n,
+                    int64_t *dense_off, double *dense_data,
+                    int32_t *lr_row0, int32_t *lr_col0, int32_t *lr_m, int32_t *lr_n, int32_t *lr_k,
+                    int64_t *lr_uoff, int64_t *lr_voff, double *lr_u, double *lr_v);
+/* Replace operator op by caller-provided block lists (same layout as the export). */
+int dfn_hmat_import(dfn_handle h, int32_t op, int32_t n, const int32_t *perm,
+                    int32_t nb_dense, const int32_t *dense_row0, const int32_t *dense_col0,
+                    const int32_t *dense_m, const int32_t *dense_n, const int64_t *dense_off,
+                    const double *dense_data,
+                    int32_t nb_lowrank, const int32_t *lr_row0, const int32_t *lr_col0,
+                    const int32_t *lr_m, const int32_t *lr_n, const int32_t *lr_k,
+                    const int64_t *lr_uoff, const int64_t *lr_voff,
+                    const double *lr_u, const double *lr_v);
+
+/* ---- state ---------------------------------------------------------------- */
+int dfn_set_state(dfn_handle h, const double *Y, const double *P, double time);
+int dfn_get_state(dfn_handle h, double *Y, double *P, double *P_dot, double *time);
+int dfn_get_stats(dfn_handle h, dfn_run_stats *stats);
+
+/* ---- the reference's hot-path interfaces ------------------------------------ */
+/* kernel procedure pointer (src/variables.f90:17-26): stress rates from slip rate.
+ * Replaces hmatIII (hmatrixModeIII.f90:9-60), hmatII (hmatrixModeII.f90:9-74),
+ * directIII (directModeIII.f:7-74), directII (directModeII.f90:7-107).
+ * stressN may be NULL in Mode III (the reference leaves it untouched there). */
+int dfn_kernel(dfn_handle h, const double *slip, double *stressT, double *stressN);
+
+/* y <- alpha*H*x + beta*y for operator op in H order (Hgemv_right,
+ * src/hmatrix/calculation_hmat.f90:3-165), x and y of length N. */
+int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const double *x, double *y);
+
+/* friction_law procedure pointer (src/variables.f90:9-14): Y_dot = f(x, Y) with the
+ * pore pressure of compute_P(x) (RateStateAgingModeIII.f90:19-121, ModeII.f90:19-92). */
+int dfn_friction_law(dfn_handle h, double x, const double *Y, double *Y_dot);
+
+/* compute_P (src/fluid_diffusion.f90:24-152): backward-Euler pressure after dt from the
+ * handle's step-start pressure.  P_temp/P_dot may be NULL.  *ier = 1 on CG failure
+ * (then P_temp and P_dot are not written, as in the reference). */
+int dfn_compute_P(dfn_handle h, double dt, double *P_temp, double *P_dot, int32_t *ier);
+
+/* rkqs (src/ode_solver/ode_bs.f90:125-174) with the handle's friction law as derivs.
+ * Returns DFN_ERR_UNDERFLOW on 'stepsize underflow'. */
+int dfn_rkqs(dfn_handle h, double *y, const double *dydx, double *x, double htry,
+             double eps, const double *yscal, double *hdid, double *hnext);
+
+/* The solver loop of src/solver.f90:59-198 on the handle's device-resident state:
+ * runs until nsteps more accepted steps are done, or time >= t_final (t_final <= 0:
+ * no time limit).  The loop state (time, t_try, counters) persists between calls. */
+int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Time `reps` applications of the stress-transfer kernel on device-resident input
+ * (the handle's current V) with HIP events on the handle's stream.  ms[0] = mean ms
+ * per application of the whole kernel (all operators), ms[1] = mean ms of the
+ * dominant device kernel alone, ms[2] = algorithmic bytes per application. */
+int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms);
+/* Device/library description string (NUL-terminated, at most len bytes). */
+int dfn_device_info(char *buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFN_H */

@@ -1,0 +1,1563 @@
+/*
+ * ORACLE -- test infrastructure only (see oracle_internal.h).
+ *
+ * Plain-C FP64 restatement of DemystiFicatioN's per-RK-stage hot path.  Each
+ * function cites the reference file:line it restates.  The complex128 arithmetic
+ * of the reference H-matrix is carried in real FP64: every factor entry is
+ * CMPLX(x,0) (src/hmatrix/eval_kernel_hmatrix.f90:47,114,194), and with zero
+ * imaginary parts (a+0i)(b+0i)=ab and (a+0i)+(b+0i)=a+b exactly, so the real
+ * parts evolve bit-for-bit as in the complex code for the same operation order.
+ *
+ * Expressions keep the reference's left-to-right evaluation order and the build
+ * disables floating-point contraction (-ffp-contract=off), so that the oracle
+ * reproduces the reference arithmetic rather than a re-associated variant.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "oracle_internal.h"
+
+#define PI_REF (4.0 * atan(1.0)) /* real(kind=8), parameter :: pi=4*atan(1.d0) */
+
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz);
+  return p;
+}
+static double *dup_or_zero(const double *src, int n) {
+  double *d = (double *)xcalloc((size_t)n, sizeof(double));
+  if (d && src) memcpy(d, src, sizeof(double) * (size_t)n);
+  return d;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Geometry, fake elements, loading projections: src/initialisation.f90:139-206,
+ * layout src/configuration.f90:126-136,185-237.                               */
+/* ------------------------------------------------------------------------- */
+static void ini_geometry(oracle_t *o) {
+  int N = o->N;
+  for (int k = 0; k < N; k++) {
+    const double *n0 = o->node + 2 * k, *n1 = o->node + 2 * (k + 1);
+    o->elem[2 * k] = (n1[0] + n0[0]) * 0.5;          /* :146 */
+    o->elem[2 * k + 1] = (n1[1] + n0[1]) * 0.5;      /* :147 */
+    double dx = n1[0] - n0[0], dy = n1[1] - n0[1];
+    o->ds[k] = sqrt(dx * dx + dy * dy);              /* :148 */
+    o->tang[2 * k] = (n1[0] - n0[0]) / o->ds[k];     /* :149 */
+    o->tang[2 * k + 1] = (n1[1] - n0[1]) / o->ds[k]; /* :150 */
+    o->nrm[2 * k] = -o->tang[2 * k + 1];             /* :151 */
+    o->nrm[2 * k + 1] = o->tang[2 * k];              /* :152 */
+  }
+}
+
+static void ini_loading(oracle_t *o) {
+  int N = o->N;
+  double *sigmaN = o->Y + 2 * N;
+  for (int k = 0; k < N; k++) {
+    double t1 = o->tang[2 * k], t2 = o->tang[2 * k + 1];
+    double n1 = o->nrm[2 * k], n2 = o->nrm[2 * k + 1];
+    if (o->mode == DFN_MODE_III) {
+      /* src/initialisation.f90:161-171 (note: the normal-stress term uses the
+       * stress *rates* Sigma11_dot.. as the reference does) */
+      o->sld[k] = o->sld[k] - t2 * o->s31d + t1 * o->s32d;
+      sigmaN[k] = sigmaN[k] + n1 * (n1 * o->s11d + n2 * o->s12d) + n2 * (n1 * o->s12d + n2 * o->s22d);
+    } else {
+      /* :175-193; Sigma11/12/22 (static) are never read from the config and are 0 */
+      sigmaN[k] = sigmaN[k] + n1 * (n1 * 0.0 + n2 * 0.0) + n2 * (n1 * 0.0 + n2 * 0.0);
+      o->nld[k] = o->nld[k] + n1 * (n1 * o->s11d + n2 * o->s12d) + n2 * (n1 * o->s12d + n2 * o->s22d);
+      o->sld[k] = o->sld[k] + t1 * (n1 * o->s11d + n2 * o->s12d) + t2 * (n1 * o->s12d + n2 * o->s22d);
+    }
+  }
+  if (o->mode == DFN_MODE_II) { /* :199-200 */
+    memcpy(o->nrm_left, o->nrm, sizeof(double) * 2 * (size_t)N);
+    memcpy(o->nrm_right, o->nrm, sizeof(double) * 2 * (size_t)N);
+  }
+}
+
+/* find_fault_crossing, src/initialisation.f90:279-344.  Pairs of faults whose
+ * node bounding boxes are farther apart than the threshold cannot produce a
+ * crossing and are skipped; the crossing list and its order are unchanged. */
+static int find_crossings(oracle_t *o) {
+  int nf = o->nfault;
+  double *bb = (double *)xcalloc((size_t)nf * 4, sizeof(double));
+  double *minds = (double *)xcalloc((size_t)nf, sizeof(double));
+  for (int f = 0; f < nf; f++) {
+    int64_t m1 = o->map1[f];
+    double mn = INFINITY;
+    for (int64_t e = 0; e < o->fnel[f]; e++) mn = fmin(mn, o->ds[m1 - 1 + e]);
+    minds[f] = mn;
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int64_t n = 0; n <= o->fnel[f]; n++) {
+      const double *p = o->node + 2 * (m1 - 1 + n);
+      x0 = fmin(x0, p[0]); x1 = fmax(x1, p[0]); y0 = fmin(y0, p[1]); y1 = fmax(y1, p[1]);
+    }
+    bb[4 * f] = x0; bb[4 * f + 1] = x1; bb[4 * f + 2] = y0; bb[4 * f + 3] = y1;
+  }
+  int cap = 16, nc = 0;
+  int64_t *cr = (int64_t *)xcalloc((size_t)cap * 6, sizeof(int64_t));
+  for (int f1 = 0; f1 < nf; f1++) {
+    for (int f2 = f1 + 1; f2 < nf; f2++) {
+      double thr = (double)0.01f * fmin(minds[f1], minds[f2]); /* :299 single-precision 0.01 */
+      double gx = fmax(0.0, fmax(bb[4 * f1] - bb[4 * f2 + 1], bb[4 * f2] - bb[4 * f1 + 1]));
+      double gy = fmax(0.0, fmax(bb[4 * f1 + 2] - bb[4 * f2 + 3], bb[4 * f2 + 2] - bb[4 * f1 + 3]));
+      if (gx > thr || gy > thr) continue;
+      for (int64_t n1 = 1; n1 <= o->fnel[f1] + 1; n1++) {
+        const double *p1 = o->node + 2 * (o->map1[f1] - 1 + n1 - 1);
+        for (int64_t n2 = 1; n2 <= o->fnel[f2] + 1; n2++) {
+          const double *p2 = o->node + 2 * (o->map1[f2] - 1 + n2 - 1);
+          double dist = sqrt((p1[0] - p2[0]) * (p1[0] - p2[0]) + (p1[1] - p2[1]) * (p1[1] - p2[1]));
+          if (dist < thr) {
+            if (nc == cap) {
+              cap *= 2;
+              cr = (int64_t *)realloc(cr, sizeof(int64_t) * 6 * (size_t)cap);
+            }
+            int64_t *c = cr + 6 * nc++;
+            c[0] = f1 + 1; c[1] = f2 + 1;
+            c[2] = n1 + o->map1[f1] - 1; c[3] = n2 + o->map1[f2] - 1;
+            c[4] = n1; c[5] = n2;
+          }
+        }
+      }
+    }
+  }
+  free(bb); free(minds);
+  o->ncross = nc; o->cross = cr;
+  /* element indices node-1 and node must exist (the reference reads out of bounds otherwise) */
+  for (int c = 0; c < nc; c++)
+    for (int q = 2; q < 4; q++)
+      if (cr[6 * c + q] - 1 < 1 || cr[6 * c + q] > o->N) return DFN_ERR_ARG;
+  return DFN_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Kernels                                                                     */
+/* ------------------------------------------------------------------------- */
+/* directIII, src/kernel_calculation/directModeIII.f:7-74 */
+static void directIII(const oracle_t *o, const double *slip, double *stressT) {
+  int N = o->N;
+  const double *node = o->node;
+  double *t = (double *)xcalloc(2 * (size_t)N, sizeof(double));
+  double *el = (double *)xcalloc(2 * (size_t)N, sizeof(double));
+  for (int k = 0; k < N; k++) { /* :162-173 */
+    t[2 * k] = node[2 * (k + 1)] - node[2 * k];
+    t[2 * k + 1] = node[2 * (k + 1) + 1] - node[2 * k + 1];
+    double n_t = sqrt(t[2 * k] * t[2 * k] + t[2 * k + 1] * t[2 * k + 1]);
+    t[2 * k] = t[2 * k] / n_t;
+    t[2 * k + 1] = t[2 * k + 1] / n_t;
+    el[2 * k] = node[2 * k] + 0.5 * (node[2 * (k + 1)] - node[2 * k]);
+    el[2 * k + 1] = node[2 * k + 1] + 0.5 * (node[2 * (k + 1) + 1] - node[2 * k + 1]);
+  }
+  double scale_mu = o->mu, twopi = 2 * PI_REF;
+#pragma omp parallel for schedule(static) num_threads(o->nthreads)
+  for (int k = 0; k < N; k++) { /* :180-202 */
+    double s = 0.0;
+    for (int l = 1; l <= N + 1; l++) {
+      double dx = el[2 * k] - node[2 * (l - 1)], dy = el[2 * k + 1] - node[2 * (l - 1) + 1];
+      double r1 = sqrt(dx * dx + dy * dy);
+      double I1 = t[2 * k] / (r1 * r1) * dx + t[2 * k + 1] / (r1 * r1) * dy;
+      if ((l - 1 >= 1) && (l <= N)) s = s - (slip[l - 1] - slip[l - 2]) * I1;
+      else if (l == 1) s = s - slip[0] * I1;
+      else s = s + slip[l - 2] * I1;
+    }
+    stressT[k] = s * scale_mu / twopi; /* :206 */
+  }
+  free(t); free(el);
+}
+
+/* directII, src/kernel_calculation/directModeII.f90:7-107 */
+static void directII(const oracle_t *o, const double *slip, double *stressT, double *stressN) {
+  int N = o->N;
+  const double *node = o->node, *el = o->elem, *nm = o->nrm, *nl = o->nrm_left, *nr = o->nrm_right;
+#pragma omp parallel for schedule(static) num_threads(o->nthreads)
+  for (int k = 0; k < N; k++) {
+    double sT = 0.0, sN = 0.0;
+    double n1 = nm[2 * k], n2 = nm[2 * k + 1];
+    for (int l = 0; l < N; l++) {
+      double exl = el[2 * k] - node[2 * l], eyl = el[2 * k + 1] - node[2 * l + 1];
+      double exr = el[2 * k] - node[2 * (l + 1)], eyr = el[2 * k + 1] - node[2 * (l + 1) + 1];
+      double r_left = sqrt(exl * exl + eyl * eyl);
+      double r_right = sqrt(exr * exr + eyr * eyr);
+      double g1l = exl / r_left, g2l = eyl / r_left, g1r = exr / r_right, g2r = eyr / r_right;
+      double cl = 4.0 * n1 * n2 * g1l * g2l + (n2 * n2 - n1 * n1) * (g2l * g2l - g1l * g1l);
+      double cr = 4.0 * n1 * n2 * g1r * g2r + (n2 * n2 - n1 * n1) * (g2r * g2r - g1r * g1r);
+      double I1tl = cl * g1l / r_left, I2tl = cl * g2l / r_left;
+      double I1tr = cr * g1r / r_right, I2tr = cr * g2r / r_right;
+      double dl = 2.0 * n1 * n2 * (g2l * g2l - g1l * g1l) - (n2 * n2 - n1 * n1) * 2.0 * g1l * g2l;
+      double dr = 2.0 * n1 * n2 * (g2r * g2r - g1r * g1r) - (n2 * n2 - n1 * n1) * 2.0 * g1r * g2r;
+      double I1nl = g1l / r_left - g2l / r_left * dl, I2nl = g2l / r_left + g1l / r_left * dl;
+      double I1nr = g1r / r_right - g2r / r_right * dr, I2nr = g2r / r_right + g1r / r_right * dr;
+      sT = sT + slip[l] * (nl[2 * l + 1] * I1tl - nr[2 * l + 1] * I1tr - nl[2 * l] * I2tl + nr[2 * l] * I2tr);
+      sN = sN + slip[l] * (nl[2 * l] * I1nl - nr[2 * l] * I1nr + nl[2 * l + 1] * I2nl - nr[2 * l + 1] * I2nr);
+    }
+    double sc = o->mu / PI_REF * (1.0 - o->cs * o->cs / (o->cp * o->cp)); /* :102-103 */
+    stressN[k] = sc * sN;
+    stressT[k] = sc * sT;
+  }
+}
+
+/* H-order geometry for the matrix-entry functions. */
+typedef struct {
+  const double *el, *nl_node, *nr_node, *tang, *nrm, *nrm_l, *nrm_r; /* H order, [2*n] */
+} hgeom;
+
+/* eval_kernel_ModeIII, src/hmatrix/eval_kernel_hmatrix.f90:7-52; k,l 0-based H order */
+static double eval_III(const hgeom *g, int k, int l) {
+  double e1 = g->el[2 * k], e2 = g->el[2 * k + 1];
+  double a1 = g->nl_node[2 * l] - e1, a2 = g->nl_node[2 * l + 1] - e2;
+  double b1 = g->nr_node[2 * l] - e1, b2 = g->nr_node[2 * l + 1] - e2;
+  double r1 = sqrt(a1 * a1 + a2 * a2);
+  double r2 = sqrt(b1 * b1 + b2 * b2);
+  double t1 = g->tang[2 * k], t2 = g->tang[2 * k + 1];
+  double I1 = t1 / (r1 * r1) * (e1 - g->nl_node[2 * l]) + t2 / (r1 * r1) * (e2 - g->nl_node[2 * l + 1]);
+  double I2 = t1 / (r2 * r2) * (e1 - g->nr_node[2 * l]) + t2 / (r2 * r2) * (e2 - g->nr_node[2 * l + 1]);
+  return -I1 + I2;
+}
+
+/* eval_kernel_ModeII_T, eval_kernel_hmatrix.f90:57-120 */
+static double eval_II_T(const hgeom *g, int k, int l) {
+  double e1 = g->el[2 * k], e2 = g->el[2 * k + 1];
+  double exl = e1 - g->nl_node[2 * l], eyl = e2 - g->nl_node[2 * l + 1];
+  double exr = e1 - g->nr_node[2 * l], eyr = e2 - g->nr_node[2 * l + 1];
+  double r_left = sqrt(exl * exl + eyl * eyl), r_right = sqrt(exr * exr + eyr * eyr);
+  double g1l = exl / r_left, g2l = eyl / r_left, g1r = exr / r_right, g2r = eyr / r_right;
+  double n1 = g->nrm[2 * k], n2 = g->nrm[2 * k + 1];
+  double cl = 4.0 * n1 * n2 * g1l * g2l + (n2 * n2 - n1 * n1) * (g2l * g2l - g1l * g1l);
+  double cr = 4.0 * n1 * n2 * g1r * g2r + (n2 * n2 - n1 * n1) * (g2r * g2r - g1r * g1r);
+  double I1tl = cl * g1l / r_left, I2tl = cl * g2l / r_left;
+  double I1tr = cr * g1r / r_right, I2tr = cr * g2r / r_right;
+  return g->nrm_l[2 * l + 1] * I1tl - g->nrm_r[2 * l + 1] * I1tr - g->nrm_l[2 * l] * I2tl + g->nrm_r[2 * l] * I2tr;
+}
+
+/* eval_kernel_ModeII_N, eval_kernel_hmatrix.f90:125-199 */
+static double eval_II_N(const hgeom *g, int k, int l) {
+  double e1 = g->el[2 * k], e2 = g->el[2 * k + 1];
+  double exl = e1 - g->nl_node[2 * l], eyl = e2 - g->nl_node[2 * l + 1];
+  double exr = e1 - g->nr_node[2 * l], eyr = e2 - g->nr_node[2 * l + 1];
+  double r_left = sqrt(exl * exl + eyl * eyl), r_right = sqrt(exr * exr + eyr * eyr);
+  double g1l = exl / r_left, g2l = eyl / r_left, g1r = exr / r_right, g2r = eyr / r_right;
+  double n1 = g->nrm[2 * k], n2 = g->nrm[2 * k + 1];
+  double dl = 2.0 * n1 * n2 * (g2l * g2l - g1l * g1l) - (n2 * n2 - n1 * n1) * 2.0 * g1l * g2l;
+  double dr = 2.0 * n1 * n2 * (g2r * g2r - g1r * g1r) - (n2 * n2 - n1 * n1) * 2.0 * g1r * g2r;
+  double I1nl = g1l / r_left - g2l / r_left * dl, I2nl = g2l / r_left + g1l / r_left * dl;
+  double I1nr = g1r / r_right - g2r / r_right * dr, I2nr = g2r / r_right + g1r / r_right * dr;
+  return g->nrm_l[2 * l] * I1nl - g->nrm_r[2 * l] * I1nr + g->nrm_l[2 * l + 1] * I2nl - g->nrm_r[2 * l + 1] * I2nr;
+}
+
+typedef double (*eval_fn)(const hgeom *, int, int);
+
+/* ------------------------------------------------------------------------- */
+/* Cluster tree, src/hmatrix/cluster_tree.f90                                  */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  double *cnod; /* [2*n] */
+  int *mat_corr;
+  int nleaf;
+  int *ind, *sons;
+  int ind_rec, nbclus;
+  int *buf;
+  double *cbuf;
+  int fail;
+} ctree_ctx;
+
+/* cluster_affiliation + compute_maximum_direction + compute_cluster_affiliation
+ * (cluster_tree.f90:261-534) and reordering_unknowns (:595-712). f,l 1-based. */
+static void affiliation(ctree_ctx *c, int f, int l, int reorder, int sons[4]) {
+  double *cn = c->cnod;
+  double mnx = INFINITY, mxx = -INFINITY, mny = INFINITY, mxy = -INFINITY;
+  for (int i = f; i <= l; i++) {
+    mnx = fmin(mnx, cn[2 * (i - 1)]); mxx = fmax(mxx, cn[2 * (i - 1)]);
+    mny = fmin(mny, cn[2 * (i - 1) + 1]); mxy = fmax(mxy, cn[2 * (i - 1) + 1]);
+  }
+  int dir = ((mxy - mny) >= (mxx - mnx)) ? 1 : 0; /* :384-392 ties -> y */
+  double min_bd = dir ? mny : mnx, max_bd = dir ? mxy : mxx;
+  double mid = min_bd + (max_bd - min_bd) / 2; /* :472-473 */
+  int size = l - f + 1;
+  int *son1 = c->buf, *son2 = c->buf + size;
+  int d1 = 0, d2 = 0;
+  for (int i = f; i <= l; i++) {
+    if (cn[2 * (i - 1) + dir] >= mid) son1[d1++] = i;
+    else son2[d2++] = i;
+  }
+  if (d1 == 0 || d2 == 0) { c->fail = 1; sons[0] = f; sons[1] = l; sons[2] = l + 1; sons[3] = l; return; }
+  if (reorder) {
+    /* ordering_vector is a no-op on the already increasing lists; the group that
+     * holds the smallest index goes first (:662-707) */
+    int *first = son1, *second = son2, df = d1, dsec = d2;
+    if (!(son1[0] <= son2[0])) { first = son2; second = son1; df = d2; dsec = d1; }
+    double *tmpc = c->cbuf;
+    int *tmpm = c->buf + 2 * size;
+    for (int t = 0; t < df; t++) {
+      tmpc[2 * t] = cn[2 * (first[t] - 1)]; tmpc[2 * t + 1] = cn[2 * (first[t] - 1) + 1];
+      tmpm[t] = c->mat_corr[first[t] - 1];
+    }
+    for (int t = 0; t < dsec; t++) {
+      tmpc[2 * (df + t)] = cn[2 * (second[t] - 1)]; tmpc[2 * (df + t) + 1] = cn[2 * (second[t] - 1) + 1];
+      tmpm[df + t] = c->mat_corr[second[t] - 1];
+    }
+    memcpy(cn + 2 * (f - 1), tmpc, sizeof(double) * 2 * (size_t)size);
+    memcpy(c->mat_corr + (f - 1), tmpm, sizeof(int) * (size_t)size);
+    sons[0] = f; sons[1] = f + df - 1; sons[2] = f + df; sons[3] = l;
+  } else {
+    if (son1[0] < son2[0]) { sons[0] = son1[0]; sons[1] = son1[d1 - 1]; sons[2] = son2[0]; sons[3] = son2[d2 - 1]; }
+    else { sons[0] = son2[0]; sons[1] = son2[d2 - 1]; sons[2] = son1[0]; sons[3] = son1[d1 - 1]; }
+  }
+}
+
+/* parameters_cluster_tree, cluster_tree.f90:186-257 */
+static void params_ctree(ctree_ctx *c, int f, int l) {
+  if (l - f + 1 <= c->nleaf || c->fail) return;
+  int s[4];
+  affiliation(c, f, l, 1, s);
+  c->nbclus += 2;
+  params_ctree(c, s[0], s[1]);
+  params_ctree(c, s[2], s[3]);
+}
+
+/* create_cluster_tree, cluster_tree.f90:69-182 (preorder numbering) */
+static void create_ctree(ctree_ctx *c, int f, int l) {
+  int me = c->ind_rec;
+  c->ind[2 * me - 2] = f;
+  c->ind[2 * me - 1] = l;
+  if (l - f + 1 <= c->nleaf || c->fail) { c->sons[2 * me - 2] = 0; c->sons[2 * me - 1] = 0; return; }
+  int s[4];
+  affiliation(c, f, l, 0, s);
+  c->ind_rec++;
+  c->sons[2 * me - 2] = c->ind_rec;
+  create_ctree(c, s[0], s[1]);
+  c->ind_rec++;
+  c->sons[2 * me - 1] = c->ind_rec;
+  create_ctree(c, s[2], s[3]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Block cluster tree, src/hmatrix/block_cluster_tree.f90                      */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  const int *ind, *sons;
+  double *bbox; /* per cluster: minx maxx miny maxy */
+  double eta;
+  int *type, *row, *col;
+  int64_t *info;
+  int ind_rec, count, cap;
+} btree_ctx;
+
+/* admissibility_condition / distance / diameter, block_cluster_tree.f90:281-476 */
+static int admissible(const btree_ctx *b, int cr, int cc) {
+  const double *R = b->bbox + 4 * (cr - 1), *C = b->bbox + 4 * (cc - 1);
+  double diam_row = 0.0, diam_col = 0.0, dist = 0.0;
+  diam_row = diam_row + (R[1] - R[0]) * (R[1] - R[0]);
+  diam_row = diam_row + (R[3] - R[2]) * (R[3] - R[2]);
+  diam_row = sqrt(diam_row);
+  diam_col = diam_col + (C[1] - C[0]) * (C[1] - C[0]);
+  diam_col = diam_col + (C[3] - C[2]) * (C[3] - C[2]);
+  diam_col = sqrt(diam_col);
+  for (int d = 0; d < 2; d++) {
+    double a = fmax(0.0, R[2 * d] - C[2 * d + 1]);
+    double bb = fmax(0.0, C[2 * d] - R[2 * d + 1]);
+    dist = dist + a * a + bb * bb;
+  }
+  dist = sqrt(dist);
+  return b->eta * dist >= fmax(diam_row, diam_col);
+}
+
+static void count_btree(btree_ctx *b, int cr, int cc) { /* parameters_blcluster_tree :187-277 */
+  if (!admissible(b, cr, cc) && b->sons[2 * cr - 1] != 0 && b->sons[2 * cc - 1] != 0) {
+    for (int ir = -1; ir <= 0; ir++)
+      for (int ic = -1; ic <= 0; ic++) {
+        b->count++;
+        count_btree(b, b->sons[2 * cr + ir - 1], b->sons[2 * cc + ic - 1]);
+      }
+  }
+}
+
+static void create_btree(btree_ctx *b, int cr, int cc) { /* create_blcluster_tree :57-183 */
+  int me = b->ind_rec;
+  b->row[me - 1] = cr;
+  b->col[me - 1] = cc;
+  if (admissible(b, cr, cc)) { b->type[me - 1] = 1; return; }
+  if (b->sons[2 * cr - 1] != 0 && b->sons[2 * cc - 1] != 0) {
+    b->type[me - 1] = 2;
+    int ns = 0;
+    for (int ir = -1; ir <= 0; ir++)
+      for (int ic = -1; ic <= 0; ic++) {
+        b->ind_rec++;
+        b->info[6 * (me - 1) + ns++] = b->ind_rec;
+        create_btree(b, b->sons[2 * cr + ir - 1], b->sons[2 * cc + ic - 1]);
+      }
+  } else {
+    b->type[me - 1] = 0;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Small dense linear algebra for the recompression (no LAPACK in the image):  */
+/* Householder QR and one-sided Jacobi SVD.                                    */
+/* ------------------------------------------------------------------------- */
+/* In-place thin QR of M (m x k, column-major, m >= k).  On exit Q (m x k) and
+ * R (k x k, upper, column-major). */
+static void qr_thin(int m, int k, const double *M, double *Q, double *R) {
+  double *W = (double *)xcalloc((size_t)m * k, sizeof(double));
+  double *v = (double *)xcalloc((size_t)m * k, sizeof(double));
+  double *beta = (double *)xcalloc((size_t)k, sizeof(double));
+  memcpy(W, M, sizeof(double) * (size_t)m * k);
+  for (int j = 0; j < k; j++) {
+    double nrm = 0.0;
+    for (int i = j; i < m; i++) nrm += W[i + (size_t)j * m] * W[i + (size_t)j * m];
+    nrm = sqrt(nrm);
+    double *vj = v + (size_t)j * m;
+    if (nrm == 0.0) { beta[j] = 0.0; continue; }
+    double alpha = W[j + (size_t)j * m] >= 0 ? -nrm : nrm;
+    for (int i = j; i < m; i++) vj[i] = W[i + (size_t)j * m];
+    vj[j] -= alpha;
+    double vv = 0.0;
+    for (int i = j; i < m; i++) vv += vj[i] * vj[i];
+    beta[j] = vv > 0 ? 2.0 / vv : 0.0;
+    for (int c = j; c < k; c++) {
+      double s = 0.0;
+      for (int i = j; i < m; i++) s += vj[i] * W[i + (size_t)c * m];
+      s *= beta[j];
+      for (int i = j; i < m; i++) W[i + (size_t)c * m] -= s * vj[i];
+    }
+  }
+  for (int c = 0; c < k; c++)
+    for (int r = 0; r < k; r++) R[r + (size_t)c * k] = (r <= c) ? W[r + (size_t)c * m] : 0.0;
+  /* Q = H_0 H_1 ... H_{k-1} applied to the first k columns of I */
+  memset(Q, 0, sizeof(double) * (size_t)m * k);
+  for (int c = 0; c < k; c++) Q[c + (size_t)c * m] = 1.0;
+  for (int j = k - 1; j >= 0; j--) {
+    if (beta[j] == 0.0) continue;
+    double *vj = v + (size_t)j * m;
+    for (int c = 0; c < k; c++) {
+      double s = 0.0;
+      for (int i = j; i < m; i++) s += vj[i] * Q[i + (size_t)c * m];
+      s *= beta[j];
+      for (int i = j; i < m; i++) Q[i + (size_t)c * m] -= s * vj[i];
+    }
+  }
+  free(W); free(v); free(beta);
+}
+
+/* SVD of a k x k matrix A = U diag(S) V^T by one-sided Jacobi (Hestenes).
+ * S descending, U and V column-major k x k. */
+static void svd_jacobi(int k, const double *A, double *U, double *S, double *V) {
+  double *W = (double *)xcalloc((size_t)k * k, sizeof(double));
+  memcpy(W, A, sizeof(double) * (size_t)k * k);
+  memset(V, 0, sizeof(double) * (size_t)k * k);
+  for (int i = 0; i < k; i++) V[i + (size_t)i * k] = 1.0;
+  for (int sweep = 0; sweep < 60; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < k - 1; p++)
+      for (int q = p + 1; q < k; q++) {
+        double alpha = 0, beta = 0, gamma = 0;
+        for (int i = 0; i < k; i++) {
+          double wp = W[i + (size_t)p * k], wq = W[i + (size_t)q * k];
+          alpha += wp * wp; beta += wq * wq; gamma += wp * wq;
+        }
+        if (gamma == 0.0) continue;
+        double c0 = fabs(gamma) / sqrt(alpha * beta);
+        if (!(c0 > 1e-15)) continue;
+        off = fmax(off, c0);
+        double zeta = (beta - alpha) / (2.0 * gamma);
+        double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+        for (int i = 0; i < k; i++) {
+          double wp = W[i + (size_t)p * k], wq = W[i + (size_t)q * k];
+          W[i + (size_t)p * k] = c * wp - s * wq;
+          W[i + (size_t)q * k] = s * wp + c * wq;
+          double vp = V[i + (size_t)p * k], vq = V[i + (size_t)q * k];
+          V[i + (size_t)p * k] = c * vp - s * vq;
+          V[i + (size_t)q * k] = s * vp + c * vq;
+        }
+      }
+    if (off <= 1e-15) break;
+  }
+  int *ord = (int *)xcalloc((size_t)k, sizeof(int));
+  double *sv = (double *)xcalloc((size_t)k, sizeof(double));
+  for (int j = 0; j < k; j++) {
+    double s = 0.0;
+    for (int i = 0; i < k; i++) s += W[i + (size_t)j * k] * W[i + (size_t)j * k];
+    sv[j] = sqrt(s);
+    ord[j] = j;
+  }
+  for (int a = 0; a < k; a++) /* sort descending (stable) */
+    for (int b = a + 1; b < k; b++)
+      if (sv[ord[b]] > sv[ord[a]]) { int t = ord[a]; ord[a] = ord[b]; ord[b] = t; }
+  double *V2 = (double *)xcalloc((size_t)k * k, sizeof(double));
+  for (int j = 0; j < k; j++) {
+    int o = ord[j];
+    S[j] = sv[o];
+    for (int i = 0; i < k; i++) {
+      U[i + (size_t)j * k] = sv[o] > 0 ? W[i + (size_t)o * k] / sv[o] : (i == j ? 1.0 : 0.0);
+      V2[i + (size_t)j * k] = V[i + (size_t)o * k];
+    }
+  }
+  memcpy(V, V2, sizeof(double) * (size_t)k * k);
+  free(W); free(ord); free(sv); free(V2);
+}
+
+/* compression_low_rank, src/hmatrix/H_matrix_recompression.f90:145-301.
+ * vA (m x inrank), vB (n x inrank) are overwritten with the recompressed factors. */
+static int compression_low_rank(int m, int n, int inrank, double *vA, double *vB, double eps) {
+  int outrank = 0;
+  int mn = m < n ? m : n;
+  if (inrank < mn) {
+    int k = inrank;
+    double *Qa = xcalloc((size_t)m * k, sizeof(double)), *Ra = xcalloc((size_t)k * k, sizeof(double));
+    double *Qb = xcalloc((size_t)n * k, sizeof(double)), *Rb = xcalloc((size_t)k * k, sizeof(double));
+    qr_thin(m, k, vA, Qa, Ra);
+    qr_thin(n, k, vB, Qb, Rb);
+    double *R = xcalloc((size_t)k * k, sizeof(double));
+    for (int c = 0; c < k; c++) /* :199-217, R = Ra Rb^T via ZDOTU over the upper rows */
+      for (int r = 0; r < k; r++) {
+        double s = 0.0;
+        for (int l = 0; l < k; l++) {
+          double ra = (l >= r) ? Ra[r + (size_t)l * k] : 0.0;
+          double rb = (l >= c) ? Rb[c + (size_t)l * k] : 0.0;
+          s = s + ra * rb;
+        }
+        R[r + (size_t)c * k] = s;
+      }
+    double *U = xcalloc((size_t)k * k, sizeof(double)), *S = xcalloc((size_t)k, sizeof(double));
+    double *V = xcalloc((size_t)k * k, sizeof(double));
+    svd_jacobi(k, R, U, S, V);
+    /* SVD_factorization :485-596: keep through the first sigma_i <= eps*sigma_1 */
+    for (int i = 0; i < k; i++) {
+      outrank++;
+      if (S[i] <= eps * S[0]) break;
+    }
+    /* A = Qa (U S), B = Qb V  (ZGEMM :247-269) */
+    memset(vA, 0, sizeof(double) * (size_t)m * inrank);
+    memset(vB, 0, sizeof(double) * (size_t)n * inrank);
+    for (int j = 0; j < outrank; j++) {
+      for (int l = 0; l < k; l++) {
+        double tA = U[l + (size_t)j * k] * S[j];
+        for (int i = 0; i < m; i++) vA[i + (size_t)j * m] += tA * Qa[i + (size_t)l * m];
+        double tB = V[l + (size_t)j * k];
+        for (int i = 0; i < n; i++) vB[i + (size_t)j * n] += tB * Qb[i + (size_t)l * n];
+      }
+    }
+    free(Qa); free(Ra); free(Qb); free(Rb); free(R); free(U); free(S); free(V);
+  } else {
+    /* F = A B^T (ZGEMM 'N','T' :283-284) then ACA_full_pivot (:305-423) */
+    double *F = xcalloc((size_t)m * n, sizeof(double));
+    for (int j = 0; j < n; j++)
+      for (int l = 0; l < inrank; l++) {
+        double t = vB[j + (size_t)l * n];
+        for (int i = 0; i < m; i++) F[i + (size_t)j * m] = F[i + (size_t)j * m] + t * vA[i + (size_t)l * m];
+      }
+    memset(vA, 0, sizeof(double) * (size_t)m * inrank);
+    memset(vB, 0, sizeof(double) * (size_t)n * inrank);
+    size_t mnF = (size_t)m * n;
+    double nF = 0.0;
+    for (size_t t = 0; t < mnF; t++) nF = nF + F[t] * F[t];
+    nF = sqrt(nF);
+    int rank;
+    int done = 0;
+    for (rank = 1; rank <= mn; rank++) {
+      double maxR = 0.0;
+      size_t ind = 0;
+      for (size_t t = 0; t < mnF; t++)
+        if (fabs(F[t]) > maxR) { maxR = fabs(F[t]); ind = t; } /* IZAMAX: first max */
+      if (maxR == 0.0) { done = 1; break; }
+      int ii = (int)(ind % (size_t)m), jj = (int)(ind / (size_t)m);
+      double gamma = 1.0 / F[ind];
+      for (int i = 0; i < m; i++) vA[i + (size_t)(rank - 1) * m] = gamma * F[i + (size_t)jj * m];
+      for (int j = 0; j < n; j++) vB[j + (size_t)(rank - 1) * n] = F[ii + (size_t)j * m];
+      for (int j = 0; j < n; j++) {
+        double t = -vB[j + (size_t)(rank - 1) * n];
+        for (int i = 0; i < m; i++) F[i + (size_t)j * m] = F[i + (size_t)j * m] + t * vA[i + (size_t)(rank - 1) * m];
+      }
+      double nR = 0.0;
+      for (size_t t = 0; t < mnF; t++) nR = nR + F[t] * F[t];
+      nR = sqrt(nR);
+      if (nR <= eps * nF) { done = 1; break; }
+    }
+    (void)done;
+    if (rank > mn) rank = mn; /* completed loop: the extra column would be zero */
+    outrank = rank;
+    free(F);
+  }
+  return outrank;
+}
+
+/* compute_full_ACA, src/hmatrix/ACA_algorithm.f90:665-826, on block rows
+ * r0..r0+m-1, cols c0..c0+n-1 (0-based H order).  A (m x max_rank) and
+ * B (n x max_rank) must be zeroed by the caller.  Returns low_rank. */
+static int full_aca(const hgeom *g, eval_fn ev, int r0, int m, int c0, int n, double eps,
+                    int max_rank, double *A, double *B) {
+  size_t mn = (size_t)m * n;
+  double *R = (double *)malloc(sizeof(double) * mn);
+  for (int i = 0; i < m; i++) /* :90-95 */
+    for (int j = 0; j < n; j++) R[i + (size_t)j * m] = ev(g, r0 + i, c0 + j);
+  double nA = 0.0; /* ZLANGE('F') */
+  for (size_t t = 0; t < mn; t++) nA = nA + R[t] * R[t];
+  nA = sqrt(nA);
+  const double piv_tol = (double)1.0e-14f; /* 10.0**(-14) in single precision, :117 */
+  int low_rank = 0, ind_rec;
+  int mnr = m < n ? m : n;
+  for (ind_rec = 1; ind_rec <= mnr; ind_rec++) {
+    /* ind_rec > max_low_rank would STOP (:106-109); unreachable, see the memory check below */
+    double best = -1.0;
+    size_t p = 0;
+    for (size_t t = 0; t < mn; t++)
+      if (fabs(R[t]) > best) { best = fabs(R[t]); p = t; } /* maxloc: first max, column-major */
+    int ip = (int)(p % (size_t)m), jp = (int)(p / (size_t)m);
+    if (fabs(R[p]) <= piv_tol) {
+      for (int j = 0; j < n; j++) B[j + (size_t)(ind_rec - 1) * n] = 0.0;
+      low_rank = ind_rec - 1;
+      break;
+    }
+    double piv = R[ip + (size_t)jp * m];
+    for (int i = 0; i < m; i++) A[i + (size_t)(ind_rec - 1) * m] = R[i + (size_t)jp * m];
+    for (int j = 0; j < n; j++) B[j + (size_t)(ind_rec - 1) * n] = R[ip + (size_t)j * m] / piv;
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < n; j++)
+        R[i + (size_t)j * m] = R[i + (size_t)j * m] - A[i + (size_t)(ind_rec - 1) * m] * B[j + (size_t)(ind_rec - 1) * n];
+    double nR = 0.0;
+    for (size_t t = 0; t < mn; t++) nR = nR + R[t] * R[t];
+    nR = sqrt(nR);
+    if (nR <= eps * nA) { low_rank = ind_rec; break; }
+    if (ind_rec + 1 > max_rank) { low_rank = ind_rec; break; } /* memory check :153-159 */
+  }
+  if (low_rank == 0) low_rank = ind_rec; /* :162-166 */
+  free(R);
+  return low_rank;
+}
+
+static int cl_first(const ohmat *H, int c) { return H->ind_clusters[2 * c - 2]; }
+static int cl_last(const ohmat *H, int c) { return H->ind_clusters[2 * c - 1]; }
+
+static void ohmat_free(ohmat *H) {
+  if (!H) return;
+  free(H->mat_corr); free(H->ind_clusters); free(H->sons); free(H->bl_type); free(H->bl_row);
+  free(H->bl_col); free(H->info); free(H->A); free(H->B); free(H->F);
+  free(H);
+}
+
+/* initiate_hmat, src/hmatrix/initialisation_hmat.f90:13-100 (cluster tree,
+ * reorder, block tree, build_H_matrix, compress_H_matrix). */
+static int build_ohmat(oracle_t *o, int op, ohmat **out) {
+  int n = o->N;
+  ohmat *H = (ohmat *)xcalloc(1, sizeof(ohmat));
+  H->n = n;
+  /* --- cluster tree on the element midpoints --- */
+  ctree_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.cnod = dup_or_zero(o->elem, 2 * n);
+  c.mat_corr = (int *)xcalloc((size_t)n, sizeof(int));
+  for (int i = 0; i < n; i++) c.mat_corr[i] = i + 1;
+  c.nleaf = o->nleaf;
+  c.buf = (int *)xcalloc(3 * (size_t)n, sizeof(int));
+  c.cbuf = (double *)xcalloc(2 * (size_t)n, sizeof(double));
+  c.nbclus = 1;
+  params_ctree(&c, 1, n);
+  if (c.fail) { free(c.cnod); free(c.mat_corr); free(c.buf); free(c.cbuf); free(H); return DFN_ERR_ARG; }
+  c.ind = (int *)xcalloc(2 * (size_t)c.nbclus, sizeof(int));
+  c.sons = (int *)xcalloc(2 * (size_t)c.nbclus, sizeof(int));
+  c.ind_rec = 1;
+  create_ctree(&c, 1, n);
+  H->nbclus = c.nbclus;
+  H->mat_corr = c.mat_corr;
+  H->ind_clusters = c.ind;
+  H->sons = c.sons;
+  /* --- geometry in H order (forward_reorder, initialisation_hmat.f90:65-72) --- */
+  double *nl = xcalloc(2 * (size_t)n, sizeof(double)), *nr = xcalloc(2 * (size_t)n, sizeof(double));
+  double *tg = xcalloc(2 * (size_t)n, sizeof(double)), *nm = xcalloc(2 * (size_t)n, sizeof(double));
+  double *nml = xcalloc(2 * (size_t)n, sizeof(double)), *nmr = xcalloc(2 * (size_t)n, sizeof(double));
+  for (int k = 0; k < n; k++) {
+    int s = H->mat_corr[k] - 1;
+    for (int d = 0; d < 2; d++) {
+      nl[2 * k + d] = o->node[2 * s + d];        /* node_left = node(:,1:N) */
+      nr[2 * k + d] = o->node[2 * (s + 1) + d];  /* node_right = node(:,2:N+1) */
+      tg[2 * k + d] = o->tang[2 * s + d];
+      nm[2 * k + d] = o->nrm[2 * s + d];
+      nml[2 * k + d] = o->nrm[2 * s + d];
+      nmr[2 * k + d] = o->nrm[2 * s + d];
+    }
+  }
+  hgeom g = {c.cnod, nl, nr, tg, nm, nml, nmr};
+  eval_fn ev = (o->mode == DFN_MODE_III) ? eval_III : (op == DFN_OP_T ? eval_II_T : eval_II_N);
+  /* --- block cluster tree --- */
+  btree_ctx b;
+  memset(&b, 0, sizeof(b));
+  b.ind = H->ind_clusters;
+  b.sons = H->sons;
+  b.eta = o->eta;
+  b.bbox = (double *)xcalloc(4 * (size_t)H->nbclus, sizeof(double));
+  for (int cl = 1; cl <= H->nbclus; cl++) {
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int i = cl_first(H, cl); i <= cl_last(H, cl); i++) {
+      x0 = fmin(x0, c.cnod[2 * (i - 1)]); x1 = fmax(x1, c.cnod[2 * (i - 1)]);
+      y0 = fmin(y0, c.cnod[2 * (i - 1) + 1]); y1 = fmax(y1, c.cnod[2 * (i - 1) + 1]);
+    }
+    b.bbox[4 * (cl - 1)] = x0; b.bbox[4 * (cl - 1) + 1] = x1;
+    b.bbox[4 * (cl - 1) + 2] = y0; b.bbox[4 * (cl - 1) + 3] = y1;
+  }
+  b.count = 1;
+  count_btree(&b, 1, 1);
+  H->nbblclus = b.count;
+  b.type = H->bl_type = (int *)xcalloc((size_t)b.count, sizeof(int));
+  b.row = H->bl_row = (int *)xcalloc((size_t)b.count, sizeof(int));
+  b.col = H->bl_col = (int *)xcalloc((size_t)b.count, sizeof(int));
+  b.info = H->info = (int64_t *)xcalloc(6 * (size_t)b.count, sizeof(int64_t));
+  b.ind_rec = 1;
+  create_btree(&b, 1, 1);
+  free(b.bbox);
+  /* --- build_H_matrix (H_matrix_construction.f90:14-110): ranks and sizes, then factors.
+   * The reference runs compute_full_ACA twice per block (the second time with the
+   * first run's rank as the memory bound); both runs are deterministic and return
+   * the same factors, so one run per block is kept here. */
+  int nb = H->nbblclus;
+  double **Af = (double **)xcalloc((size_t)nb, sizeof(double *));
+  double **Bf = (double **)xcalloc((size_t)nb, sizeof(double *));
+  int64_t dimA = 0, dimB = 0, dimF = 0;
+  int status = DFN_OK;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(o->nthreads) reduction(+ : dimA, dimB, dimF)
+  for (int bl = 0; bl < nb; bl++) {
+    int cr = H->bl_row[bl], cc = H->bl_col[bl];
+    int r0 = cl_first(H, cr), m = cl_last(H, cr) - r0 + 1;
+    int c0 = cl_first(H, cc), nn = cl_last(H, cc) - c0 + 1;
+    if (H->bl_type[bl] == 1) {
+      double *A = xcalloc((size_t)m * o->max_rank, sizeof(double));
+      double *B = xcalloc((size_t)nn * o->max_rank, sizeof(double));
+      int k = full_aca(&g, ev, r0 - 1, m, c0 - 1, nn, o->eps_aca, o->max_rank, A, B);
+      Af[bl] = A; Bf[bl] = B;
+      H->info[6 * bl + 4] = k;
+      H->info[6 * bl + 5] = k;
+      dimA += 4 * (int64_t)m * k; /* :245-246 */
+      dimB += 4 * (int64_t)nn * k;
+    } else if (H->bl_type[bl] == 0) {
+      dimF += (int64_t)m * nn;
+    }
+  }
+  H->dimA = dimA; H->dimB = dimB; H->dimF = dimF;
+  H->A = xcalloc((size_t)dimA, sizeof(double));
+  H->B = xcalloc((size_t)dimB, sizeof(double));
+  H->F = xcalloc((size_t)dimF, sizeof(double));
+  if (!H->A || !H->B || !H->F) status = DFN_ERR_NOMEM;
+  int64_t posA = 1, posB = 1, posF = 0;
+  for (int bl = 0; bl < nb && status == DFN_OK; bl++) { /* compute_H_matrix :281-379 */
+    int cr = H->bl_row[bl], cc = H->bl_col[bl];
+    int r0 = cl_first(H, cr), m = cl_last(H, cr) - r0 + 1;
+    int c0 = cl_first(H, cc), nn = cl_last(H, cc) - c0 + 1;
+    if (H->bl_type[bl] == 1) {
+      int k = (int)H->info[6 * bl + 4];
+      memcpy(H->A + posA - 1, Af[bl], sizeof(double) * (size_t)m * k);
+      memcpy(H->B + posB - 1, Bf[bl], sizeof(double) * (size_t)nn * k);
+      H->info[6 * bl + 0] = posA;                   /* :538-543 */
+      H->info[6 * bl + 1] = posA + (int64_t)m * k - 1;
+      H->info[6 * bl + 2] = posB;
+      H->info[6 * bl + 3] = posB + (int64_t)nn * k - 1;
+      posA += 4 * (int64_t)m * k;
+      posB += 4 * (int64_t)nn * k;
+    } else if (H->bl_type[bl] == 0) {               /* compute_full_matrices :555-704 */
+      H->info[6 * bl + 0] = posF + 1;
+      for (int j = 0; j < nn; j++)
+        for (int i = 0; i < m; i++) H->F[posF + (int64_t)j * m + i] = ev(&g, r0 - 1 + i, c0 - 1 + j);
+      posF += (int64_t)m * nn;
+      H->info[6 * bl + 1] = posF;
+    }
+  }
+  for (int bl = 0; bl < nb; bl++) { free(Af[bl]); free(Bf[bl]); }
+  free(Af); free(Bf);
+  /* --- compress_H_matrix (H_matrix_recompression.f90:13-141) --- */
+  if (status == DFN_OK) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(o->nthreads)
+    for (int bl = 0; bl < nb; bl++) {
+      if (H->bl_type[bl] != 1) continue;
+      int cr = H->bl_row[bl], cc = H->bl_col[bl];
+      int m = cl_last(H, cr) - cl_first(H, cr) + 1, nn = cl_last(H, cc) - cl_first(H, cc) + 1;
+      int inrank = (int)H->info[6 * bl + 4];
+      int64_t sA = H->info[6 * bl + 0], sB = H->info[6 * bl + 2];
+      int outr = compression_low_rank(m, nn, inrank, H->A + sA - 1, H->B + sB - 1, o->eps_aca);
+      H->info[6 * bl + 1] = sA + (int64_t)m * outr - 1;
+      H->info[6 * bl + 3] = sB + (int64_t)nn * outr - 1;
+      H->info[6 * bl + 4] = outr;
+    }
+  }
+  free(c.cnod); free(c.buf); free(c.cbuf);
+  free(nl); free(nr); free(tg); free(nm); free(nml); free(nmr);
+  if (status != DFN_OK) { ohmat_free(H); return status; }
+  *out = H;
+  return DFN_OK;
+}
+
+/* Hgemv_right, src/hmatrix/calculation_hmat.f90:3-165, real FP64 with the
+ * reference BLAS ZGEMV loop order. */
+static int nbrow_of(const ohmat *H, int b) { return cl_last(H, H->bl_row[b - 1]) - cl_first(H, H->bl_row[b - 1]) + 1; }
+static int nbcol_of(const ohmat *H, int b) { return cl_last(H, H->bl_col[b - 1]) - cl_first(H, H->bl_col[b - 1]) + 1; }
+
+static void gemv_n(int m, int n, double alpha, const double *A, const double *x, double beta, double *y) {
+  if (beta != 1.0) {
+    if (beta == 0.0) for (int i = 0; i < m; i++) y[i] = 0.0;
+    else for (int i = 0; i < m; i++) y[i] = beta * y[i];
+  }
+  for (int j = 0; j < n; j++) {
+    double t = alpha * x[j];
+    for (int i = 0; i < m; i++) y[i] = y[i] + t * A[i + (size_t)j * m];
+  }
+}
+
+static void hgemv_rec(const ohmat *H, int ind, double alpha, double beta, int nrow, int ncol,
+                      const double *x, double *y, double *zbuf) {
+  int t = H->bl_type[ind - 1];
+  const int64_t *inf = H->info + 6 * (ind - 1);
+  if (t == 2) {
+    int s1 = (int)inf[0], s2 = (int)inf[1], s3 = (int)inf[2], s4 = (int)inf[3];
+    int r11 = nbrow_of(H, s1), c11 = nbcol_of(H, s1);
+    hgemv_rec(H, s1, alpha, beta, r11, c11, x, y, zbuf);
+    hgemv_rec(H, s2, alpha, 1.0, r11, ncol - c11, x + c11, y, zbuf);
+    hgemv_rec(H, s3, alpha, beta, nrow - r11, c11, x, y + r11, zbuf);
+    hgemv_rec(H, s4, alpha, 1.0, nrow - r11, ncol - c11, x + c11, y + r11, zbuf);
+  } else if (t == 1) {
+    int k = (int)inf[4];
+    const double *B = H->B + inf[2] - 1, *A = H->A + inf[0] - 1;
+    for (int j = 0; j < k; j++) { /* ZGEMV 'T', beta = 0 */
+      double tmp = 0.0;
+      for (int i = 0; i < ncol; i++) tmp = tmp + B[i + (size_t)j * ncol] * x[i];
+      zbuf[j] = 0.0 + 1.0 * tmp;
+    }
+    gemv_n(nrow, k, alpha, A, zbuf, beta, y);
+  } else {
+    gemv_n(nrow, ncol, alpha, H->F + inf[0] - 1, x, beta, y);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Flat block lists (exchange format shared with the device library)          */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  int n, nd, nl;
+  int *perm;
+  int *d_r0, *d_c0, *d_m, *d_n; int64_t *d_off; double *d_data; int64_t d_size;
+  int *l_r0, *l_c0, *l_m, *l_n, *l_k; int64_t *l_uoff, *l_voff; double *l_u, *l_v;
+  int64_t u_size, v_size;
+} flat_hmat;
+
+typedef struct dfn_handle_s {
+  oracle_t o;
+  flat_hmat *flat[2];
+} ohandle;
+
+static void flat_free(flat_hmat *f) {
+  if (!f) return;
+  free(f->perm); free(f->d_r0); free(f->d_c0); free(f->d_m); free(f->d_n); free(f->d_off); free(f->d_data);
+  free(f->l_r0); free(f->l_c0); free(f->l_m); free(f->l_n); free(f->l_k); free(f->l_uoff); free(f->l_voff);
+  free(f->l_u); free(f->l_v); free(f);
+}
+
+/* y_h = H x_h over a flat list: dense blocks then low-rank blocks, each with the
+ * ZGEMV loop order.  Threads split the row range into disjoint stripes so each
+ * output entry has a single writer and a fixed summation order. */
+static void flat_apply(const flat_hmat *f, const double *x, double *y, int nthreads) {
+  int n = f->n;
+  for (int i = 0; i < n; i++) y[i] = 0.0;
+  int nt = nthreads < 1 ? 1 : nthreads;
+  int maxk = 1;
+  for (int b = 0; b < f->nl; b++) if (f->l_k[b] > maxk) maxk = f->l_k[b];
+  double *z = xcalloc((size_t)f->nl * maxk, sizeof(double));
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nt)
+  for (int b = 0; b < f->nl; b++) {
+    const double *V = f->l_v + f->l_voff[b];
+    for (int j = 0; j < f->l_k[b]; j++) {
+      double t = 0.0;
+      for (int i = 0; i < f->l_n[b]; i++) t = t + V[i + (size_t)j * f->l_n[b]] * x[f->l_c0[b] + i];
+      z[(size_t)b * maxk + j] = t;
+    }
+  }
+#pragma omp parallel num_threads(nt)
+  {
+    int tid = 0, nth = 1;
+#ifdef _OPENMP
+    tid = omp_get_thread_num(); nth = omp_get_num_threads();
+#endif
+    int lo = (int)((int64_t)n * tid / nth), hi = (int)((int64_t)n * (tid + 1) / nth);
+    for (int b = 0; b < f->nd; b++) {
+      int r0 = f->d_r0[b], m = f->d_m[b];
+      int a = r0 > lo ? r0 : lo, e = (r0 + m) < hi ? (r0 + m) : hi;
+      if (a >= e) continue;
+      const double *F = f->d_data + f->d_off[b];
+      for (int j = 0; j < f->d_n[b]; j++) {
+        double t = x[f->d_c0[b] + j];
+        for (int i = a; i < e; i++) y[i] = y[i] + t * F[(i - r0) + (size_t)j * m];
+      }
+    }
+    for (int b = 0; b < f->nl; b++) {
+      int r0 = f->l_r0[b], m = f->l_m[b];
+      int a = r0 > lo ? r0 : lo, e = (r0 + m) < hi ? (r0 + m) : hi;
+      if (a >= e) continue;
+      const double *U = f->l_u + f->l_uoff[b];
+      for (int j = 0; j < f->l_k[b]; j++) {
+        double t = z[(size_t)b * maxk + j];
+        for (int i = a; i < e; i++) y[i] = y[i] + t * U[(i - r0) + (size_t)j * m];
+      }
+    }
+  }
+  free(z);
+}
+
+/* ------------------------------------------------------------------------- */
+/* kernel_ptr adapters: hmatIII (hmatrixModeIII.f90:9-60), hmatII (:9-74)     */
+/* ------------------------------------------------------------------------- */
+static int apply_op(ohandle *h, int op, const double *xh, double *yh) {
+  oracle_t *o = &h->o;
+  if (h->flat[op]) { flat_apply(h->flat[op], xh, yh, o->nthreads); return DFN_OK; }
+  if (!o->H[op]) return DFN_ERR_STATE;
+  double *zb = xcalloc((size_t)(o->max_rank > 64 ? o->max_rank : 64) + o->N, sizeof(double));
+  hgemv_rec(o->H[op], 1, 1.0, 0.0, o->N, o->N, xh, yh, zb);
+  free(zb);
+  return DFN_OK;
+}
+
+static const int *op_perm(ohandle *h, int op) { return h->flat[op] ? h->flat[op]->perm : NULL; }
+
+static int kernel_eval(ohandle *h, const double *slip, double *sT, double *sN) {
+  oracle_t *o = &h->o;
+  int N = o->N;
+  if (o->kernel == DFN_KERNEL_DIRECT) {
+    if (o->mode == DFN_MODE_III) directIII(o, slip, sT);
+    else directII(o, slip, sT, sN);
+    return DFN_OK;
+  }
+  double *xh = xcalloc((size_t)N, sizeof(double)), *yh = xcalloc((size_t)N, sizeof(double));
+  int nops = (o->mode == DFN_MODE_II) ? 2 : 1;
+  int st = DFN_OK;
+  for (int op = 0; op < nops && st == DFN_OK; op++) {
+    const int *pf = op_perm(h, op);
+    for (int k = 0; k < N; k++) xh[k] = slip[pf ? pf[k] : o->H[op]->mat_corr[k] - 1];
+    st = apply_op(h, op, xh, yh);
+    double sc = (o->mode == DFN_MODE_III) ? o->mu / (2.0 * PI_REF)
+                                          : o->mu / PI_REF * (1.0 - o->cs * o->cs / (o->cp * o->cp));
+    double *dst = (op == DFN_OP_T) ? sT : sN;
+    if (dst)
+      for (int k = 0; k < N; k++) dst[pf ? pf[k] : o->H[op]->mat_corr[k] - 1] = sc * yh[k];
+  }
+  free(xh); free(yh);
+  return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* compute_P, src/fluid_diffusion.f90:24-405                                   */
+/* ------------------------------------------------------------------------- */
+#define P1(arr, i) ((arr)[(i) - 1]) /* 1-based access */
+
+static double hmean(double pa, double pb) { return 2.0 * pa * pb / (pa + pb); }
+
+static void compute_diag(const oracle_t *o, double dt, double *diag) { /* :308-405 */
+  int N = o->N;
+  double c = o->por * o->visc * (o->fcomp + o->rcomp);
+  const double *kx = o->perm_x; /* 1-based, [N+1] */
+  for (int k = 1; k <= N; k++) P1(diag, k) = 0.0;
+  P1(diag, 1) = 1.0 + P1(kx, 2) * dt / (c * (P1(o->ds, 1) * P1(o->ds, 1)));
+  P1(diag, N) = 1.0 + P1(kx, N) * dt / (c * (P1(o->ds, N) * P1(o->ds, N)));
+  for (int k = 2; k <= N - 1; k++)
+    P1(diag, k) = 1.0 + (P1(kx, k) + P1(kx, k + 1)) * dt / (c * (P1(o->ds, k) * P1(o->ds, k)));
+  double d1 = c * (P1(o->ds, 1) * P1(o->ds, 1));
+  const double *p = o->perm;
+  for (int q = 0; q < o->ncross; q++) {
+    int64_t n1 = o->cross[6 * q + 2], n2 = o->cross[6 * q + 3];
+    double kl, kr;
+    kl = hmean(P1(p, n1 - 1), P1(p, n2 - 1)); kr = hmean(P1(p, n1 - 1), P1(p, n2));
+    P1(diag, n1 - 1) = P1(diag, n1 - 1) + kl * dt / d1 + kr * dt / d1;
+    kl = hmean(P1(p, n1), P1(p, n2 - 1)); kr = hmean(P1(p, n1), P1(p, n2));
+    P1(diag, n1) = P1(diag, n1) + kl * dt / d1 + kr * dt / d1;
+    kl = hmean(P1(p, n1 - 1), P1(p, n2 - 1)); kr = hmean(P1(p, n1), P1(p, n2 - 1));
+    P1(diag, n2 - 1) = P1(diag, n2 - 1) + kl * dt / d1 + kr * dt / d1;
+    kl = hmean(P1(p, n1 - 1), P1(p, n2)); kr = hmean(P1(p, n1), P1(p, n2));
+    P1(diag, n2) = P1(diag, n2) + kl * dt / d1 + kr * dt / d1;
+  }
+}
+
+static void compute_MV(const oracle_t *o, const double *v, double dt, const double *diag, double *out) { /* :163-298 */
+  int N = o->N;
+  double c = o->por * o->visc * (o->fcomp + o->rcomp);
+  const double *kx = o->perm_x, *ds = o->ds;
+  double A1, A2, A3;
+  A2 = 1.0 + P1(kx, 2) * dt / (c * (P1(ds, 1) * P1(ds, 1)));
+  A3 = -(P1(kx, 2) * dt / (c * (P1(ds, 1) * P1(ds, 1))));
+  P1(out, 1) = (A2 * P1(v, 1) + A3 * P1(v, 2)) / P1(diag, 1);
+  A1 = -(P1(kx, N) * dt / (c * (P1(ds, N) * P1(ds, N))));
+  A2 = 1.0 + P1(kx, N) * dt / (c * (P1(ds, N) * P1(ds, N)));
+  P1(out, N) = (A1 * P1(v, N - 1) + A2 * P1(v, N)) / P1(diag, N);
+  for (int k = 2; k <= N - 1; k++) {
+    A1 = -(P1(kx, k) * dt / (c * (P1(ds, k - 1) * P1(ds, k - 1))));
+    A2 = 1.0 + (P1(kx, k) + P1(kx, k + 1)) * dt / (c * (P1(ds, k) * P1(ds, k)));
+    A3 = -(P1(kx, k + 1) * dt / (c * (P1(ds, k + 1) * P1(ds, k + 1))));
+    P1(out, k) = (A1 * P1(v, k - 1) + A2 * P1(v, k) + A3 * P1(v, k + 1)) / P1(diag, k);
+  }
+  /* crossing terms use ds(k-1) with k the exhausted loop index (= max(N,2)) */
+  int kq = N > 2 ? N : 2;
+  double dq = c * (P1(ds, kq - 1) * P1(ds, kq - 1));
+  const double *p = o->perm;
+  for (int q = 0; q < o->ncross; q++) {
+    int64_t n1 = o->cross[6 * q + 2], n2 = o->cross[6 * q + 3];
+    double kl, kr;
+    kl = hmean(P1(p, n1 - 1), P1(p, n2 - 1)); kr = hmean(P1(p, n1 - 1), P1(p, n2));
+    A1 = -(kl * dt / dq); A3 = -(kr * dt / dq); A2 = kl * dt / dq + kr * dt / dq;
+    P1(out, n1 - 1) = P1(out, n1 - 1) + (A1 * P1(v, n2 - 1) + A2 * P1(v, n1 - 1) + A3 * P1(v, n2)) / P1(diag, n1 - 1);
+    kl = hmean(P1(p, n1), P1(p, n2 - 1)); kr = hmean(P1(p, n1), P1(p, n2));
+    A1 = -(kl * dt / dq); A3 = -(kr * dt / dq); A2 = kl * dt / dq + kr * dt / dq;
+    /* :252 divides by the node index itself */
+    P1(out, n1) = P1(out, n1) + (A1 * P1(v, n2 - 1) + A2 * P1(v, n1) + A3 * P1(v, n2)) / (double)n1;
+    kl = hmean(P1(p, n1 - 1), P1(p, n2 - 1)); kr = hmean(P1(p, n1), P1(p, n2 - 1));
+    A1 = -(kl * dt / dq); A3 = -(kr * dt / dq); A2 = kl * dt / dq + kr * dt / dq;
+    P1(out, n2 - 1) = P1(out, n2 - 1) + (A1 * P1(v, n1 - 1) + A2 * P1(v, n2 - 1) + A3 * P1(v, n1)) / P1(diag, n2 - 1);
+    kl = hmean(P1(p, n1 - 1), P1(p, n2)); kr = hmean(P1(p, n1), P1(p, n2));
+    A1 = -(kl * dt / dq); A3 = -(kr * dt / dq); A2 = kl * dt / dq + kr * dt / dq;
+    P1(out, n2) = P1(out, n2) + (A1 * P1(v, n1 - 1) + A2 * P1(v, n2) + A3 * P1(v, n1)) / P1(diag, n2);
+  }
+}
+
+static void update_permeability(oracle_t *o) { /* :67-75 */
+  int N = o->N;
+  const double *sigmaN = o->Y + 2 * N;
+  if (o->coupling == 1)
+    for (int k = 0; k < N; k++)
+      o->perm[k] = (o->perm_star[k] - o->kmin[k]) * exp(-fabs(sigmaN[k] + o->P[k]) / fabs(o->Snk[k])) + o->kmin[k];
+  for (int k = 2; k <= N; k++)
+    P1(o->perm_x, k) = 2.0 * P1(o->perm, k - 1) * P1(o->perm, k) / (P1(o->perm, k - 1) + P1(o->perm, k));
+}
+
+/* Returns ier (0 ok, 1 failure); *iters the CG iteration count. */
+static int compute_P(oracle_t *o, double dt, double *P_temp, int64_t *iters) {
+  int N = o->N;
+  double *diag = xcalloc((size_t)N, sizeof(double)), *b1 = xcalloc((size_t)N, sizeof(double));
+  double *xk = xcalloc((size_t)N, sizeof(double)), *rk = xcalloc((size_t)N, sizeof(double));
+  double *rk1 = xcalloc((size_t)N, sizeof(double)), *pk = xcalloc((size_t)N, sizeof(double));
+  double *Ap = xcalloc((size_t)N, sizeof(double));
+  int ier = 0;
+  compute_diag(o, dt, diag);
+  for (int k = 0; k < N; k++) b1[k] = o->P[k] / diag[k];
+  for (int s = 0; s < o->nsrc; s++) { /* :52-64 */
+    int f = (int)o->src_fault[s] - 1;
+    if ((o->time + dt >= o->tb[s]) && (o->time + dt < o->te[s])) {
+      int64_t idx = o->map1[f] - 1 + o->src_idx[s]; /* 1-based */
+      P1(b1, idx) = P1(b1, idx) + (dt * o->Q[s] / (o->por * (o->fcomp + o->rcomp)) / P1(o->ds, idx)) / P1(diag, idx);
+    }
+  }
+  update_permeability(o);
+  for (int k = 0; k < N; k++) xk[k] = o->P[k];
+  compute_MV(o, xk, dt, diag, Ap);
+  for (int k = 0; k < N; k++) rk[k] = b1[k] - Ap[k];
+  for (int k = 0; k < N; k++) pk[k] = rk[k];
+  int64_t it = 0;
+  double errmax = 0.0;
+  for (int k = 0; k < N; k++) errmax = fmax(errmax, fabs(rk[k] / 1.0));
+  errmax = errmax / o->tol;
+  while (errmax >= 1.0) {
+    it++;
+    compute_MV(o, pk, dt, diag, Ap);
+    double rr = 0.0, pAp = 0.0;
+    for (int k = 0; k < N; k++) rr = rr + rk[k] * rk[k];
+    for (int k = 0; k < N; k++) pAp = pAp + pk[k] * Ap[k];
+    double alpha = rr / pAp;
+    for (int k = 0; k < N; k++) xk[k] = xk[k] + alpha * pk[k];
+    for (int k = 0; k < N; k++) rk1[k] = rk[k] - alpha * Ap[k];
+    double r1r1 = 0.0;
+    for (int k = 0; k < N; k++) r1r1 = r1r1 + rk1[k] * rk1[k];
+    double beta = r1r1 / rr;
+    for (int k = 0; k < N; k++) pk[k] = rk1[k] + beta * pk[k];
+    for (int k = 0; k < N; k++) rk[k] = rk1[k];
+    if (it > 200) { ier = 1; break; }
+    errmax = 0.0;
+    for (int k = 0; k < N; k++) errmax = fmax(errmax, fabs(rk[k] / 1.0));
+    errmax = errmax / o->tol;
+  }
+  if (!ier) {
+    if (dt < (double)1e-9f) for (int k = 0; k < N; k++) o->Pdot[k] = 0.0; /* :136-140 */
+    else for (int k = 0; k < N; k++) o->Pdot[k] = (xk[k] - o->P[k]) / dt;
+    double mx = 0.0;
+    for (int k = 0; k < N; k++) mx = fmax(mx, fabs(xk[k]));
+    if (P_temp) memcpy(P_temp, xk, sizeof(double) * (size_t)N);
+    if (mx > DBL_MAX) ier = 1;
+  }
+  if (iters) *iters = it;
+  o->st.cg_solves++;
+  o->st.cg_iterations += it;
+  free(diag); free(b1); free(xk); free(rk); free(rk1); free(pk); free(Ap);
+  return ier;
+}
+
+/* ------------------------------------------------------------------------- */
+/* friction_law: RateStateAgingModeIII.f90:19-121 / ModeII.f90:19-92          */
+/* ------------------------------------------------------------------------- */
+static int friction_law(ohandle *h, double x, const double *Y1, double *Yd) {
+  oracle_t *o = &h->o;
+  int N = o->N;
+  int st = kernel_eval(h, Y1, o->sT, o->sN);
+  if (st) return st;
+  double *Pt = xcalloc((size_t)N, sizeof(double));
+  int ier = compute_P(o, x, Pt, NULL);
+  o->st.rhs_evaluations++;
+  if (ier) { free(Pt); return DFN_ERR_CG; }
+  const double *V = Y1, *psi = Y1 + N, *sig = Y1 + 2 * N;
+  for (int k = 0; k < N; k++)
+    Yd[2 * N + k] = (o->mode == DFN_MODE_III) ? o->nld[k] : o->sN[k] + o->nld[k];
+  for (int k = 0; k < N; k++) {
+    double B1 = V[k] / (2.0 * o->V0[k]);
+    double C1 = exp((o->f0[k] + o->b[k] * psi[k]) / o->a[k]);
+    double D1 = 1.0 / (C1 * C1) + B1 * B1;
+    D1 = 1.0 / sqrt(D1);
+    double thdot = o->V0[k] / o->Dc[k] * exp(-psi[k]) - fabs(V[k]) / o->Dc[k];
+    Yd[k] = (o->sld[k] + o->sT[k] + (Yd[2 * N + k] + o->Pdot[k]) * o->a[k] * asinh(B1 * C1) +
+             B1 * D1 * (sig[k] + Pt[k]) * o->b[k] * thdot) /
+            (o->mu / (2.0 * o->cs) - D1 * (sig[k] + Pt[k]) * o->a[k] / (2.0 * o->V0[k]));
+    Yd[N + k] = o->V0[k] / o->Dc[k] * exp(-psi[k]) - fabs(V[k]) / o->Dc[k];
+  }
+  for (int q = 0; q < o->nfake; q++) { /* :104-113 */
+    int64_t i = o->fake[q];
+    Yd[i] = 0; Yd[i + N] = 0; Yd[i + 2 * N] = 0;
+  }
+  free(Pt);
+  return DFN_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* rkck / rkqs, src/ode_solver/ode_bs.f90:125-226                               */
+/* ------------------------------------------------------------------------- */
+static const double A2 = 0.2, A3 = 0.3, A4 = 0.6, A5 = 1.0, A6 = 0.875, B21 = 0.2, B31 = 3.0 / 40.0,
+                    B32 = 9.0 / 40.0, B41 = 0.3, B42 = -0.9, B43 = 1.2, B51 = -11.0 / 54.0, B52 = 2.5,
+                    B53 = -70.0 / 27.0, B54 = 35.0 / 27.0, B61 = 1631.0 / 55296.0, B62 = 175.0 / 512.0,
+                    B63 = 575.0 / 13824.0, B64 = 44275.0 / 110592.0, B65 = 253.0 / 4096.0,
+                    C1 = 37.0 / 378.0, C3 = 250.0 / 621.0, C4 = 125.0 / 594.0, C6 = 512.0 / 1771.0;
+#define DC1 (C1 - 2825.0 / 27648.0)
+#define DC3 (C3 - 18575.0 / 48384.0)
+#define DC4 (C4 - 13525.0 / 55296.0)
+#define DC5 (-277.0 / 14336.0)
+#define DC6 (C6 - 0.25)
+
+static int rkck(ohandle *h, int n, const double *y, const double *dydx, double x, double hh,
+                double *yout, double *yerr) {
+  double *ak2 = xcalloc((size_t)n, 8), *ak3 = xcalloc((size_t)n, 8), *ak4 = xcalloc((size_t)n, 8);
+  double *ak5 = xcalloc((size_t)n, 8), *ak6 = xcalloc((size_t)n, 8), *yt = xcalloc((size_t)n, 8);
+  int st;
+  for (int i = 0; i < n; i++) yt[i] = y[i] + B21 * hh * dydx[i];
+  if ((st = friction_law(h, x + A2 * hh, yt, ak2))) goto done;
+  for (int i = 0; i < n; i++) yt[i] = y[i] + hh * (B31 * dydx[i] + B32 * ak2[i]);
+  if ((st = friction_law(h, x + A3 * hh, yt, ak3))) goto done;
+  for (int i = 0; i < n; i++) yt[i] = y[i] + hh * (B41 * dydx[i] + B42 * ak2[i] + B43 * ak3[i]);
+  if ((st = friction_law(h, x + A4 * hh, yt, ak4))) goto done;
+  for (int i = 0; i < n; i++) yt[i] = y[i] + hh * (B51 * dydx[i] + B52 * ak2[i] + B53 * ak3[i] + B54 * ak4[i]);
+  if ((st = friction_law(h, x + A5 * hh, yt, ak5))) goto done;
+  for (int i = 0; i < n; i++)
+    yt[i] = y[i] + hh * (B61 * dydx[i] + B62 * ak2[i] + B63 * ak3[i] + B64 * ak4[i] + B65 * ak5[i]);
+  if ((st = friction_law(h, x + A6 * hh, yt, ak6))) goto done;
+  for (int i = 0; i < n; i++) yout[i] = y[i] + hh * (C1 * dydx[i] + C3 * ak3[i] + C4 * ak4[i] + C6 * ak6[i]);
+  for (int i = 0; i < n; i++) yerr[i] = hh * (DC1 * dydx[i] + DC3 * ak3[i] + DC4 * ak4[i] + DC5 * ak5[i] + DC6 * ak6[i]);
+done:
+  free(ak2); free(ak3); free(ak4); free(ak5); free(ak6); free(yt);
+  return st;
+}
+
+static int rkqs(ohandle *h, int n, double *y, const double *dydx, double *x, double htry, double eps,
+                const double *yscal, double *hdid, double *hnext) {
+  const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = (double)1.89e-4f; /* :149-150 */
+  double *yerr = xcalloc((size_t)n, 8), *ytemp = xcalloc((size_t)n, 8);
+  double hh = htry, errmax;
+  int st = DFN_OK;
+  for (;;) {
+    if ((st = rkck(h, n, y, dydx, *x, hh, ytemp, yerr))) goto done;
+    errmax = 0.0;
+    for (int i = 0; i < n; i++) errmax = fmax(errmax, fabs(yerr[i] / yscal[i]));
+    errmax = errmax / eps;
+    if (errmax <= 1.0) break;
+    h->o.st.rejected_attempts++;
+    double htemp = SAFETY * hh * pow(errmax, PSHRNK);
+    hh = copysign(fmax(fabs(htemp), 0.1 * fabs(hh)), hh);
+    double xnew = *x + hh;
+    if (xnew == *x) { st = DFN_ERR_UNDERFLOW; goto done; }
+  }
+  if (errmax > ERRCON) *hnext = SAFETY * hh * pow(errmax, PGROW);
+  else *hnext = 5.0 * hh;
+  *hdid = hh;
+  *x = *x + hh;
+  memcpy(y, ytemp, sizeof(double) * (size_t)n);
+done:
+  free(yerr); free(ytemp);
+  return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Public (dfno_) API                                                          */
+/* ------------------------------------------------------------------------- */
+#define API __attribute__((visibility("default")))
+
+API int dfno_set_threads(int n) {
+  (void)n;
+  return DFN_OK;
+}
+
+API int dfno_handle_set_threads(dfn_handle hh, int n) {
+  if (!hh || n < 1) return DFN_ERR_ARG;
+  ((ohandle *)hh)->o.nthreads = n;
+  return DFN_OK;
+}
+
+API int dfno_create(const dfn_problem *p, dfn_handle *out) {
+  if (!p || !out || p->nb_fault < 1 || p->nb_element < 2 || !p->node || !p->fault_nb_element) return DFN_ERR_ARG;
+  if (p->fracture_mode != DFN_MODE_II && p->fracture_mode != DFN_MODE_III) return DFN_ERR_ARG;
+  int64_t tot = -1;
+  for (int f = 0; f < p->nb_fault; f++) tot += p->fault_nb_element[f] + 1;
+  if (tot != p->nb_element) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)xcalloc(1, sizeof(ohandle));
+  oracle_t *o = &h->o;
+  int N = p->nb_element;
+  o->mode = p->fracture_mode; o->kernel = p->static_kernel; o->nfault = p->nb_fault; o->N = N;
+  o->nthreads = 1;
+  o->fnel = xcalloc((size_t)o->nfault, sizeof(int64_t));
+  o->map1 = xcalloc((size_t)o->nfault, sizeof(int64_t));
+  o->map2 = xcalloc((size_t)o->nfault, sizeof(int64_t));
+  int64_t nc = 0;
+  for (int f = 0; f < o->nfault; f++) { /* configuration.f90:237-245 */
+    o->fnel[f] = p->fault_nb_element[f];
+    o->map1[f] = nc + 1;
+    o->map2[f] = nc + o->fnel[f];
+    nc += o->fnel[f] + 1;
+  }
+  o->node = dup_or_zero(p->node, 2 * (N + 1));
+  o->elem = xcalloc(2 * (size_t)N, 8); o->ds = xcalloc((size_t)N, 8); o->tang = xcalloc(2 * (size_t)N, 8);
+  o->nrm = xcalloc(2 * (size_t)N, 8); o->nrm_left = xcalloc(2 * (size_t)N, 8); o->nrm_right = xcalloc(2 * (size_t)N, 8);
+  o->a = dup_or_zero(p->a, N); o->b = dup_or_zero(p->b, N); o->Dc = dup_or_zero(p->Dc, N);
+  o->f0 = dup_or_zero(p->f0, N); o->V0 = dup_or_zero(p->V0, N);
+  o->nld = dup_or_zero(p->normal_loading_dot, N); o->sld = dup_or_zero(p->shear_loading_dot, N);
+  o->perm = dup_or_zero(p->permeability, N); o->perm_star = xcalloc((size_t)N, 8);
+  o->perm_x = xcalloc((size_t)N + 1, 8);
+  o->kmin = dup_or_zero(p->kmin, N); o->kmax = dup_or_zero(p->kmax, N);
+  o->Snk = dup_or_zero(p->Snk, N); o->Lk = dup_or_zero(p->Lk, N); o->Tk = dup_or_zero(p->Tk, N);
+  o->Y = xcalloc(3 * (size_t)N, 8); o->P = dup_or_zero(p->P, N); o->Pdot = xcalloc((size_t)N, 8);
+  o->sT = xcalloc((size_t)N, 8); o->sN = xcalloc((size_t)N, 8);
+  if (p->V) memcpy(o->Y, p->V, 8 * (size_t)N);
+  if (p->theta) memcpy(o->Y + N, p->theta, 8 * (size_t)N);
+  if (p->sigmaN) memcpy(o->Y + 2 * N, p->sigmaN, 8 * (size_t)N);
+  o->mu = p->mu; o->cs = p->cs; o->cp = p->cp;
+  o->s11d = p->Sigma11_dot; o->s12d = p->Sigma12_dot; o->s22d = p->Sigma22_dot;
+  o->s31d = p->Sigma31_dot; o->s32d = p->Sigma32_dot;
+  o->por = p->porosity; o->fcomp = p->fluid_comp; o->rcomp = p->rock_comp; o->visc = p->dyn_viscosity;
+  o->tol = p->tol_solver; o->coupling = p->permeability_coupling;
+  o->nsrc = p->nb_source;
+  o->src_fault = xcalloc((size_t)o->nsrc, 8); o->src_idx = xcalloc((size_t)o->nsrc, 8);
+  o->Q = xcalloc((size_t)o->nsrc, 8); o->tb = xcalloc((size_t)o->nsrc, 8); o->te = xcalloc((size_t)o->nsrc, 8);
+  for (int s = 0; s < o->nsrc; s++) {
+    o->src_fault[s] = p->source_fault[s]; o->src_idx[s] = p->source_index[s];
+    o->Q[s] = p->source_Q[s]; o->tb[s] = p->source_t_beg[s]; o->te[s] = p->source_t_end[s];
+    if (o->src_fault[s] < 1 || o->src_fault[s] > o->nfault || o->src_idx[s] < 1 ||
+        o->src_idx[s] > o->fnel[o->src_fault[s] - 1]) { *out = (dfn_handle)h; return DFN_ERR_ARG; }
+  }
+  o->eta = p->eta > 0 ? p->eta : 3.0; o->eps_aca = p->eps_aca > 0 ? p->eps_aca : 1e-8;
+  o->nleaf = p->nleaf > 0 ? p->nleaf : 100; o->max_rank = p->max_rank > 0 ? p->max_rank : 20;
+  /* fake elements between faults (RateStateAgingModeIII.f90:105-113 index rule) */
+  o->fake = xcalloc((size_t)o->nfault, 8);
+  int64_t idx = 0;
+  for (int f = 0; f < o->nfault; f++) {
+    idx += o->fnel[f] + 1;
+    if (idx <= N) o->fake[o->nfake++] = idx - 1;
+  }
+  for (int q = 0; q < o->nfake; q++) {
+    int64_t i = o->fake[q];
+    o->a[i] = o->b[i] = o->Dc[i] = o->f0[i] = o->V0[i] = 1.0;
+    o->Snk[i] = o->Lk[i] = o->Tk[i] = 1.0;
+    o->nld[i] = o->sld[i] = o->perm[i] = o->kmin[i] = o->kmax[i] = o->P[i] = 0.0;
+    o->Y[i] = o->Y[N + i] = o->Y[2 * N + i] = 0.0;
+  }
+  ini_geometry(o);
+  ini_loading(o);
+  /* initialisation.f90:252-266 */
+  if (o->coupling == 1)
+    for (int k = 0; k < N; k++)
+      o->perm_star[k] = o->kmin[k] + (o->perm[k] - o->kmin[k]) * exp(fabs(o->Y[2 * N + k] + o->P[k]) / fabs(o->Snk[k]));
+  for (int k = 2; k <= N; k++)
+    P1(o->perm_x, k) = 2.0 * P1(o->perm, k - 1) * P1(o->perm, k) / (P1(o->perm, k - 1) + P1(o->perm, k));
+  int st = find_crossings(o);
+  o->t_try = 100.0;
+  *out = (dfn_handle)h;
+  return st;
+}
+
+static void oracle_free(oracle_t *o) {
+  free(o->fnel); free(o->map1); free(o->map2); free(o->node); free(o->elem); free(o->ds); free(o->tang);
+  free(o->nrm); free(o->nrm_left); free(o->nrm_right); free(o->a); free(o->b); free(o->Dc); free(o->f0);
+  free(o->V0); free(o->nld); free(o->sld); free(o->perm); free(o->perm_star); free(o->perm_x); free(o->kmin);
+  free(o->kmax); free(o->Snk); free(o->Lk); free(o->Tk); free(o->src_fault); free(o->src_idx); free(o->Q);
+  free(o->tb); free(o->te); free(o->cross); free(o->fake); free(o->Y); free(o->P); free(o->Pdot);
+  free(o->sT); free(o->sN);
+  ohmat_free(o->H[0]); ohmat_free(o->H[1]);
+}
+
+API int dfno_destroy(dfn_handle hh) {
+  if (!hh) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  oracle_free(&h->o);
+  flat_free(h->flat[0]); flat_free(h->flat[1]);
+  free(h);
+  return DFN_OK;
+}
+
+API int dfno_build_hmat(dfn_handle hh) {
+  if (!hh) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  oracle_t *o = &h->o;
+  if (o->kernel != DFN_KERNEL_HMATRIX) return DFN_OK;
+  int st;
+  if (o->mode == DFN_MODE_II) {
+    if ((st = build_ohmat(o, DFN_OP_N, &o->H[DFN_OP_N]))) return st;
+  }
+  return build_ohmat(o, DFN_OP_T, &o->H[DFN_OP_T]);
+}
+
+API int dfno_hmat_stats(dfn_handle hh, int32_t op, int64_t *stats) {
+  if (!hh || !stats || op < 0 || op > 1) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  ohmat *H = h->o.H[op];
+  if (!H) return DFN_ERR_STATE;
+  int64_t nd = 0, nl = 0, sd = 0, su = 0, sv = 0, mr = 0, sr = 0;
+  for (int b = 0; b < H->nbblclus; b++) {
+    int m = nbrow_of(H, b + 1), n = nbcol_of(H, b + 1);
+    if (H->bl_type[b] == 0) { nd++; sd += (int64_t)m * n; }
+    else if (H->bl_type[b] == 1) {
+      int k = (int)H->info[6 * b + 4];
+      nl++; su += (int64_t)m * k; sv += (int64_t)n * k; sr += k;
+      if (k > mr) mr = k;
+    }
+  }
+  int64_t s[10] = {nd, nl, sd, su, sv, mr, H->nbclus, H->nbblclus, H->n, sr};
+  memcpy(stats, s, sizeof(s));
+  return DFN_OK;
+}
+
+/* Export in block order (dense and low-rank lists each in increasing block id). */
+API int dfno_hmat_export(dfn_handle hh, int32_t op, int32_t *perm, int32_t *d_r0, int32_t *d_c0,
+                         int32_t *d_m, int32_t *d_n, int64_t *d_off, double *d_data, int32_t *l_r0,
+                         int32_t *l_c0, int32_t *l_m, int32_t *l_n, int32_t *l_k, int64_t *l_uoff,
+                         int64_t *l_voff, double *l_u, double *l_v) {
+  if (!hh || op < 0 || op > 1) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  ohmat *H = h->o.H[op];
+  if (!H) return DFN_ERR_STATE;
+  for (int k = 0; k < H->n; k++) perm[k] = H->mat_corr[k] - 1;
+  int id = 0, il = 0;
+  int64_t od = 0, ou = 0, ov = 0;
+  for (int b = 0; b < H->nbblclus; b++) {
+    int r0 = cl_first(H, H->bl_row[b]) - 1, c0 = cl_first(H, H->bl_col[b]) - 1;
+    int m = nbrow_of(H, b + 1), n = nbcol_of(H, b + 1);
+    if (H->bl_type[b] == 0) {
+      d_r0[id] = r0; d_c0[id] = c0; d_m[id] = m; d_n[id] = n; d_off[id] = od;
+      memcpy(d_data + od, H->F + H->info[6 * b] - 1, sizeof(double) * (size_t)m * n);
+      od += (int64_t)m * n; id++;
+    } else if (H->bl_type[b] == 1) {
+      int k = (int)H->info[6 * b + 4];
+      l_r0[il] = r0; l_c0[il] = c0; l_m[il] = m; l_n[il] = n; l_k[il] = k;
+      l_uoff[il] = ou; l_voff[il] = ov;
+      memcpy(l_u + ou, H->A + H->info[6 * b] - 1, sizeof(double) * (size_t)m * k);
+      memcpy(l_v + ov, H->B + H->info[6 * b + 2] - 1, sizeof(double) * (size_t)n * k);
+      ou += (int64_t)m * k; ov += (int64_t)n * k; il++;
+    }
+  }
+  return DFN_OK;
+}
+
+API int dfno_hmat_import(dfn_handle hh, int32_t op, int32_t n, const int32_t *perm, int32_t nd,
+                         const int32_t *d_r0, const int32_t *d_c0, const int32_t *d_m, const int32_t *d_n,
+                         const int64_t *d_off, const double *d_data, int32_t nl, const int32_t *l_r0,
+                         const int32_t *l_c0, const int32_t *l_m, const int32_t *l_n, const int32_t *l_k,
+                         const int64_t *l_uoff, const int64_t *l_voff, const double *l_u, const double *l_v) {
+  if (!hh || op < 0 || op > 1) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  if (n != h->o.N) return DFN_ERR_ARG;
+  flat_hmat *f = xcalloc(1, sizeof(flat_hmat));
+  f->n = n; f->nd = nd; f->nl = nl;
+  f->perm = xcalloc((size_t)n, 4); memcpy(f->perm, perm, 4 * (size_t)n);
+#define CP32(dst, src, cnt) do { dst = xcalloc((size_t)(cnt), 4); memcpy(dst, src, 4 * (size_t)(cnt)); } while (0)
+#define CP64(dst, src, cnt) do { dst = xcalloc((size_t)(cnt), 8); memcpy(dst, src, 8 * (size_t)(cnt)); } while (0)
+  CP32(f->d_r0, d_r0, nd); CP32(f->d_c0, d_c0, nd); CP32(f->d_m, d_m, nd); CP32(f->d_n, d_n, nd);
+  CP64(f->d_off, d_off, nd);
+  int64_t dsz = 0, usz = 0, vsz = 0;
+  for (int b = 0; b < nd; b++) if (d_off[b] + (int64_t)d_m[b] * d_n[b] > dsz) dsz = d_off[b] + (int64_t)d_m[b] * d_n[b];
+  CP64(f->d_data, d_data, dsz);
+  CP32(f->l_r0, l_r0, nl); CP32(f->l_c0, l_c0, nl); CP32(f->l_m, l_m, nl); CP32(f->l_n, l_n, nl);
+  CP32(f->l_k, l_k, nl); CP64(f->l_uoff, l_uoff, nl); CP64(f->l_voff, l_voff, nl);
+  for (int b = 0; b < nl; b++) {
+    if (l_uoff[b] + (int64_t)l_m[b] * l_k[b] > usz) usz = l_uoff[b] + (int64_t)l_m[b] * l_k[b];
+    if (l_voff[b] + (int64_t)l_n[b] * l_k[b] > vsz) vsz = l_voff[b] + (int64_t)l_n[b] * l_k[b];
+  }
+  CP64(f->l_u, l_u, usz); CP64(f->l_v, l_v, vsz);
+  f->d_size = dsz; f->u_size = usz; f->v_size = vsz;
+  flat_free(h->flat[op]);
+  h->flat[op] = f;
+  return DFN_OK;
+}
+
+API int dfno_set_state(dfn_handle hh, const double *Y, const double *P, double time) {
+  if (!hh) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  if (Y) memcpy(o->Y, Y, 8 * 3 * (size_t)o->N);
+  if (P) memcpy(o->P, P, 8 * (size_t)o->N);
+  o->time = time;
+  return DFN_OK;
+}
+
+API int dfno_get_state(dfn_handle hh, double *Y, double *P, double *Pdot, double *time) {
+  if (!hh) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  if (Y) memcpy(Y, o->Y, 8 * 3 * (size_t)o->N);
+  if (P) memcpy(P, o->P, 8 * (size_t)o->N);
+  if (Pdot) memcpy(Pdot, o->Pdot, 8 * (size_t)o->N);
+  if (time) *time = o->time;
+  return DFN_OK;
+}
+
+API int dfno_get_stats(dfn_handle hh, dfn_run_stats *s) {
+  if (!hh || !s) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  *s = o->st;
+  s->time = o->time;
+  s->t_try = o->t_try;
+  double mv = -INFINITY;
+  for (int k = 0; k < o->N; k++) mv = fmax(mv, o->Y[k]);
+  s->max_V = mv;
+  return DFN_OK;
+}
+
+API int dfno_kernel(dfn_handle hh, const double *slip, double *sT, double *sN) {
+  if (!hh || !slip || !sT) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  if (h->o.mode == DFN_MODE_II && !sN) return DFN_ERR_ARG;
+  return kernel_eval(h, slip, sT, sN);
+}
+
+API int dfno_hgemv(dfn_handle hh, int32_t op, double alpha, double beta, const double *x, double *y) {
+  if (!hh || op < 0 || op > 1 || !x || !y) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  oracle_t *o = &h->o;
+  if (h->flat[op]) {
+    double *t = xcalloc((size_t)o->N, 8);
+    flat_apply(h->flat[op], x, t, o->nthreads);
+    for (int i = 0; i < o->N; i++) y[i] = alpha * t[i] + (beta == 0.0 ? 0.0 : beta * y[i]);
+    free(t);
+    return DFN_OK;
+  }
+  if (!o->H[op]) return DFN_ERR_STATE;
+  double *zb = xcalloc((size_t)o->max_rank + 64, 8);
+  hgemv_rec(o->H[op], 1, alpha, beta, o->N, o->N, x, y, zb);
+  free(zb);
+  return DFN_OK;
+}
+
+API int dfno_friction_law(dfn_handle hh, double x, const double *Y, double *Yd) {
+  if (!hh || !Y || !Yd) return DFN_ERR_ARG;
+  return friction_law((ohandle *)hh, x, Y, Yd);
+}
+
+API int dfno_compute_P(dfn_handle hh, double dt, double *P_temp, double *P_dot, int32_t *ier) {
+  if (!hh) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  int e = compute_P(o, dt, P_temp, NULL);
+  if (ier) *ier = e;
+  if (!e && P_dot) memcpy(P_dot, o->Pdot, 8 * (size_t)o->N);
+  return DFN_OK;
+}
+
+API int dfno_rkqs(dfn_handle hh, double *y, const double *dydx, double *x, double htry, double eps,
+                  const double *yscal, double *hdid, double *hnext) {
+  if (!hh || !y || !dydx || !x || !yscal || !hdid || !hnext) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  return rkqs(h, 3 * h->o.N, y, dydx, x, htry, eps, yscal, hdid, hnext);
+}
+
+/* solve(), src/solver.f90:42-206 (output writing excluded). */
+API int dfno_solve(dfn_handle hh, int64_t nsteps, double t_final, dfn_run_stats *stats) {
+  if (!hh) return DFN_ERR_ARG;
+  ohandle *h = (ohandle *)hh;
+  oracle_t *o = &h->o;
+  int N = o->N, n = 3 * N;
+  if (o->kernel == DFN_KERNEL_HMATRIX && !o->H[0] && !h->flat[0]) return DFN_ERR_STATE;
+  double *Yd = xcalloc((size_t)n, 8), *yscal = xcalloc((size_t)n, 8), *Pt = xcalloc((size_t)N, 8);
+  int st = DFN_OK;
+  for (int64_t it = 0; it < nsteps; it++) {
+    if (t_final > 0 && o->time >= t_final) break;
+    if (o->t_try > 10.0) { /* :64-77 */
+      for (int s = 0; s < o->nsrc; s++)
+        if ((o->time < o->tb[s]) && (o->time + o->t_try > o->tb[s])) o->t_try = 10.0;
+    }
+    int ier = compute_P(o, o->t_try, Pt, NULL); /* :81-92 */
+    while (ier == 1) {
+      o->t_try = o->t_try / 2.0;
+      o->st.cg_halvings++;
+      ier = compute_P(o, o->t_try, Pt, NULL);
+    }
+    double time_temp = 0.0; /* :97 */
+    if ((st = friction_law(h, o->t_try, o->Y, Yd))) break; /* :101 */
+    for (int i = 0; i < n; i++) yscal[i] = fabs(o->Y[i]) + fabs(Yd[i] * o->t_try) + o->tol; /* :102 */
+    double t_did, t_next;
+    if ((st = rkqs(h, n, o->Y, Yd, &time_temp, o->t_try, o->tol, yscal, &t_did, &t_next))) break;
+    o->t_try = t_next; /* :117-120 */
+    if (o->t_try > 24 * 60 * 60) o->t_try = 24 * 60 * 60;
+    ier = compute_P(o, t_did, Pt, NULL); /* :174 commit */
+    if (!ier) memcpy(o->P, Pt, 8 * (size_t)N);
+    o->time = o->time + t_did; /* :177 */
+    o->st.accepted_steps++;
+    int tensile = 0; /* check_errors :412-420 */
+    for (int k = 0; k < N; k++) if (o->Y[2 * N + k] > 0) tensile = 1;
+    if (tensile) { st = DFN_ERR_TENSILE; break; }
+  }
+  free(Yd); free(yscal); free(Pt);
+  if (stats) dfno_get_stats(hh, stats);
+  return st;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Introspection used by the parity tests                                     */
+/* ------------------------------------------------------------------------- */
+API int dfno_debug_geometry(dfn_handle hh, double *elem, double *ds, double *tang, double *nrm, double *sld,
+                            double *nld) {
+  if (!hh) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  size_t N = (size_t)o->N;
+  if (elem) memcpy(elem, o->elem, 16 * N);
+  if (ds) memcpy(ds, o->ds, 8 * N);
+  if (tang) memcpy(tang, o->tang, 16 * N);
+  if (nrm) memcpy(nrm, o->nrm, 16 * N);
+  if (sld) memcpy(sld, o->sld, 8 * N);
+  if (nld) memcpy(nld, o->nld, 8 * N);
+  return DFN_OK;
+}
+
+API int dfno_debug_crossings(dfn_handle hh, int64_t *cross, int32_t cap, int32_t *count) {
+  if (!hh || !count) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  *count = o->ncross;
+  if (cross && cap >= o->ncross) memcpy(cross, o->cross, sizeof(int64_t) * 6 * (size_t)o->ncross);
+  return DFN_OK;
+}
+
+/* M[k + l*N] = kernel entry (receiver k, source l) in the original element order. */
+API int dfno_eval_matrix(dfn_handle hh, int32_t op, double *M) {
+  if (!hh || !M) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  int N = o->N;
+  hgeom g = {o->elem, o->node, o->node + 2, o->tang, o->nrm, o->nrm, o->nrm};
+  eval_fn ev = (o->mode == DFN_MODE_III) ? eval_III : (op == DFN_OP_T ? eval_II_T : eval_II_N);
+  for (int l = 0; l < N; l++)
+    for (int k = 0; k < N; k++) M[k + (size_t)l * N] = ev(&g, k, l);
+  return DFN_OK;
+}
+
+/* Cluster/block trees of operator op as the reference stores them (1-based). */
+API int dfno_hmat_tree(dfn_handle hh, int32_t op, int32_t *mat_corr, int32_t *nbclus, int32_t *ind_clusters,
+                       int32_t cap_cl, int32_t *nbblclus, int32_t *bltree, int64_t *blinfo, int32_t cap_bl) {
+  if (!hh || op < 0 || op > 1) return DFN_ERR_ARG;
+  ohmat *H = ((ohandle *)hh)->o.H[op];
+  if (!H) return DFN_ERR_STATE;
+  if (mat_corr) memcpy(mat_corr, H->mat_corr, 4 * (size_t)H->n);
+  *nbclus = H->nbclus;
+  *nbblclus = H->nbblclus;
+  if (ind_clusters && cap_cl >= H->nbclus) memcpy(ind_clusters, H->ind_clusters, 8 * (size_t)H->nbclus);
+  if (bltree && blinfo && cap_bl >= H->nbblclus)
+    for (int b = 0; b < H->nbblclus; b++) {
+      bltree[3 * b] = H->bl_type[b]; bltree[3 * b + 1] = H->bl_row[b]; bltree[3 * b + 2] = H->bl_col[b];
+      for (int j = 0; j < 6; j++) blinfo[6 * b + j] = H->info[6 * b + j];
+    }
+  return DFN_OK;
+}

@@ -1,0 +1,39 @@
+"""Problems the golden fixtures are generated on (shared by make_golden.py and the tests)."""
+import numpy as np
+
+from demystification_dfn_amd import problems
+
+
+def single_fault_small():
+    """BP6-QD-A-like fault, 300 elements over 3 km, dense Mode III, tol 1e-8."""
+    return problems.single_fault(n=300, length=3000.0, tol=1e-8, name="golden_single_fault_300")
+
+
+def _seg(x0, y0, x1, y1, n):
+    t = np.linspace(0, 1, n + 1)
+    return np.stack([x0 + (x1 - x0) * t, y0 + (y1 - y0) * t], 1)
+
+
+def crossing_network():
+    """Three Mode II faults, two of them sharing the node (0,0) (one crossing), an
+    injection source on fault 2, a 10 MPa pulse and graded permeability."""
+    fs = []
+    for nd, a in [(_seg(-50, 0, 50, 0, 10), 0.01), (_seg(0, -50, 0, 50, 10), 0.011),
+                  (_seg(30, 40, 120, 60, 9), 0.009)]:
+        n = nd.shape[0] - 1
+        one = np.ones(n)
+        fs.append(dict(node=nd, a=a * one, b=0.012 * one, Dc=1e-3 * one, f0=0.6 * one, V0=1e-6 * one,
+                       sigmaN=-30e6 * one, V=1e-12 * one, theta=np.log(1e6) * one, P=0 * one,
+                       permeability=1e-15 * one * (1 + np.arange(n) / 10), Snk=one, Lk=one, Tk=one))
+    nels, node, arr = problems.concatenate_faults(fs)
+    arr["P"][5] = 10e6
+    return problems.Problem("golden_crossing_network", 2, 0, nels, node, arr, tol_solver=1e-6,
+                            sources=[(2, 3, 1e-5, 0.0, 1e5)])
+
+
+def tree_network():
+    """Seeded 20-fault network for the cluster / block cluster trees (leaf 16, eta 2)."""
+    q = problems.network_small(n_faults=20, seed=3)
+    q.nleaf = 16
+    q.eta = 2.0
+    return q
