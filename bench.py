@@ -1,0 +1,145 @@
+"""Benchmark of the per-RK-stage hot path on one MI355X.
+
+A step is one accepted step of the solver loop (src/solver.f90:59-198): CG
+pressure solves, the stress-transfer H-matvec and the rate-state RHS at every
+Cash-Karp stage, the embedded error norm and the step controller, all on the
+device.  The default workload is the seeded 800-fault Mode II network (~80k
+elements, H-matrix eta 2 / ACA 1e-6 / leaf 64, FP64).
+
+Prints ONE JSON line: simulated years per wall-clock hour (whole job), the
+roofline of the dominant kernel (the H-matvec panel kernel, HIP events on the
+library's stream) and a CPU baseline (the oracle, timed on this host's cores on
+a bounded sample continuing from the GPU's final state).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs plain FP64"
+YEAR = 365.25 * 86400.0
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name, seed):
+    from demystification_dfn_amd import _abi, problems
+    if name == "network80k":
+        p = problems.network_80k(seed=seed)
+        desc = dict(workload="network800_80k", n_faults=800, box_m=7000, length_law="p(L)~L^-0.1 on [50 m, 2 km]",
+                    element_m=10, fracture_mode="II", kernel="hmatrix", eta=2.0, eps_aca=1e-6, leaf=64,
+                    friction="aging, a~U[0.008,0.012], b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa",
+                    pulse="10 MPa on one seeded element", seed=seed)
+    elif name == "single_fault":
+        p = problems.single_fault(n=4000, length=40e3, static_kernel=_abi.KERNEL_DIRECT, tol=1e-8)
+        desc = dict(workload="single_fault_4000", fracture_mode="III", kernel="direct", length_m=40e3,
+                    injection="Q=1.25e-6 for 100 days", tol=1e-8)
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    desc["N"] = p.N
+    return p, desc
+
+
+def cpu_baseline(sim, p, seconds, threads):
+    """The oracle (plain-C restatement) continuing from the device's state on the
+    same operator block lists, timed for ~`seconds` of wall time."""
+    from oracle.oracle import OracleSimulation, _lib
+    o = OracleSimulation(p, threads=threads)
+    if p.static_kernel == 1:
+        for op in ((0, 1) if p.fracture_mode == 2 else (0,)):
+            o.hmat_import(op, sim.hmat_export(op))
+    Y, P, _, t = sim.get_state()
+    o.set_state(Y, P, t)
+    _lib.dfno_set_t_try.argtypes = [type(o._h), __import__("ctypes").c_double]
+    _lib.dfno_set_t_try(o._h, float(sim.stats()["t_try"]))
+    t_start = o.stats()["time"]
+    steps = 0
+    w0 = time.perf_counter()
+    while time.perf_counter() - w0 < seconds:
+        o.solve(1)
+        steps += 1
+    wall = time.perf_counter() - w0
+    sim_dt = o.stats()["time"] - t_start
+    return dict(value=(sim_dt / YEAR) / (wall / 3600.0), unit="simulated yr / wall hr", cores=threads,
+                kind="port", sample=f"{steps} accepted steps continuing from the GPU's final state "
+                                   f"({wall:.1f} s wall, {sim_dt:.3e} s simulated, OpenMP matvec)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="network80k")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this path has no multi-GPU form (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from demystification_dfn_amd.device import DeviceSimulation, device_info
+    log("[bench]", device_info())
+    p, desc = make_workload(args.workload, args.seed)
+    log(f"[bench] workload {desc['workload']} N={p.N}")
+    sim = DeviceSimulation(p)
+    t0 = time.perf_counter()
+    sim.initiate_hmat()
+    setup_s = time.perf_counter() - t0
+    hm = {}
+    if p.static_kernel == 1:
+        ops = (0, 1) if p.fracture_mode == 2 else (0,)
+        st = [sim.hmat_stats(op) for op in ops]
+        hm = dict(S_entries=int(sum(s["S_dense"] + s["S_lowrank_U"] + s["S_lowrank_V"] for s in st)),
+                  max_rank=int(max(s["max_rank"] for s in st)),
+                  lowrank_blocks=int(sum(s["nb_lowrank"] for s in st)),
+                  dense_blocks=int(sum(s["nb_dense"] for s in st)))
+    log(f"[bench] H-matrix setup {setup_s:.1f} s {hm}")
+    sim.solve(args.warmup)
+    s0 = sim.stats()
+    log(f"[bench] warmup done t={s0['time']:.3e}")
+    w0 = time.perf_counter()
+    sim.solve(args.steps)
+    wall = time.perf_counter() - w0
+    s1 = sim.stats()
+    sim_dt = s1["time"] - s0["time"]
+    value = (sim_dt / YEAR) / (wall / 3600.0)
+    log(f"[bench] {args.steps} steps in {wall:.2f} s, simulated {sim_dt:.3e} s")
+    kb = sim.bench_kernel(reps=20)
+    achieved = kb["dominant_bytes"] / (kb["ms_dominant"] * 1e-3) / 1e9
+    whole = kb["alg_bytes"] / (kb["ms_kernel"] * 1e-3) / 1e9
+    out = dict(
+        metric=METRIC, value=value, unit="simulated yr / wall hr", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+        data="synthetic (seeded generator)", config=desc,
+        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                      traffic=None, kernel="panel_kernel<double> (H-matvec phase 2)",
+                      bytes_per_launch=kb["dominant_bytes"], ms_per_launch=kb["ms_dominant"]),
+        hmatvec=dict(ms=kb["ms_kernel"], alg_bytes=kb["alg_bytes"], GB_s=whole, frac_hbm=whole / HBM_PEAK_GBS, **hm),
+        run=dict(accepted=s1["accepted_steps"] - s0["accepted_steps"],
+                 rejected=s1["rejected_attempts"] - s0["rejected_attempts"],
+                 cg_iterations=s1["cg_iterations"] - s0["cg_iterations"], sim_time_s=s1["time"],
+                 max_V=s1["max_V"], setup_s=setup_s),
+    )
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"[bench] cpu baseline on {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(sim, p, args.cpu_seconds, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,116 @@
+// Device-side data layouts shared by the kernels and the host engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dfn {
+
+constexpr int kMaxRank = 32;          // largest low-rank block rank the phase-1 kernel keeps in registers
+constexpr int kPanelMaxCols = 2048;   // columns per phase-2 panel task (w staged in 16 KB of LDS)
+constexpr int kVChunkRows = 512;      // rows of V per phase-1 wave task
+constexpr int kCgMaxIter = 200;       // compute_P gives up after 200 iterations (fluid_diffusion.f90:120)
+
+// Phase 1: zpart[zp + j] = scale * sum_{i<len} V[row0+i, j] * w[xcol+row0+i], j < k
+struct VTask {
+  int64_t voff;   // offset of the block's V (column-major, leading dimension ld)
+  int32_t ld;     // rows of V (the block's column count)
+  int32_t row0;   // first row of this chunk
+  int32_t len;    // rows in this chunk
+  int32_t k;      // rank
+  int32_t xcol;   // H index of the block's first column
+  int32_t zp;     // output offset in zpart
+  float scale;
+  int32_t pad;
+};
+
+// z of one low-rank block: w[zout + j] = sum_t zpart[zp_first + t*k + j]
+struct ZRed {
+  int32_t zp_first, ntask, k, zout;
+};
+
+// Phase 2: ypart[yp + r] = scale * sum_j P[r, j] * w[widx[j]] over a row-major m x L panel
+struct PTask {
+  int64_t poff;
+  int64_t widx;
+  int32_t m;
+  int32_t L;
+  int32_t yp;
+  float scale;
+};
+
+struct RhsParams {
+  const double *a, *b, *Dc, *f0, *V0, *nld, *sld;
+  const uint8_t *fake;
+  double eta_rad;  // mu / (2 cs), radiation damping
+  int mode3;
+};
+
+struct StatusWord {
+  unsigned long long errmax_bits;  // max |yerr/yscal| as the bits of a non-negative double
+  int tensile;
+  int pad;
+};
+
+struct StageArgs {
+  int n;
+  RhsParams p;
+  const double *y;      // step-start state (3N)
+  const double *ytemp;  // this stage's input state (3N)
+  const double *sT, *sN, *Pt, *Pd;
+  double *k[6];
+  double *ynext;        // next stage input, or the 5th-order solution at stage 6
+  double *yscal;
+  double h, t_try, tol;
+  StatusWord *status;
+};
+
+struct CgSys {
+  double dt;
+  double *diag, *A1, *A2, *A3, *b, *x, *r, *p, *Ap;
+  double *Pt, *Pd;
+  int *result;  // [ier, iterations]
+};
+
+struct CgArgs {
+  int n;
+  const CgSys *sys;
+  const double *perm_x;       // permeability_x (0-based: perm_x[k] = permeability_x(k+1))
+  const double *perm_x_diag;  // permeability_x in effect for compute_diag
+  const double *perm;         // permeability for the crossing terms of the stencil
+  const double *perm_diag;    // permeability for the crossing terms of compute_diag
+  const double *ds;
+  const double *P;            // step-start pressure
+  double cfac;                // porosity * dyn_viscosity * (fluid_comp + rock_comp)
+  double time, tol, por, beta;
+  int nsrc;
+  const int64_t *src_elem;
+  const double *src_Q, *src_tb, *src_te;
+  int ncross;
+  const int64_t *cross;
+};
+
+struct PermArgs {
+  int n, coupling;
+  double *perm, *perm_x;
+  const double *perm_star, *kmin, *sigmaN, *P, *Snk;
+};
+
+// launchers (kernels.hip)
+void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s);
+void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, const double *w, double *zpart, hipStream_t s);
+void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
+void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const double *w,
+                  double *ypart, hipStream_t s);
+void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
+                     double scale, double *yT, double *yN, hipStream_t s);
+void launch_direct3(const double *node, const double *slip, double *cbuf, int n, double scale, double *sT,
+                    hipStream_t s);
+void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, int n, double scale,
+                    double *sT, double *sN, hipStream_t s);
+void launch_stage(int stage, const StageArgs &a, hipStream_t s);
+void launch_restart_stage2(const double *y, const double *k1, double h, int n3, double *yt, hipStream_t s);
+void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
+void launch_perm_update(const PermArgs &a, hipStream_t s);
+
+}  // namespace dfn

@@ -1,0 +1,925 @@
+// Host engine and C ABI (include/dfn.h): device-resident state of one problem,
+// the device layout of the stress-transfer operators, and the solver loop of
+// src/solver.f90:59-198 driving the kernels of kernels.hip on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "dfn_device.hpp"
+#include "dfn_host.hpp"
+
+namespace dfn {
+
+#define DFN_HIP(expr)                                       \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) {                                 \
+      std::fprintf(stderr, "dfn: %s failed: %s\n", #expr,   \
+                   hipGetErrorString(e_));                  \
+      return DFN_ERR_HIP;                                   \
+    }                                                       \
+  } while (0)
+
+#define DFN_TRY(expr)           \
+  do {                          \
+    int s_ = (expr);            \
+    if (s_ != DFN_OK) return s_; \
+  } while (0)
+
+// owning device allocation
+struct DMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DMem() = default;
+  DMem(const DMem &) = delete;
+  DMem &operator=(const DMem &) = delete;
+  ~DMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t nbytes) {
+    release();
+    if (nbytes == 0) nbytes = 8;
+    if (hipMalloc(&p, nbytes) != hipSuccess) { p = nullptr; return DFN_ERR_NOMEM; }
+    bytes = nbytes;
+    return DFN_OK;
+  }
+  template <class T>
+  int upload(const std::vector<T> &v) {
+    DFN_TRY(alloc(v.size() * sizeof(T)));
+    if (!v.empty()) DFN_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return DFN_OK;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// Device layout of the stress-transfer operator(s) of one problem (1 in Mode III,
+// T and N in Mode II, sharing the permutation and the input vector).
+struct DevOps {
+  int n = 0, nops = 0;
+  bool fp32 = false;
+  DMem perm, vtasks, vdata, zred, zpart, ptasks, pdata, widx, w, ypart, fin_first, fin_idx;
+  int nvt = 0, nzr = 0, npt = 0;
+  double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
+  double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
+  int64_t S_total = 0;   // matrix entries streamed per application
+};
+
+template <class T>
+static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
+  const int n = ops[0]->n;
+  for (int o = 1; o < nops; o++)
+    if (ops[o]->perm != ops[0]->perm) return DFN_ERR_UNSUPPORTED;  // T and N share the cluster tree
+  d.n = n;
+  d.nops = nops;
+  d.fp32 = sizeof(T) == 4;
+  // w = [x_h | z of op 0 | z of op 1]
+  std::vector<int64_t> zbase(nops + 1, n);
+  std::vector<std::vector<int>> zoff(nops);
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    zoff[o].resize(H.l_k.size());
+    int64_t z = 0;
+    for (size_t b = 0; b < H.l_k.size(); b++) { zoff[o][b] = (int)z; z += H.l_k[b]; }
+    zbase[o + 1] = zbase[o] + z;
+  }
+  const int64_t wsize = zbase[nops];
+  // phase 1 tasks and V data
+  std::vector<VTask> vt;
+  std::vector<ZRed> zr;
+  std::vector<T> vdata;
+  int zp = 0;
+  int64_t S_total = 0;
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    const int64_t vbase = (int64_t)vdata.size();
+    vdata.insert(vdata.end(), H.l_v.begin(), H.l_v.end());
+    for (size_t b = 0; b < H.l_k.size(); b++) {
+      const int k = H.l_k[b], nb = H.l_n[b];
+      if (k > kMaxRank) return DFN_ERR_UNSUPPORTED;
+      ZRed r{zp, 0, k, (int)(zbase[o] + zoff[o][b])};
+      for (int r0 = 0; r0 < nb; r0 += kVChunkRows) {
+        VTask t{};
+        t.voff = vbase + H.l_voff[b];
+        t.ld = nb;
+        t.row0 = r0;
+        t.len = std::min(kVChunkRows, nb - r0);
+        t.k = k;
+        t.xcol = H.l_c0[b];
+        t.zp = zp;
+        t.scale = 1.0f;
+        vt.push_back(t);
+        zp += k;
+        r.ntask++;
+      }
+      zr.push_back(r);
+    }
+    S_total += (int64_t)H.l_v.size() + (int64_t)H.l_u.size() + (int64_t)H.d_data.size();
+  }
+  // phase 2 panels: per operator and leaf row cluster, all covering blocks
+  const HostOperator &H0 = *ops[0];
+  const int nleaf = (int)H0.leaf_r0.size();
+  std::vector<int> row_leaf(n);
+  for (int l = 0; l < nleaf; l++)
+    for (int i = 0; i < H0.leaf_m[l]; i++) row_leaf[H0.leaf_r0[l] + i] = l;
+  std::vector<PTask> pt;
+  std::vector<T> pdata;
+  std::vector<int> widx;
+  std::vector<std::vector<int>> fin((size_t)n * nops);
+  int yp = 0;
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    // covering lists: (kind, block) per leaf; dense first then low-rank, block order
+    std::vector<std::vector<std::pair<int, int>>> cover(nleaf);
+    for (size_t b = 0; b < H.d_m.size(); b++)
+      for (int l = row_leaf[H.d_r0[b]]; l < nleaf && H0.leaf_r0[l] < H.d_r0[b] + H.d_m[b]; l++)
+        cover[l].push_back({0, (int)b});
+    for (size_t b = 0; b < H.l_m.size(); b++)
+      for (int l = row_leaf[H.l_r0[b]]; l < nleaf && H0.leaf_r0[l] < H.l_r0[b] + H.l_m[b]; l++)
+        cover[l].push_back({1, (int)b});
+    for (int l = 0; l < nleaf; l++) {
+      const int r0 = H0.leaf_r0[l], m = H0.leaf_m[l];
+      // columns of this panel: (w index, source pointer into the block, column stride)
+      struct Col { int64_t wi; const double *src; };
+      std::vector<Col> cols;
+      for (auto [kind, b] : cover[l]) {
+        if (kind == 0) {
+          const double *F = H.d_data.data() + H.d_off[b] + (r0 - H.d_r0[b]);
+          for (int j = 0; j < H.d_n[b]; j++) cols.push_back({(int64_t)H.d_c0[b] + j, F + (int64_t)j * H.d_m[b]});
+        } else {
+          const double *U = H.l_u.data() + H.l_uoff[b] + (r0 - H.l_r0[b]);
+          for (int j = 0; j < H.l_k[b]; j++) cols.push_back({zbase[o] + zoff[o][b] + j, U + (int64_t)j * H.l_m[b]});
+        }
+      }
+      for (size_t c0 = 0; c0 < cols.size(); c0 += kPanelMaxCols) {
+        const int L = (int)std::min<size_t>(kPanelMaxCols, cols.size() - c0);
+        PTask t{};
+        t.poff = (int64_t)pdata.size();
+        t.widx = (int64_t)widx.size();
+        t.m = m;
+        t.L = L;
+        t.yp = yp;
+        t.scale = 1.0f;
+        for (int j = 0; j < L; j++) widx.push_back((int)cols[c0 + j].wi);
+        for (int r = 0; r < m; r++)
+          for (int j = 0; j < L; j++) pdata.push_back((T)cols[c0 + j].src[r]);
+        for (int r = 0; r < m; r++) fin[(size_t)o * n + r0 + r].push_back(yp + r);
+        yp += m;
+        pt.push_back(t);
+      }
+    }
+  }
+  std::vector<int> fin_first((size_t)n * nops + 1, 0), fin_idx;
+  for (size_t g = 0; g < fin.size(); g++) {
+    fin_first[g] = (int)fin_idx.size();
+    fin_idx.insert(fin_idx.end(), fin[g].begin(), fin[g].end());
+  }
+  fin_first[fin.size()] = (int)fin_idx.size();
+  if (widx.size() > (size_t)INT32_MAX || zp < 0 || yp < 0) return DFN_ERR_UNSUPPORTED;
+  DFN_TRY(d.perm.upload(H0.perm));
+  DFN_TRY(d.vtasks.upload(vt));
+  DFN_TRY(d.vdata.upload(vdata));
+  DFN_TRY(d.zred.upload(zr));
+  DFN_TRY(d.zpart.alloc(sizeof(double) * std::max(zp, 1)));
+  DFN_TRY(d.ptasks.upload(pt));
+  DFN_TRY(d.pdata.upload(pdata));
+  DFN_TRY(d.widx.upload(widx));
+  DFN_TRY(d.w.alloc(sizeof(double) * wsize));
+  DFN_TRY(d.ypart.alloc(sizeof(double) * std::max(yp, 1)));
+  DFN_TRY(d.fin_first.upload(fin_first));
+  DFN_TRY(d.fin_idx.upload(fin_idx));
+  d.nvt = (int)vt.size();
+  d.nzr = (int)zr.size();
+  d.npt = (int)pt.size();
+  d.S_total = S_total;
+  // bytes: factors + x read + y written, per operator (x shared in Mode II)
+  d.alg_bytes = (double)sizeof(T) * (double)S_total + 8.0 * n + 8.0 * n * nops;
+  d.panel_bytes = (double)sizeof(T) * (double)pdata.size() + 8.0 * (double)yp;
+  return DFN_OK;
+}
+
+struct Engine {
+  HostProblem hp;
+  HostOperator hop[2];
+  bool have_op[2] = {false, false};
+  int precision = DFN_PREC_FP64;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<DevOps> ops64, ops32;
+  // problem data on the device
+  DMem node, elem, nrm, ds, a, b, Dc, f0, V0, nld, sld, fake, perm_star, kmin, Snk, src_elem, src_Q, src_tb, src_te,
+      cross, dcbuf;
+  DMem perm_cur, perm_next, permx_cur, permx_next;
+  // state and stage buffers
+  DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot, scratch3, scratch3b;
+  DMem cg_work, cg_sys, cg_res;
+  DMem status;
+  StatusWord *h_status = nullptr;  // pinned
+  int *h_cgres = nullptr;          // pinned [kSlots*2]
+  static constexpr int kSlots = 6;
+  int last_pd_slot = -1;
+  double time = 0, t_try = 100.0;
+  dfn_run_stats st{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  double *k(int i) const { return kbuf.as<double>() + (size_t)i * 3 * hp.N; }
+  double *cg_ptr(int slot, int which) const {  // 12 vectors of N per slot
+    return cg_work.as<double>() + ((size_t)slot * 12 + which) * hp.N;
+  }
+  double *Pt(int slot) const { return cg_ptr(slot, 10); }
+  double *Pd(int slot) const { return cg_ptr(slot, 11); }
+
+  ~Engine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_status) (void)hipHostFree(h_status);
+    if (h_cgres) (void)hipHostFree(h_cgres);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    ops64.reset();
+    ops32.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  int init() {
+    int dev = 0;
+    DFN_HIP(hipGetDevice(&dev));
+    DFN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    DFN_HIP(hipEventCreate(&ev0));
+    DFN_HIP(hipEventCreate(&ev1));
+    const int N = hp.N;
+    DFN_TRY(node.upload(hp.node));
+    DFN_TRY(elem.upload(hp.elem));
+    DFN_TRY(nrm.upload(hp.nrm));
+    DFN_TRY(ds.upload(hp.ds));
+    DFN_TRY(a.upload(hp.a));
+    DFN_TRY(b.upload(hp.b));
+    DFN_TRY(Dc.upload(hp.Dc));
+    DFN_TRY(f0.upload(hp.f0));
+    DFN_TRY(V0.upload(hp.V0));
+    DFN_TRY(nld.upload(hp.nld));
+    DFN_TRY(sld.upload(hp.sld));
+    DFN_TRY(fake.upload(hp.is_fake));
+    DFN_TRY(perm_star.upload(hp.perm_star));
+    DFN_TRY(kmin.upload(hp.kmin));
+    DFN_TRY(Snk.upload(hp.Snk));
+    DFN_TRY(src_elem.upload(hp.src_elem));
+    DFN_TRY(src_Q.upload(hp.src_Q));
+    DFN_TRY(src_tb.upload(hp.src_tb));
+    DFN_TRY(src_te.upload(hp.src_te));
+    DFN_TRY(cross.upload(hp.cross));
+    DFN_TRY(dcbuf.alloc(sizeof(double) * (N + 1)));
+    // permeability_x (initialisation.f90:252-266): perm_x[k] = hmean(perm[k-1], perm[k]), k = 1..N-1
+    std::vector<double> px(N + 1, 0.0);
+    for (int kk = 1; kk < N; kk++)
+      px[kk] = 2.0 * hp.perm0[kk - 1] * hp.perm0[kk] / (hp.perm0[kk - 1] + hp.perm0[kk]);
+    DFN_TRY(perm_cur.upload(hp.perm0));
+    DFN_TRY(perm_next.upload(hp.perm0));
+    DFN_TRY(permx_cur.upload(px));
+    DFN_TRY(permx_next.upload(px));
+    const size_t n3 = 3 * (size_t)N * sizeof(double);
+    DFN_TRY(Y.upload(hp.Y0));
+    DFN_TRY(Ynew.alloc(n3));
+    DFN_TRY(yscal.alloc(n3));
+    DFN_TRY(sT.alloc(sizeof(double) * N));
+    DFN_TRY(sN.alloc(sizeof(double) * N));
+    DFN_HIP(hipMemset(sN.p, 0, sizeof(double) * N));
+    DFN_TRY(kbuf.alloc(6 * n3));
+    DFN_TRY(ytA.alloc(n3));
+    DFN_TRY(ytB.alloc(n3));
+    DFN_TRY(scratch3.alloc(n3));
+    DFN_TRY(scratch3b.alloc(n3));
+    DFN_TRY(P.upload(hp.P0));
+    DFN_TRY(Pdot.alloc(sizeof(double) * N));
+    DFN_HIP(hipMemset(Pdot.p, 0, sizeof(double) * N));
+    DFN_TRY(cg_work.alloc(sizeof(double) * (size_t)kSlots * 12 * N));
+    DFN_TRY(cg_res.alloc(sizeof(int) * kSlots * 2));
+    DFN_TRY(cg_sys.alloc(sizeof(CgSys) * kSlots));
+    DFN_TRY(status.alloc(sizeof(StatusWord)));
+    DFN_HIP(hipHostMalloc((void **)&h_status, sizeof(StatusWord), hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_cgres, sizeof(int) * kSlots * 2, hipHostMallocDefault));
+    time = 0.0;
+    t_try = 100.0;
+    return DFN_OK;
+  }
+
+  double kernel_scale() const {
+    const double pi = 4.0 * std::atan(1.0);
+    if (hp.mode == DFN_MODE_III) return hp.mu / (2.0 * pi);
+    return hp.mu / pi * (1.0 - hp.cs * hp.cs / (hp.cp * hp.cp));
+  }
+
+  int relayout() {
+    ops64.reset();
+    ops32.reset();
+    const int nops = hp.mode == DFN_MODE_II ? 2 : 1;
+    const HostOperator *list[2] = {&hop[0], &hop[1]};
+    for (int o = 0; o < nops; o++)
+      if (!have_op[o]) return DFN_OK;  // incomplete until every operator exists
+    ops64.reset(new DevOps);
+    DFN_TRY(layout_ops<double>(list, nops, *ops64));
+    if (precision == DFN_PREC_FP32) {
+      ops32.reset(new DevOps);
+      DFN_TRY(layout_ops<float>(list, nops, *ops32));
+    }
+    return DFN_OK;
+  }
+
+  const DevOps *active_ops() const { return (precision == DFN_PREC_FP32 && ops32) ? ops32.get() : ops64.get(); }
+
+  // stress rates sT (and sN) from the slip-rate vector d_V (device, original order)
+  int apply_kernel(const double *d_V, double *d_sT, double *d_sN) {
+    const int N = hp.N;
+    if (hp.kernel == DFN_KERNEL_DIRECT) {
+      if (hp.mode == DFN_MODE_III)
+        launch_direct3(node.as<double>(), d_V, dcbuf.as<double>(), N, kernel_scale(), d_sT, stream);
+      else
+        launch_direct2(node.as<double>(), elem.as<double>(), nrm.as<double>(), d_V, N, kernel_scale(), d_sT, d_sN,
+                       stream);
+      return DFN_OK;
+    }
+    const DevOps *d = active_ops();
+    if (!d) return DFN_ERR_STATE;
+    double *w = d->w.as<double>();
+    launch_gather(d_V, d->perm.as<int>(), w, N, stream);
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream);
+    launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
+    launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), w, d->ypart.as<double>(),
+                 stream);
+    launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
+                    d->nops, kernel_scale(), d_sT, d->nops > 1 ? d_sN : d_sT, stream);
+    return DFN_OK;
+  }
+
+  // compute_P for the time steps dts[0..nsys) into CG slots slot0.. (no host sync)
+  int cg_launch(const double *dts, int nsys, int slot0) {
+    const int N = hp.N;
+    std::vector<CgSys> sys(nsys);
+    for (int s = 0; s < nsys; s++) {
+      const int sl = slot0 + s;
+      CgSys &c = sys[s];
+      c.dt = dts[s];
+      c.diag = cg_ptr(sl, 0); c.A1 = cg_ptr(sl, 1); c.A2 = cg_ptr(sl, 2); c.A3 = cg_ptr(sl, 3);
+      c.b = cg_ptr(sl, 4); c.x = cg_ptr(sl, 5); c.r = cg_ptr(sl, 6); c.p = cg_ptr(sl, 7); c.Ap = cg_ptr(sl, 8);
+      c.Pt = Pt(sl); c.Pd = Pd(sl);
+      c.result = cg_res.as<int>() + 2 * sl;
+    }
+    DFN_HIP(hipMemcpyAsync(cg_sys.as<CgSys>() + slot0, sys.data(), sizeof(CgSys) * nsys, hipMemcpyHostToDevice,
+                           stream));
+    // permeability update (fluid_diffusion.f90:67-75); diag keeps the values before it
+    const double *px_diag = permx_cur.as<double>(), *p_diag = perm_cur.as<double>();
+    const double *px = px_diag, *pp = p_diag;
+    if (hp.coupling == 1) {
+      DFN_HIP(hipMemcpyAsync(perm_next.p, perm_cur.p, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+      PermArgs pa{N, 1, perm_next.as<double>(), permx_next.as<double>(), perm_star.as<double>(), kmin.as<double>(),
+                  Y.as<double>() + 2 * N, P.as<double>(), Snk.as<double>()};
+      launch_perm_update(pa, stream);
+      px = permx_next.as<double>();
+      pp = perm_next.as<double>();
+    }
+    CgArgs A{};
+    A.n = N;
+    A.sys = cg_sys.as<CgSys>() + slot0;
+    A.perm_x = px;
+    A.perm_x_diag = px_diag;
+    A.perm = pp;
+    A.perm_diag = p_diag;
+    A.ds = ds.as<double>();
+    A.P = P.as<double>();
+    A.cfac = hp.por * hp.visc * (hp.fcomp + hp.rcomp);
+    A.time = time;
+    A.tol = hp.tol;
+    A.por = hp.por;
+    A.beta = hp.fcomp + hp.rcomp;
+    A.nsrc = (int)hp.src_elem.size();
+    A.src_elem = src_elem.as<int64_t>();
+    A.src_Q = src_Q.as<double>();
+    A.src_tb = src_tb.as<double>();
+    A.src_te = src_te.as<double>();
+    A.ncross = hp.ncross();
+    A.cross = cross.as<int64_t>();
+    launch_cg(A, nsys, stream);
+    if (hp.coupling == 1) {  // the updated permeability becomes the state (swap buffers)
+      std::swap(perm_cur.p, perm_next.p);
+      std::swap(permx_cur.p, permx_next.p);
+    }
+    st.cg_solves += nsys;
+    last_pd_slot = slot0 + nsys - 1;
+    return DFN_OK;
+  }
+
+  int cg_collect(int slot0, int nsys, int *any_fail) {
+    DFN_HIP(hipMemcpyAsync(h_cgres + 2 * slot0, cg_res.as<int>() + 2 * slot0, sizeof(int) * 2 * nsys,
+                           hipMemcpyDeviceToHost, stream));
+    DFN_HIP(hipStreamSynchronize(stream));
+    *any_fail = 0;
+    for (int s = slot0; s < slot0 + nsys; s++) {
+      st.cg_iterations += h_cgres[2 * s + 1];
+      if (h_cgres[2 * s]) *any_fail = 1;
+    }
+    return DFN_OK;
+  }
+
+  StageArgs stage_args(const double *ytemp, int slot, double *ynext, double h) {
+    const int N = hp.N;
+    StageArgs s{};
+    s.n = N;
+    s.p = RhsParams{a.as<double>(), b.as<double>(), Dc.as<double>(), f0.as<double>(), V0.as<double>(),
+                    nld.as<double>(), sld.as<double>(), fake.as<uint8_t>(), hp.mu / (2.0 * hp.cs),
+                    hp.mode == DFN_MODE_III ? 1 : 0};
+    s.y = Y.as<double>();
+    s.ytemp = ytemp;
+    s.sT = sT.as<double>();
+    s.sN = hp.mode == DFN_MODE_II ? sN.as<double>() : nullptr;
+    s.Pt = Pt(slot);
+    s.Pd = Pd(slot);
+    for (int i = 0; i < 6; i++) s.k[i] = k(i);
+    s.ynext = ynext;
+    s.yscal = yscal.as<double>();
+    s.h = h;
+    s.t_try = t_try;
+    s.tol = hp.tol;
+    s.status = status.as<StatusWord>();
+    return s;
+  }
+
+  // friction_law(x, Y) for a device Y into device Ydot (uses CG slot 0)
+  int friction_law_dev(double x, const double *dY, double *dYd) {
+    DFN_TRY(apply_kernel(dY, sT.as<double>(), sN.as<double>()));
+    DFN_TRY(cg_launch(&x, 1, 0));
+    int fail = 0;
+    DFN_TRY(cg_collect(0, 1, &fail));
+    st.rhs_evaluations++;
+    if (fail) return DFN_ERR_CG;
+    StageArgs s = stage_args(dY, 0, nullptr, 0.0);
+    double *k0 = s.k[0];
+    s.k[0] = dYd;
+    launch_stage(0, s, stream);
+    s.k[0] = k0;
+    DFN_HIP(hipStreamSynchronize(stream));
+    return DFN_OK;
+  }
+
+  // rkqs (ode_bs.f90:125-174) on the device state Y with k1 = dydx already in k(0)
+  // and yscal set; x is the step-start abscissa.  On success Y holds the new state.
+  int rkqs_dev(double x, double htry, double eps, double *hdid, double *hnext) {
+    const int N = hp.N, n3 = 3 * N;
+    const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = (double)1.89e-4f;
+    double h = htry, errmax = 0.0;
+    launch_restart_stage2(Y.as<double>(), k(0), h, n3, ytA.as<double>(), stream);
+    for (;;) {
+      static const double Ai[5] = {0.2, 0.3, 0.6, 1.0, 0.875};
+      double dts[5];
+      for (int s = 0; s < 5; s++) dts[s] = x + Ai[s] * h;
+      DFN_TRY(cg_launch(dts, 5, 1));
+      DFN_HIP(hipMemsetAsync(status.p, 0, sizeof(StatusWord), stream));
+      double *in = ytA.as<double>(), *out = ytB.as<double>();
+      for (int s = 2; s <= 6; s++) {
+        DFN_TRY(apply_kernel(in, sT.as<double>(), sN.as<double>()));
+        StageArgs a = stage_args(in, s - 1, s == 6 ? Ynew.as<double>() : out, h);
+        launch_stage(s, a, stream);
+        st.rhs_evaluations++;
+        std::swap(in, out);
+      }
+      DFN_HIP(hipMemcpyAsync(h_status, status.p, sizeof(StatusWord), hipMemcpyDeviceToHost, stream));
+      int fail = 0;
+      DFN_TRY(cg_collect(1, 5, &fail));
+      if (fail) return DFN_ERR_CG;
+      double em;
+      std::memcpy(&em, &h_status->errmax_bits, sizeof(double));
+      errmax = em / eps;
+      if (errmax <= 1.0) break;
+      st.rejected_attempts++;
+      const double htemp = SAFETY * h * std::pow(errmax, PSHRNK);
+      h = std::copysign(std::fmax(std::fabs(htemp), 0.1 * std::fabs(h)), h);
+      const double xnew = x + h;
+      if (xnew == x) return DFN_ERR_UNDERFLOW;
+      launch_restart_stage2(Y.as<double>(), k(0), h, n3, ytA.as<double>(), stream);
+    }
+    if (errmax > ERRCON) *hnext = SAFETY * h * std::pow(errmax, PGROW);
+    else *hnext = 5.0 * h;
+    *hdid = h;
+    std::swap(Y.p, Ynew.p);
+    return DFN_OK;
+  }
+
+  // one iteration of src/solver.f90:59-198
+  int step() {
+    const int N = hp.N;
+    if (t_try > 10.0)  // :64-77
+      for (size_t s = 0; s < hp.src_tb.size(); s++)
+        if ((time < hp.src_tb[s]) && (time + t_try > hp.src_tb[s])) t_try = 10.0;
+    int fail = 0;
+    DFN_TRY(cg_launch(&t_try, 1, 0));  // :81-92
+    DFN_TRY(cg_collect(0, 1, &fail));
+    while (fail) {
+      t_try = t_try / 2.0;
+      st.cg_halvings++;
+      DFN_TRY(cg_launch(&t_try, 1, 0));
+      DFN_TRY(cg_collect(0, 1, &fail));
+    }
+    // k1 = f(t_try, Y) and yscal (:101-102); compute_P(t_try) of the RHS equals the
+    // call above unless the permeability evolves between calls
+    DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
+    if (hp.coupling == 1) {
+      DFN_TRY(cg_launch(&t_try, 1, 0));
+      DFN_TRY(cg_collect(0, 1, &fail));
+      if (fail) return DFN_ERR_CG;
+    }
+    StageArgs s1 = stage_args(Y.as<double>(), 0, ytA.as<double>(), t_try);
+    launch_stage(1, s1, stream);
+    st.rhs_evaluations++;
+    double t_did = 0, t_next = 0;
+    DFN_TRY(rkqs_dev(0.0, t_try, hp.tol, &t_did, &t_next));  // :111
+    t_try = t_next;  // :117-120
+    if (t_try > 24 * 60 * 60) t_try = 24 * 60 * 60;
+    // commit compute_P(t_did) (:174): the stage-5 solve used dt = 0 + 1.0*h = t_did
+    if (hp.coupling == 1) {
+      DFN_TRY(cg_launch(&t_did, 1, 5));
+      DFN_TRY(cg_collect(5, 1, &fail));
+    }
+    if (!fail) {
+      DFN_HIP(hipMemcpyAsync(P.p, Pt(4), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+      DFN_HIP(hipMemcpyAsync(Pdot.p, Pd(4), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+      if (hp.coupling == 1) {
+        DFN_HIP(hipMemcpyAsync(P.p, Pt(5), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+        DFN_HIP(hipMemcpyAsync(Pdot.p, Pd(5), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+      }
+    }
+    last_pd_slot = -1;
+    time = time + t_did;  // :177
+    st.accepted_steps++;
+    if (h_status->tensile) return DFN_ERR_TENSILE;  // check_errors :412-420 on the accepted state
+    return DFN_OK;
+  }
+
+  int sync_pdot() {
+    if (last_pd_slot >= 0) {
+      DFN_HIP(hipMemcpyAsync(Pdot.p, Pd(last_pd_slot), sizeof(double) * hp.N, hipMemcpyDeviceToDevice, stream));
+      last_pd_slot = -1;
+    }
+    return DFN_OK;
+  }
+};
+
+static std::string g_last_error;
+
+}  // namespace dfn
+
+using dfn::Engine;
+
+struct dfn_handle_s {
+  Engine e;
+};
+
+#define API extern "C" __attribute__((visibility("default")))
+
+static int device_ok() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return DFN_ERR_HIP;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return DFN_ERR_HIP;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    std::fprintf(stderr, "dfn: device is %s, this library is built for gfx950\n", prop.gcnArchName);
+    return DFN_ERR_HIP;
+  }
+  return DFN_OK;
+}
+
+API int dfn_device_info(char *buf, int32_t len) {
+  if (!buf || len < 1) return DFN_ERR_ARG;
+  int n = 0;
+  std::string s;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    s = "no HIP device";
+  } else {
+    hipDeviceProp_t prop;
+    (void)hipGetDeviceProperties(&prop, 0);
+    char tmp[512];
+    std::snprintf(tmp, sizeof(tmp), "%s %s CUs=%d mem=%.1fGB", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+                  prop.totalGlobalMem / 1e9);
+    s = tmp;
+  }
+  std::strncpy(buf, s.c_str(), (size_t)len - 1);
+  buf[len - 1] = 0;
+  return DFN_OK;
+}
+
+API int dfn_create(const dfn_problem *p, dfn_handle *out) {
+  if (!out) return DFN_ERR_ARG;
+  *out = nullptr;
+  int st = device_ok();
+  if (st) return st;
+  dfn_handle h = new (std::nothrow) dfn_handle_s;
+  if (!h) return DFN_ERR_NOMEM;
+  st = dfn::setup_problem(p, h->e.hp);
+  if (st == DFN_OK) st = h->e.init();
+  if (st != DFN_OK) { delete h; return st; }
+  *out = h;
+  return DFN_OK;
+}
+
+API int dfn_destroy(dfn_handle h) {
+  if (!h) return DFN_ERR_ARG;
+  delete h;
+  return DFN_OK;
+}
+
+API int dfn_build_hmat(dfn_handle h) {
+  if (!h) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  if (e.hp.kernel != DFN_KERNEL_HMATRIX) return DFN_OK;
+  if (e.hp.mode == DFN_MODE_II) {  // N then T (src/initialisation.f90:222-246)
+    DFN_TRY(dfn::build_operator(e.hp, DFN_OP_N, e.hop[DFN_OP_N]));
+    e.have_op[DFN_OP_N] = true;
+  }
+  DFN_TRY(dfn::build_operator(e.hp, DFN_OP_T, e.hop[DFN_OP_T]));
+  e.have_op[DFN_OP_T] = true;
+  return e.relayout();
+}
+
+API int dfn_set_precision(dfn_handle h, int32_t prec) {
+  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32)) return DFN_ERR_ARG;
+  h->e.precision = prec;
+  if (prec == DFN_PREC_FP32 && !h->e.ops32) return h->e.relayout();
+  return DFN_OK;
+}
+
+API int dfn_hmat_stats(dfn_handle h, int32_t op, int64_t *stats) {
+  if (!h || !stats || op < 0 || op > 1) return DFN_ERR_ARG;
+  const dfn::HostOperator &H = h->e.hop[op];
+  if (!h->e.have_op[op]) return DFN_ERR_STATE;
+  int64_t mr = 0, sr = 0;
+  for (int k : H.l_k) { mr = std::max<int64_t>(mr, k); sr += k; }
+  const int64_t s[10] = {(int64_t)H.d_m.size(), (int64_t)H.l_m.size(), (int64_t)H.d_data.size(),
+                         (int64_t)H.l_u.size(), (int64_t)H.l_v.size(), mr, H.nbclus, H.nbblclus, H.n, sr};
+  std::memcpy(stats, s, sizeof(s));
+  return DFN_OK;
+}
+
+API int dfn_hmat_export(dfn_handle h, int32_t op, int32_t *perm, int32_t *d_r0, int32_t *d_c0, int32_t *d_m,
+                        int32_t *d_n, int64_t *d_off, double *d_data, int32_t *l_r0, int32_t *l_c0, int32_t *l_m,
+                        int32_t *l_n, int32_t *l_k, int64_t *l_uoff, int64_t *l_voff, double *l_u, double *l_v) {
+  if (!h || op < 0 || op > 1) return DFN_ERR_ARG;
+  if (!h->e.have_op[op]) return DFN_ERR_STATE;
+  const dfn::HostOperator &H = h->e.hop[op];
+  auto cp = [](auto *dst, const auto &v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+  };
+  cp(perm, H.perm); cp(d_r0, H.d_r0); cp(d_c0, H.d_c0); cp(d_m, H.d_m); cp(d_n, H.d_n); cp(d_off, H.d_off);
+  cp(d_data, H.d_data); cp(l_r0, H.l_r0); cp(l_c0, H.l_c0); cp(l_m, H.l_m); cp(l_n, H.l_n); cp(l_k, H.l_k);
+  cp(l_uoff, H.l_uoff); cp(l_voff, H.l_voff); cp(l_u, H.l_u); cp(l_v, H.l_v);
+  return DFN_OK;
+}
+
+API int dfn_hmat_import(dfn_handle h, int32_t op, int32_t n, const int32_t *perm, int32_t nd, const int32_t *d_r0,
+                        const int32_t *d_c0, const int32_t *d_m, const int32_t *d_n, const int64_t *d_off,
+                        const double *d_data, int32_t nl, const int32_t *l_r0, const int32_t *l_c0,
+                        const int32_t *l_m, const int32_t *l_n, const int32_t *l_k, const int64_t *l_uoff,
+                        const int64_t *l_voff, const double *l_u, const double *l_v) {
+  if (!h || op < 0 || op > 1 || !perm || n != h->e.hp.N || nd < 0 || nl < 0) return DFN_ERR_ARG;
+  if (op == DFN_OP_N && h->e.hp.mode != DFN_MODE_II) return DFN_ERR_ARG;
+  dfn::HostOperator H;
+  H.n = n;
+  H.perm.assign(perm, perm + n);
+  std::vector<int> seen(n, 0);
+  for (int v : H.perm) {
+    if (v < 0 || v >= n || seen[v]) return DFN_ERR_ARG;
+    seen[v] = 1;
+  }
+  H.d_r0.assign(d_r0, d_r0 + nd); H.d_c0.assign(d_c0, d_c0 + nd); H.d_m.assign(d_m, d_m + nd);
+  H.d_n.assign(d_n, d_n + nd); H.d_off.assign(d_off, d_off + nd);
+  H.l_r0.assign(l_r0, l_r0 + nl); H.l_c0.assign(l_c0, l_c0 + nl); H.l_m.assign(l_m, l_m + nl);
+  H.l_n.assign(l_n, l_n + nl); H.l_k.assign(l_k, l_k + nl); H.l_uoff.assign(l_uoff, l_uoff + nl);
+  H.l_voff.assign(l_voff, l_voff + nl);
+  int64_t ds = 0, us = 0, vs = 0;
+  std::vector<int> cut(n + 1, 0);  // row starts of all blocks -> leaf row clusters
+  for (int b = 0; b < nd; b++) {
+    if (d_m[b] < 1 || d_n[b] < 1 || d_r0[b] < 0 || d_c0[b] < 0 || d_r0[b] + d_m[b] > n || d_c0[b] + d_n[b] > n)
+      return DFN_ERR_ARG;
+    ds = std::max<int64_t>(ds, d_off[b] + (int64_t)d_m[b] * d_n[b]);
+    cut[d_r0[b]] = 1; cut[d_r0[b] + d_m[b]] = 1;
+  }
+  for (int b = 0; b < nl; b++) {
+    if (l_m[b] < 1 || l_n[b] < 1 || l_k[b] < 0 || l_k[b] > dfn::kMaxRank || l_r0[b] < 0 || l_c0[b] < 0 ||
+        l_r0[b] + l_m[b] > n || l_c0[b] + l_n[b] > n)
+      return DFN_ERR_ARG;
+    us = std::max<int64_t>(us, l_uoff[b] + (int64_t)l_m[b] * l_k[b]);
+    vs = std::max<int64_t>(vs, l_voff[b] + (int64_t)l_n[b] * l_k[b]);
+    cut[l_r0[b]] = 1; cut[l_r0[b] + l_m[b]] = 1;
+  }
+  H.d_data.assign(d_data, d_data + ds);
+  H.l_u.assign(l_u, l_u + us);
+  H.l_v.assign(l_v, l_v + vs);
+  cut[0] = 1; cut[n] = 1;
+  int prev = 0;
+  for (int i = 1; i <= n; i++)
+    if (cut[i]) { H.leaf_r0.push_back(prev); H.leaf_m.push_back(i - prev); prev = i; }
+  Engine &e = h->e;
+  // in Mode II both operators must share row splits: merge with the other operator's
+  const int other = 1 - op;
+  if (e.hp.mode == DFN_MODE_II && e.have_op[other]) {
+    std::vector<int> c2(n + 1, 0);
+    for (size_t l = 0; l < H.leaf_r0.size(); l++) c2[H.leaf_r0[l]] = 1;
+    for (size_t l = 0; l < e.hop[other].leaf_r0.size(); l++) c2[e.hop[other].leaf_r0[l]] = 1;
+    c2[n] = 1;
+    std::vector<int> r0s, ms;
+    prev = 0;
+    for (int i = 1; i <= n; i++)
+      if (c2[i]) { r0s.push_back(prev); ms.push_back(i - prev); prev = i; }
+    H.leaf_r0 = r0s; H.leaf_m = ms;
+    e.hop[other].leaf_r0 = r0s; e.hop[other].leaf_m = ms;
+  }
+  e.hop[op] = std::move(H);
+  e.have_op[op] = true;
+  return e.relayout();
+}
+
+API int dfn_set_state(dfn_handle h, const double *Yh, const double *Ph, double time) {
+  if (!h) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const size_t N = e.hp.N;
+  if (Yh) DFN_HIP(hipMemcpy(e.Y.p, Yh, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+  if (Ph) DFN_HIP(hipMemcpy(e.P.p, Ph, N * sizeof(double), hipMemcpyHostToDevice));
+  e.time = time;
+  return DFN_OK;
+}
+
+API int dfn_get_state(dfn_handle h, double *Yh, double *Ph, double *Pdh, double *time) {
+  if (!h) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const size_t N = e.hp.N;
+  DFN_TRY(e.sync_pdot());
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  if (Yh) DFN_HIP(hipMemcpy(Yh, e.Y.p, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+  if (Ph) DFN_HIP(hipMemcpy(Ph, e.P.p, N * sizeof(double), hipMemcpyDeviceToHost));
+  if (Pdh) DFN_HIP(hipMemcpy(Pdh, e.Pdot.p, N * sizeof(double), hipMemcpyDeviceToHost));
+  if (time) *time = e.time;
+  return DFN_OK;
+}
+
+API int dfn_get_stats(dfn_handle h, dfn_run_stats *s) {
+  if (!h || !s) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  *s = e.st;
+  s->time = e.time;
+  s->t_try = e.t_try;
+  std::vector<double> V(e.hp.N);
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  DFN_HIP(hipMemcpy(V.data(), e.Y.p, V.size() * sizeof(double), hipMemcpyDeviceToHost));
+  s->max_V = *std::max_element(V.begin(), V.end());
+  return DFN_OK;
+}
+
+API int dfn_kernel(dfn_handle h, const double *slip, double *sTh, double *sNh) {
+  if (!h || !slip || !sTh) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const size_t N = e.hp.N;
+  if (e.hp.mode == DFN_MODE_II && !sNh) return DFN_ERR_ARG;
+  double *dV = e.scratch3.as<double>();
+  DFN_HIP(hipMemcpyAsync(dV, slip, N * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  DFN_TRY(e.apply_kernel(dV, e.sT.as<double>(), e.sN.as<double>()));
+  DFN_HIP(hipMemcpyAsync(sTh, e.sT.p, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  if (e.hp.mode == DFN_MODE_II)
+    DFN_HIP(hipMemcpyAsync(sNh, e.sN.p, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  return DFN_OK;
+}
+
+API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const double *x, double *y) {
+  if (!h || !x || !y || op < 0 || op > 1) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  if (e.hp.kernel != DFN_KERNEL_HMATRIX || !e.active_ops()) return DFN_ERR_STATE;
+  const dfn::DevOps *d = e.active_ops();
+  if (op >= d->nops) return DFN_ERR_ARG;
+  const int N = e.hp.N;
+  // x is in H order: apply with the identity scatter by undoing the permutation on the host
+  const std::vector<int> &perm = e.hop[0].perm;
+  std::vector<double> xo(N), yT(N), yN(N);
+  for (int i = 0; i < N; i++) xo[perm[i]] = x[i];
+  double *dV = e.scratch3.as<double>();
+  double *dT = e.scratch3b.as<double>(), *dN = dT + N;
+  DFN_HIP(hipMemcpyAsync(dV, xo.data(), N * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  double *w = d->w.as<double>();
+  dfn::launch_gather(dV, d->perm.as<int>(), w, N, e.stream);
+  dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), e.stream);
+  dfn::launch_zreduce(d->zred.as<dfn::ZRed>(), d->nzr, d->zpart.as<double>(), w, e.stream);
+  dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), w,
+               d->ypart.as<double>(), e.stream);
+  dfn::launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
+                  d->nops, 1.0, dT, d->nops > 1 ? dN : dT, e.stream);
+  DFN_HIP(hipMemcpyAsync(yT.data(), op == 0 ? dT : dN, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  for (int i = 0; i < N; i++) {
+    const double hx = yT[perm[i]];
+    y[i] = alpha * hx + (beta == 0.0 ? 0.0 : beta * y[i]);
+  }
+  return DFN_OK;
+}
+
+API int dfn_friction_law(dfn_handle h, double x, const double *Yh, double *Ydh) {
+  if (!h || !Yh || !Ydh) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const size_t n3 = 3 * (size_t)e.hp.N;
+  DFN_HIP(hipMemcpyAsync(e.scratch3.p, Yh, n3 * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  DFN_TRY(e.friction_law_dev(x, e.scratch3.as<double>(), e.scratch3b.as<double>()));
+  DFN_HIP(hipMemcpy(Ydh, e.scratch3b.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  return DFN_OK;
+}
+
+API int dfn_compute_P(dfn_handle h, double dt, double *Pth, double *Pdh, int32_t *ier) {
+  if (!h) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const size_t N = e.hp.N;
+  DFN_TRY(e.cg_launch(&dt, 1, 0));
+  int fail = 0;
+  DFN_TRY(e.cg_collect(0, 1, &fail));
+  if (ier) *ier = fail;
+  if (!fail) {
+    if (Pth) DFN_HIP(hipMemcpy(Pth, e.Pt(0), N * sizeof(double), hipMemcpyDeviceToHost));
+    if (Pdh) DFN_HIP(hipMemcpy(Pdh, e.Pd(0), N * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    e.last_pd_slot = -1;  // the reference leaves P_dot untouched on failure
+  }
+  return DFN_OK;
+}
+
+API int dfn_rkqs(dfn_handle h, double *y, const double *dydx, double *x, double htry, double eps,
+                 const double *yscal, double *hdid, double *hnext) {
+  if (!h || !y || !dydx || !x || !yscal || !hdid || !hnext) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const size_t n3 = 3 * (size_t)e.hp.N;
+  // run on a copy of the handle's state slot: Y <- y
+  DFN_HIP(hipMemcpyAsync(e.Y.p, y, n3 * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  DFN_HIP(hipMemcpyAsync(e.k(0), dydx, n3 * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  DFN_HIP(hipMemcpyAsync(e.yscal.p, yscal, n3 * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  DFN_TRY(e.rkqs_dev(*x, htry, eps, hdid, hnext));
+  *x = *x + *hdid;
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  DFN_HIP(hipMemcpy(y, e.Y.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  return DFN_OK;
+}
+
+API int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats) {
+  if (!h || nsteps < 0) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  if (e.hp.kernel == DFN_KERNEL_HMATRIX && !e.active_ops()) return DFN_ERR_STATE;
+  int st = DFN_OK;
+  for (int64_t i = 0; i < nsteps; i++) {
+    if (t_final > 0 && e.time >= t_final) break;
+    st = e.step();
+    if (st != DFN_OK) break;
+  }
+  if (stats) {
+    int s2 = dfn_get_stats(h, stats);
+    if (st == DFN_OK) st = s2;
+  }
+  return st;
+}
+
+API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
+  if (!h || !ms || reps < 1) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const double *dV = e.Y.as<double>();
+  DFN_TRY(e.apply_kernel(dV, e.sT.as<double>(), e.sN.as<double>()));  // warm
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  DFN_HIP(hipEventRecord(e.ev0, e.stream));
+  for (int r = 0; r < reps; r++) DFN_TRY(e.apply_kernel(dV, e.sT.as<double>(), e.sN.as<double>()));
+  DFN_HIP(hipEventRecord(e.ev1, e.stream));
+  DFN_HIP(hipEventSynchronize(e.ev1));
+  float t = 0;
+  DFN_HIP(hipEventElapsedTime(&t, e.ev0, e.ev1));
+  ms[0] = (double)t / reps;
+  ms[1] = 0.0;
+  ms[2] = 0.0;
+  ms[3] = 0.0;
+  const dfn::DevOps *d = e.active_ops();
+  if (e.hp.kernel == DFN_KERNEL_HMATRIX && d) {
+    // the phase-2 panel kernel alone
+    double *w = d->w.as<double>();
+    DFN_HIP(hipEventRecord(e.ev0, e.stream));
+    for (int r = 0; r < reps; r++)
+      dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), w,
+                        d->ypart.as<double>(), e.stream);
+    DFN_HIP(hipEventRecord(e.ev1, e.stream));
+    DFN_HIP(hipEventSynchronize(e.ev1));
+    DFN_HIP(hipEventElapsedTime(&t, e.ev0, e.ev1));
+    ms[1] = (double)t / reps;
+    ms[2] = d->alg_bytes;
+    ms[3] = d->panel_bytes;
+  } else {
+    ms[1] = ms[0];
+    const double N = e.hp.N;
+    ms[2] = 8.0 * N * N + 16.0 * N;  // dense operator bytes (the direct kernel recomputes them)
+    ms[3] = ms[2];
+  }
+  return DFN_OK;
+}

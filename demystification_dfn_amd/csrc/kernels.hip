@@ -1,0 +1,602 @@
+// gfx950 kernels of the per-RK-stage hot path.
+//
+//   * stress-transfer H-matvec in two phases over device-resident block lists:
+//       phase 1  z_b = V_b^T x      one wave per (low-rank block, row chunk)
+//       phase 2  y_R = [F | U] w    one workgroup per leaf-row panel (row-major,
+//                                   w = [x_h | z] gathered through LDS)
+//       finalize y(perm) = scale * sum of panel partials (fixed order, no atomics)
+//   * directIII / directII on the fly (dense dislocation kernels)
+//   * rate-state right-hand side fused with the Cash-Karp stage combinations and
+//     the max-norm error reduction
+//   * compute_P: row-scaled CG, several time steps solved side by side
+//
+// Kernels whose results feed the ODE state are compiled without floating-point
+// contraction so that every +,-,*,/ rounds as in the reference's expressions.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dfn_device.hpp"
+
+namespace dfn {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// H-matvec
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restrict__ x, const int *__restrict__ perm,
+                                                          double *__restrict__ xh, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xh[i] = x[perm[i]];
+}
+
+// Phase 1: one wave per task; lanes stride the chunk's rows, each lane keeps the
+// k partial dot products in registers, then a wave reduction per column.
+template <typename TV>
+__global__ void __launch_bounds__(256) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
+                                                          const TV *__restrict__ vdata, const double *__restrict__ w,
+                                                          double *__restrict__ zpart) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (t >= ntasks) return;
+  const VTask task = tasks[t];
+  double acc[kMaxRank];
+#pragma unroll
+  for (int j = 0; j < kMaxRank; j++) acc[j] = 0.0;
+  const TV *V = vdata + task.voff + task.row0;
+  const double *x = w + task.xcol + task.row0;
+  for (int i = lane; i < task.len; i += 64) {
+    const double xi = x[i];
+#pragma unroll
+    for (int j = 0; j < kMaxRank; j++)
+      if (j < task.k) acc[j] += (double)V[i + (int64_t)j * task.ld] * xi;
+  }
+#pragma unroll
+  for (int j = 0; j < kMaxRank; j++) {
+    if (j < task.k) {
+      const double s = wave_sum(acc[j]);
+      if (lane == j) zpart[task.zp + j] = s * (double)task.scale;
+    }
+  }
+}
+
+// z of each low-rank block = sum of its chunk partials, in chunk order.
+__global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ zr, int nblocks,
+                                                      const double *__restrict__ zpart, double *__restrict__ w) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const ZRed r = zr[b];
+  for (int j = 0; j < r.k; j++) {
+    double s = 0.0;
+    for (int t = 0; t < r.ntask; t++) s += zpart[r.zp_first + t * r.k + j];
+    w[r.zout + j] = s;
+  }
+}
+
+// Phase 2: one workgroup per panel task.  w entries of the panel's columns are
+// gathered into LDS once; each wave then streams whole rows (row-major panel,
+// coalesced 512 B per wave load), two rows at a time, and reduces per row.
+template <typename TP>
+__global__ void __launch_bounds__(256) panel_kernel(const PTask *__restrict__ tasks, const TP *__restrict__ pdata,
+                                                    const int *__restrict__ widx, const double *__restrict__ w,
+                                                    double *__restrict__ ypart) {
+  __shared__ double wl[kPanelMaxCols];
+  const PTask task = tasks[blockIdx.x];
+  for (int j = threadIdx.x; j < task.L; j += 256) wl[j] = w[widx[task.widx + j]];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const TP *P = pdata + task.poff;
+  const int L = task.L;
+  int r = wave * 2;
+  for (; r + 1 < task.m; r += 8) {
+    const TP *row0 = P + (int64_t)r * L, *row1 = row0 + L;
+    double a0 = 0.0, a1 = 0.0;
+    for (int j = lane; j < L; j += 64) {
+      const double wj = wl[j];
+      a0 += (double)row0[j] * wj;
+      a1 += (double)row1[j] * wj;
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    if (lane == 0) {
+      ypart[task.yp + r] = a0 * (double)task.scale;
+      ypart[task.yp + r + 1] = a1 * (double)task.scale;
+    }
+  }
+  if (r < task.m) {
+    const TP *row0 = P + (int64_t)r * L;
+    double a0 = 0.0;
+    for (int j = lane; j < L; j += 64) a0 += (double)row0[j] * wl[j];
+    a0 = wave_sum(a0);
+    if (lane == 0) ypart[task.yp + r] = a0 * (double)task.scale;
+  }
+}
+
+// y(perm(i)) = scale * sum of the partials of H row i (one thread per row and operator).
+__global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ fin_first, const int *__restrict__ fin_idx,
+                                                       const int *__restrict__ perm, const double *__restrict__ ypart,
+                                                       int n, int nops, double scale, double *__restrict__ yT,
+                                                       double *__restrict__ yN) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * nops) return;
+  const int op = g / n, i = g - op * n;
+  double s = 0.0;
+  for (int q = fin_first[g]; q < fin_first[g + 1]; q++) s += ypart[fin_idx[q]];
+  double *y = op == 0 ? yT : yN;
+  y[perm[i]] = scale * s;
+}
+
+// ---------------------------------------------------------------------------
+// Direct kernels (src/kernel_calculation/directModeIII.f:7-74, directModeII.f90:7-107)
+// ---------------------------------------------------------------------------
+// c_l: weights of the node dislocations: l=0: -s_1, 1..N-1: -(s_{l+1}-s_l), N: +s_N
+__global__ void __launch_bounds__(256) dislocation_weights_kernel(const double *__restrict__ slip,
+                                                                  double *__restrict__ c, int n) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l > n) return;
+  if (l == 0) c[0] = -slip[0];
+  else if (l == n) c[n] = slip[n - 1];
+  else c[l] = -(slip[l] - slip[l - 1]);
+}
+
+__global__ void __launch_bounds__(256) direct3_kernel(const double *__restrict__ node, const double *__restrict__ c,
+                                                      int n, double scale, double *__restrict__ sT) {
+  __shared__ double sx[256], sy[256], sc[256];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double ex = 0, ey = 0, t1 = 0, t2 = 0;
+  if (k < n) {
+    const double x0 = node[2 * k], y0 = node[2 * k + 1], x1 = node[2 * k + 2], y1 = node[2 * k + 3];
+    double tx = x1 - x0, ty = y1 - y0;
+    const double nt = sqrt(tx * tx + ty * ty);
+    t1 = tx / nt;
+    t2 = ty / nt;
+    ex = x0 + 0.5 * (x1 - x0);
+    ey = y0 + 0.5 * (y1 - y0);
+  }
+  double s = 0.0;
+  for (int base = 0; base <= n; base += 256) {
+    const int l = base + threadIdx.x;
+    if (l <= n) { sx[threadIdx.x] = node[2 * l]; sy[threadIdx.x] = node[2 * l + 1]; sc[threadIdx.x] = c[l]; }
+    __syncthreads();
+    const int cnt = min(256, n + 1 - base);
+    for (int q = 0; q < cnt; q++) {
+      const double dx = ex - sx[q], dy = ey - sy[q];
+      const double r2 = dx * dx + dy * dy;
+      s += sc[q] * ((t1 * dx + t2 * dy) / r2);
+    }
+    __syncthreads();
+  }
+  if (k < n) sT[k] = s * scale;
+}
+
+__global__ void __launch_bounds__(256) direct2_kernel(const double *__restrict__ node, const double *__restrict__ elem,
+                                                      const double *__restrict__ nrm, const double *__restrict__ slip,
+                                                      int n, double scale, double *__restrict__ sT,
+                                                      double *__restrict__ sN) {
+  __shared__ double sx[257], sy[257], snx[256], sny[256], ss[256];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double ex = 0, ey = 0, n1 = 0, n2 = 0;
+  if (k < n) { ex = elem[2 * k]; ey = elem[2 * k + 1]; n1 = nrm[2 * k]; n2 = nrm[2 * k + 1]; }
+  const double c4 = 4.0 * n1 * n2, c2 = 2.0 * n1 * n2, dn = n2 * n2 - n1 * n1;
+  double aT = 0.0, aN = 0.0;
+  for (int base = 0; base < n; base += 256) {
+    const int l = base + threadIdx.x;
+    if (l < n) {
+      sx[threadIdx.x] = node[2 * l]; sy[threadIdx.x] = node[2 * l + 1];
+      snx[threadIdx.x] = nrm[2 * l]; sny[threadIdx.x] = nrm[2 * l + 1]; ss[threadIdx.x] = slip[l];
+    }
+    if (threadIdx.x == 0) {
+      const int lr = min(base + 256, n);
+      sx[256] = node[2 * lr]; sy[256] = node[2 * lr + 1];
+    }
+    __syncthreads();
+    const int cnt = min(256, n - base);
+    for (int q = 0; q < cnt; q++) {
+      const double xr = (q + 1 < cnt) ? sx[q + 1] : sx[256], yr = (q + 1 < cnt) ? sy[q + 1] : sy[256];
+      const double exl = ex - sx[q], eyl = ey - sy[q], exr = ex - xr, eyr = ey - yr;
+      const double rl = sqrt(exl * exl + eyl * eyl), rr = sqrt(exr * exr + eyr * eyr);
+      const double g1l = exl / rl, g2l = eyl / rl, g1r = exr / rr, g2r = eyr / rr;
+      const double cl = c4 * g1l * g2l + dn * (g2l * g2l - g1l * g1l);
+      const double cr = c4 * g1r * g2r + dn * (g2r * g2r - g1r * g1r);
+      const double dl = c2 * (g2l * g2l - g1l * g1l) - dn * 2.0 * g1l * g2l;
+      const double dr = c2 * (g2r * g2r - g1r * g1r) - dn * 2.0 * g1r * g2r;
+      const double m1 = snx[q], m2 = sny[q];
+      const double tT = m2 * (cl * g1l / rl) - m2 * (cr * g1r / rr) - m1 * (cl * g2l / rl) + m1 * (cr * g2r / rr);
+      const double tN = m1 * (g1l / rl - g2l / rl * dl) - m1 * (g1r / rr - g2r / rr * dr) +
+                        m2 * (g2l / rl + g1l / rl * dl) - m2 * (g2r / rr + g1r / rr * dr);
+      aT += ss[q] * tT;
+      aN += ss[q] * tN;
+    }
+    __syncthreads();
+  }
+  if (k < n) { sT[k] = scale * aT; sN[k] = scale * aN; }
+}
+
+// ---------------------------------------------------------------------------
+// Rate-state RHS fused with the Cash-Karp stage combinations
+// (src/friction/RateStateAgingModeIII.f90:66-113, ModeII.f90:169-212,
+//  src/ode_solver/ode_bs.f90:199-224)
+// ---------------------------------------------------------------------------
+#pragma clang fp contract(off)
+
+struct CK {
+  static constexpr double A2 = 0.2, A3 = 0.3, A4 = 0.6, A5 = 1.0, A6 = 0.875;
+  static constexpr double B21 = 0.2, B31 = 3.0 / 40.0, B32 = 9.0 / 40.0, B41 = 0.3, B42 = -0.9, B43 = 1.2;
+  static constexpr double B51 = -11.0 / 54.0, B52 = 2.5, B53 = -70.0 / 27.0, B54 = 35.0 / 27.0;
+  static constexpr double B61 = 1631.0 / 55296.0, B62 = 175.0 / 512.0, B63 = 575.0 / 13824.0;
+  static constexpr double B64 = 44275.0 / 110592.0, B65 = 253.0 / 4096.0;
+  static constexpr double C1 = 37.0 / 378.0, C3 = 250.0 / 621.0, C4 = 125.0 / 594.0, C6 = 512.0 / 1771.0;
+  static constexpr double DC1 = C1 - 2825.0 / 27648.0, DC3 = C3 - 18575.0 / 48384.0;
+  static constexpr double DC4 = C4 - 13525.0 / 55296.0, DC5 = -277.0 / 14336.0, DC6 = C6 - 0.25;
+};
+
+__device__ __forceinline__ void rate_state(const RhsParams &p, int k, double V, double psi, double sig, double sT,
+                                           double sN, double Pt, double Pd, double &dV, double &dpsi, double &dsig) {
+  dsig = p.mode3 ? p.nld[k] : sN + p.nld[k];
+  const double a = p.a[k], b = p.b[k], Dc = p.Dc[k], V0 = p.V0[k];
+  const double B1 = V / (2.0 * V0);
+  const double C1 = exp((p.f0[k] + b * psi) / a);
+  double D1 = 1.0 / (C1 * C1) + B1 * B1;
+  D1 = 1.0 / sqrt(D1);
+  const double th = V0 / Dc * exp(-psi) - fabs(V) / Dc;
+  dV = (p.sld[k] + sT + (dsig + Pd) * a * asinh(B1 * C1) + B1 * D1 * (sig + Pt) * b * th) /
+       (p.eta_rad - D1 * (sig + Pt) * a / (2.0 * V0));
+  dpsi = V0 / Dc * exp(-psi) - fabs(V) / Dc;
+  if (p.fake[k]) { dV = 0.0; dpsi = 0.0; dsig = 0.0; }
+}
+
+// Stage s of rkck: k_s = f(ytemp_s); then the next stage input (or, at s = 6, the
+// 5th-order solution, the embedded error and its scaled max norm).  At s = 1 the
+// kernel evaluates k1 at x = t_try and forms yscal (src/solver.f90:101-102).
+template <int S>
+__global__ void __launch_bounds__(256) stage_kernel(StageArgs A) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = A.n;
+  double emax = 0.0;
+  int tensile = 0;
+  if (k < n) {
+    const double *yt = A.ytemp;
+    double d[3];
+    rate_state(A.p, k, yt[k], yt[n + k], yt[2 * n + k], A.sT[k], A.sN ? A.sN[k] : 0.0, A.Pt[k], A.Pd[k], d[0], d[1],
+               d[2]);
+    const double h = A.h;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const int i = c * n + k;
+      const double y = A.y[i];
+      if (S == 0) {
+        A.k[0][i] = d[c];
+      } else if (S == 1) {
+        A.k[0][i] = d[c];
+        A.yscal[i] = fabs(y) + fabs(d[c] * A.t_try) + A.tol;
+        A.ynext[i] = y + CK::B21 * h * d[c];
+      } else if (S == 2) {
+        A.k[1][i] = d[c];
+        A.ynext[i] = y + h * (CK::B31 * A.k[0][i] + CK::B32 * d[c]);
+      } else if (S == 3) {
+        A.k[2][i] = d[c];
+        A.ynext[i] = y + h * (CK::B41 * A.k[0][i] + CK::B42 * A.k[1][i] + CK::B43 * d[c]);
+      } else if (S == 4) {
+        A.k[3][i] = d[c];
+        A.ynext[i] = y + h * (CK::B51 * A.k[0][i] + CK::B52 * A.k[1][i] + CK::B53 * A.k[2][i] + CK::B54 * d[c]);
+      } else if (S == 5) {
+        A.k[4][i] = d[c];
+        A.ynext[i] = y + h * (CK::B61 * A.k[0][i] + CK::B62 * A.k[1][i] + CK::B63 * A.k[2][i] +
+                              CK::B64 * A.k[3][i] + CK::B65 * d[c]);
+      } else {
+        const double k1 = A.k[0][i], k3 = A.k[2][i], k4 = A.k[3][i], k5 = A.k[4][i];
+        const double yo = y + h * (CK::C1 * k1 + CK::C3 * k3 + CK::C4 * k4 + CK::C6 * d[c]);
+        const double ye = h * (CK::DC1 * k1 + CK::DC3 * k3 + CK::DC4 * k4 + CK::DC5 * k5 + CK::DC6 * d[c]);
+        A.ynext[i] = yo;
+        double e = fabs(ye / A.yscal[i]);
+        if (!(e == e)) e = INFINITY;
+        emax = fmax(emax, e);
+        if (c == 2 && yo > 0) tensile = 1;
+      }
+    }
+  }
+  if (S == 6) {
+    __shared__ double red[4];
+    __shared__ int tred[4];
+    double m = wave_max(emax);
+    int tf = __any(tensile) ? 1 : 0;
+    if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = m; tred[threadIdx.x >> 6] = tf; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double bm = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+      // non-negative doubles order like their bit patterns
+      atomicMax(reinterpret_cast<unsigned long long *>(&A.status->errmax_bits), __double_as_longlong(bm));
+      if (tred[0] | tred[1] | tred[2] | tred[3]) atomicOr(&A.status->tensile, 1);
+    }
+  }
+}
+
+// ytemp2 = y + B21*h*k1 for a retried step size (ode_bs.f90:213)
+__global__ void __launch_bounds__(256) restart_stage2_kernel(const double *__restrict__ y, const double *__restrict__ k1,
+                                                             double h, int n3, double *__restrict__ yt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n3) yt[i] = y[i] + CK::B21 * h * k1[i];
+}
+
+// ---------------------------------------------------------------------------
+// compute_P (src/fluid_diffusion.f90:24-405): one workgroup per time step dt.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  v = wave_sum(v);
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < nw; q++) s += red[q];
+  return s;
+}
+
+__device__ __forceinline__ double block_max(double v, double *red) {
+  v = wave_max(v);
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < nw; q++) s = fmax(s, red[q]);
+  return s;
+}
+
+__device__ __forceinline__ double hmean(double pa, double pb) { return 2.0 * pa * pb / (pa + pb); }
+
+// M v for system s: rows (A1 v_{k-1} + A2 v_k + A3 v_{k+1}) / diag_k plus the crossing terms.
+__device__ void cg_matvec(const CgArgs &A, const CgSys &S, double dt, const double *v, double *out) {
+  const int n = A.n;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    double r;
+    if (k == 0) r = S.A2[0] * v[0] + S.A3[0] * v[1];
+    else if (k == n - 1) r = S.A1[k] * v[k - 1] + S.A2[k] * v[k];
+    else r = S.A1[k] * v[k - 1] + S.A2[k] * v[k] + S.A3[k] * v[k + 1];
+    out[k] = r / S.diag[k];
+  }
+  if (A.ncross > 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double *p = A.perm, *ds = A.ds;
+      const int kq = n > 2 ? n : 2;
+      const double dq = A.cfac * (ds[kq - 2] * ds[kq - 2]);
+      for (int q = 0; q < A.ncross; q++) {
+        const int64_t n1 = A.cross[6 * q + 2], n2 = A.cross[6 * q + 3];  // 1-based node ids
+        // element e (1-based) -> index e-1
+        double kl, kr, a1, a2, a3;
+        kl = hmean(p[n1 - 2], p[n2 - 2]); kr = hmean(p[n1 - 2], p[n2 - 1]);
+        a1 = -(kl * dt / dq); a3 = -(kr * dt / dq); a2 = kl * dt / dq + kr * dt / dq;
+        out[n1 - 2] = out[n1 - 2] + (a1 * v[n2 - 2] + a2 * v[n1 - 2] + a3 * v[n2 - 1]) / S.diag[n1 - 2];
+        kl = hmean(p[n1 - 1], p[n2 - 2]); kr = hmean(p[n1 - 1], p[n2 - 1]);
+        a1 = -(kl * dt / dq); a3 = -(kr * dt / dq); a2 = kl * dt / dq + kr * dt / dq;
+        out[n1 - 1] = out[n1 - 1] + (a1 * v[n2 - 2] + a2 * v[n1 - 1] + a3 * v[n2 - 1]) / (double)n1;
+        kl = hmean(p[n1 - 2], p[n2 - 2]); kr = hmean(p[n1 - 1], p[n2 - 2]);
+        a1 = -(kl * dt / dq); a3 = -(kr * dt / dq); a2 = kl * dt / dq + kr * dt / dq;
+        out[n2 - 2] = out[n2 - 2] + (a1 * v[n1 - 2] + a2 * v[n2 - 2] + a3 * v[n1 - 1]) / S.diag[n2 - 2];
+        kl = hmean(p[n1 - 2], p[n2 - 1]); kr = hmean(p[n1 - 1], p[n2 - 1]);
+        a1 = -(kl * dt / dq); a3 = -(kr * dt / dq); a2 = kl * dt / dq + kr * dt / dq;
+        out[n2 - 1] = out[n2 - 1] + (a1 * v[n1 - 2] + a2 * v[n2 - 1] + a3 * v[n1 - 1]) / S.diag[n2 - 1];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) cg_kernel(CgArgs A) {
+  __shared__ double red[16];
+  const CgSys S = A.sys[blockIdx.x];
+  const double dt = S.dt;
+  const int n = A.n;
+  const double c = A.cfac;
+  const double *kx = A.perm_x;  // 0-based: kx[k] is permeability_x(k+1)
+  const double *ds = A.ds;
+  // diag (compute_diag :308-405) with the permeability_x in effect before this call
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    double d;
+    if (k == 0) d = 1.0 + A.perm_x_diag[1] * dt / (c * (ds[0] * ds[0]));
+    else if (k == n - 1) d = 1.0 + A.perm_x_diag[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1]));
+    else d = 1.0 + (A.perm_x_diag[k] + A.perm_x_diag[k + 1]) * dt / (c * (ds[k] * ds[k]));
+    S.diag[k] = d;
+    // stencil coefficients of compute_MatrixVector (:175-204) with the current permeability_x
+    if (k == 0) {
+      S.A2[0] = 1.0 + kx[1] * dt / (c * (ds[0] * ds[0]));
+      S.A3[0] = -(kx[1] * dt / (c * (ds[0] * ds[0])));
+    } else if (k == n - 1) {
+      S.A1[k] = -(kx[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1])));
+      S.A2[k] = 1.0 + kx[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1]));
+    } else {
+      S.A1[k] = -(kx[k] * dt / (c * (ds[k - 1] * ds[k - 1])));
+      S.A2[k] = 1.0 + (kx[k] + kx[k + 1]) * dt / (c * (ds[k] * ds[k]));
+      S.A3[k] = -(kx[k + 1] * dt / (c * (ds[k + 1] * ds[k + 1])));
+    }
+  }
+  __syncthreads();
+  if (A.ncross > 0 && threadIdx.x == 0) {
+    const double *p = A.perm_diag;
+    const double d1 = c * (ds[0] * ds[0]);
+    for (int q = 0; q < A.ncross; q++) {
+      const int64_t n1 = A.cross[6 * q + 2], n2 = A.cross[6 * q + 3];
+      const int64_t e[4] = {n1 - 2, n1 - 1, n2 - 2, n2 - 1};
+      const double kls[4] = {hmean(p[n1 - 2], p[n2 - 2]), hmean(p[n1 - 1], p[n2 - 2]), hmean(p[n1 - 2], p[n2 - 2]),
+                             hmean(p[n1 - 2], p[n2 - 1])};
+      const double krs[4] = {hmean(p[n1 - 2], p[n2 - 1]), hmean(p[n1 - 1], p[n2 - 1]), hmean(p[n1 - 1], p[n2 - 2]),
+                             hmean(p[n1 - 1], p[n2 - 1])};
+      for (int z = 0; z < 4; z++) S.diag[e[z]] = S.diag[e[z]] + kls[z] * dt / d1 + krs[z] * dt / d1;
+    }
+  }
+  __syncthreads();
+  // right-hand side b1 = P / diag (+ sources, :52-64) and the initial residual
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    S.b[k] = A.P[k] / S.diag[k];
+    S.x[k] = A.P[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 0; q < A.nsrc; q++) {
+      const double tt = A.time + dt;
+      if (tt >= A.src_tb[q] && tt < A.src_te[q]) {
+        const int64_t e = A.src_elem[q] - 1;
+        S.b[e] = S.b[e] + (dt * A.src_Q[q] / (A.por * A.beta) / ds[e]) / S.diag[e];
+      }
+    }
+  __syncthreads();
+  cg_matvec(A, S, dt, S.x, S.Ap);
+  double emax = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const double r = S.b[k] - S.Ap[k];
+    S.r[k] = r;
+    S.p[k] = r;
+    emax = fmax(emax, fabs(r / 1.0));
+  }
+  double errmax = block_max(emax, red) / A.tol;
+  double rr = 0.0;
+  {
+    double part = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) part += S.r[k] * S.r[k];
+    rr = block_sum(part, red);
+  }
+  int it = 0, ier = 0;
+  while (errmax >= 1.0) {
+    it++;
+    cg_matvec(A, S, dt, S.p, S.Ap);
+    double part = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) part += S.p[k] * S.Ap[k];
+    const double pAp = block_sum(part, red);
+    const double alpha = rr / pAp;
+    double p1 = 0.0, em = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      S.x[k] = S.x[k] + alpha * S.p[k];
+      const double r1 = S.r[k] - alpha * S.Ap[k];
+      S.r[k] = r1;
+      p1 += r1 * r1;
+      em = fmax(em, fabs(r1 / 1.0));
+    }
+    const double r1r1 = block_sum(p1, red);
+    const double beta = r1r1 / rr;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) S.p[k] = S.r[k] + beta * S.p[k];
+    rr = r1r1;
+    if (it > kCgMaxIter) { ier = 1; break; }
+    errmax = block_max(em, red) / A.tol;
+  }
+  double xmax = 0.0;
+  if (!ier) {
+    const bool small = dt < (double)1e-9f;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const double xk = S.x[k];
+      S.Pt[k] = xk;
+      S.Pd[k] = small ? 0.0 : (xk - A.P[k]) / dt;
+      xmax = fmax(xmax, fabs(xk));
+    }
+  }
+  xmax = block_max(xmax, red);
+  if (threadIdx.x == 0) {
+    if (!ier && xmax > 1.7976931348623157e308) ier = 1;
+    S.result[0] = ier;
+    S.result[1] = it;
+  }
+}
+
+// permeability update of compute_P (:67-75); writes perm and perm_x
+__global__ void __launch_bounds__(256) perm_update_kernel(PermArgs A) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.n) return;
+  if (A.coupling == 1)
+    A.perm[k] = (A.perm_star[k] - A.kmin[k]) * exp(-fabs(A.sigmaN[k] + A.P[k]) / fabs(A.Snk[k])) + A.kmin[k];
+}
+
+__global__ void __launch_bounds__(256) perm_x_kernel(const double *__restrict__ perm, double *__restrict__ perm_x, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 1 && k < n) perm_x[k] = 2.0 * perm[k - 1] * perm[k] / (perm[k - 1] + perm[k]);
+}
+
+#pragma clang fp contract(on)
+
+// ---------------------------------------------------------------------------
+// Launch helpers (called from the host engine)
+// ---------------------------------------------------------------------------
+static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s) {
+  hipLaunchKernelGGL(gather_perm_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, x, perm, xh, n);
+}
+
+void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, const double *w, double *zpart, hipStream_t s) {
+  if (nt <= 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(lowrank_vtx_kernel<float>, dim3(nblk(nt, 4)), dim3(256), 0, s, t, nt, (const float *)vdata, w,
+                       zpart);
+  else
+    hipLaunchKernelGGL(lowrank_vtx_kernel<double>, dim3(nblk(nt, 4)), dim3(256), 0, s, t, nt, (const double *)vdata,
+                       w, zpart);
+}
+
+void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(zreduce_kernel, dim3(nblk(nb, 256)), dim3(256), 0, s, zr, nb, zpart, w);
+}
+
+void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const double *w,
+                  double *ypart, hipStream_t s) {
+  if (nt <= 0) return;
+  if (fp32)
+    hipLaunchKernelGGL(panel_kernel<float>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, w, ypart);
+  else
+    hipLaunchKernelGGL(panel_kernel<double>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
+}
+
+void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
+                     double scale, double *yT, double *yN, hipStream_t s) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(nblk((int64_t)n * nops, 256)), dim3(256), 0, s, fin_first, fin_idx, perm,
+                     ypart, n, nops, scale, yT, yN);
+}
+
+void launch_direct3(const double *node, const double *slip, double *cbuf, int n, double scale, double *sT,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(dislocation_weights_kernel, dim3(nblk(n + 1, 256)), dim3(256), 0, s, slip, cbuf, n);
+  hipLaunchKernelGGL(direct3_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, node, cbuf, n, scale, sT);
+}
+
+void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, int n, double scale,
+                    double *sT, double *sN, hipStream_t s) {
+  hipLaunchKernelGGL(direct2_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, node, elem, nrm, slip, n, scale, sT, sN);
+}
+
+void launch_stage(int stage, const StageArgs &a, hipStream_t s) {
+  const dim3 g(nblk(a.n, 256)), b(256);
+  switch (stage) {
+    case 0: hipLaunchKernelGGL(stage_kernel<0>, g, b, 0, s, a); break;
+    case 1: hipLaunchKernelGGL(stage_kernel<1>, g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(stage_kernel<2>, g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(stage_kernel<3>, g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(stage_kernel<4>, g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL(stage_kernel<5>, g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL(stage_kernel<6>, g, b, 0, s, a); break;
+  }
+}
+
+void launch_restart_stage2(const double *y, const double *k1, double h, int n3, double *yt, hipStream_t s) {
+  hipLaunchKernelGGL(restart_stage2_kernel, dim3(nblk(n3, 256)), dim3(256), 0, s, y, k1, h, n3, yt);
+}
+
+void launch_cg(const CgArgs &a, int nsys, hipStream_t s) {
+  hipLaunchKernelGGL(cg_kernel, dim3(nsys), dim3(1024), 0, s, a);
+}
+
+void launch_perm_update(const PermArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(perm_update_kernel, dim3(nblk(a.n, 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(perm_x_kernel, dim3(nblk(a.n, 256)), dim3(256), 0, s, a.perm, a.perm_x, a.n);
+}
+
+}  // namespace dfn

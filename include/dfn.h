@@ -166,7 +166,9 @@ int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats
 /* Time `reps` applications of the stress-transfer kernel on device-resident input
  * (the handle's current V) with HIP events on the handle's stream.  ms[0] = mean ms
  * per application of the whole kernel (all operators), ms[1] = mean ms of the
- * dominant device kernel alone, ms[2] = algorithmic bytes per application. */
+ * dominant device kernel alone (H-matrix: the phase-2 panel kernel), ms[2] =
+ * algorithmic bytes per application, ms[3] = algorithmic bytes of the dominant
+ * kernel per launch.  ms must hold 4 doubles. */
 int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms);
 /* Device/library description string (NUL-terminated, at most len bytes). */
 int dfn_device_info(char *buf, int32_t len);

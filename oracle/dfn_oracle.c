@@ -1561,3 +1561,11 @@ API int dfno_hmat_tree(dfn_handle hh, int32_t op, int32_t *mat_corr, int32_t *nb
     }
   return DFN_OK;
 }
+
+/* Loop state of the solver (t_try of src/solver.f90:31), for continuing a run
+ * from a state produced elsewhere. */
+API int dfno_set_t_try(dfn_handle hh, double t_try) {
+  if (!hh) return DFN_ERR_ARG;
+  ((ohandle *)hh)->o.t_try = t_try;
+  return DFN_OK;
+}

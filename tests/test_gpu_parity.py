@@ -1,0 +1,156 @@
+"""Parity of the gfx950 library (through its C ABI) with the CPU oracle on the same
+seeded inputs.
+
+Bars: stress-transfer matvec relative error <= 1e-12 in
+plain FP64; slip rate / log state within 1e-8 relative over interseismic steps.
+compute_P is a CG iteration whose sums are reduced in a different order on the
+device, so pressures are compared to the CG stopping tolerance (max residual of
+the row-scaled system below tol_solver Pa)."""
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from demystification_dfn_amd import _abi, problems
+from oracle.oracle import OracleSimulation
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(problem):
+    from demystification_dfn_amd.device import DeviceSimulation
+    print(f"[gpu] {problem.name} N={problem.N} mode={problem.fracture_mode} kernel={problem.static_kernel}",
+          flush=True)
+    sys.stdout.flush()
+    return DeviceSimulation(problem)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def close_elementwise(a, b, rtol, floor):
+    return np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), floor))
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network])
+def test_direct_kernel_and_rhs(make):
+    p = make()
+    d, o = dev(p), OracleSimulation(p)
+    V = np.random.default_rng(0).random(p.N) * 1e-9
+    for a, b in zip(d.kernel(V), o.kernel(V)):
+        if np.linalg.norm(b) > 0:
+            assert rel(a, b) <= 1e-12
+    Y0 = p.initial_Y()
+    for x in (1.0, 1000.0, 20000.0):
+        yd, yo = d.friction_law(x, Y0), o.friction_law(x, Y0)
+        assert close_elementwise(yd, yo, 1e-9, 1e-300 + np.abs(yo).max() * 1e-12), x
+
+
+@pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network])
+def test_compute_P(make):
+    p = make()
+    d, o = dev(p), OracleSimulation(p)
+    for dt in (10.0, 5000.0, 86400.0):
+        a, ad, ea = d.compute_P(dt)
+        b, bd, eb = o.compute_P(dt)
+        assert ea == eb, dt
+        if ea == 0:
+            scale = max(np.abs(b).max(), 1.0)
+            assert np.max(np.abs(a - b)) <= 1e-9 * scale + 10 * p.tol_solver, dt
+            assert np.max(np.abs(ad - bd)) <= (1e-9 * scale + 10 * p.tol_solver) / dt, dt
+
+
+@pytest.mark.parametrize("make,nsteps", [(cases.single_fault_small, 20), (cases.crossing_network, 30)])
+def test_solver_trajectory(make, nsteps):
+    p = make()
+    d, o = dev(p), OracleSimulation(p)
+    sd, so = d.solve(nsteps), o.solve(nsteps)
+    Yd, Pd, _, td = d.get_state()
+    Yo, Po, _, to = o.get_state()
+    assert sd["accepted_steps"] == so["accepted_steps"] == nsteps
+    assert sd["rejected_attempts"] == so["rejected_attempts"]
+    assert abs(td - to) <= 1e-8 * to
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    assert close_elementwise(Yd[:N][real], Yo[:N][real], 1e-8, 1e-300)          # slip rate
+    assert close_elementwise(Yd[N:2 * N][real], Yo[N:2 * N][real], 1e-8, 1e-300)  # log state
+    assert close_elementwise(Yd[2 * N:][real], Yo[2 * N:][real], 1e-8, 1e-300)   # normal stress
+
+
+def test_rkqs_single_call():
+    p = cases.single_fault_small()
+    d, o = dev(p), OracleSimulation(p)
+    Y0 = p.initial_Y()
+    yd = o.friction_law(200.0, Y0)
+    yscal = np.abs(Y0) + np.abs(yd * 200.0) + p.tol_solver
+    y1, x1, h1, n1 = d.rkqs(Y0, yd, 0.0, 200.0, p.tol_solver, yscal)
+    y2, x2, h2, n2 = o.rkqs(Y0, yd, 0.0, 200.0, p.tol_solver, yscal)
+    assert h1 == pytest.approx(h2, rel=1e-12) and n1 == pytest.approx(n2, rel=1e-9)
+    assert close_elementwise(y1, y2, 1e-10, 1e-300)
+
+
+# ---------------------------------------------------------------------------
+def _hnet(mode, n_faults=14, seed=5):
+    return problems.network_small(n_faults=n_faults, seed=seed, mode=mode, eta=2.0, eps_aca=1e-6, nleaf=16,
+                                  static_kernel=_abi.KERNEL_HMATRIX)
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_hmatvec_on_identical_block_lists(mode):
+    """Device H-matvec vs the oracle applying the very same block lists."""
+    p = _hnet(mode)
+    d, o = dev(p), OracleSimulation(p)
+    d.initiate_hmat()
+    ops = (0, 1) if mode == 2 else (0,)
+    for op in ops:
+        o.hmat_import(op, d.hmat_export(op))
+    rng = np.random.default_rng(1)
+    for trial in range(3):
+        V = 10.0 ** rng.uniform(-12, 0, p.N)
+        (aT, aN), (bT, bN) = d.kernel(V), o.kernel(V)
+        assert rel(aT, bT) <= 1e-12
+        if mode == 2:
+            assert rel(aN, bN) <= 1e-12
+    x = rng.standard_normal(p.N)
+    for op in ops:
+        assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_device_built_operator_matches_oracle_construction(mode):
+    """Trees, ranks and factors of the library's construction vs the oracle's
+    restatement of src/hmatrix (cluster tree, block tree, full-pivot ACA,
+    QR+SVD recompression)."""
+    p = _hnet(mode)
+    d, o = dev(p), OracleSimulation(p)
+    d.initiate_hmat()
+    o.initiate_hmat()
+    for op in ((0, 1) if mode == 2 else (0,)):
+        a, b = d.hmat_export(op), o.hmat_export(op)
+        for key in ("perm", "dense_row0", "dense_col0", "dense_m", "dense_n", "lr_row0", "lr_col0", "lr_m", "lr_n",
+                    "lr_k"):
+            np.testing.assert_array_equal(a[key], b[key], err_msg=key)
+        np.testing.assert_array_equal(a["dense_data"], b["dense_data"])
+        x = np.random.default_rng(op).standard_normal(p.N)
+        assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
+
+
+def test_hmatrix_solver_trajectory_mode2():
+    p = _hnet(2)
+    d, o = dev(p), OracleSimulation(p)
+    d.initiate_hmat()
+    for op in (0, 1):
+        o.hmat_import(op, d.hmat_export(op))
+    sd, so = d.solve(25), o.solve(25)
+    Yd = d.get_state()[0]
+    Yo = o.get_state()[0]
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    assert sd["rejected_attempts"] == so["rejected_attempts"]
+    assert close_elementwise(Yd[:N][real], Yo[:N][real], 1e-8, 1e-300)
+    assert close_elementwise(Yd[N:2 * N][real], Yo[N:2 * N][real], 1e-8, 1e-300)
