@@ -52,15 +52,14 @@ def make_workload(name, seed):
 def cpu_baseline(sim, p, seconds, threads):
     """The oracle (plain-C restatement) continuing from the device's state on the
     same operator block lists, timed for ~`seconds` of wall time."""
-    from oracle.oracle import OracleSimulation, _lib
+    from oracle.oracle import OracleSimulation
     o = OracleSimulation(p, threads=threads)
     if p.static_kernel == 1:
         for op in ((0, 1) if p.fracture_mode == 2 else (0,)):
             o.hmat_import(op, sim.hmat_export(op))
     Y, P, _, t = sim.get_state()
     o.set_state(Y, P, t)
-    _lib.dfno_set_t_try.argtypes = [type(o._h), __import__("ctypes").c_double]
-    _lib.dfno_set_t_try(o._h, float(sim.stats()["t_try"]))
+    o.set_t_try(sim.stats()["t_try"])
     t_start = o.stats()["time"]
     steps = 0
     w0 = time.perf_counter()

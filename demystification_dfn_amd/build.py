@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(CSRC, "libdfn.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
+FORT = os.path.join(HERE, "fortran")
 CXX = os.environ.get("CXX_HOST", "g++")
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
@@ -40,7 +42,20 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_fortran()
     return LIB
+
+
+def build_fortran():
+    """The ISO_C_BINDING layer (fortran/dfn_hip.f90) and the Fortran host driver
+    that calls the library through it (fortran/drive_solver.f90)."""
+    bld = os.path.join(FORT, "build")
+    os.makedirs(bld, exist_ok=True)
+    mod_o = os.path.join(bld, "dfn_hip.o")
+    subprocess.run([FLANG, "-O2", "-J", bld, "-c", os.path.join(FORT, "dfn_hip.f90"), "-o", mod_o], check=True)
+    subprocess.run([FLANG, "-O2", "-J", bld, "-I", bld, os.path.join(FORT, "drive_solver.f90"), mod_o,
+                    "-L" + CSRC, "-ldfn", "-Wl,-rpath,$ORIGIN/../../csrc", "-o", os.path.join(bld, "drive_solver")],
+                   check=True)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,75 @@
+! Fortran host driving the device library through the dfn_hip ISO_C_BINDING layer
+! with the reference's own call sequence of src/solver.f90:59-198: compute_P at
+! t_try with step halving, the friction law at t_try for the error scale, rkqs,
+! the 1-day cap and the pressure commit.  Reads a problem written by
+! demystification_dfn_amd.fortran_io.write_problem, writes time, Y and P.
+program drive_solver
+use iso_c_binding
+use dfn_hip
+implicit none
+character(len=512) :: fin, fout
+integer(c_int64_t) :: hdr(6)
+integer(c_int64_t), allocatable :: fnel(:), sfault(:), sidx(:)
+real(c_double), allocatable :: node(:,:), arr(:,:), sQ(:), stb(:), ste(:)
+real(c_double) :: scal(14)
+real(c_double), allocatable :: Y(:), Y_dot(:), yscal(:), P(:), P_dot(:), P_temp(:)
+real(c_double) :: time, t_try, t_did, t_next, time_temp
+integer :: N, nf, mode, kernel, nsrc, ier, it, f, s, u
+integer(c_int64_t) :: nsteps
+integer(c_int) :: st
+
+call get_command_argument(1, fin)
+call get_command_argument(2, fout)
+open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+read(u) hdr
+N = int(hdr(1)); nf = int(hdr(2)); mode = int(hdr(3)); kernel = int(hdr(4)); nsrc = int(hdr(5)); nsteps = hdr(6)
+allocate(fnel(nf), node(2, N+1), arr(N, 17), sfault(max(nsrc,1)), sidx(max(nsrc,1)))
+allocate(sQ(max(nsrc,1)), stb(max(nsrc,1)), ste(max(nsrc,1)))
+read(u) fnel
+read(u) node
+read(u) arr
+read(u) scal
+if (nsrc > 0) read(u) sfault(1:nsrc), sidx(1:nsrc), sQ(1:nsrc), stb(1:nsrc), ste(1:nsrc)
+close(u)
+! arr columns: a b Dc f0 V0 nld sld V theta sigmaN P permeability kmin kmax Snk Lk Tk
+! scal: mu cs cp S11d S12d S22d S31d S32d porosity fluid_comp rock_comp viscosity tol eta_adm
+call dfn_hip_setup(mode, kernel, nf, N, fnel, node, arr(:,1), arr(:,2), arr(:,3), arr(:,4), arr(:,5), arr(:,6), &
+                   arr(:,7), arr(:,8), arr(:,9), arr(:,10), arr(:,11), arr(:,12), arr(:,13), arr(:,14), arr(:,15), &
+                   arr(:,16), arr(:,17), scal(1), scal(2), scal(3), scal(4:8), scal(9), scal(10), scal(11), scal(12), &
+                   scal(13), 0, nsrc, sfault, sidx, sQ, stb, ste, scal(14), 1.d-6, 64, ier)
+if (ier /= 0) then
+  print *, 'dfn_hip_setup failed', ier
+  stop 1
+end if
+allocate(Y(3*N), Y_dot(3*N), yscal(3*N), P(N), P_dot(N), P_temp(N))
+st = dfn_get_state(dfn_hip_handle, Y, P, P_dot, time)
+t_try = 100._c_double
+do it = 1, int(nsteps)
+  if (t_try > 10._c_double) then
+    do s = 1, nsrc
+      if ((time < stb(s)) .and. (time + t_try > stb(s))) t_try = 10._c_double
+    end do
+  end if
+  call compute_P_hip(t_try, P_temp, P_dot, ier)
+  do while (ier == 1)
+    t_try = t_try/2._c_double
+    call compute_P_hip(t_try, P_temp, P_dot, ier)
+  end do
+  time_temp = 0._c_double
+  call ode_RateStateAgingHIP(t_try, 3*N, Y, Y_dot)
+  yscal = abs(Y) + abs(Y_dot*t_try) + scal(13)
+  call rkqs_hip(Y, Y_dot, 3*N, time_temp, t_try, scal(13), yscal, t_did, t_next, ode_RateStateAgingHIP)
+  t_try = t_next
+  if (t_try > 24*60*60) t_try = 24*60*60
+  call compute_P_hip(t_did, P_temp, P_dot, ier)
+  if (ier == 0) P = P_temp
+  time = time + t_did
+  st = dfn_set_state(dfn_hip_handle, Y, P, time)
+  if (any(Y(2*N+1:3*N) > 0)) exit
+end do
+open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+write(u) time, Y, P
+close(u)
+call dfn_hip_close()
+f = 0
+end program

@@ -39,6 +39,12 @@ class OracleSimulation(Simulation):
     def set_threads(self, n):
         _lib.dfno_handle_set_threads(self._h, int(n))
 
+    def set_t_try(self, t_try):
+        """Trial step of the solver loop (t_try, src/solver.f90:31)."""
+        _lib.dfno_set_t_try.argtypes = [C.c_void_p, C.c_double]
+        _lib.dfno_set_t_try.restype = C.c_int
+        assert _lib.dfno_set_t_try(self._h, float(t_try)) == 0
+
 
 # --------------------------------------------------------------------------
 # Introspection of the oracle and the reference build
