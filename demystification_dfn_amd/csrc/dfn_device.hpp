@@ -19,9 +19,9 @@ struct VTask {
   int32_t len;    // rows in this chunk
   int32_t k;      // rank
   int32_t xcol;   // H index of the block's first column
-  int32_t zp;     // output offset in zpart
+  int32_t zp;     // output offset in zpart (blocks split in several chunks)
   float scale;
-  int32_t pad;
+  int32_t wout;   // >= 0: single-chunk block, z written straight to w[wout + j]
 };
 
 // z of one low-rank block: w[zout + j] = sum_t zpart[zp_first + t*k + j]
@@ -90,6 +90,16 @@ struct CgArgs {
   const int64_t *cross;
 };
 
+// Crossing rows of the fluid-diffusion stencil: entries (crossing, slot) of each
+// element, in crossing order (slot 0: element node1-1, 1: node1, 2: node2-1, 3: node2).
+struct CrossRows {
+  const int *start;  // [n+1]
+  const int *ent;    // [2*entries] (crossing index, slot)
+};
+
+constexpr int kCgEPT = 8;                   // elements per thread in the cooperative CG
+constexpr int kCgSpan = kCgEPT * 256;       // elements per workgroup
+
 struct PermArgs {
   int n, coupling;
   double *perm, *perm_x;
@@ -98,7 +108,7 @@ struct PermArgs {
 
 // launchers (kernels.hip)
 void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s);
-void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, const double *w, double *zpart, hipStream_t s);
+void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const double *w,
                   double *ypart, hipStream_t s);
@@ -111,6 +121,8 @@ void launch_direct2(const double *node, const double *elem, const double *nrm, c
 void launch_stage(int stage, const StageArgs &a, hipStream_t s);
 void launch_restart_stage2(const double *y, const double *k1, double h, int n3, double *yt, hipStream_t s);
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
+// cooperative multi-workgroup CG; returns false if the grid cannot be co-resident
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *partial, int *sysflag, hipStream_t s);
 void launch_perm_update(const PermArgs &a, hipStream_t s);
 
 }  // namespace dfn

@@ -107,7 +107,9 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
     for (size_t b = 0; b < H.l_k.size(); b++) {
       const int k = H.l_k[b], nb = H.l_n[b];
       if (k > kMaxRank) return DFN_ERR_UNSUPPORTED;
-      ZRed r{zp, 0, k, (int)(zbase[o] + zoff[o][b])};
+      const int wz = (int)(zbase[o] + zoff[o][b]);
+      const bool single = nb <= kVChunkRows;  // one chunk: z goes straight into w
+      ZRed r{zp, 0, k, wz};
       for (int r0 = 0; r0 < nb; r0 += kVChunkRows) {
         VTask t{};
         t.voff = vbase + H.l_voff[b];
@@ -116,13 +118,14 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
         t.len = std::min(kVChunkRows, nb - r0);
         t.k = k;
         t.xcol = H.l_c0[b];
-        t.zp = zp;
+        t.zp = single ? -1 : zp;
         t.scale = 1.0f;
+        t.wout = single ? wz : -1;
         vt.push_back(t);
-        zp += k;
+        if (!single) zp += k;
         r.ntask++;
       }
-      zr.push_back(r);
+      if (!single) zr.push_back(r);
     }
     S_total += (int64_t)H.l_v.size() + (int64_t)H.l_u.size() + (int64_t)H.d_data.size();
   }
@@ -221,7 +224,8 @@ struct Engine {
   DMem perm_cur, perm_next, permx_cur, permx_next;
   // state and stage buffers
   DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot, scratch3, scratch3b;
-  DMem cg_work, cg_sys, cg_res;
+  DMem cg_work, cg_sys, cg_res, cg_partial, cg_flags, xrow_start, xrow_ent;
+  bool coop_ok = true;
   DMem status;
   StatusWord *h_status = nullptr;  // pinned
   int *h_cgres = nullptr;          // pinned [kSlots*2]
@@ -304,6 +308,27 @@ struct Engine {
     DFN_TRY(cg_res.alloc(sizeof(int) * kSlots * 2));
     DFN_TRY(cg_sys.alloc(sizeof(CgSys) * kSlots));
     DFN_TRY(status.alloc(sizeof(StatusWord)));
+    const int wps = (N + kCgSpan - 1) / kCgSpan;
+    DFN_TRY(cg_partial.alloc(sizeof(double) * 4 * (size_t)kSlots * wps));
+    DFN_TRY(cg_flags.alloc(sizeof(int) * kSlots));
+    // crossing rows of the stencil, per element in crossing order
+    {
+      std::vector<std::vector<std::pair<int, int>>> rows(N);
+      for (int c = 0; c < hp.ncross(); c++) {
+        const int64_t n1 = hp.cross[6 * c + 2], n2 = hp.cross[6 * c + 3];
+        const int64_t e[4] = {n1 - 2, n1 - 1, n2 - 2, n2 - 1};
+        for (int z = 0; z < 4; z++) rows[e[z]].push_back({c, z});
+      }
+      std::vector<int> start(N + 1, 0), ent;
+      for (int k = 0; k < N; k++) {
+        start[k] = (int)(ent.size() / 2);
+        for (auto [c, z] : rows[k]) { ent.push_back(c); ent.push_back(z); }
+      }
+      start[N] = (int)(ent.size() / 2);
+      if (ent.empty()) ent.assign(2, 0);
+      DFN_TRY(xrow_start.upload(start));
+      DFN_TRY(xrow_ent.upload(ent));
+    }
     DFN_HIP(hipHostMalloc((void **)&h_status, sizeof(StatusWord), hipHostMallocDefault));
     DFN_HIP(hipHostMalloc((void **)&h_cgres, sizeof(int) * kSlots * 2, hipHostMallocDefault));
     time = 0.0;
@@ -406,7 +431,13 @@ struct Engine {
     A.src_te = src_te.as<double>();
     A.ncross = hp.ncross();
     A.cross = cross.as<int64_t>();
-    launch_cg(A, nsys, stream);
+    bool launched = false;
+    if (coop_ok) {
+      CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
+      launched = launch_cg_coop(A, X, nsys, cg_partial.as<double>() + 0, cg_flags.as<int>(), stream);
+      if (!launched) coop_ok = false;  // grid too large to be co-resident: one workgroup per system
+    }
+    if (!launched) launch_cg(A, nsys, stream);
     if (hp.coupling == 1) {  // the updated permeability becomes the state (swap buffers)
       std::swap(perm_cur.p, perm_next.p);
       std::swap(permx_cur.p, permx_next.p);

@@ -12,6 +12,7 @@
 //
 // Kernels whose results feed the ODE state are compiled without floating-point
 // contraction so that every +,-,*,/ rounds as in the reference's expressions.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -41,47 +42,77 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
   if (i < n) xh[i] = x[perm[i]];
 }
 
-// Phase 1: one wave per task; lanes stride the chunk's rows, each lane keeps the
-// k partial dot products in registers, then a wave reduction per column.
+// Phase 1: one wave per task.  The body is specialised on the rank K (a
+// wave-uniform switch), so the K column loads of a row strip are all in flight
+// together; lanes stride the rows, K partial sums stay in registers and the K
+// wave reductions run interleaved.
+template <int K, typename TV>
+__device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
+                                         double *__restrict__ zpart, double *__restrict__ wz, int lane) {
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) acc[j] = 0.0;
+  const TV *V = vdata + task.voff + task.row0;
+  const double *x = w + task.xcol + task.row0;
+  const int64_t ld = task.ld;
+  for (int i = lane; i < task.len; i += 64) {
+    const double xi = x[i];
+    TV v[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) v[j] = V[i + j * ld];
+#pragma unroll
+    for (int j = 0; j < K; j++) acc[j] += (double)v[j] * xi;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int j = 0; j < K; j++) acc[j] += __shfl_xor(acc[j], off, 64);
+  }
+  double mine = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; j++)
+    if (lane == j) mine = acc[j];
+  if (lane < K) {
+    const double v = mine * (double)task.scale;
+    if (task.wout >= 0) wz[task.wout + lane] = v;
+    else zpart[task.zp + lane] = v;
+  }
+}
+
 template <typename TV>
 __global__ void __launch_bounds__(256) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
-                                                          const TV *__restrict__ vdata, const double *__restrict__ w,
+                                                          const TV *__restrict__ vdata, double *__restrict__ w,
                                                           double *__restrict__ zpart) {
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (t >= ntasks) return;
   const VTask task = tasks[t];
-  double acc[kMaxRank];
-#pragma unroll
-  for (int j = 0; j < kMaxRank; j++) acc[j] = 0.0;
-  const TV *V = vdata + task.voff + task.row0;
-  const double *x = w + task.xcol + task.row0;
-  for (int i = lane; i < task.len; i += 64) {
-    const double xi = x[i];
-#pragma unroll
-    for (int j = 0; j < kMaxRank; j++)
-      if (j < task.k) acc[j] += (double)V[i + (int64_t)j * task.ld] * xi;
-  }
-#pragma unroll
-  for (int j = 0; j < kMaxRank; j++) {
-    if (j < task.k) {
-      const double s = wave_sum(acc[j]);
-      if (lane == j) zpart[task.zp + j] = s * (double)task.scale;
-    }
+  switch (task.k) {
+#define DFN_VTX_CASE(K) \
+  case K: vtx_body<K, TV>(task, vdata, w, zpart, w, lane); break;
+    DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
+    DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
+    DFN_VTX_CASE(13) DFN_VTX_CASE(14) DFN_VTX_CASE(15) DFN_VTX_CASE(16) DFN_VTX_CASE(17) DFN_VTX_CASE(18)
+    DFN_VTX_CASE(19) DFN_VTX_CASE(20) DFN_VTX_CASE(21) DFN_VTX_CASE(22) DFN_VTX_CASE(23) DFN_VTX_CASE(24)
+    DFN_VTX_CASE(25) DFN_VTX_CASE(26) DFN_VTX_CASE(27) DFN_VTX_CASE(28) DFN_VTX_CASE(29) DFN_VTX_CASE(30)
+    DFN_VTX_CASE(31) DFN_VTX_CASE(32)
+#undef DFN_VTX_CASE
+    default: break;
   }
 }
 
-// z of each low-rank block = sum of its chunk partials, in chunk order.
+// z of each multi-chunk low-rank block = sum of its chunk partials, in chunk
+// order; one thread per (block, column).
 __global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ zr, int nblocks,
                                                       const double *__restrict__ zpart, double *__restrict__ w) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = g >> 5, j = g & 31;
   if (b >= nblocks) return;
   const ZRed r = zr[b];
-  for (int j = 0; j < r.k; j++) {
-    double s = 0.0;
-    for (int t = 0; t < r.ntask; t++) s += zpart[r.zp_first + t * r.k + j];
-    w[r.zout + j] = s;
-  }
+  if (j >= r.k) return;
+  double s = 0.0;
+  for (int t = 0; t < r.ntask; t++) s += zpart[r.zp_first + t * r.k + j];
+  w[r.zout + j] = s;
 }
 
 // Phase 2: one workgroup per panel task.  w entries of the panel's columns are
@@ -508,6 +539,245 @@ __global__ void __launch_bounds__(1024) cg_kernel(CgArgs A) {
   }
 }
 
+// Cooperative compute_P: every system (time step dt) is spread over `wps`
+// workgroups of 256 threads, kCgEPT elements per thread held in registers (x, r,
+// p, A p and the stencil coefficients).  Neighbour and crossing values of p go
+// through global memory; the dot products are per-workgroup partials summed in
+// a fixed order after each grid barrier, so every workgroup of a system computes
+// the same alpha, beta and stopping decision.  Three grid barriers per iteration.
+__device__ __forceinline__ void cross_ends(int z, int64_t n1, int64_t n2, int64_t &ea, int64_t &eb, int64_t &ec,
+                                           int64_t &pl0, int64_t &pl1, int64_t &pr0, int64_t &pr1) {
+  // for slot z: the row's A1/A2/A3 operands (0-based elements) and the permeability pairs
+  // of the left and right harmonic means (src/fluid_diffusion.f90:221-291, :349-401)
+  switch (z) {
+    case 0: ea = n2 - 2; eb = n1 - 2; ec = n2 - 1; pl0 = n1 - 2; pl1 = n2 - 2; pr0 = n1 - 2; pr1 = n2 - 1; break;
+    case 1: ea = n2 - 2; eb = n1 - 1; ec = n2 - 1; pl0 = n1 - 1; pl1 = n2 - 2; pr0 = n1 - 1; pr1 = n2 - 1; break;
+    case 2: ea = n1 - 2; eb = n2 - 2; ec = n1 - 1; pl0 = n1 - 2; pl1 = n2 - 2; pr0 = n1 - 1; pr1 = n2 - 2; break;
+    default: ea = n1 - 2; eb = n2 - 1; ec = n1 - 1; pl0 = n1 - 2; pl1 = n2 - 1; pr0 = n1 - 1; pr1 = n2 - 1; break;
+  }
+}
+
+// Values of a vector at other elements: either read directly (v1 == nullptr) or
+// formed as v0[e] + beta * v1[e] (the new search direction from the previous
+// residual and direction, as their owner forms it).
+struct NbVec {
+  const double *v0, *v1;
+  double beta;
+  __device__ __forceinline__ double operator()(int64_t e) const { return v1 ? v0[e] + beta * v1[e] : v0[e]; }
+};
+
+// M v at element k (row-scaled stencil plus crossing terms)
+__device__ __forceinline__ double coop_mv(const CgArgs &A, const CrossRows &X, int k, double vk, const NbVec &v,
+                                          double a1, double a2, double a3, double dg, double dt, double dq) {
+  const int n = A.n;
+  double r;
+  if (k == 0) r = a2 * vk + a3 * v(1);
+  else if (k == n - 1) r = a1 * v(k - 1) + a2 * vk;
+  else r = a1 * v(k - 1) + a2 * vk + a3 * v(k + 1);
+  r = r / dg;
+  for (int t = X.start[k]; t < X.start[k + 1]; t++) {
+    const int cidx = X.ent[2 * t], z = X.ent[2 * t + 1];
+    const int64_t n1 = A.cross[6 * cidx + 2], n2 = A.cross[6 * cidx + 3];
+    int64_t ea, eb, ec, pl0, pl1, pr0, pr1;
+    cross_ends(z, n1, n2, ea, eb, ec, pl0, pl1, pr0, pr1);
+    const double kl = hmean(A.perm[pl0], A.perm[pl1]), kr = hmean(A.perm[pr0], A.perm[pr1]);
+    const double c1 = -(kl * dt / dq), c3 = -(kr * dt / dq), c2 = kl * dt / dq + kr * dt / dq;
+    const double vb = (eb == k) ? vk : v(eb);
+    const double num = c1 * v(ea) + c2 * vb + c3 * v(ec);
+    r = r + num / (z == 1 ? (double)n1 : dg);
+  }
+  return r;
+}
+
+constexpr int kCgMaxSys = 8;
+
+// Per iteration: A p (neighbours of p_i formed from r_{i-1} and p_{i-1} in global
+// memory), grid barrier on p.Ap, update of x and r (r and p_i published), grid
+// barrier on r.r and max|r|, new direction in registers.  S.r and S.p / S.Ap hold
+// the published residual and the two alternating copies of the direction.
+__global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int wps, double *partial, int *sysflag) {
+  namespace cgr = cooperative_groups;
+  cgr::grid_group grid = cgr::this_grid();
+  __shared__ double red[8];
+  const int nsys = gridDim.x / wps;
+  const int sys = blockIdx.x / wps, part = blockIdx.x - sys * wps;
+  const CgSys S = A.sys[sys];
+  const int n = A.n;
+  const double dt = S.dt, c = A.cfac;
+  const double *ds = A.ds, *kx = A.perm_x, *kxd = A.perm_x_diag, *P = A.P;
+  const int base = part * kCgSpan + threadIdx.x;
+  const int kq = n > 2 ? n : 2;
+  const double dq = c * (ds[kq - 2] * ds[kq - 2]);
+  const double d1 = c * (ds[0] * ds[0]);
+  double *pbuf[2] = {S.p, S.Ap};
+  double a1[kCgEPT], a2[kCgEPT], a3[kCgEPT], dg[kCgEPT], x[kCgEPT], r[kCgEPT], p[kCgEPT], ap[kCgEPT];
+  double part_rr = 0.0, part_em = 0.0;
+#pragma unroll
+  for (int q = 0; q < kCgEPT; q++) {
+    const int k = base + q * 256;
+    a1[q] = 0.0; a2[q] = 0.0; a3[q] = 0.0; dg[q] = 1.0; x[q] = 0.0; r[q] = 0.0; p[q] = 0.0; ap[q] = 0.0;
+    if (k < n) {
+      // compute_diag (:308-405) with the permeability in effect before this call
+      double d;
+      if (k == 0) d = 1.0 + kxd[1] * dt / (c * (ds[0] * ds[0]));
+      else if (k == n - 1) d = 1.0 + kxd[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1]));
+      else d = 1.0 + (kxd[k] + kxd[k + 1]) * dt / (c * (ds[k] * ds[k]));
+      for (int t = X.start[k]; t < X.start[k + 1]; t++) {
+        const int cidx = X.ent[2 * t], z = X.ent[2 * t + 1];
+        const int64_t n1 = A.cross[6 * cidx + 2], n2 = A.cross[6 * cidx + 3];
+        int64_t ea, eb, ec, pl0, pl1, pr0, pr1;
+        cross_ends(z, n1, n2, ea, eb, ec, pl0, pl1, pr0, pr1);
+        const double kl = hmean(A.perm_diag[pl0], A.perm_diag[pl1]), kr = hmean(A.perm_diag[pr0], A.perm_diag[pr1]);
+        d = d + kl * dt / d1 + kr * dt / d1;
+      }
+      dg[q] = d;
+      // compute_MatrixVector coefficients (:175-204)
+      if (k == 0) {
+        a2[q] = 1.0 + kx[1] * dt / (c * (ds[0] * ds[0]));
+        a3[q] = -(kx[1] * dt / (c * (ds[0] * ds[0])));
+      } else if (k == n - 1) {
+        a1[q] = -(kx[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1])));
+        a2[q] = 1.0 + kx[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1]));
+      } else {
+        a1[q] = -(kx[k] * dt / (c * (ds[k - 1] * ds[k - 1])));
+        a2[q] = 1.0 + (kx[k] + kx[k + 1]) * dt / (c * (ds[k] * ds[k]));
+        a3[q] = -(kx[k + 1] * dt / (c * (ds[k + 1] * ds[k + 1])));
+      }
+      // b1 = P/diag plus the active sources (:45-64)
+      double b = P[k] / d;
+      for (int s2 = 0; s2 < A.nsrc; s2++) {
+        const double tt = A.time + dt;
+        if (A.src_elem[s2] - 1 == k && tt >= A.src_tb[s2] && tt < A.src_te[s2])
+          b = b + (dt * A.src_Q[s2] / (A.por * A.beta) / ds[k]) / d;
+      }
+      x[q] = P[k];
+      const NbVec vP{P, nullptr, 0.0};
+      const double mv = coop_mv(A, X, k, P[k], vP, a1[q], a2[q], a3[q], d, dt, dq);
+      r[q] = b - mv;
+      p[q] = r[q];
+      S.r[k] = r[q];
+      part_rr += r[q] * r[q];
+      part_em = fmax(part_em, fabs(r[q] / 1.0));
+    }
+  }
+  double *mypart = partial + 4 * (int64_t)(sys * wps + part);
+  part_rr = block_sum(part_rr, red);
+  part_em = block_max(part_em, red);
+  if (threadIdx.x == 0) { mypart[0] = part_rr; mypart[1] = part_em; }
+  grid.sync();
+  // every workgroup tracks every system's state from the shared partials
+  bool sdone[kCgMaxSys];
+  int sit[kCgMaxSys];
+#pragma unroll
+  for (int s2 = 0; s2 < kCgMaxSys; s2++) {
+    sit[s2] = 0;
+    sdone[s2] = true;
+    if (s2 < nsys) {
+      double em2 = 0.0;
+      for (int pp = 0; pp < wps; pp++) em2 = fmax(em2, partial[4 * (int64_t)(s2 * wps + pp) + 1]);
+      sdone[s2] = !(em2 / A.tol >= 1.0);
+    }
+  }
+  const double *sp = partial + 4 * (int64_t)(sys * wps);
+  double rr = 0.0;
+  for (int pp = 0; pp < wps; pp++) rr += sp[4 * pp];
+  bool done = false, fail = false;
+  int it = 0;
+  double beta_prev = 0.0;
+  for (;;) {
+    bool all_done = true;
+#pragma unroll
+    for (int s2 = 0; s2 < kCgMaxSys; s2++) all_done = all_done && sdone[s2];
+    if (all_done) break;
+#pragma unroll
+    for (int s2 = 0; s2 < kCgMaxSys; s2++)
+      if (s2 == sys) done = sdone[s2];
+    const double *pprev = pbuf[(it + 1) & 1];  // p_{i-1}
+    double *pcur = pbuf[it & 1];               // receives p_i
+    double part_pap = 0.0;
+    if (!done) {
+      // p_1 = r_0 is read as is; later p_i = r_{i-1} + beta_{i-1} p_{i-1}
+      const NbVec vp{S.r, it == 0 ? nullptr : pprev, beta_prev};
+#pragma unroll
+      for (int q = 0; q < kCgEPT; q++) {
+        const int k = base + q * 256;
+        if (k < n) {
+          ap[q] = coop_mv(A, X, k, p[q], vp, a1[q], a2[q], a3[q], dg[q], dt, dq);
+          part_pap += p[q] * ap[q];
+        }
+      }
+    }
+    part_pap = block_sum(part_pap, red);
+    if (threadIdx.x == 0) mypart[2] = part_pap;
+    grid.sync();
+    double part_r1 = 0.0, part_m = 0.0;
+    if (!done) {
+      double pAp = 0.0;
+      for (int pp = 0; pp < wps; pp++) pAp += sp[4 * pp + 2];
+      const double alpha = rr / pAp;
+#pragma unroll
+      for (int q = 0; q < kCgEPT; q++) {
+        const int k = base + q * 256;
+        if (k < n) {
+          x[q] = x[q] + alpha * p[q];
+          r[q] = r[q] - alpha * ap[q];
+          S.r[k] = r[q];
+          pcur[k] = p[q];
+          part_r1 += r[q] * r[q];
+          part_m = fmax(part_m, fabs(r[q] / 1.0));
+        }
+      }
+    }
+    part_r1 = block_sum(part_r1, red);
+    part_m = block_max(part_m, red);
+    if (threadIdx.x == 0) { mypart[0] = part_r1; mypart[1] = part_m; }
+    grid.sync();
+#pragma unroll
+    for (int s2 = 0; s2 < kCgMaxSys; s2++) {
+      if (s2 < nsys && !sdone[s2]) {
+        sit[s2]++;
+        double em2 = 0.0;
+        for (int pp = 0; pp < wps; pp++) em2 = fmax(em2, partial[4 * (int64_t)(s2 * wps + pp) + 1]);
+        sdone[s2] = (sit[s2] > kCgMaxIter) || !(em2 / A.tol >= 1.0);
+      }
+    }
+    if (!done) {
+      double r1r1 = 0.0;
+      for (int pp = 0; pp < wps; pp++) r1r1 += sp[4 * pp];
+      const double beta = r1r1 / rr;
+#pragma unroll
+      for (int q = 0; q < kCgEPT; q++) p[q] = r[q] + beta * p[q];
+      rr = r1r1;
+      beta_prev = beta;
+      it++;
+      if (it > kCgMaxIter) fail = true;
+    }
+  }
+  int bad = 0;
+  if (!fail) {
+    const bool small = dt < (double)1e-9f;
+#pragma unroll
+    for (int q = 0; q < kCgEPT; q++) {
+      const int k = base + q * 256;
+      if (k < n) {
+        S.Pt[k] = x[q];
+        S.Pd[k] = small ? 0.0 : (x[q] - P[k]) / dt;
+        if (fabs(x[q]) > 1.7976931348623157e308) bad = 1;
+      }
+    }
+  }
+  const double b2 = block_max(bad ? 1.0 : 0.0, red);
+  if (threadIdx.x == 0) mypart[3] = b2;
+  grid.sync();
+  if (part == 0 && threadIdx.x == 0) {
+    int anybad = 0;
+    for (int pp = 0; pp < wps; pp++) anybad |= (sp[4 * pp + 3] > 0.0);
+    S.result[0] = (fail || anybad) ? 1 : 0;
+    S.result[1] = it;
+  }
+  (void)sysflag;
+}
+
 // permeability update of compute_P (:67-75); writes perm and perm_x
 __global__ void __launch_bounds__(256) perm_update_kernel(PermArgs A) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -532,7 +802,7 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
   hipLaunchKernelGGL(gather_perm_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, x, perm, xh, n);
 }
 
-void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, const double *w, double *zpart, hipStream_t s) {
+void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s) {
   if (nt <= 0) return;
   if (fp32)
     hipLaunchKernelGGL(lowrank_vtx_kernel<float>, dim3(nblk(nt, 4)), dim3(256), 0, s, t, nt, (const float *)vdata, w,
@@ -544,7 +814,7 @@ void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, const doub
 
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s) {
   if (nb <= 0) return;
-  hipLaunchKernelGGL(zreduce_kernel, dim3(nblk(nb, 256)), dim3(256), 0, s, zr, nb, zpart, w);
+  hipLaunchKernelGGL(zreduce_kernel, dim3(nblk((int64_t)nb * 32, 256)), dim3(256), 0, s, zr, nb, zpart, w);
 }
 
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const double *w,
@@ -592,6 +862,27 @@ void launch_restart_stage2(const double *y, const double *k1, double h, int n3, 
 
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s) {
   hipLaunchKernelGGL(cg_kernel, dim3(nsys), dim3(1024), 0, s, a);
+}
+
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *partial, int *sysflag, hipStream_t s) {
+  static int max_blocks = -1;
+  if (max_blocks < 0) {
+    int per_cu = 0, dev = 0, cus = 0, coop = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_coop_kernel, 256, 0) != hipSuccess)
+      max_blocks = 0;
+    else
+      max_blocks = per_cu * cus;
+  }
+  int wps = (a.n + kCgSpan - 1) / kCgSpan;
+  if (nsys * wps > max_blocks) return false;
+  CgArgs A = a;
+  CrossRows X = xr;
+  void *args[] = {&A, &X, &wps, &partial, &sysflag};
+  return hipLaunchCooperativeKernel((const void *)cg_coop_kernel, dim3(nsys * wps), dim3(256), args, 0, s) ==
+         hipSuccess;
 }
 
 void launch_perm_update(const PermArgs &a, hipStream_t s) {

@@ -37,3 +37,25 @@ def tree_network():
     q.nleaf = 16
     q.eta = 2.0
     return q
+
+
+def grid_network(ds=2.0):
+    """Three horizontal and three vertical Mode II faults whose nodes coincide at
+    the nine intersections (crossings), ~2.7k elements: the pressure solve spans
+    several device workgroups and carries crossing terms.  A source on fault 1."""
+    fs = []
+    for y in (0.0, 300.0, 600.0):
+        fs.append(_seg(0.0, y, 1000.0, y, int(round(1000.0 / ds))))
+    for x in (100.0, 500.0, 900.0):
+        fs.append(_seg(x, -100.0, x, 700.0, int(round(800.0 / ds))))
+    faults = []
+    for i, nd in enumerate(fs):
+        n = nd.shape[0] - 1
+        one = np.ones(n)
+        faults.append(dict(node=nd, a=(0.009 + 0.0005 * i) * one, b=0.012 * one, Dc=1e-3 * one, f0=0.6 * one,
+                           V0=1e-6 * one, sigmaN=-30e6 * one, V=1e-12 * one, theta=np.log(1e6) * one, P=0 * one,
+                           permeability=1e-14 * one * (1 + 0.5 * np.sin(np.arange(n) / 37.0)), Snk=one, Lk=one,
+                           Tk=one))
+    nels, node, arr = problems.concatenate_faults(faults)
+    return problems.Problem("grid_network", 2, 0, nels, node, arr, tol_solver=1e-6,
+                            sources=[(1, 75, 2e-4, 0.0, 1e6)])

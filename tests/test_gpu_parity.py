@@ -49,7 +49,7 @@ def test_direct_kernel_and_rhs(make):
         assert close_elementwise(yd, yo, 1e-9, 1e-300 + np.abs(yo).max() * 1e-12), x
 
 
-@pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network])
+@pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network, cases.grid_network])
 def test_compute_P(make):
     p = make()
     d, o = dev(p), OracleSimulation(p)
@@ -63,7 +63,8 @@ def test_compute_P(make):
             assert np.max(np.abs(ad - bd)) <= (1e-9 * scale + 10 * p.tol_solver) / dt, dt
 
 
-@pytest.mark.parametrize("make,nsteps", [(cases.single_fault_small, 20), (cases.crossing_network, 30)])
+@pytest.mark.parametrize("make,nsteps", [(cases.single_fault_small, 20), (cases.crossing_network, 30),
+                                          (cases.grid_network, 4)])
 def test_solver_trajectory(make, nsteps):
     p = make()
     d, o = dev(p), OracleSimulation(p)
