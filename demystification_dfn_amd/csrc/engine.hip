@@ -501,7 +501,9 @@ struct Engine {
 
   // rkqs (ode_bs.f90:125-174) on the device state Y with k1 = dydx already in k(0)
   // and yscal set; x is the step-start abscissa.  On success Y holds the new state.
-  int rkqs_dev(double x, double htry, double eps, double *hdid, double *hnext) {
+  // stages_ready: CG slots 1..5 already hold compute_P(x + A_i*htry) (launched with the
+  // step's t_try solve)
+  int rkqs_dev(double x, double htry, double eps, double *hdid, double *hnext, bool stages_ready = false) {
     const int N = hp.N, n3 = 3 * N;
     const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = (double)1.89e-4f;
     double h = htry, errmax = 0.0;
@@ -510,7 +512,9 @@ struct Engine {
       static const double Ai[5] = {0.2, 0.3, 0.6, 1.0, 0.875};
       double dts[5];
       for (int s = 0; s < 5; s++) dts[s] = x + Ai[s] * h;
-      DFN_TRY(cg_launch(dts, 5, 1));
+      if (!stages_ready) DFN_TRY(cg_launch(dts, 5, 1));
+      else st.cg_solves += 5;
+      stages_ready = false;
       DFN_HIP(hipMemsetAsync(status.p, 0, sizeof(StatusWord), stream));
       double *in = ytA.as<double>(), *out = ytB.as<double>();
       for (int s = 2; s <= 6; s++) {
@@ -549,14 +553,25 @@ struct Engine {
       for (size_t s = 0; s < hp.src_tb.size(); s++)
         if ((time < hp.src_tb[s]) && (time + t_try > hp.src_tb[s])) t_try = 10.0;
     int fail = 0;
-    DFN_TRY(cg_launch(&t_try, 1, 0));  // :81-92
+    // compute_P(t_try) (:81-92).  Without permeability evolution every compute_P of the
+    // step depends on dt alone, so the five stage solves of a first attempt with
+    // h = t_try (x + A_i h, x = 0) run in the same launch.
+    const bool batch = hp.coupling != 1;
+    auto launch_try = [&]() -> int {
+      if (!batch) return cg_launch(&t_try, 1, 0);
+      const double dts[6] = {t_try, 0.0 + 0.2 * t_try, 0.0 + 0.3 * t_try, 0.0 + 0.6 * t_try, 0.0 + 1.0 * t_try,
+                             0.0 + 0.875 * t_try};
+      return cg_launch(dts, 6, 0);
+    };
+    DFN_TRY(launch_try());
     DFN_TRY(cg_collect(0, 1, &fail));
     while (fail) {
       t_try = t_try / 2.0;
       st.cg_halvings++;
-      DFN_TRY(cg_launch(&t_try, 1, 0));
+      DFN_TRY(launch_try());
       DFN_TRY(cg_collect(0, 1, &fail));
     }
+    if (batch) st.cg_solves -= 5;  // counted when the stages use them
     // k1 = f(t_try, Y) and yscal (:101-102); compute_P(t_try) of the RHS equals the
     // call above unless the permeability evolves between calls
     DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
@@ -569,7 +584,7 @@ struct Engine {
     launch_stage(1, s1, stream);
     st.rhs_evaluations++;
     double t_did = 0, t_next = 0;
-    DFN_TRY(rkqs_dev(0.0, t_try, hp.tol, &t_did, &t_next));  // :111
+    DFN_TRY(rkqs_dev(0.0, t_try, hp.tol, &t_did, &t_next, batch));  // :111
     t_try = t_next;  // :117-120
     if (t_try > 24 * 60 * 60) t_try = 24 * 60 * 60;
     // commit compute_P(t_did) (:174): the stage-5 solve used dt = 0 + 1.0*h = t_did
