@@ -73,6 +73,43 @@ def cpu_baseline(sim, p, seconds, threads):
                                    f"({wall:.1f} s wall, {sim_dt:.3e} s simulated, OpenMP matvec)")
 
 
+def mixed_precision(sim, p, args, kb64):
+    """FP32 operator storage (per-block power-of-two scales, FP32 accumulation,
+    compensated near-field sums) with FP64 state, RHS, CG and error norm: stress-rate
+    error against the FP64 operator over random log-uniform slip rates in [1e-12, 1]
+    m/s, then the same number of solver steps continued from the FP64 run's state."""
+    from demystification_dfn_amd import _abi
+    rng = np.random.default_rng(7)
+    errs, sig = [], []
+    for _ in range(args.mixed_vectors):
+        V = 10.0 ** rng.uniform(-12.0, 0.0, p.N)
+        sim.set_precision(_abi.PREC_FP64)
+        a = sim.kernel(V)
+        sim.set_precision(_abi.PREC_FP32)
+        b = sim.kernel(V)
+        for y64, y32 in zip(a, b):
+            errs.append(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
+            big = np.abs(y64) > 1e-3 * np.abs(y64).max()
+            sig.append(np.max(np.abs(y32[big] - y64[big]) / np.abs(y64[big])))
+    sim.set_precision(_abi.PREC_FP32)
+    kb = sim.bench_kernel(reps=20)
+    s0 = sim.stats()
+    w0 = time.perf_counter()
+    sim.solve(args.mixed_steps)
+    wall = time.perf_counter() - w0
+    s1 = sim.stats()
+    sim.set_precision(_abi.PREC_FP64)
+    sim_dt = s1["time"] - s0["time"]
+    return dict(value=(sim_dt / YEAR) / (wall / 3600.0), unit="simulated yr / wall hr", steps=args.mixed_steps,
+                ms_per_step=1e3 * wall / args.mixed_steps,
+                hmatvec_ms=kb["ms_kernel"], hmatvec_GB_s=kb["alg_bytes"] / (kb["ms_kernel"] * 1e-3) / 1e9,
+                hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kb["ms_kernel"],
+                panel_GB_s=kb["dominant_bytes"] / (kb["ms_dominant"] * 1e-3) / 1e9,
+                stress_rate_rel_err=dict(vectors=args.mixed_vectors, norm_max=float(np.max(errs)),
+                                         norm_median=float(np.median(errs)), significant_rows_max=float(np.max(sig))),
+                note="continues from the FP64 run's final state; operators FP32, state FP64")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,13 +119,19 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed-steps", type=int, default=300,
+                    help="steps of the FP32-operator (mixed precision) run on the same workload; 0 skips it")
+    ap.add_argument("--mixed-vectors", type=int, default=200,
+                    help="random log-uniform slip-rate vectors for the FP32-vs-FP64 stress-rate error")
+    ap.add_argument("--copy-probe-gb", type=float, default=2.0,
+                    help="size of the copy kernel whose rate is reported as the achievable HBM bandwidth")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this path has no multi-GPU form (see DESIGN.md, sharding)",
               file=sys.stderr)
         sys.exit(2)
 
-    from demystification_dfn_amd.device import DeviceSimulation, device_info
+    from demystification_dfn_amd.device import DeviceSimulation, bench_stream, device_info
     log("[bench]", device_info())
     p, desc = make_workload(args.workload, args.seed)
     log(f"[bench] workload {desc['workload']} N={p.N}")
@@ -116,6 +159,7 @@ def main():
     value = (sim_dt / YEAR) / (wall / 3600.0)
     log(f"[bench] {args.steps} steps in {wall:.2f} s, simulated {sim_dt:.3e} s")
     kb = sim.bench_kernel(reps=20)
+    copy_gbs = bench_stream(int(args.copy_probe_gb * (1 << 30)), 20) if args.copy_probe_gb > 0 else None
     achieved = kb["dominant_bytes"] / (kb["ms_dominant"] * 1e-3) / 1e9
     whole = kb["alg_bytes"] / (kb["ms_kernel"] * 1e-3) / 1e9
     out = dict(
@@ -126,11 +170,14 @@ def main():
                       traffic=None, kernel="panel_kernel<double> (H-matvec phase 2)",
                       bytes_per_launch=kb["dominant_bytes"], ms_per_launch=kb["ms_dominant"]),
         hmatvec=dict(ms=kb["ms_kernel"], alg_bytes=kb["alg_bytes"], GB_s=whole, frac_hbm=whole / HBM_PEAK_GBS, **hm),
+        copy_kernel_GB_s=copy_gbs,
         run=dict(accepted=s1["accepted_steps"] - s0["accepted_steps"],
                  rejected=s1["rejected_attempts"] - s0["rejected_attempts"],
                  cg_iterations=s1["cg_iterations"] - s0["cg_iterations"], sim_time_s=s1["time"],
                  max_V=s1["max_V"], setup_s=setup_s),
     )
+    if args.mixed_steps > 0 and p.static_kernel == 1:
+        out["mixed"] = mixed_precision(sim, p, args, kb)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] cpu baseline on {threads} threads")

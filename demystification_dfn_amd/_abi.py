@@ -106,6 +106,7 @@ SIGNATURES = {
     "rkqs": [_H, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
     "solve": [_H, C.c_int64, C.c_double, C.POINTER(DfnRunStats)],
     "bench_kernel": [_H, C.c_int32, _dp],
+    "bench_stream": [C.c_int64, C.c_int32, _dp],
     "device_info": [C.c_char_p, C.c_int32],
 }
 

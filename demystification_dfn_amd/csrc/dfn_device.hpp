@@ -11,17 +11,19 @@ constexpr int kPanelMaxCols = 2048;   // columns per phase-2 panel task (w stage
 constexpr int kVChunkRows = 512;      // rows of V per phase-1 wave task
 constexpr int kCgMaxIter = 200;       // compute_P gives up after 200 iterations (fluid_diffusion.f90:120)
 
-// Phase 1: zpart[zp + j] = scale * sum_{i<len} V[row0+i, j] * w[xcol+row0+i], j < k
+// Phase 1: out[j] = scale * sum_{i<len} V[row0+i, j] * w[xcol+row0+i], j < k.
+// FP32 storage holds V / scale with a power-of-two scale per block.
 struct VTask {
   int64_t voff;   // offset of the block's V (column-major, leading dimension ld)
+  double scale;
   int32_t ld;     // rows of V (the block's column count)
   int32_t row0;   // first row of this chunk
   int32_t len;    // rows in this chunk
   int32_t k;      // rank
   int32_t xcol;   // H index of the block's first column
   int32_t zp;     // output offset in zpart (blocks split in several chunks)
-  float scale;
   int32_t wout;   // >= 0: single-chunk block, z written straight to w[wout + j]
+  int32_t pad;
 };
 
 // z of one low-rank block: w[zout + j] = sum_t zpart[zp_first + t*k + j]
@@ -29,14 +31,16 @@ struct ZRed {
   int32_t zp_first, ntask, k, zout;
 };
 
-// Phase 2: ypart[yp + r] = scale * sum_j P[r, j] * w[widx[j]] over a row-major m x L panel
+// Phase 2: ypart[yp + r] = sum_j P[r, j] * w[widx[j]] (* cscale[j] in FP32 storage,
+// where the first nd columns are near-field entries summed with compensation)
+// over a row-major m x L panel.
 struct PTask {
   int64_t poff;
   int64_t widx;
   int32_t m;
   int32_t L;
   int32_t yp;
-  float scale;
+  int32_t nd;
 };
 
 struct RhsParams {
@@ -110,8 +114,8 @@ struct PermArgs {
 void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s);
 void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
-void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const double *w,
-                  double *ypart, hipStream_t s);
+void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
+                  const double *w, double *ypart, hipStream_t s);
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
                      double scale, double *yT, double *yN, hipStream_t s);
 void launch_direct3(const double *node, const double *slip, double *cbuf, int n, double scale, double *sT,
@@ -124,5 +128,6 @@ void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
 // cooperative multi-workgroup CG; returns false if the grid cannot be co-resident
 bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *partial, int *sysflag, hipStream_t s);
 void launch_perm_update(const PermArgs &a, hipStream_t s);
+void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 
 }  // namespace dfn

@@ -68,15 +68,26 @@ struct DMem {
 struct DevOps {
   int n = 0, nops = 0;
   bool fp32 = false;
-  DMem perm, vtasks, vdata, zred, zpart, ptasks, pdata, widx, w, ypart, fin_first, fin_idx;
+  DMem perm, vtasks, vdata, zred, zpart, ptasks, pdata, widx, cscale, w, ypart, fin_first, fin_idx;
   int nvt = 0, nzr = 0, npt = 0;
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
   int64_t S_total = 0;   // matrix entries streamed per application
 };
 
+// power-of-two scale 2^e >= max|v| (1 for an all-zero block): dividing by it is exact
+static double pow2_scale(const double *v, size_t n) {
+  double m = 0.0;
+  for (size_t i = 0; i < n; i++) m = std::fmax(m, std::fabs(v[i]));
+  if (!(m > 0.0) || !std::isfinite(m)) return 1.0;
+  int e = 0;
+  (void)std::frexp(m, &e);
+  return std::ldexp(1.0, e);
+}
+
 template <class T>
 static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
+  const bool scaled = sizeof(T) == 4;  // FP32 storage keeps each block divided by its power-of-two scale
   const int n = ops[0]->n;
   for (int o = 1; o < nops; o++)
     if (ops[o]->perm != ops[0]->perm) return DFN_ERR_UNSUPPORTED;  // T and N share the cluster tree
@@ -103,10 +114,14 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   for (int o = 0; o < nops; o++) {
     const HostOperator &H = *ops[o];
     const int64_t vbase = (int64_t)vdata.size();
-    vdata.insert(vdata.end(), H.l_v.begin(), H.l_v.end());
+    vdata.resize(vdata.size() + H.l_v.size());
     for (size_t b = 0; b < H.l_k.size(); b++) {
       const int k = H.l_k[b], nb = H.l_n[b];
       if (k > kMaxRank) return DFN_ERR_UNSUPPORTED;
+      const double *vb = H.l_v.data() + H.l_voff[b];
+      const size_t cnt = (size_t)nb * k;
+      const double vs = scaled ? pow2_scale(vb, cnt) : 1.0;
+      for (size_t i = 0; i < cnt; i++) vdata[vbase + H.l_voff[b] + i] = (T)(vb[i] / vs);
       const int wz = (int)(zbase[o] + zoff[o][b]);
       const bool single = nb <= kVChunkRows;  // one chunk: z goes straight into w
       ZRed r{zp, 0, k, wz};
@@ -119,7 +134,7 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
         t.k = k;
         t.xcol = H.l_c0[b];
         t.zp = single ? -1 : zp;
-        t.scale = 1.0f;
+        t.scale = vs;
         t.wout = single ? wz : -1;
         vt.push_back(t);
         if (!single) zp += k;
@@ -138,6 +153,22 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   std::vector<PTask> pt;
   std::vector<T> pdata;
   std::vector<int> widx;
+  std::vector<float> cscale;
+  std::vector<std::vector<double>> dscale(nops), uscale(nops);
+  int64_t reserve = 0;
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    reserve += (int64_t)H.d_data.size() + (int64_t)H.l_u.size();
+    dscale[o].assign(H.d_m.size(), 1.0);
+    uscale[o].assign(H.l_m.size(), 1.0);
+    if (scaled) {
+      for (size_t b = 0; b < H.d_m.size(); b++)
+        dscale[o][b] = pow2_scale(H.d_data.data() + H.d_off[b], (size_t)H.d_m[b] * H.d_n[b]);
+      for (size_t b = 0; b < H.l_m.size(); b++)
+        uscale[o][b] = pow2_scale(H.l_u.data() + H.l_uoff[b], (size_t)H.l_m[b] * H.l_k[b]);
+    }
+  }
+  pdata.reserve(reserve);
   std::vector<std::vector<int>> fin((size_t)n * nops);
   int yp = 0;
   for (int o = 0; o < nops; o++) {
@@ -152,16 +183,22 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
         cover[l].push_back({1, (int)b});
     for (int l = 0; l < nleaf; l++) {
       const int r0 = H0.leaf_r0[l], m = H0.leaf_m[l];
-      // columns of this panel: (w index, source pointer into the block, column stride)
-      struct Col { int64_t wi; const double *src; };
+      // columns of this panel: (w index, source pointer into the block, block scale);
+      // near-field (dense) columns first
+      struct Col { int64_t wi; const double *src; double scale; };
       std::vector<Col> cols;
+      int ndense = 0;
       for (auto [kind, b] : cover[l]) {
         if (kind == 0) {
+          const double sc = scaled ? dscale[o][b] : 1.0;
           const double *F = H.d_data.data() + H.d_off[b] + (r0 - H.d_r0[b]);
-          for (int j = 0; j < H.d_n[b]; j++) cols.push_back({(int64_t)H.d_c0[b] + j, F + (int64_t)j * H.d_m[b]});
+          for (int j = 0; j < H.d_n[b]; j++) cols.push_back({(int64_t)H.d_c0[b] + j, F + (int64_t)j * H.d_m[b], sc});
+          ndense += H.d_n[b];
         } else {
+          const double sc = scaled ? uscale[o][b] : 1.0;
           const double *U = H.l_u.data() + H.l_uoff[b] + (r0 - H.l_r0[b]);
-          for (int j = 0; j < H.l_k[b]; j++) cols.push_back({zbase[o] + zoff[o][b] + j, U + (int64_t)j * H.l_m[b]});
+          for (int j = 0; j < H.l_k[b]; j++)
+            cols.push_back({zbase[o] + zoff[o][b] + j, U + (int64_t)j * H.l_m[b], sc});
         }
       }
       for (size_t c0 = 0; c0 < cols.size(); c0 += kPanelMaxCols) {
@@ -172,10 +209,13 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
         t.m = m;
         t.L = L;
         t.yp = yp;
-        t.scale = 1.0f;
-        for (int j = 0; j < L; j++) widx.push_back((int)cols[c0 + j].wi);
+        t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, (int64_t)ndense - (int64_t)c0));
+        for (int j = 0; j < L; j++) {
+          widx.push_back((int)cols[c0 + j].wi);
+          if (scaled) cscale.push_back((float)cols[c0 + j].scale);
+        }
         for (int r = 0; r < m; r++)
-          for (int j = 0; j < L; j++) pdata.push_back((T)cols[c0 + j].src[r]);
+          for (int j = 0; j < L; j++) pdata.push_back((T)(cols[c0 + j].src[r] / cols[c0 + j].scale));
         for (int r = 0; r < m; r++) fin[(size_t)o * n + r0 + r].push_back(yp + r);
         yp += m;
         pt.push_back(t);
@@ -197,6 +237,7 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   DFN_TRY(d.ptasks.upload(pt));
   DFN_TRY(d.pdata.upload(pdata));
   DFN_TRY(d.widx.upload(widx));
+  DFN_TRY(d.cscale.upload(cscale));
   DFN_TRY(d.w.alloc(sizeof(double) * wsize));
   DFN_TRY(d.ypart.alloc(sizeof(double) * std::max(yp, 1)));
   DFN_TRY(d.fin_first.upload(fin_first));
@@ -207,7 +248,7 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   d.S_total = S_total;
   // bytes: factors + x read + y written, per operator (x shared in Mode II)
   d.alg_bytes = (double)sizeof(T) * (double)S_total + 8.0 * n + 8.0 * n * nops;
-  d.panel_bytes = (double)sizeof(T) * (double)pdata.size() + 8.0 * (double)yp;
+  d.panel_bytes = (double)sizeof(T) * (double)pdata.size() + 4.0 * (double)cscale.size() + 8.0 * (double)yp;
   return DFN_OK;
 }
 
@@ -377,8 +418,8 @@ struct Engine {
     launch_gather(d_V, d->perm.as<int>(), w, N, stream);
     launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream);
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
-    launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), w, d->ypart.as<double>(),
-                 stream);
+    launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
+                 d->ypart.as<double>(), stream);
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                     d->nops, kernel_scale(), d_sT, d->nops > 1 ? d_sN : d_sT, stream);
     return DFN_OK;
@@ -640,6 +681,34 @@ static int device_ok() {
   return DFN_OK;
 }
 
+API int dfn_bench_stream(int64_t bytes, int32_t reps, double *gbps) {
+  if (bytes < 16 || reps < 1 || !gbps) return DFN_ERR_ARG;
+  int st = device_ok();
+  if (st) return st;
+  bytes &= ~(int64_t)15;
+  dfn::DMem a, b;
+  DFN_TRY(a.alloc((size_t)bytes));
+  DFN_TRY(b.alloc((size_t)bytes));
+  hipStream_t s;
+  hipEvent_t e0, e1;
+  DFN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  DFN_HIP(hipEventCreate(&e0));
+  DFN_HIP(hipEventCreate(&e1));
+  DFN_HIP(hipMemsetAsync(a.p, 0, (size_t)bytes, s));
+  dfn::launch_stream_copy(a.p, b.p, bytes, s);
+  DFN_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; r++) dfn::launch_stream_copy(a.p, b.p, bytes, s);
+  DFN_HIP(hipEventRecord(e1, s));
+  DFN_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  DFN_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * (double)bytes / (ms * 1e-3 / reps) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  return DFN_OK;
+}
+
 API int dfn_device_info(char *buf, int32_t len) {
   if (!buf || len < 1) return DFN_ERR_ARG;
   int n = 0;
@@ -858,8 +927,8 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
   dfn::launch_gather(dV, d->perm.as<int>(), w, N, e.stream);
   dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), e.stream);
   dfn::launch_zreduce(d->zred.as<dfn::ZRed>(), d->nzr, d->zpart.as<double>(), w, e.stream);
-  dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), w,
-               d->ypart.as<double>(), e.stream);
+  dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(),
+                    d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream);
   dfn::launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                   d->nops, 1.0, dT, d->nops > 1 ? dN : dT, e.stream);
   DFN_HIP(hipMemcpyAsync(yT.data(), op == 0 ? dT : dN, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
@@ -953,8 +1022,8 @@ API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
     double *w = d->w.as<double>();
     DFN_HIP(hipEventRecord(e.ev0, e.stream));
     for (int r = 0; r < reps; r++)
-      dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), w,
-                        d->ypart.as<double>(), e.stream);
+      dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(),
+                        d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream);
     DFN_HIP(hipEventRecord(e.ev1, e.stream));
     DFN_HIP(hipEventSynchronize(e.ev1));
     DFN_HIP(hipEventElapsedTime(&t, e.ev0, e.ev1));

@@ -49,31 +49,32 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 template <int K, typename TV>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
                                          double *__restrict__ zpart, double *__restrict__ wz, int lane) {
-  double acc[K];
+  // FP64 storage accumulates in FP64; FP32 storage in FP32 (TV)
+  TV acc[K];
 #pragma unroll
-  for (int j = 0; j < K; j++) acc[j] = 0.0;
+  for (int j = 0; j < K; j++) acc[j] = (TV)0;
   const TV *V = vdata + task.voff + task.row0;
   const double *x = w + task.xcol + task.row0;
   const int64_t ld = task.ld;
   for (int i = lane; i < task.len; i += 64) {
-    const double xi = x[i];
+    const TV xi = (TV)x[i];
     TV v[K];
 #pragma unroll
     for (int j = 0; j < K; j++) v[j] = V[i + j * ld];
 #pragma unroll
-    for (int j = 0; j < K; j++) acc[j] += (double)v[j] * xi;
+    for (int j = 0; j < K; j++) acc[j] += v[j] * xi;
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
     for (int j = 0; j < K; j++) acc[j] += __shfl_xor(acc[j], off, 64);
   }
-  double mine = 0.0;
+  TV mine = (TV)0;
 #pragma unroll
   for (int j = 0; j < K; j++)
     if (lane == j) mine = acc[j];
   if (lane < K) {
-    const double v = mine * (double)task.scale;
+    const double v = (double)mine * task.scale;
     if (task.wout >= 0) wz[task.wout + lane] = v;
     else zpart[task.zp + lane] = v;
   }
@@ -118,39 +119,73 @@ __global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ z
 // Phase 2: one workgroup per panel task.  w entries of the panel's columns are
 // gathered into LDS once; each wave then streams whole rows (row-major panel,
 // coalesced 512 B per wave load), two rows at a time, and reduces per row.
-template <typename TP>
-__global__ void __launch_bounds__(256) panel_kernel(const PTask *__restrict__ tasks, const TP *__restrict__ pdata,
-                                                    const int *__restrict__ widx, const double *__restrict__ w,
-                                                    double *__restrict__ ypart) {
+__global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict__ tasks, const double *__restrict__ pdata,
+                                                        const int *__restrict__ widx, const double *__restrict__ w,
+                                                        double *__restrict__ ypart) {
   __shared__ double wl[kPanelMaxCols];
   const PTask task = tasks[blockIdx.x];
   for (int j = threadIdx.x; j < task.L; j += 256) wl[j] = w[widx[task.widx + j]];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const TP *P = pdata + task.poff;
+  const double *P = pdata + task.poff;
   const int L = task.L;
   int r = wave * 2;
   for (; r + 1 < task.m; r += 8) {
-    const TP *row0 = P + (int64_t)r * L, *row1 = row0 + L;
+    const double *row0 = P + (int64_t)r * L, *row1 = row0 + L;
     double a0 = 0.0, a1 = 0.0;
     for (int j = lane; j < L; j += 64) {
       const double wj = wl[j];
-      a0 += (double)row0[j] * wj;
-      a1 += (double)row1[j] * wj;
+      a0 += row0[j] * wj;
+      a1 += row1[j] * wj;
     }
     a0 = wave_sum(a0);
     a1 = wave_sum(a1);
     if (lane == 0) {
-      ypart[task.yp + r] = a0 * (double)task.scale;
-      ypart[task.yp + r + 1] = a1 * (double)task.scale;
+      ypart[task.yp + r] = a0;
+      ypart[task.yp + r + 1] = a1;
     }
   }
   if (r < task.m) {
-    const TP *row0 = P + (int64_t)r * L;
+    const double *row0 = P + (int64_t)r * L;
     double a0 = 0.0;
-    for (int j = lane; j < L; j += 64) a0 += (double)row0[j] * wl[j];
+    for (int j = lane; j < L; j += 64) a0 += row0[j] * wl[j];
     a0 = wave_sum(a0);
-    if (lane == 0) ypart[task.yp + r] = a0 * (double)task.scale;
+    if (lane == 0) ypart[task.yp + r] = a0;
+  }
+}
+
+// FP32 storage: entries are block / 2^e per block, the column scales 2^e fold into
+// the gathered input; FP32 accumulation, with a compensated (Kahan) sum over the
+// near-field columns j < nd of each row.  Lane partials are combined in FP64.
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+__global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict__ tasks, const float *__restrict__ pdata,
+                                                        const int *__restrict__ widx, const float *__restrict__ cscale,
+                                                        const double *__restrict__ w, double *__restrict__ ypart) {
+  __shared__ float wl[kPanelMaxCols];
+  const PTask task = tasks[blockIdx.x];
+  for (int j = threadIdx.x; j < task.L; j += 256)
+    wl[j] = (float)(w[widx[task.widx + j]] * (double)cscale[task.widx + j]);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float *P = pdata + task.poff;
+  const int L = task.L, nd = task.nd;
+  for (int r = wave; r < task.m; r += 4) {
+    const float *row = P + (int64_t)r * L;
+    float s = 0.0f, c = 0.0f, a = 0.0f;
+    for (int j = lane; j < L; j += 64) {
+      const float v = row[j] * wl[j];
+      if (j < nd) kahan_add(s, c, v);
+      else a += v;
+    }
+    double tot = ((double)s - (double)c) + (double)a;
+    tot = wave_sum(tot);
+    if (lane == 0) ypart[task.yp + r] = tot;
   }
 }
 
@@ -817,13 +852,13 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
   hipLaunchKernelGGL(zreduce_kernel, dim3(nblk((int64_t)nb * 32, 256)), dim3(256), 0, s, zr, nb, zpart, w);
 }
 
-void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const double *w,
-                  double *ypart, hipStream_t s) {
+void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
+                  const double *w, double *ypart, hipStream_t s) {
   if (nt <= 0) return;
   if (fp32)
-    hipLaunchKernelGGL(panel_kernel<float>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, w, ypart);
+    hipLaunchKernelGGL(panel_kernel_f32, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart);
   else
-    hipLaunchKernelGGL(panel_kernel<double>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
+    hipLaunchKernelGGL(panel_kernel_f64, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
 }
 
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
@@ -862,6 +897,17 @@ void launch_restart_stage2(const double *y, const double *k1, double h, int n3, 
 
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s) {
   hipLaunchKernelGGL(cg_kernel, dim3(nsys), dim3(1024), 0, s, a);
+}
+
+__global__ void __launch_bounds__(256) stream_copy_kernel(const double2 *__restrict__ a, double2 *__restrict__ b,
+                                                          int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s) {
+  const int64_t n = bytes / 16;
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
 }
 
 bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *partial, int *sysflag, hipStream_t s) {

@@ -32,6 +32,13 @@ def device_info():
     return buf.value.decode()
 
 
+def bench_stream(nbytes=1 << 30, reps=20):
+    """GB/s (read + write) of a plain copy kernel over nbytes on the device."""
+    g = np.zeros(1)
+    _abi.check(load()["bench_stream"](int(nbytes), int(reps), _abi.dptr(g)), "dfn_bench_stream")
+    return float(g[0])
+
+
 class DeviceSimulation(Simulation):
     """A problem resident on the MI355X: operators, state and solver loop on the device."""
 

@@ -170,6 +170,10 @@ int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats
  * algorithmic bytes per application, ms[3] = algorithmic bytes of the dominant
  * kernel per launch.  ms must hold 4 doubles. */
 int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms);
+/* Measurement helper: a plain device-to-device copy kernel over `bytes` bytes,
+ * run `reps` times on a private stream; *gbps = 2*bytes / mean time (read + write).
+ * Used to scale hardware traffic counters and to state the achievable HBM rate. */
+int dfn_bench_stream(int64_t bytes, int32_t reps, double *gbps);
 /* Device/library description string (NUL-terminated, at most len bytes). */
 int dfn_device_info(char *buf, int32_t len);
 

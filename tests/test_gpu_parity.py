@@ -155,3 +155,39 @@ def test_hmatrix_solver_trajectory_mode2():
     assert sd["rejected_attempts"] == so["rejected_attempts"]
     assert close_elementwise(Yd[:N][real], Yo[:N][real], 1e-8, 1e-300)
     assert close_elementwise(Yd[N:2 * N][real], Yo[N:2 * N][real], 1e-8, 1e-300)
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_fp32_operator_storage_error(mode):
+    """Precision form: FP32 factors with per-block scaling vs the FP64 operator, on
+    log-uniform slip rates in [1e-12, 1] m/s; the bar is 1e-5 relative."""
+    p = _hnet(mode, n_faults=20, seed=9)
+    d = dev(p)
+    d.initiate_hmat()
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        V = 10.0 ** rng.uniform(-12, 0, p.N)
+        d.set_precision(_abi.PREC_FP64)
+        a = d.kernel(V)
+        d.set_precision(_abi.PREC_FP32)
+        b = d.kernel(V)
+        for y64, y32 in zip(a, b):
+            if np.linalg.norm(y64) > 0:
+                err = rel(y32, y64)
+                assert err <= 1e-5, err
+                assert err > 0  # really a different storage
+
+
+def test_fp32_operator_trajectory_tracks_fp64():
+    p = _hnet(2)
+    a, b = dev(p), dev(p)
+    a.initiate_hmat()
+    b.initiate_hmat()
+    b.set_precision(_abi.PREC_FP32)
+    a.solve(15)
+    b.solve(15)
+    Ya, Yb = a.get_state()[0], b.get_state()[0]
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    assert close_elementwise(Yb[:N][real], Ya[:N][real], 1e-4, 1e-300)
