@@ -23,6 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs plain FP64"
+# HBM bytes per launch of the dominant kernel, from two rocprofv3 PMC runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a 1 GiB copy kernel;
+# summary and commands in profiles/pmc_traffic_network80k_fp64.md.
+MEASURED_TRAFFIC = {("network800_80k", 2024): 2.496313e9}
 YEAR = 365.25 * 86400.0
 HBM_PEAK_GBS = 8000.0
 
@@ -167,7 +171,8 @@ def main():
         ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
         data="synthetic (seeded generator)", config=desc,
         roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=None, kernel="panel_kernel<double> (H-matvec phase 2)",
+                      traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed)),
+                      kernel="panel_kernel_f64 (H-matvec phase 2)",
                       bytes_per_launch=kb["dominant_bytes"], ms_per_launch=kb["ms_dominant"]),
         hmatvec=dict(ms=kb["ms_kernel"], alg_bytes=kb["alg_bytes"], GB_s=whole, frac_hbm=whole / HBM_PEAK_GBS, **hm),
         copy_kernel_GB_s=copy_gbs,
