@@ -169,6 +169,7 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
     }
   }
   pdata.reserve(reserve);
+  const int64_t panel_entries = reserve;  // dense + U entries, every one stored once
   std::vector<std::vector<int>> fin((size_t)n * nops);
   int yp = 0;
   for (int o = 0; o < nops; o++) {
@@ -201,21 +202,24 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
             cols.push_back({zbase[o] + zoff[o][b] + j, U + (int64_t)j * H.l_m[b], sc});
         }
       }
+      // rows padded to 16 bytes (2 doubles / 4 floats) with zero entries on column 0 of w
+      const int pad = 16 / (int)sizeof(T);
       for (size_t c0 = 0; c0 < cols.size(); c0 += kPanelMaxCols) {
         const int L = (int)std::min<size_t>(kPanelMaxCols, cols.size() - c0);
+        const int Lp = (L + pad - 1) / pad * pad;
         PTask t{};
         t.poff = (int64_t)pdata.size();
         t.widx = (int64_t)widx.size();
         t.m = m;
-        t.L = L;
+        t.L = Lp;
         t.yp = yp;
         t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, (int64_t)ndense - (int64_t)c0));
-        for (int j = 0; j < L; j++) {
-          widx.push_back((int)cols[c0 + j].wi);
-          if (scaled) cscale.push_back((float)cols[c0 + j].scale);
+        for (int j = 0; j < Lp; j++) {
+          widx.push_back(j < L ? (int)cols[c0 + j].wi : 0);
+          if (scaled) cscale.push_back(j < L ? (float)cols[c0 + j].scale : 1.0f);
         }
         for (int r = 0; r < m; r++)
-          for (int j = 0; j < L; j++) pdata.push_back((T)(cols[c0 + j].src[r] / cols[c0 + j].scale));
+          for (int j = 0; j < Lp; j++) pdata.push_back(j < L ? (T)(cols[c0 + j].src[r] / cols[c0 + j].scale) : (T)0);
         for (int r = 0; r < m; r++) fin[(size_t)o * n + r0 + r].push_back(yp + r);
         yp += m;
         pt.push_back(t);
@@ -248,7 +252,7 @@ static int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   d.S_total = S_total;
   // bytes: factors + x read + y written, per operator (x shared in Mode II)
   d.alg_bytes = (double)sizeof(T) * (double)S_total + 8.0 * n + 8.0 * n * nops;
-  d.panel_bytes = (double)sizeof(T) * (double)pdata.size() + 4.0 * (double)cscale.size() + 8.0 * (double)yp;
+  d.panel_bytes = (double)sizeof(T) * (double)panel_entries + 4.0 * (double)cscale.size() + 8.0 * (double)yp;
   return DFN_OK;
 }
 

@@ -117,26 +117,28 @@ __global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ z
 }
 
 // Phase 2: one workgroup per panel task.  w entries of the panel's columns are
-// gathered into LDS once; each wave then streams whole rows (row-major panel,
-// coalesced 512 B per wave load), two rows at a time, and reduces per row.
+// gathered into LDS once; each wave then streams two whole rows at a time
+// (row-major panel, rows padded to a multiple of 2 doubles / 4 floats so every
+// lane issues one 16-byte load per row: 1 KiB per wave instruction).
 __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict__ tasks, const double *__restrict__ pdata,
                                                         const int *__restrict__ widx, const double *__restrict__ w,
                                                         double *__restrict__ ypart) {
-  __shared__ double wl[kPanelMaxCols];
+  __shared__ double2 wl2[kPanelMaxCols / 2];
+  double *wl = reinterpret_cast<double *>(wl2);
   const PTask task = tasks[blockIdx.x];
   for (int j = threadIdx.x; j < task.L; j += 256) wl[j] = w[widx[task.widx + j]];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const double *P = pdata + task.poff;
-  const int L = task.L;
+  const int L2 = task.L >> 1;
+  const double2 *P = reinterpret_cast<const double2 *>(pdata + task.poff);
   int r = wave * 2;
   for (; r + 1 < task.m; r += 8) {
-    const double *row0 = P + (int64_t)r * L, *row1 = row0 + L;
+    const double2 *row0 = P + (int64_t)r * L2, *row1 = row0 + L2;
     double a0 = 0.0, a1 = 0.0;
-    for (int j = lane; j < L; j += 64) {
-      const double wj = wl[j];
-      a0 += row0[j] * wj;
-      a1 += row1[j] * wj;
+    for (int j = lane; j < L2; j += 64) {
+      const double2 wv = wl2[j], v0 = row0[j], v1 = row1[j];
+      a0 += v0.x * wv.x + v0.y * wv.y;
+      a1 += v1.x * wv.x + v1.y * wv.y;
     }
     a0 = wave_sum(a0);
     a1 = wave_sum(a1);
@@ -146,9 +148,12 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
     }
   }
   if (r < task.m) {
-    const double *row0 = P + (int64_t)r * L;
+    const double2 *row0 = P + (int64_t)r * L2;
     double a0 = 0.0;
-    for (int j = lane; j < L; j += 64) a0 += row0[j] * wl[j];
+    for (int j = lane; j < L2; j += 64) {
+      const double2 wv = wl2[j], v0 = row0[j];
+      a0 += v0.x * wv.x + v0.y * wv.y;
+    }
     a0 = wave_sum(a0);
     if (lane == 0) ypart[task.yp + r] = a0;
   }
@@ -164,28 +169,53 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
   s = t;
 }
 
+__device__ __forceinline__ void acc4(float4 v, float4 wv, int j, int nd, float &s, float &c, float &a) {
+  if (j + 3 < nd) {
+    kahan_add(s, c, v.x * wv.x);
+    kahan_add(s, c, v.y * wv.y);
+    kahan_add(s, c, v.z * wv.z);
+    kahan_add(s, c, v.w * wv.w);
+  } else if (j >= nd) {
+    a += v.x * wv.x + v.y * wv.y + v.z * wv.z + v.w * wv.w;
+  } else {
+    const float p[4] = {v.x * wv.x, v.y * wv.y, v.z * wv.z, v.w * wv.w};
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (j + q < nd) kahan_add(s, c, p[q]);
+      else a += p[q];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict__ tasks, const float *__restrict__ pdata,
                                                         const int *__restrict__ widx, const float *__restrict__ cscale,
                                                         const double *__restrict__ w, double *__restrict__ ypart) {
-  __shared__ float wl[kPanelMaxCols];
+  __shared__ float4 wl4[kPanelMaxCols / 4];
+  float *wl = reinterpret_cast<float *>(wl4);
   const PTask task = tasks[blockIdx.x];
   for (int j = threadIdx.x; j < task.L; j += 256)
     wl[j] = (float)(w[widx[task.widx + j]] * (double)cscale[task.widx + j]);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float *P = pdata + task.poff;
-  const int L = task.L, nd = task.nd;
-  for (int r = wave; r < task.m; r += 4) {
-    const float *row = P + (int64_t)r * L;
-    float s = 0.0f, c = 0.0f, a = 0.0f;
-    for (int j = lane; j < L; j += 64) {
-      const float v = row[j] * wl[j];
-      if (j < nd) kahan_add(s, c, v);
-      else a += v;
+  const int L4 = task.L >> 2, nd = task.nd;
+  const float4 *P = reinterpret_cast<const float4 *>(pdata + task.poff);
+  for (int r = wave * 2; r < task.m; r += 8) {
+    const bool two = r + 1 < task.m;
+    const float4 *row0 = P + (int64_t)r * L4, *row1 = row0 + (two ? L4 : 0);
+    float s0 = 0.0f, c0 = 0.0f, a0 = 0.0f, s1 = 0.0f, c1 = 0.0f, a1 = 0.0f;
+    for (int j = lane; j < L4; j += 64) {
+      const float4 wv = wl4[j], v0 = row0[j], v1 = row1[j];
+      acc4(v0, wv, 4 * j, nd, s0, c0, a0);
+      acc4(v1, wv, 4 * j, nd, s1, c1, a1);
     }
-    double tot = ((double)s - (double)c) + (double)a;
-    tot = wave_sum(tot);
-    if (lane == 0) ypart[task.yp + r] = tot;
+    double t0 = ((double)s0 - (double)c0) + (double)a0;
+    double t1 = ((double)s1 - (double)c1) + (double)a1;
+    t0 = wave_sum(t0);
+    t1 = wave_sum(t1);
+    if (lane == 0) {
+      ypart[task.yp + r] = t0;
+      if (two) ypart[task.yp + r + 1] = t1;
+    }
   }
 }
 
