@@ -29,6 +29,7 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 MEASURED_TRAFFIC = {("network800_80k", 2024): 2.496313e9}
 YEAR = 365.25 * 86400.0
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
 
 
 def log(*a):
@@ -53,10 +54,32 @@ def make_workload(name, seed):
     return p, desc
 
 
+def cpu_baseline_reference(sim, p, seconds):
+    """The reference's own Fortran modules (oracle/_ref, built with the reference's
+    -O1 serial flags) continuing from the device's state, timed for ~`seconds`."""
+    from oracle.oracle import RefSimulation
+    r = RefSimulation(p)
+    Y, P, _, t = sim.get_state()
+    r.set_state(Y, P, t)
+    r.set_t_try(sim.stats()["t_try"])
+    steps = 0
+    w0 = time.perf_counter()
+    while time.perf_counter() - w0 < seconds:
+        steps += r.solve(1)
+    wall = time.perf_counter() - w0
+    sim_dt = r.get_state()[3] - t
+    return dict(value=(sim_dt / YEAR) / (wall / 3600.0), unit="simulated yr / wall hr", cores=1, kind="reference",
+                sample=f"{steps} accepted steps of the reference Fortran (directIII, friction law, compute_P, rkqs; "
+                       f"-O1, one thread) continuing from the GPU's final state ({wall:.1f} s wall, {sim_dt:.3e} s "
+                       f"simulated)")
+
+
 def cpu_baseline(sim, p, seconds, threads):
     """The oracle (plain-C restatement) continuing from the device's state on the
     same operator block lists, timed for ~`seconds` of wall time."""
-    from oracle.oracle import OracleSimulation
+    from oracle.oracle import OracleSimulation, RefSimulation
+    if p.static_kernel == 0 and RefSimulation.available():
+        return cpu_baseline_reference(sim, p, seconds)
     o = OracleSimulation(p, threads=threads)
     if p.static_kernel == 1:
         for op in ((0, 1) if p.fracture_mode == 2 else (0,)):
@@ -163,18 +186,30 @@ def main():
     value = (sim_dt / YEAR) / (wall / 3600.0)
     log(f"[bench] {args.steps} steps in {wall:.2f} s, simulated {sim_dt:.3e} s")
     kb = sim.bench_kernel(reps=20)
+    dominant = "panel_kernel_f64 (H-matvec phase 2)" if p.static_kernel == 1 else (
+        "direct3_tile_kernel + reduce (dense dislocation kernel evaluated on the fly, 11 flops per "
+        "element-node pair)" if p.fracture_mode == 3 else
+        "direct2_tile_kernel + reduces (dense dislocation kernel evaluated on the fly, 40 flops per pair)")
     copy_gbs = bench_stream(int(args.copy_probe_gb * (1 << 30)), 20) if args.copy_probe_gb > 0 else None
     achieved = kb["dominant_bytes"] / (kb["ms_dominant"] * 1e-3) / 1e9
     whole = kb["alg_bytes"] / (kb["ms_kernel"] * 1e-3) / 1e9
+    if p.static_kernel == 1:
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed)), kernel=dominant,
+                    bytes_per_launch=kb["dominant_bytes"], ms_per_launch=kb["ms_dominant"])
+        matvec = dict(ms=kb["ms_kernel"], alg_bytes=kb["alg_bytes"], GB_s=whole, frac_hbm=whole / HBM_PEAK_GBS, **hm)
+    else:
+        tflops = kb["dominant_bytes"] / (kb["ms_dominant"] * 1e-3) / 1e12
+        roof = dict(bound="valu-fp64", achieved=tflops, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=tflops / FP64_PEAK_TFLOPS, traffic=None, kernel=dominant,
+                    flops_per_launch=kb["dominant_bytes"], ms_per_launch=kb["ms_dominant"])
+        matvec = dict(ms=kb["ms_kernel"], flops=kb["alg_bytes"], TFLOP_s=tflops)
     out = dict(
         metric=METRIC, value=value, unit="simulated yr / wall hr", n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
         data="synthetic (seeded generator)", config=desc,
-        roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                      traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed)),
-                      kernel="panel_kernel_f64 (H-matvec phase 2)",
-                      bytes_per_launch=kb["dominant_bytes"], ms_per_launch=kb["ms_dominant"]),
-        hmatvec=dict(ms=kb["ms_kernel"], alg_bytes=kb["alg_bytes"], GB_s=whole, frac_hbm=whole / HBM_PEAK_GBS, **hm),
+        roofline=roof,
+        hmatvec=matvec,
         copy_kernel_GB_s=copy_gbs,
         run=dict(accepted=s1["accepted_steps"] - s0["accepted_steps"],
                  rejected=s1["rejected_attempts"] - s0["rejected_attempts"],

@@ -101,6 +101,8 @@ struct CrossRows {
   const int *ent;    // [2*entries] (crossing index, slot)
 };
 
+constexpr int kCgLdsThreads = 512;          // threads of the one-workgroup-per-system CG
+constexpr int kCgLdsMax = kCgLdsThreads * 8;  // largest system it takes (elements)
 constexpr int kCgEPT = 8;                   // elements per thread in the cooperative CG
 constexpr int kCgSpan = kCgEPT * 256;       // elements per workgroup
 
@@ -118,14 +120,20 @@ void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const in
                   const double *w, double *ypart, hipStream_t s);
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
                      double scale, double *yT, double *yN, hipStream_t s);
-void launch_direct3(const double *node, const double *slip, double *cbuf, int n, double scale, double *sT,
+// dense direct kernels: node columns per tile, number of tiles along the nodes,
+// and the partial-sum buffer each launch needs (doubles): direct_chunks(n) * n
+// for Mode III, twice that for Mode II
+constexpr int kDirectCols = 64;
+int direct_chunks(int n);
+void launch_direct3(const double *node, const double *slip, double *part, int n, double scale, double *sT,
                     hipStream_t s);
-void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, int n, double scale,
-                    double *sT, double *sN, hipStream_t s);
+void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, double *part, int n,
+                    double scale, double *sT, double *sN, hipStream_t s);
 void launch_stage(int stage, const StageArgs &a, hipStream_t s);
 void launch_restart_stage2(const double *y, const double *k1, double h, int n3, double *yt, hipStream_t s);
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
 // cooperative multi-workgroup CG; returns false if the grid cannot be co-resident
+bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s);
 bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *partial, int *sysflag, hipStream_t s);
 void launch_perm_update(const PermArgs &a, hipStream_t s);
 void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -265,12 +266,18 @@ struct Engine {
   std::unique_ptr<DevOps> ops64, ops32;
   // problem data on the device
   DMem node, elem, nrm, ds, a, b, Dc, f0, V0, nld, sld, fake, perm_star, kmin, Snk, src_elem, src_Q, src_tb, src_te,
-      cross, dcbuf;
+      cross, dpart;
   DMem perm_cur, perm_next, permx_cur, permx_next;
   // state and stage buffers
   DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot, scratch3, scratch3b;
   DMem cg_work, cg_sys, cg_res, cg_partial, cg_flags, xrow_start, xrow_ent;
   bool coop_ok = true;
+  // compute_P kernel choice: 0 = by size (one workgroup per system up to
+  // kCgLdsMax elements, else cooperative, else the plain one-workgroup kernel);
+  // 1/2/3 = prefer the LDS / cooperative / plain kernel.  Set from the
+  // environment variable DFN_CG_PATH ("lds", "coop", "plain") at dfn_create, so
+  // that every kernel can be checked on small problems.
+  int cg_path = 0;
   DMem status;
   StatusWord *h_status = nullptr;  // pinned
   int *h_cgres = nullptr;          // pinned [kSlots*2]
@@ -299,6 +306,10 @@ struct Engine {
   }
 
   int init() {
+    if (const char *cp = std::getenv("DFN_CG_PATH")) {
+      const std::string v(cp);
+      cg_path = v == "lds" ? 1 : v == "coop" ? 2 : v == "plain" ? 3 : 0;
+    }
     int dev = 0;
     DFN_HIP(hipGetDevice(&dev));
     DFN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -325,7 +336,8 @@ struct Engine {
     DFN_TRY(src_tb.upload(hp.src_tb));
     DFN_TRY(src_te.upload(hp.src_te));
     DFN_TRY(cross.upload(hp.cross));
-    DFN_TRY(dcbuf.alloc(sizeof(double) * (N + 1)));
+    if (hp.kernel == DFN_KERNEL_DIRECT)
+      DFN_TRY(dpart.alloc(sizeof(double) * (size_t)direct_chunks(N) * N * (hp.mode == DFN_MODE_II ? 2 : 1)));
     // permeability_x (initialisation.f90:252-266): perm_x[k] = hmean(perm[k-1], perm[k]), k = 1..N-1
     std::vector<double> px(N + 1, 0.0);
     for (int kk = 1; kk < N; kk++)
@@ -410,10 +422,10 @@ struct Engine {
     const int N = hp.N;
     if (hp.kernel == DFN_KERNEL_DIRECT) {
       if (hp.mode == DFN_MODE_III)
-        launch_direct3(node.as<double>(), d_V, dcbuf.as<double>(), N, kernel_scale(), d_sT, stream);
+        launch_direct3(node.as<double>(), d_V, dpart.as<double>(), N, kernel_scale(), d_sT, stream);
       else
-        launch_direct2(node.as<double>(), elem.as<double>(), nrm.as<double>(), d_V, N, kernel_scale(), d_sT, d_sN,
-                       stream);
+        launch_direct2(node.as<double>(), elem.as<double>(), nrm.as<double>(), d_V, dpart.as<double>(), N,
+                       kernel_scale(), d_sT, d_sN, stream);
       return DFN_OK;
     }
     const DevOps *d = active_ops();
@@ -477,7 +489,11 @@ struct Engine {
     A.ncross = hp.ncross();
     A.cross = cross.as<int64_t>();
     bool launched = false;
-    if (coop_ok) {
+    if (N <= kCgLdsMax && cg_path <= 1) {
+      CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
+      launched = launch_cg_lds(A, X, nsys, stream);
+    }
+    if (!launched && coop_ok && cg_path != 3) {
       CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
       launched = launch_cg_coop(A, X, nsys, cg_partial.as<double>() + 0, cg_flags.as<int>(), stream);
       if (!launched) coop_ok = false;  // grid too large to be co-resident: one workgroup per system
@@ -1035,9 +1051,12 @@ API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
     ms[2] = d->alg_bytes;
     ms[3] = d->panel_bytes;
   } else {
+    // the direct kernels are FP64-arithmetic bound: report algorithmic flops per
+    // application, counting sqrt and divide as one each (Mode III: 11 per
+    // element-node pair; Mode II: 40 per pair for the two outputs)
     ms[1] = ms[0];
     const double N = e.hp.N;
-    ms[2] = 8.0 * N * N + 16.0 * N;  // dense operator bytes (the direct kernel recomputes them)
+    ms[2] = (e.hp.mode == DFN_MODE_III ? 11.0 : 40.0) * N * (N + 1.0);
     ms[3] = ms[2];
   }
   return DFN_OK;

@@ -33,6 +33,46 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// Wave reductions through DPP lane moves, for call sites where all 64 lanes are
+// active: xor-1 and xor-2 quad permutes, the half-row and row mirrors give every
+// lane its 16-lane row total (a + b and b + a are the same double, so the lanes
+// of a row agree bit for bit); the four row totals are then read from lanes 0,
+// 16, 32, 48 and added in that order.  Every lane returns the same value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+__device__ __forceinline__ void wave_sum_max_full(double &sum, double &mx) {
+  sum += dpp_move<kDppXor1>(sum);
+  mx = fmax(mx, dpp_move<kDppXor1>(mx));
+  sum += dpp_move<kDppXor2>(sum);
+  mx = fmax(mx, dpp_move<kDppXor2>(mx));
+  sum += dpp_move<kDppHalfMirror>(sum);
+  mx = fmax(mx, dpp_move<kDppHalfMirror>(mx));
+  sum += dpp_move<kDppMirror>(sum);
+  mx = fmax(mx, dpp_move<kDppMirror>(mx));
+  sum = (lane_value(sum, 0) + lane_value(sum, 16)) + (lane_value(sum, 32) + lane_value(sum, 48));
+  mx = fmax(fmax(lane_value(mx, 0), lane_value(mx, 16)), fmax(lane_value(mx, 32), lane_value(mx, 48)));
+}
+
+__device__ __forceinline__ double wave_sum_full(double sum) {
+  sum += dpp_move<kDppXor1>(sum);
+  sum += dpp_move<kDppXor2>(sum);
+  sum += dpp_move<kDppHalfMirror>(sum);
+  sum += dpp_move<kDppMirror>(sum);
+  return (lane_value(sum, 0) + lane_value(sum, 16)) + (lane_value(sum, 32) + lane_value(sum, 48));
+}
+
 // ---------------------------------------------------------------------------
 // H-matvec
 // ---------------------------------------------------------------------------
@@ -237,86 +277,109 @@ __global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ f
 // Direct kernels (src/kernel_calculation/directModeIII.f:7-74, directModeII.f90:7-107)
 // ---------------------------------------------------------------------------
 // c_l: weights of the node dislocations: l=0: -s_1, 1..N-1: -(s_{l+1}-s_l), N: +s_N
-__global__ void __launch_bounds__(256) dislocation_weights_kernel(const double *__restrict__ slip,
-                                                                  double *__restrict__ c, int n) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l > n) return;
-  if (l == 0) c[0] = -slip[0];
-  else if (l == n) c[n] = slip[n - 1];
-  else c[l] = -(slip[l] - slip[l - 1]);
+// Dense direct kernels (src/kernel_calculation/directModeIII.f:7-74,
+// directModeII.f90:7-107).  Both sum over the N+1 nodes rather than the N
+// elements: the reference adds each element's contribution as (left node term)
+// - (right node term), so the node terms are gathered once per node with the
+// slip differences of the two elements sharing it ("dislocation weights").
+// The N x (N+1) node plane is tiled as 256 receiver rows x kDirectCols nodes
+// per workgroup, so a 4000-element fault still launches ~1000 workgroups; each
+// tile writes one partial per row and direct_reduce_kernel sums the partials of
+// a row in a fixed order (results do not depend on scheduling).  The weights
+// are formed from the slip as the tiles load their nodes.
+// ---------------------------------------------------------------------------
+// Mode III weight of node l: -(s_l - s_{l-1}) with s_{-1} = s_N = 0
+__device__ __forceinline__ double dislocation_weight3(const double *__restrict__ slip, int l, int n) {
+  if (l == 0) return -slip[0];
+  if (l == n) return slip[n - 1];
+  return -(slip[l] - slip[l - 1]);
 }
 
-__global__ void __launch_bounds__(256) direct3_kernel(const double *__restrict__ node, const double *__restrict__ c,
-                                                      int n, double scale, double *__restrict__ sT) {
-  __shared__ double sx[256], sy[256], sc[256];
+__global__ void __launch_bounds__(256) direct3_tile_kernel(const double *__restrict__ node,
+                                                           const double *__restrict__ slip, int n,
+                                                           double *__restrict__ part) {
+  __shared__ double sx[kDirectCols], sy[kDirectCols], sc[kDirectCols];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double ex = 0, ey = 0, t1 = 0, t2 = 0;
-  if (k < n) {
-    const double x0 = node[2 * k], y0 = node[2 * k + 1], x1 = node[2 * k + 2], y1 = node[2 * k + 3];
-    double tx = x1 - x0, ty = y1 - y0;
-    const double nt = sqrt(tx * tx + ty * ty);
-    t1 = tx / nt;
-    t2 = ty / nt;
-    ex = x0 + 0.5 * (x1 - x0);
-    ey = y0 + 0.5 * (y1 - y0);
+  const int j0 = blockIdx.y * kDirectCols, cnt = min(kDirectCols, n + 1 - j0);
+  if (threadIdx.x < cnt) {
+    const int l = j0 + threadIdx.x;
+    sx[threadIdx.x] = node[2 * l];
+    sy[threadIdx.x] = node[2 * l + 1];
+    sc[threadIdx.x] = dislocation_weight3(slip, l, n);
   }
+  __syncthreads();
+  if (k >= n) return;
+  const double x0 = node[2 * k], y0 = node[2 * k + 1], x1 = node[2 * k + 2], y1 = node[2 * k + 3];
+  const double tx = x1 - x0, ty = y1 - y0;
+  const double nt = sqrt(tx * tx + ty * ty);
+  const double t1 = tx / nt, t2 = ty / nt;
+  const double ex = x0 + 0.5 * (x1 - x0), ey = y0 + 0.5 * (y1 - y0);
   double s = 0.0;
-  for (int base = 0; base <= n; base += 256) {
-    const int l = base + threadIdx.x;
-    if (l <= n) { sx[threadIdx.x] = node[2 * l]; sy[threadIdx.x] = node[2 * l + 1]; sc[threadIdx.x] = c[l]; }
-    __syncthreads();
-    const int cnt = min(256, n + 1 - base);
-    for (int q = 0; q < cnt; q++) {
-      const double dx = ex - sx[q], dy = ey - sy[q];
-      const double r2 = dx * dx + dy * dy;
-      s += sc[q] * ((t1 * dx + t2 * dy) / r2);
-    }
-    __syncthreads();
+  for (int q = 0; q < cnt; q++) {
+    const double dx = ex - sx[q], dy = ey - sy[q];
+    const double r2 = dx * dx + dy * dy;
+    s += sc[q] * ((t1 * dx + t2 * dy) / r2);
   }
-  if (k < n) sT[k] = s * scale;
+  part[(int64_t)blockIdx.y * n + k] = s;
 }
 
-__global__ void __launch_bounds__(256) direct2_kernel(const double *__restrict__ node, const double *__restrict__ elem,
-                                                      const double *__restrict__ nrm, const double *__restrict__ slip,
-                                                      int n, double scale, double *__restrict__ sT,
-                                                      double *__restrict__ sN) {
-  __shared__ double sx[257], sy[257], snx[256], sny[256], ss[256];
+// Mode II weights of node l: w1 = s_l m1_l - s_{l-1} m1_{l-1}, w2 likewise with m2
+// (m = element normal, s = slip; s_{-1} = s_N = 0)
+__global__ void __launch_bounds__(256) direct2_tile_kernel(const double *__restrict__ node,
+                                                           const double *__restrict__ elem,
+                                                           const double *__restrict__ nrm,
+                                                           const double *__restrict__ slip, int n,
+                                                           double *__restrict__ partT, double *__restrict__ partN) {
+  __shared__ double sx[kDirectCols], sy[kDirectCols], sw1[kDirectCols], sw2[kDirectCols];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  double ex = 0, ey = 0, n1 = 0, n2 = 0;
-  if (k < n) { ex = elem[2 * k]; ey = elem[2 * k + 1]; n1 = nrm[2 * k]; n2 = nrm[2 * k + 1]; }
+  const int j0 = blockIdx.y * kDirectCols, cnt = min(kDirectCols, n + 1 - j0);
+  if (threadIdx.x < cnt) {
+    const int l = j0 + threadIdx.x;
+    sx[threadIdx.x] = node[2 * l];
+    sy[threadIdx.x] = node[2 * l + 1];
+    double w1 = 0.0, w2 = 0.0;
+    if (l < n) { w1 = slip[l] * nrm[2 * l]; w2 = slip[l] * nrm[2 * l + 1]; }
+    if (l > 0) { w1 -= slip[l - 1] * nrm[2 * l - 2]; w2 -= slip[l - 1] * nrm[2 * l - 1]; }
+    sw1[threadIdx.x] = w1;
+    sw2[threadIdx.x] = w2;
+  }
+  __syncthreads();
+  if (k >= n) return;
+  const double ex = elem[2 * k], ey = elem[2 * k + 1], n1 = nrm[2 * k], n2 = nrm[2 * k + 1];
   const double c4 = 4.0 * n1 * n2, c2 = 2.0 * n1 * n2, dn = n2 * n2 - n1 * n1;
   double aT = 0.0, aN = 0.0;
-  for (int base = 0; base < n; base += 256) {
-    const int l = base + threadIdx.x;
-    if (l < n) {
-      sx[threadIdx.x] = node[2 * l]; sy[threadIdx.x] = node[2 * l + 1];
-      snx[threadIdx.x] = nrm[2 * l]; sny[threadIdx.x] = nrm[2 * l + 1]; ss[threadIdx.x] = slip[l];
-    }
-    if (threadIdx.x == 0) {
-      const int lr = min(base + 256, n);
-      sx[256] = node[2 * lr]; sy[256] = node[2 * lr + 1];
-    }
-    __syncthreads();
-    const int cnt = min(256, n - base);
-    for (int q = 0; q < cnt; q++) {
-      const double xr = (q + 1 < cnt) ? sx[q + 1] : sx[256], yr = (q + 1 < cnt) ? sy[q + 1] : sy[256];
-      const double exl = ex - sx[q], eyl = ey - sy[q], exr = ex - xr, eyr = ey - yr;
-      const double rl = sqrt(exl * exl + eyl * eyl), rr = sqrt(exr * exr + eyr * eyr);
-      const double g1l = exl / rl, g2l = eyl / rl, g1r = exr / rr, g2r = eyr / rr;
-      const double cl = c4 * g1l * g2l + dn * (g2l * g2l - g1l * g1l);
-      const double cr = c4 * g1r * g2r + dn * (g2r * g2r - g1r * g1r);
-      const double dl = c2 * (g2l * g2l - g1l * g1l) - dn * 2.0 * g1l * g2l;
-      const double dr = c2 * (g2r * g2r - g1r * g1r) - dn * 2.0 * g1r * g2r;
-      const double m1 = snx[q], m2 = sny[q];
-      const double tT = m2 * (cl * g1l / rl) - m2 * (cr * g1r / rr) - m1 * (cl * g2l / rl) + m1 * (cr * g2r / rr);
-      const double tN = m1 * (g1l / rl - g2l / rl * dl) - m1 * (g1r / rr - g2r / rr * dr) +
-                        m2 * (g2l / rl + g1l / rl * dl) - m2 * (g2r / rr + g1r / rr * dr);
-      aT += ss[q] * tT;
-      aN += ss[q] * tN;
-    }
-    __syncthreads();
+  for (int q = 0; q < cnt; q++) {
+    const double gx = ex - sx[q], gy = ey - sy[q];
+    const double r = sqrt(gx * gx + gy * gy), ir = 1.0 / r;
+    const double g1 = gx * ir, g2 = gy * ir;
+    const double d21 = g2 * g2 - g1 * g1, p12 = g1 * g2;
+    const double cc = c4 * p12 + dn * d21;
+    const double dd = c2 * d21 - dn * 2.0 * p12;
+    const double I1t = cc * g1 * ir, I2t = cc * g2 * ir;
+    const double I1n = g1 * ir - g2 * ir * dd, I2n = g2 * ir + g1 * ir * dd;
+    const double w1 = sw1[q], w2 = sw2[q];
+    aT += I1t * w2 - I2t * w1;
+    aN += I1n * w1 + I2n * w2;
   }
-  if (k < n) { sT[k] = scale * aT; sN[k] = scale * aN; }
+  partT[(int64_t)blockIdx.y * n + k] = aT;
+  partN[(int64_t)blockIdx.y * n + k] = aN;
+}
+
+// out[k] = scale * sum_c part[c][k]: 64 rows per workgroup, four waves each
+// summing every fourth tile, then the four sums added in wave order
+__global__ void __launch_bounds__(256) direct_reduce_kernel(const double *__restrict__ part, int nchunk, int n,
+                                                            double scale, double *__restrict__ out) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (k < n) {
+#pragma unroll 4
+    for (int c = g; c < nchunk; c += 4) s += part[(int64_t)c * n + k];
+  }
+  red[g][lane] = s;
+  __syncthreads();
+  if (g == 0 && k < n) out[k] = (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * scale;
 }
 
 // ---------------------------------------------------------------------------
@@ -631,16 +694,24 @@ struct NbVec {
   __device__ __forceinline__ double operator()(int64_t e) const { return v1 ? v0[e] + beta * v1[e] : v0[e]; }
 };
 
+// v at other elements, read from a workgroup-shared copy
+struct LdsVec {
+  const double *v;
+  __device__ __forceinline__ double operator()(int64_t e) const { return v[e]; }
+};
+
 // M v at element k (row-scaled stencil plus crossing terms)
-__device__ __forceinline__ double coop_mv(const CgArgs &A, const CrossRows &X, int k, double vk, const NbVec &v,
-                                          double a1, double a2, double a3, double dg, double dt, double dq) {
+template <class Vec>
+__device__ __forceinline__ double coop_mv(const CgArgs &A, const CrossRows &X, int k, int t0, int t1, double vk,
+                                          const Vec &v, double a1, double a2, double a3, double dg, double dt,
+                                          double dq) {
   const int n = A.n;
   double r;
   if (k == 0) r = a2 * vk + a3 * v(1);
   else if (k == n - 1) r = a1 * v(k - 1) + a2 * vk;
   else r = a1 * v(k - 1) + a2 * vk + a3 * v(k + 1);
   r = r / dg;
-  for (int t = X.start[k]; t < X.start[k + 1]; t++) {
+  for (int t = t0; t < t1; t++) {
     const int cidx = X.ent[2 * t], z = X.ent[2 * t + 1];
     const int64_t n1 = A.cross[6 * cidx + 2], n2 = A.cross[6 * cidx + 3];
     int64_t ea, eb, ec, pl0, pl1, pr0, pr1;
@@ -676,18 +747,22 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   const double d1 = c * (ds[0] * ds[0]);
   double *pbuf[2] = {S.p, S.Ap};
   double a1[kCgEPT], a2[kCgEPT], a3[kCgEPT], dg[kCgEPT], x[kCgEPT], r[kCgEPT], p[kCgEPT], ap[kCgEPT];
+  int xs[kCgEPT], xe[kCgEPT];  // this element's range of crossing entries
   double part_rr = 0.0, part_em = 0.0;
 #pragma unroll
   for (int q = 0; q < kCgEPT; q++) {
     const int k = base + q * 256;
     a1[q] = 0.0; a2[q] = 0.0; a3[q] = 0.0; dg[q] = 1.0; x[q] = 0.0; r[q] = 0.0; p[q] = 0.0; ap[q] = 0.0;
+    xs[q] = 0; xe[q] = 0;
     if (k < n) {
+      xs[q] = X.start[k];
+      xe[q] = X.start[k + 1];
       // compute_diag (:308-405) with the permeability in effect before this call
       double d;
       if (k == 0) d = 1.0 + kxd[1] * dt / (c * (ds[0] * ds[0]));
       else if (k == n - 1) d = 1.0 + kxd[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1]));
       else d = 1.0 + (kxd[k] + kxd[k + 1]) * dt / (c * (ds[k] * ds[k]));
-      for (int t = X.start[k]; t < X.start[k + 1]; t++) {
+      for (int t = xs[q]; t < xe[q]; t++) {
         const int cidx = X.ent[2 * t], z = X.ent[2 * t + 1];
         const int64_t n1 = A.cross[6 * cidx + 2], n2 = A.cross[6 * cidx + 3];
         int64_t ea, eb, ec, pl0, pl1, pr0, pr1;
@@ -717,7 +792,7 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       }
       x[q] = P[k];
       const NbVec vP{P, nullptr, 0.0};
-      const double mv = coop_mv(A, X, k, P[k], vP, a1[q], a2[q], a3[q], d, dt, dq);
+      const double mv = coop_mv(A, X, k, xs[q], xe[q], P[k], vP, a1[q], a2[q], a3[q], d, dt, dq);
       r[q] = b - mv;
       p[q] = r[q];
       S.r[k] = r[q];
@@ -767,7 +842,7 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       for (int q = 0; q < kCgEPT; q++) {
         const int k = base + q * 256;
         if (k < n) {
-          ap[q] = coop_mv(A, X, k, p[q], vp, a1[q], a2[q], a3[q], dg[q], dt, dq);
+          ap[q] = coop_mv(A, X, k, xs[q], xe[q], p[q], vp, a1[q], a2[q], a3[q], dg[q], dt, dq);
           part_pap += p[q] * ap[q];
         }
       }
@@ -843,6 +918,174 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   (void)sysflag;
 }
 
+// compute_P for systems that fit one workgroup (n <= kCgLdsThreads * EPT): one
+// workgroup per system, EPT elements per thread in registers (interleaved,
+// k = q * kCgLdsThreads + tid), the search direction mirrored in LDS for the
+// neighbour and crossing reads.  Same recurrences, stopping rule and failure
+// rule as cg_coop_kernel; three workgroup barriers per iteration and no grid
+// barrier, which is what makes it the faster path for faults of a few thousand
+// elements.  The row scaling is folded into the stencil coefficients
+// (A1/diag, A2/diag, A3/diag), so a row costs three multiply-adds and no divide.
+template <class Vec>
+__device__ __forceinline__ double lds_mv(const CgArgs &A, const CrossRows &X, int k, int t0, int t1, double vk,
+                                         const Vec &v, double s1, double s2, double s3, double dg, double dt,
+                                         double dq) {
+  const int km = k > 0 ? k - 1 : k, kp = k < A.n - 1 ? k + 1 : k;  // end rows have a zero coefficient there
+  double r = s1 * v(km) + s2 * vk + s3 * v(kp);
+  for (int t = t0; t < t1; t++) {
+    const int cidx = X.ent[2 * t], z = X.ent[2 * t + 1];
+    const int64_t n1 = A.cross[6 * cidx + 2], n2 = A.cross[6 * cidx + 3];
+    int64_t ea, eb, ec, pl0, pl1, pr0, pr1;
+    cross_ends(z, n1, n2, ea, eb, ec, pl0, pl1, pr0, pr1);
+    const double kl = hmean(A.perm[pl0], A.perm[pl1]), kr = hmean(A.perm[pr0], A.perm[pr1]);
+    const double c1 = -(kl * dt / dq), c3 = -(kr * dt / dq), c2 = kl * dt / dq + kr * dt / dq;
+    const double vb = (eb == k) ? vk : v(eb);
+    const double num = c1 * v(ea) + c2 * vb + c3 * v(ec);
+    r = r + num / (z == 1 ? (double)n1 : dg);
+  }
+  return r;
+}
+
+template <int EPT>
+__global__ void __launch_bounds__(kCgLdsThreads) cg_lds_kernel(CgArgs A, CrossRows X) {
+  extern __shared__ double lds_p[];  // [n]
+  __shared__ double red1[kCgLdsThreads / 64], red2[kCgLdsThreads / 64], red3[kCgLdsThreads / 64];
+  const CgSys S = A.sys[blockIdx.x];
+  const int n = A.n, tid = threadIdx.x, wid = tid >> 6, nw = kCgLdsThreads / 64;
+  const double dt = S.dt, c = A.cfac;
+  const double *ds = A.ds, *kx = A.perm_x, *kxd = A.perm_x_diag, *P = A.P;
+  const int kq = n > 2 ? n : 2;
+  const double dq = c * (ds[kq - 2] * ds[kq - 2]);
+  const double d1 = c * (ds[0] * ds[0]);
+  double a1[EPT], a2[EPT], a3[EPT], dg[EPT], x[EPT], r[EPT], p[EPT];
+  int xs[EPT], xe[EPT];  // this element's range of crossing entries
+  double part_rr = 0.0, part_em = 0.0;
+#pragma unroll
+  for (int q = 0; q < EPT; q++) {
+    const int k = q * kCgLdsThreads + tid;
+    a1[q] = 0.0; a2[q] = 0.0; a3[q] = 0.0; dg[q] = 1.0; x[q] = 0.0; r[q] = 0.0; p[q] = 0.0;
+    xs[q] = 0; xe[q] = 0;
+    if (k < n) {
+      xs[q] = X.start[k];
+      xe[q] = X.start[k + 1];
+      double d;
+      if (k == 0) d = 1.0 + kxd[1] * dt / (c * (ds[0] * ds[0]));
+      else if (k == n - 1) d = 1.0 + kxd[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1]));
+      else d = 1.0 + (kxd[k] + kxd[k + 1]) * dt / (c * (ds[k] * ds[k]));
+      for (int t = xs[q]; t < xe[q]; t++) {
+        const int cidx = X.ent[2 * t], z = X.ent[2 * t + 1];
+        const int64_t n1 = A.cross[6 * cidx + 2], n2 = A.cross[6 * cidx + 3];
+        int64_t ea, eb, ec, pl0, pl1, pr0, pr1;
+        cross_ends(z, n1, n2, ea, eb, ec, pl0, pl1, pr0, pr1);
+        const double kl = hmean(A.perm_diag[pl0], A.perm_diag[pl1]), kr = hmean(A.perm_diag[pr0], A.perm_diag[pr1]);
+        d = d + kl * dt / d1 + kr * dt / d1;
+      }
+      dg[q] = d;
+      if (k == 0) {
+        a2[q] = 1.0 + kx[1] * dt / (c * (ds[0] * ds[0]));
+        a3[q] = -(kx[1] * dt / (c * (ds[0] * ds[0])));
+      } else if (k == n - 1) {
+        a1[q] = -(kx[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1])));
+        a2[q] = 1.0 + kx[n - 1] * dt / (c * (ds[n - 1] * ds[n - 1]));
+      } else {
+        a1[q] = -(kx[k] * dt / (c * (ds[k - 1] * ds[k - 1])));
+        a2[q] = 1.0 + (kx[k] + kx[k + 1]) * dt / (c * (ds[k] * ds[k]));
+        a3[q] = -(kx[k + 1] * dt / (c * (ds[k + 1] * ds[k + 1])));
+      }
+      a1[q] = a1[q] / d;
+      a2[q] = a2[q] / d;
+      a3[q] = a3[q] / d;
+      double b = P[k] / d;
+      for (int s2 = 0; s2 < A.nsrc; s2++) {
+        const double tt = A.time + dt;
+        if (A.src_elem[s2] - 1 == k && tt >= A.src_tb[s2] && tt < A.src_te[s2])
+          b = b + (dt * A.src_Q[s2] / (A.por * A.beta) / ds[k]) / d;
+      }
+      x[q] = P[k];
+      const NbVec vP{P, nullptr, 0.0};
+      const double mv = lds_mv(A, X, k, xs[q], xe[q], P[k], vP, a1[q], a2[q], a3[q], d, dt, dq);
+      r[q] = b - mv;
+      p[q] = r[q];
+      lds_p[k] = p[q];
+      part_rr += r[q] * r[q];
+      part_em = fmax(part_em, fabs(r[q] / 1.0));
+    }
+  }
+  wave_sum_max_full(part_rr, part_em);
+  if ((tid & 63) == 0) { red2[wid] = part_rr; red3[wid] = part_em; }
+  __syncthreads();  // lds_p and the initial partials are complete
+  double rr = 0.0, em = 0.0;
+  for (int w = 0; w < nw; w++) { rr += red2[w]; em = fmax(em, red3[w]); }
+  int it = 0;
+  bool fail = false;
+  const LdsVec vp{lds_p};
+  while (em / A.tol >= 1.0) {
+    it++;
+    double ap[EPT];
+    double part_pap = 0.0;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      const int k = q * kCgLdsThreads + tid;
+      ap[q] = 0.0;
+      if (k < n) {
+        ap[q] = lds_mv(A, X, k, xs[q], xe[q], p[q], vp, a1[q], a2[q], a3[q], dg[q], dt, dq);
+        part_pap += p[q] * ap[q];
+      }
+    }
+    part_pap = wave_sum_full(part_pap);
+    if ((tid & 63) == 0) red1[wid] = part_pap;
+    __syncthreads();  // all reads of lds_p done; pAp partials complete
+    double pAp = 0.0;
+    for (int w = 0; w < nw; w++) pAp += red1[w];
+    const double alpha = rr / pAp;
+    double part_r1 = 0.0, part_m = 0.0;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      const int k = q * kCgLdsThreads + tid;
+      if (k < n) {
+        x[q] = x[q] + alpha * p[q];
+        r[q] = r[q] - alpha * ap[q];
+        part_r1 += r[q] * r[q];
+        part_m = fmax(part_m, fabs(r[q] / 1.0));
+      }
+    }
+    wave_sum_max_full(part_r1, part_m);
+    if ((tid & 63) == 0) { red2[wid] = part_r1; red3[wid] = part_m; }
+    __syncthreads();  // r.r and max|r| partials complete
+    double r1r1 = 0.0;
+    em = 0.0;
+    for (int w = 0; w < nw; w++) { r1r1 += red2[w]; em = fmax(em, red3[w]); }
+    const double beta = r1r1 / rr;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      const int k = q * kCgLdsThreads + tid;
+      p[q] = r[q] + beta * p[q];
+      if (k < n) lds_p[k] = p[q];
+    }
+    rr = r1r1;
+    if (it > kCgMaxIter) { fail = true; break; }
+    __syncthreads();  // the new direction is in lds_p; red2/red3 are read by all
+  }
+  int bad = 0;
+  if (!fail) {
+    const bool small = dt < (double)1e-9f;
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      const int k = q * kCgLdsThreads + tid;
+      if (k < n) {
+        S.Pt[k] = x[q];
+        S.Pd[k] = small ? 0.0 : (x[q] - P[k]) / dt;
+        if (fabs(x[q]) > 1.7976931348623157e308) bad = 1;
+      }
+    }
+  }
+  bad = __syncthreads_or(bad);
+  if (tid == 0) {
+    S.result[0] = (fail || bad) ? 1 : 0;
+    S.result[1] = it;
+  }
+}
+
 // permeability update of compute_P (:67-75); writes perm and perm_x
 __global__ void __launch_bounds__(256) perm_update_kernel(PermArgs A) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -897,15 +1140,22 @@ void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, 
                      ypart, n, nops, scale, yT, yN);
 }
 
-void launch_direct3(const double *node, const double *slip, double *cbuf, int n, double scale, double *sT,
+int direct_chunks(int n) { return (n + kDirectCols) / kDirectCols; }
+
+void launch_direct3(const double *node, const double *slip, double *part, int n, double scale, double *sT,
                     hipStream_t s) {
-  hipLaunchKernelGGL(dislocation_weights_kernel, dim3(nblk(n + 1, 256)), dim3(256), 0, s, slip, cbuf, n);
-  hipLaunchKernelGGL(direct3_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, node, cbuf, n, scale, sT);
+  const int nc = direct_chunks(n);
+  hipLaunchKernelGGL(direct3_tile_kernel, dim3(nblk(n, 256), nc), dim3(256), 0, s, node, slip, n, part);
+  hipLaunchKernelGGL(direct_reduce_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, part, nc, n, scale, sT);
 }
 
-void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, int n, double scale,
-                    double *sT, double *sN, hipStream_t s) {
-  hipLaunchKernelGGL(direct2_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, node, elem, nrm, slip, n, scale, sT, sN);
+void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, double *part, int n,
+                    double scale, double *sT, double *sN, hipStream_t s) {
+  const int nc = direct_chunks(n);
+  double *pT = part, *pN = part + (int64_t)nc * n;
+  hipLaunchKernelGGL(direct2_tile_kernel, dim3(nblk(n, 256), nc), dim3(256), 0, s, node, elem, nrm, slip, n, pT, pN);
+  hipLaunchKernelGGL(direct_reduce_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, pT, nc, n, scale, sT);
+  hipLaunchKernelGGL(direct_reduce_kernel, dim3(nblk(n, 64)), dim3(256), 0, s, pN, nc, n, scale, sN);
 }
 
 void launch_stage(int stage, const StageArgs &a, hipStream_t s) {
@@ -959,6 +1209,19 @@ bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *part
   void *args[] = {&A, &X, &wps, &partial, &sysflag};
   return hipLaunchCooperativeKernel((const void *)cg_coop_kernel, dim3(nsys * wps), dim3(256), args, 0, s) ==
          hipSuccess;
+}
+
+bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s) {
+  const size_t sh = sizeof(double) * (size_t)a.n;
+  if (a.n <= kCgLdsThreads * 2)
+    hipLaunchKernelGGL(cg_lds_kernel<2>, dim3(nsys), dim3(kCgLdsThreads), sh, s, a, xr);
+  else if (a.n <= kCgLdsThreads * 4)
+    hipLaunchKernelGGL(cg_lds_kernel<4>, dim3(nsys), dim3(kCgLdsThreads), sh, s, a, xr);
+  else if (a.n <= kCgLdsThreads * 8)
+    hipLaunchKernelGGL(cg_lds_kernel<8>, dim3(nsys), dim3(kCgLdsThreads), sh, s, a, xr);
+  else
+    return false;
+  return true;
 }
 
 void launch_perm_update(const PermArgs &a, hipStream_t s) {

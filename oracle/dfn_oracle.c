@@ -925,6 +925,8 @@ static int kernel_eval(ohandle *h, const double *slip, double *sT, double *sN) {
   double *xh = xcalloc((size_t)N, sizeof(double)), *yh = xcalloc((size_t)N, sizeof(double));
   int nops = (o->mode == DFN_MODE_II) ? 2 : 1;
   int st = DFN_OK;
+  for (int op = 0; op < nops; op++)
+    if (!h->flat[op] && !o->H[op]) { free(xh); free(yh); return DFN_ERR_STATE; }
   for (int op = 0; op < nops && st == DFN_OK; op++) {
     const int *pf = op_perm(h, op);
     for (int k = 0; k < N; k++) xh[k] = slip[pf ? pf[k] : o->H[op]->mat_corr[k] - 1];

@@ -137,6 +137,7 @@ class RefSimulation:
             lib.ref_compute_P.argtypes = [C.c_double, _dp, _dp, _i32p]
             lib.ref_rkqs.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
             lib.ref_solve.argtypes = [C.c_int64, C.c_double, _i64p]
+            lib.ref_set_t_try.argtypes = [C.c_double]
             lib.ref_trees.argtypes = [C.c_int, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _i32p, _i32p, _i32p,
                                       _i32p, _i32p, _i64p]
             cls._lib = lib
@@ -212,6 +213,9 @@ class RefSimulation:
         self.lib().ref_rkqs(_p(y), _p(np.ascontiguousarray(dydx, np.float64)), _p(xx), float(htry), float(eps),
                             _p(np.ascontiguousarray(yscal, np.float64)), _p(hd), _p(hn))
         return y, float(xx[0]), float(hd[0]), float(hn[0])
+
+    def set_t_try(self, t):
+        self.lib().ref_set_t_try(float(t))
 
     def solve(self, nsteps, t_final=0.0):
         acc = C.c_int64()

@@ -214,6 +214,11 @@ subroutine ref_rkqs(yv, dydxv, x, htry, eps, yscalv, hdid, hnext) bind(C, name='
   yv(1:3*nb_element) = y1
 end subroutine
 
+subroutine ref_set_t_try(t) bind(C, name='ref_set_t_try')
+  real(c_double), value :: t
+  drv_t_try = t
+end subroutine
+
 ! nsteps iterations of the step of src/solver.f90:59-198 (no output files).
 subroutine ref_solve(nsteps, t_final, accepted) bind(C, name='ref_solve')
   integer(c_int64_t), value :: nsteps

@@ -49,8 +49,16 @@ def test_direct_kernel_and_rhs(make):
         assert close_elementwise(yd, yo, 1e-9, 1e-300 + np.abs(yo).max() * 1e-12), x
 
 
+# compute_P has three kernels (one workgroup per system with the direction in
+# LDS, cooperative multi-workgroup, plain one-workgroup); the library picks by
+# size and DFN_CG_PATH forces a choice so each is checked here.
+CG_PATHS = ["auto", "coop", "plain"]
+
+
+@pytest.mark.parametrize("cg_path", CG_PATHS)
 @pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network, cases.grid_network])
-def test_compute_P(make):
+def test_compute_P(make, cg_path, monkeypatch):
+    monkeypatch.setenv("DFN_CG_PATH", cg_path)
     p = make()
     d, o = dev(p), OracleSimulation(p)
     for dt in (10.0, 5000.0, 86400.0):
@@ -63,9 +71,11 @@ def test_compute_P(make):
             assert np.max(np.abs(ad - bd)) <= (1e-9 * scale + 10 * p.tol_solver) / dt, dt
 
 
+@pytest.mark.parametrize("cg_path", CG_PATHS)
 @pytest.mark.parametrize("make,nsteps", [(cases.single_fault_small, 20), (cases.crossing_network, 30),
                                           (cases.grid_network, 4)])
-def test_solver_trajectory(make, nsteps):
+def test_solver_trajectory(make, nsteps, cg_path, monkeypatch):
+    monkeypatch.setenv("DFN_CG_PATH", cg_path)
     p = make()
     d, o = dev(p), OracleSimulation(p)
     sd, so = d.solve(nsteps), o.solve(nsteps)
@@ -191,3 +201,35 @@ def test_fp32_operator_trajectory_tracks_fp64():
     real = np.ones(N, bool)
     real[p.fake_indices()] = False
     assert close_elementwise(Yb[:N][real], Ya[:N][real], 1e-4, 1e-300)
+
+
+# ---------------------------------------------------------------------------
+def _single_fault_4000():
+    return problems.single_fault(n=4000, length=40e3, static_kernel=_abi.KERNEL_DIRECT, tol=1e-8)
+
+
+def test_single_fault_4000_direct_kernel():
+    """The 4000-element Mode III fault with the dense direct kernel: tiled device
+    kernel vs the oracle's element loop, on smooth, random and log-uniform rates."""
+    p = _single_fault_4000()
+    d, o = dev(p), OracleSimulation(p, threads=8)
+    rng = np.random.default_rng(11)
+    x = np.linspace(-1, 1, p.N)
+    for V in (1e-9 * np.exp(-x * x / 0.01), rng.random(p.N) * 1e-9, 10.0 ** rng.uniform(-12, 0, p.N)):
+        assert rel(d.kernel(V)[0], o.kernel(V)[0]) <= 1e-12
+
+
+def test_single_fault_4000_trajectory():
+    """Injection into the 4000-element fault: 40 accepted solver steps (friction
+    law, compute_P with the source, rkqs) track the oracle to 1e-8."""
+    p = _single_fault_4000()
+    d, o = dev(p), OracleSimulation(p, threads=8)
+    sd, so = d.solve(40), o.solve(40)
+    Yd, Pd, _, td = d.get_state()
+    Yo, Po, _, to = o.get_state()
+    assert sd["rejected_attempts"] == so["rejected_attempts"]
+    assert abs(td - to) <= 1e-8 * to
+    N = p.N
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Yd[sl], Yo[sl], 1e-8, 1e-300)
+    assert np.max(np.abs(Pd - Po)) <= 1e-9 * max(np.abs(Po).max(), 1.0) + 10 * p.tol_solver
