@@ -1,6 +1,7 @@
 """In-tree build of the device library csrc/libdfn.so for gfx950.
 
-hipcc compiles the HIP translation units (kernels, engine + C ABI) for
+hipcc compiles the HIP translation units (kernels, engine + C ABI, device
+H-matrix construction) for
 --offload-arch=gfx950; g++ compiles the host-only setup (geometry, crossings,
 H-matrix construction with OpenMP).  Floating-point contraction is off in the
 host setup so the operator entries round as the reference's expressions do.
@@ -18,7 +19,7 @@ FORT = os.path.join(HERE, "fortran")
 CXX = os.environ.get("CXX_HOST", "g++")
 
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=fast-honor-pragmas"]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-mavx2",
               "-Wall", "-Wno-unused-function"]
 
@@ -33,7 +34,7 @@ def build(verbose=True):
     o = os.path.join(CSRC, "host_setup.o")
     _run([CXX] + HOST_FLAGS + ["-c", "host_setup.cpp", "-o", o])
     objs.append(o)
-    for src in ("kernels.hip", "engine.hip"):
+    for src in ("kernels.hip", "engine.hip", "hbuild.hip"):
         o = os.path.join(CSRC, src.replace(".hip", ".o"))
         _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", o])
         objs.append(o)

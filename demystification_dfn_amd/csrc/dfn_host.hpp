@@ -46,6 +46,32 @@ struct HostProblem {
   int ncross() const { return (int)(cross.size() / 6); }
 };
 
+// Geometry in H order for the matrix-entry functions of
+// src/hmatrix/eval_kernel_hmatrix.f90 (receiver k, source l): element centres,
+// left/right nodes, tangents and normals.
+struct HGeom {
+  std::vector<double> ex, ey, nlx, nly, nrx, nry, tx, ty, nx, ny;
+};
+
+// The partition of one operator (cluster tree, block cluster tree) with its
+// leaves in preorder and the geometry in H order; the matrix data are filled
+// separately (on the host by build_operator, on the device by
+// build_operator_device).
+struct HostPartition {
+  int n = 0, nbclus = 0, nbblclus = 0;
+  bool mode3 = false, normal = false;
+  std::vector<int> perm, leaf_r0, leaf_m;
+  std::vector<int> d_r0, d_c0, d_m, d_n;  // dense leaves
+  std::vector<int> l_r0, l_c0, l_m, l_n;  // admissible leaves
+  HGeom g;
+};
+
+#define DFN_TRY_HOST(x)          \
+  do {                           \
+    const int s_ = (x);          \
+    if (s_ != DFN_OK) return s_; \
+  } while (0)
+
 // Copies and validates a dfn_problem; computes geometry, loading projections,
 // fake-element defaults and fault crossings.  Returns a DFN_* status.
 int setup_problem(const dfn_problem *p, HostProblem &hp);
@@ -53,5 +79,25 @@ int setup_problem(const dfn_problem *p, HostProblem &hp);
 // Builds the H-matrix operator op (DFN_OP_T / DFN_OP_N) of the problem
 // (cluster tree, block cluster tree, full-pivot ACA, QR+SVD recompression).
 int build_operator(const HostProblem &hp, int op, HostOperator &out);
+
+// Pieces of build_operator shared with the device construction.
+int build_partition(const HostProblem &hp, int op, HostPartition &P);
+// Copies the partition's lists into out and sizes the dense data.
+void init_operator(const HostPartition &P, HostOperator &out);
+// QR + SVD recompression of one low-rank block (A m x k, B n x k, column-major,
+// replaced by the recompressed factors); returns the new rank.
+int recompress_block(int m, int n, int k, std::vector<double> &A, std::vector<double> &B, double eps);
+// Recompresses nl blocks whose factors arrive packed (block i: A at hA + doa[i],
+// m[i] x rank[i]; B at hB + dob[i], n[i] x rank[i]); rank is updated, the
+// recompressed factors land in Af / Bf.  OpenMP over blocks.
+void recompress_all(const double *hA, const int64_t *doa, const double *hB, const int64_t *dob, const int *m,
+                    const int *n, int nl, double eps, std::vector<int> &rank, std::vector<std::vector<double>> &Af,
+                    std::vector<std::vector<double>> &Bf);
+// Concatenates per-block factors (Af[i]: m x rank[i], Bf[i]: n x rank[i]) into out.
+void pack_lowrank(std::vector<std::vector<double>> &Af, std::vector<std::vector<double>> &Bf,
+                  const std::vector<int> &rank, HostOperator &out);
+// Device construction of the same operator: partition on the host, dense leaves
+// and full-pivot ACA on the GPU, recompression on the host.
+int build_operator_device(const HostProblem &hp, int op, HostOperator &out, void *stream);
 
 }  // namespace dfn

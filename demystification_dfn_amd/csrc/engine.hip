@@ -772,11 +772,18 @@ API int dfn_build_hmat(dfn_handle h) {
   if (!h) return DFN_ERR_ARG;
   Engine &e = h->e;
   if (e.hp.kernel != DFN_KERNEL_HMATRIX) return DFN_OK;
+  // construction on the GPU (dense leaves and ACA) unless DFN_HBUILD=host asks
+  // for the all-host path; both give the same operator
+  const char *hb = std::getenv("DFN_HBUILD");
+  const bool on_host = hb && std::string(hb) == "host";
+  auto build = [&](int op) {
+    return on_host ? dfn::build_operator(e.hp, op, e.hop[op]) : dfn::build_operator_device(e.hp, op, e.hop[op], e.stream);
+  };
   if (e.hp.mode == DFN_MODE_II) {  // N then T (src/initialisation.f90:222-246)
-    DFN_TRY(dfn::build_operator(e.hp, DFN_OP_N, e.hop[DFN_OP_N]));
+    DFN_TRY(build(DFN_OP_N));
     e.have_op[DFN_OP_N] = true;
   }
-  DFN_TRY(dfn::build_operator(e.hp, DFN_OP_T, e.hop[DFN_OP_T]));
+  DFN_TRY(build(DFN_OP_T));
   e.have_op[DFN_OP_T] = true;
   return e.relayout();
 }

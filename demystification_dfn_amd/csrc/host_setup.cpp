@@ -174,11 +174,6 @@ int setup_problem(const dfn_problem *p, HostProblem &hp) {
 // ---------------------------------------------------------------------------
 namespace {
 
-// Geometry in H order for the matrix-entry functions of
-// src/hmatrix/eval_kernel_hmatrix.f90 (receiver k, source l).
-struct HGeom {
-  std::vector<double> ex, ey, nlx, nly, nrx, nry, tx, ty, nx, ny;
-};
 
 inline double entry_III(const HGeom &g, int k, int l) {  // eval_kernel_ModeIII :7-52
   const double e1 = g.ex[k], e2 = g.ey[k];
@@ -516,7 +511,26 @@ int recompress(int m, int n, int k, std::vector<double> &A, std::vector<double> 
 
 }  // namespace
 
-int build_operator(const HostProblem &hp, int op, HostOperator &out) {
+int recompress_block(int m, int n, int k, std::vector<double> &A, std::vector<double> &B, double eps) {
+  return recompress(m, n, k, A, B, eps);
+}
+
+void recompress_all(const double *hA, const int64_t *doa, const double *hB, const int64_t *dob, const int *m,
+                    const int *n, int nl, double eps, std::vector<int> &rank, std::vector<std::vector<double>> &Af,
+                    std::vector<std::vector<double>> &Bf) {
+  Af.assign(nl, {});
+  Bf.assign(nl, {});
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < nl; i++) {
+    std::vector<double> a(hA + doa[i], hA + doa[i] + (int64_t)m[i] * rank[i]);
+    std::vector<double> b(hB + dob[i], hB + dob[i] + (int64_t)n[i] * rank[i]);
+    rank[i] = recompress(m[i], n[i], rank[i], a, b, eps);
+    Af[i].swap(a);
+    Bf[i].swap(b);
+  }
+}
+
+int build_partition(const HostProblem &hp, int op, HostPartition &P) {
   const int n = hp.N;
   if (op == DFN_OP_N && hp.mode != DFN_MODE_II) return DFN_ERR_ARG;
   TreeBuilder tb;
@@ -528,8 +542,12 @@ int build_operator(const HostProblem &hp, int op, HostOperator &out) {
   BlockTreeBuilder bt(tb.cl, hp.eta);
   bt.make(0, 0);
 
+  P = HostPartition();
+  P.n = n;
+  P.mode3 = hp.mode == DFN_MODE_III;
+  P.normal = op == DFN_OP_N;
   // geometry in H order (forward_reorder, src/hmatrix/initialisation_hmat.f90:65-72)
-  HGeom g;
+  HGeom &g = P.g;
   g.ex = tb.cx; g.ey = tb.cy;
   for (auto *v : {&g.nlx, &g.nly, &g.nrx, &g.nry, &g.tx, &g.ty, &g.nx, &g.ny}) v->resize(n);
   for (int k = 0; k < n; k++) {
@@ -539,61 +557,93 @@ int build_operator(const HostProblem &hp, int op, HostOperator &out) {
     g.tx[k] = hp.tang[2 * s]; g.ty[k] = hp.tang[2 * s + 1];
     g.nx[k] = hp.nrm[2 * s]; g.ny[k] = hp.nrm[2 * s + 1];
   }
-  const bool mode3 = hp.mode == DFN_MODE_III, normal = op == DFN_OP_N;
-
-  // leaves in preorder
-  std::vector<int> lr_ids, dn_ids;
-  for (int b = 0; b < (int)bt.blocks.size(); b++) {
-    if (bt.blocks[b].type == 1) lr_ids.push_back(b);
-    else if (bt.blocks[b].type == 0) dn_ids.push_back(b);
-  }
-  out = HostOperator();
-  out.n = n;
-  out.perm = tb.perm;
-  out.nbclus = (int)tb.cl.size();
-  out.nbblclus = (int)bt.blocks.size();
+  P.perm = tb.perm;
+  P.nbclus = (int)tb.cl.size();
+  P.nbblclus = (int)bt.blocks.size();
   for (const Cluster &c : tb.cl)
-    if (c.son1 < 0) { out.leaf_r0.push_back(c.first); out.leaf_m.push_back(c.size()); }
+    if (c.son1 < 0) { P.leaf_r0.push_back(c.first); P.leaf_m.push_back(c.size()); }
+  // leaves in preorder: dense (compute_full_matrices, H_matrix_construction.f90:555-704)
+  // and admissible ones
+  for (const Block &b : bt.blocks) {
+    if (b.type == 0) {
+      P.d_r0.push_back(tb.cl[b.row].first); P.d_m.push_back(tb.cl[b.row].size());
+      P.d_c0.push_back(tb.cl[b.col].first); P.d_n.push_back(tb.cl[b.col].size());
+    } else if (b.type == 1) {
+      P.l_r0.push_back(tb.cl[b.row].first); P.l_m.push_back(tb.cl[b.row].size());
+      P.l_c0.push_back(tb.cl[b.col].first); P.l_n.push_back(tb.cl[b.col].size());
+    }
+  }
+  return DFN_OK;
+}
 
+void init_operator(const HostPartition &P, HostOperator &out) {
+  out = HostOperator();
+  out.n = P.n;
+  out.perm = P.perm;
+  out.nbclus = P.nbclus;
+  out.nbblclus = P.nbblclus;
+  out.leaf_r0 = P.leaf_r0;
+  out.leaf_m = P.leaf_m;
+  const int nd = (int)P.d_r0.size();
+  out.d_r0 = P.d_r0; out.d_c0 = P.d_c0; out.d_m = P.d_m; out.d_n = P.d_n;
+  out.d_off.resize(nd);
+  int64_t off = 0;
+  for (int i = 0; i < nd; i++) {
+    out.d_off[i] = off;
+    off += (int64_t)P.d_m[i] * P.d_n[i];
+  }
+  out.d_data.resize(off);
+  out.l_r0 = P.l_r0; out.l_c0 = P.l_c0; out.l_m = P.l_m; out.l_n = P.l_n;
+}
+
+void pack_lowrank(std::vector<std::vector<double>> &Af, std::vector<std::vector<double>> &Bf,
+                  const std::vector<int> &rank, HostOperator &out) {
+  const int nl = (int)out.l_m.size();
+  out.l_k = rank;
+  out.l_uoff.resize(nl); out.l_voff.resize(nl);
+  int64_t ou = 0, ov = 0;
+  for (int i = 0; i < nl; i++) {
+    out.l_uoff[i] = ou; out.l_voff[i] = ov;
+    ou += (int64_t)out.l_m[i] * rank[i];
+    ov += (int64_t)out.l_n[i] * rank[i];
+  }
+  out.l_u.resize(ou);
+  out.l_v.resize(ov);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < nl; i++) {
+    std::copy(Af[i].begin(), Af[i].end(), out.l_u.begin() + out.l_uoff[i]);
+    std::copy(Bf[i].begin(), Bf[i].end(), out.l_v.begin() + out.l_voff[i]);
+  }
+}
+
+int build_operator(const HostProblem &hp, int op, HostOperator &out) {
+  HostPartition P;
+  DFN_TRY_HOST(build_partition(hp, op, P));
+  init_operator(P, out);
+  const HGeom &g = P.g;
+  const bool mode3 = P.mode3, normal = P.normal;
   auto assemble = [&](int r0, int m, int c0, int nn, double *dst) {
     for (int j = 0; j < nn; j++)
       for (int i = 0; i < m; i++)
         dst[i + (size_t)j * m] = mode3 ? entry_III(g, r0 + i, c0 + j) : entry_II(g, r0 + i, c0 + j, normal);
   };
-
-  // dense leaves (compute_full_matrices, H_matrix_construction.f90:555-704)
-  const int nd = (int)dn_ids.size();
-  out.d_r0.resize(nd); out.d_c0.resize(nd); out.d_m.resize(nd); out.d_n.resize(nd); out.d_off.resize(nd);
-  int64_t off = 0;
-  for (int i = 0; i < nd; i++) {
-    const Block &b = bt.blocks[dn_ids[i]];
-    out.d_r0[i] = tb.cl[b.row].first; out.d_m[i] = tb.cl[b.row].size();
-    out.d_c0[i] = tb.cl[b.col].first; out.d_n[i] = tb.cl[b.col].size();
-    out.d_off[i] = off;
-    off += (int64_t)out.d_m[i] * out.d_n[i];
-  }
-  out.d_data.resize(off);
+  const int nd = (int)out.d_r0.size();
 #pragma omp parallel for schedule(dynamic, 4)
   for (int i = 0; i < nd; i++) assemble(out.d_r0[i], out.d_m[i], out.d_c0[i], out.d_n[i], &out.d_data[out.d_off[i]]);
 
   // admissible leaves: full-pivot ACA then recompression, largest blocks first
-  const int nl = (int)lr_ids.size();
+  const int nl = (int)out.l_m.size();
   std::vector<std::vector<double>> Af(nl), Bf(nl);
   std::vector<int> rank(nl);
   std::vector<int> order(nl);
   std::iota(order.begin(), order.end(), 0);
-  auto area = [&](int i) {
-    const Block &b = bt.blocks[lr_ids[i]];
-    return (int64_t)tb.cl[b.row].size() * tb.cl[b.col].size();
-  };
+  auto area = [&](int i) { return (int64_t)out.l_m[i] * out.l_n[i]; };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return area(x) > area(y); });
   int status = DFN_OK;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int t = 0; t < nl; t++) {
     const int i = order[t];
-    const Block &b = bt.blocks[lr_ids[i]];
-    const int r0 = tb.cl[b.row].first, m = tb.cl[b.row].size();
-    const int c0 = tb.cl[b.col].first, nn = tb.cl[b.col].size();
+    const int r0 = out.l_r0[i], m = out.l_m[i], c0 = out.l_c0[i], nn = out.l_n[i];
     std::vector<double> R;
     try {
       R.resize((size_t)m * nn);
@@ -612,24 +662,7 @@ int build_operator(const HostProblem &hp, int op, HostOperator &out) {
     Bf[i].swap(B);
   }
   if (status != DFN_OK) return status;
-  out.l_r0.resize(nl); out.l_c0.resize(nl); out.l_m.resize(nl); out.l_n.resize(nl); out.l_k.resize(nl);
-  out.l_uoff.resize(nl); out.l_voff.resize(nl);
-  int64_t ou = 0, ov = 0;
-  for (int i = 0; i < nl; i++) {
-    const Block &b = bt.blocks[lr_ids[i]];
-    out.l_r0[i] = tb.cl[b.row].first; out.l_m[i] = tb.cl[b.row].size();
-    out.l_c0[i] = tb.cl[b.col].first; out.l_n[i] = tb.cl[b.col].size();
-    out.l_k[i] = rank[i];
-    out.l_uoff[i] = ou; out.l_voff[i] = ov;
-    ou += (int64_t)out.l_m[i] * rank[i];
-    ov += (int64_t)out.l_n[i] * rank[i];
-  }
-  out.l_u.resize(ou);
-  out.l_v.resize(ov);
-  for (int i = 0; i < nl; i++) {
-    std::copy(Af[i].begin(), Af[i].end(), out.l_u.begin() + out.l_uoff[i]);
-    std::copy(Bf[i].begin(), Bf[i].end(), out.l_v.begin() + out.l_voff[i]);
-  }
+  pack_lowrank(Af, Bf, rank, out);
   return DFN_OK;
 }
 

@@ -131,11 +131,13 @@ def test_hmatvec_on_identical_block_lists(mode):
         assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
 
 
+@pytest.mark.parametrize("builder", ["device", "host"])
 @pytest.mark.parametrize("mode", [2, 3])
-def test_device_built_operator_matches_oracle_construction(mode):
-    """Trees, ranks and factors of the library's construction vs the oracle's
-    restatement of src/hmatrix (cluster tree, block tree, full-pivot ACA,
-    QR+SVD recompression)."""
+def test_device_built_operator_matches_oracle_construction(mode, builder, monkeypatch):
+    """Trees, ranks and factors of the library's construction (GPU assembly and
+    ACA, or the all-host path) vs the oracle's restatement of src/hmatrix
+    (cluster tree, block tree, full-pivot ACA, QR+SVD recompression)."""
+    monkeypatch.setenv("DFN_HBUILD", builder)
     p = _hnet(mode)
     d, o = dev(p), OracleSimulation(p)
     d.initiate_hmat()
@@ -148,6 +150,30 @@ def test_device_built_operator_matches_oracle_construction(mode):
         np.testing.assert_array_equal(a["dense_data"], b["dense_data"])
         x = np.random.default_rng(op).standard_normal(p.N)
         assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
+
+
+OP_KEYS = ("perm", "dense_row0", "dense_col0", "dense_m", "dense_n", "dense_data", "lr_row0", "lr_col0", "lr_m",
+           "lr_n", "lr_k", "lr_u", "lr_v")
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_gpu_construction_equals_host_construction_larger(mode, monkeypatch):
+    """A 60-fault network (a few thousand elements, admissible leaves spanning
+    many 64 x 256 tiles): the GPU assembly + ACA gives the very same operator,
+    factor for factor, as the library's host construction."""
+    p = problems.network_small(n_faults=60, seed=21, mode=mode, eta=2.0, eps_aca=1e-6, nleaf=32,
+                               static_kernel=_abi.KERNEL_HMATRIX)
+    monkeypatch.setenv("DFN_HBUILD", "host")
+    a = dev(p)
+    a.initiate_hmat()
+    monkeypatch.setenv("DFN_HBUILD", "device")
+    b = dev(p)
+    b.initiate_hmat()
+    for op in ((0, 1) if mode == 2 else (0,)):
+        ea, eb = a.hmat_export(op), b.hmat_export(op)
+        assert a.hmat_stats(op)["nb_lowrank"] > 50
+        for key in OP_KEYS:
+            np.testing.assert_array_equal(ea[key], eb[key], err_msg=key)
 
 
 def test_hmatrix_solver_trajectory_mode2():
