@@ -108,6 +108,16 @@ SIGNATURES = {
     "bench_kernel": [_H, C.c_int32, _dp],
     "bench_stream": [C.c_int64, C.c_int32, _dp],
     "device_info": [C.c_char_p, C.c_int32],
+    # ensembles of independent networks
+    "ensemble_create": [C.POINTER(DfnProblem), C.c_int32, C.POINTER(C.c_void_p)],
+    "ensemble_destroy": [_H],
+    "ensemble_build_hmat": [_H],
+    "ensemble_set_precision": [_H, C.c_int32],
+    "ensemble_set_graph": [_H, C.c_int32],
+    "ensemble_solve": [_H, C.c_int64],
+    "ensemble_get_state": [_H, C.c_int32, _dp, _dp, _dp, C.POINTER(C.c_double)],
+    "ensemble_get_stats": [_H, C.c_int32, C.POINTER(DfnRunStats)],
+    "ensemble_info": [_H, _i64p],
 }
 
 # entry points every build of the device library must export (checked by the CPU tests)

@@ -74,6 +74,17 @@ struct CgSys {
   double *diag, *A1, *A2, *A3, *b, *x, *r, *p, *Ap;
   double *Pt, *Pd;
   int *result;  // [ier, iterations]
+  int net;      // network of the system (multi-network launches)
+  int pad;
+};
+
+// One network of a multi-network compute_P launch: its elements are [base,
+// base + n) of the concatenated arrays (permeability_x at xbase, n + 1 entries),
+// its sources and crossings the given ranges of the concatenated lists (node ids
+// and element ids stay network-local), and its own fluid constants and time.
+struct CgNet {
+  int n, base, xbase, src0, nsrc, cross0, ncross, pad;
+  double cfac, time, tol, por, beta;
 };
 
 struct CgArgs {
@@ -92,6 +103,8 @@ struct CgArgs {
   const double *src_Q, *src_tb, *src_te;
   int ncross;
   const int64_t *cross;
+  const CgNet *nets;   // null: one network described by the fields above
+  const int *active;   // multi-network launches: systems of networks with active[net] == 0 return at once
 };
 
 // Crossing rows of the fluid-diffusion stencil: entries (crossing, slot) of each
@@ -130,6 +143,24 @@ void launch_direct3(const double *node, const double *slip, double *part, int n,
 void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, double *part, int n,
                     double scale, double *sT, double *sN, hipStream_t s);
 void launch_stage(int stage, const StageArgs &a, hipStream_t s);
+
+// Ensemble of networks concatenated element-wise (network of element k: net[k]):
+// the stage kernels with per-network step sizes, masks and error norms.
+struct EnsStageArgs {
+  StageArgs a;              // n, params, buffers over all networks; a.h / a.t_try / a.tol / a.status unused
+  const int *net;           // [n]
+  const int *active;        // [nnet] 1: the network takes part in this attempt
+  const double *h, *t_try, *tol, *eta;  // [nnet]
+  StatusWord *status;       // [nnet]
+};
+void launch_ens_stage(int stage, const EnsStageArgs &a, hipStream_t s);
+void launch_ens_restart_stage2(const EnsStageArgs &a, hipStream_t s);
+void launch_ens_clear_status(StatusWord *st, const int *active, int nnet, hipStream_t s);
+// dt of every (network, slot) system of an ensemble compute_P launch: slot 0 and,
+// when with_stages, slots 1..5 (x + A_i h with x = 0), from t_try (slot 0) and h
+void launch_ens_cg_dt(CgSys *sys, int nnet, int nslot, const double *t_try, const double *h, int first_slot,
+                      hipStream_t s);
+void launch_cg_lds_multi(const CgArgs &a, const CrossRows &xr, int nsys, int max_n, hipStream_t s);
 void launch_restart_stage2(const double *y, const double *k1, double h, int n3, double *yt, hipStream_t s);
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
 // cooperative multi-workgroup CG; returns false if the grid cannot be co-resident

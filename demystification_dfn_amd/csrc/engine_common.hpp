@@ -1,0 +1,258 @@
+// Shared by the single-network engine and the ensemble engine: error macros,
+// owning device buffers and the device layout of the stress-transfer operators
+// (two-phase H-matvec task lists, see kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "dfn_device.hpp"
+#include "dfn_host.hpp"
+
+namespace dfn {
+
+#define DFN_HIP(expr)                                       \
+  do {                                                      \
+    hipError_t e_ = (expr);                                 \
+    if (e_ != hipSuccess) {                                 \
+      std::fprintf(stderr, "dfn: %s failed: %s\n", #expr,   \
+                   hipGetErrorString(e_));                  \
+      return DFN_ERR_HIP;                                   \
+    }                                                       \
+  } while (0)
+
+#define DFN_TRY(expr)           \
+  do {                          \
+    int s_ = (expr);            \
+    if (s_ != DFN_OK) return s_; \
+  } while (0)
+
+// owning device allocation
+struct DMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DMem() = default;
+  DMem(const DMem &) = delete;
+  DMem &operator=(const DMem &) = delete;
+  ~DMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  int alloc(size_t nbytes) {
+    release();
+    if (nbytes == 0) nbytes = 8;
+    if (hipMalloc(&p, nbytes) != hipSuccess) { p = nullptr; return DFN_ERR_NOMEM; }
+    bytes = nbytes;
+    return DFN_OK;
+  }
+  template <class T>
+  int upload(const std::vector<T> &v) {
+    DFN_TRY(alloc(v.size() * sizeof(T)));
+    if (!v.empty()) DFN_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return DFN_OK;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// Device layout of the stress-transfer operator(s) of one problem (1 in Mode III,
+// T and N in Mode II, sharing the permutation and the input vector).
+struct DevOps {
+  int n = 0, nops = 0;
+  bool fp32 = false;
+  DMem perm, vtasks, vdata, zred, zpart, ptasks, pdata, widx, cscale, w, ypart, fin_first, fin_idx;
+  int nvt = 0, nzr = 0, npt = 0;
+  double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
+  double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
+  int64_t S_total = 0;   // matrix entries streamed per application
+};
+
+// power-of-two scale 2^e >= max|v| (1 for an all-zero block): dividing by it is exact
+inline double pow2_scale(const double *v, size_t n) {
+  double m = 0.0;
+  for (size_t i = 0; i < n; i++) m = std::fmax(m, std::fabs(v[i]));
+  if (!(m > 0.0) || !std::isfinite(m)) return 1.0;
+  int e = 0;
+  (void)std::frexp(m, &e);
+  return std::ldexp(1.0, e);
+}
+
+template <class T>
+int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
+  const bool scaled = sizeof(T) == 4;  // FP32 storage keeps each block divided by its power-of-two scale
+  const int n = ops[0]->n;
+  for (int o = 1; o < nops; o++)
+    if (ops[o]->perm != ops[0]->perm) return DFN_ERR_UNSUPPORTED;  // T and N share the cluster tree
+  d.n = n;
+  d.nops = nops;
+  d.fp32 = sizeof(T) == 4;
+  // w = [x_h | z of op 0 | z of op 1]
+  std::vector<int64_t> zbase(nops + 1, n);
+  std::vector<std::vector<int>> zoff(nops);
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    zoff[o].resize(H.l_k.size());
+    int64_t z = 0;
+    for (size_t b = 0; b < H.l_k.size(); b++) { zoff[o][b] = (int)z; z += H.l_k[b]; }
+    zbase[o + 1] = zbase[o] + z;
+  }
+  const int64_t wsize = zbase[nops];
+  // phase 1 tasks and V data
+  std::vector<VTask> vt;
+  std::vector<ZRed> zr;
+  std::vector<T> vdata;
+  int zp = 0;
+  int64_t S_total = 0;
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    const int64_t vbase = (int64_t)vdata.size();
+    vdata.resize(vdata.size() + H.l_v.size());
+    for (size_t b = 0; b < H.l_k.size(); b++) {
+      const int k = H.l_k[b], nb = H.l_n[b];
+      if (k > kMaxRank) return DFN_ERR_UNSUPPORTED;
+      const double *vb = H.l_v.data() + H.l_voff[b];
+      const size_t cnt = (size_t)nb * k;
+      const double vs = scaled ? pow2_scale(vb, cnt) : 1.0;
+      for (size_t i = 0; i < cnt; i++) vdata[vbase + H.l_voff[b] + i] = (T)(vb[i] / vs);
+      const int wz = (int)(zbase[o] + zoff[o][b]);
+      const bool single = nb <= kVChunkRows;  // one chunk: z goes straight into w
+      ZRed r{zp, 0, k, wz};
+      for (int r0 = 0; r0 < nb; r0 += kVChunkRows) {
+        VTask t{};
+        t.voff = vbase + H.l_voff[b];
+        t.ld = nb;
+        t.row0 = r0;
+        t.len = std::min(kVChunkRows, nb - r0);
+        t.k = k;
+        t.xcol = H.l_c0[b];
+        t.zp = single ? -1 : zp;
+        t.scale = vs;
+        t.wout = single ? wz : -1;
+        vt.push_back(t);
+        if (!single) zp += k;
+        r.ntask++;
+      }
+      if (!single) zr.push_back(r);
+    }
+    S_total += (int64_t)H.l_v.size() + (int64_t)H.l_u.size() + (int64_t)H.d_data.size();
+  }
+  // phase 2 panels: per operator and leaf row cluster, all covering blocks
+  const HostOperator &H0 = *ops[0];
+  const int nleaf = (int)H0.leaf_r0.size();
+  std::vector<int> row_leaf(n);
+  for (int l = 0; l < nleaf; l++)
+    for (int i = 0; i < H0.leaf_m[l]; i++) row_leaf[H0.leaf_r0[l] + i] = l;
+  std::vector<PTask> pt;
+  std::vector<T> pdata;
+  std::vector<int> widx;
+  std::vector<float> cscale;
+  std::vector<std::vector<double>> dscale(nops), uscale(nops);
+  int64_t reserve = 0;
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    reserve += (int64_t)H.d_data.size() + (int64_t)H.l_u.size();
+    dscale[o].assign(H.d_m.size(), 1.0);
+    uscale[o].assign(H.l_m.size(), 1.0);
+    if (scaled) {
+      for (size_t b = 0; b < H.d_m.size(); b++)
+        dscale[o][b] = pow2_scale(H.d_data.data() + H.d_off[b], (size_t)H.d_m[b] * H.d_n[b]);
+      for (size_t b = 0; b < H.l_m.size(); b++)
+        uscale[o][b] = pow2_scale(H.l_u.data() + H.l_uoff[b], (size_t)H.l_m[b] * H.l_k[b]);
+    }
+  }
+  pdata.reserve(reserve);
+  const int64_t panel_entries = reserve;  // dense + U entries, every one stored once
+  std::vector<std::vector<int>> fin((size_t)n * nops);
+  int yp = 0;
+  for (int o = 0; o < nops; o++) {
+    const HostOperator &H = *ops[o];
+    // covering lists: (kind, block) per leaf; dense first then low-rank, block order
+    std::vector<std::vector<std::pair<int, int>>> cover(nleaf);
+    for (size_t b = 0; b < H.d_m.size(); b++)
+      for (int l = row_leaf[H.d_r0[b]]; l < nleaf && H0.leaf_r0[l] < H.d_r0[b] + H.d_m[b]; l++)
+        cover[l].push_back({0, (int)b});
+    for (size_t b = 0; b < H.l_m.size(); b++)
+      for (int l = row_leaf[H.l_r0[b]]; l < nleaf && H0.leaf_r0[l] < H.l_r0[b] + H.l_m[b]; l++)
+        cover[l].push_back({1, (int)b});
+    for (int l = 0; l < nleaf; l++) {
+      const int r0 = H0.leaf_r0[l], m = H0.leaf_m[l];
+      // columns of this panel: (w index, source pointer into the block, block scale);
+      // near-field (dense) columns first
+      struct Col { int64_t wi; const double *src; double scale; };
+      std::vector<Col> cols;
+      int ndense = 0;
+      for (auto [kind, b] : cover[l]) {
+        if (kind == 0) {
+          const double sc = scaled ? dscale[o][b] : 1.0;
+          const double *F = H.d_data.data() + H.d_off[b] + (r0 - H.d_r0[b]);
+          for (int j = 0; j < H.d_n[b]; j++) cols.push_back({(int64_t)H.d_c0[b] + j, F + (int64_t)j * H.d_m[b], sc});
+          ndense += H.d_n[b];
+        } else {
+          const double sc = scaled ? uscale[o][b] : 1.0;
+          const double *U = H.l_u.data() + H.l_uoff[b] + (r0 - H.l_r0[b]);
+          for (int j = 0; j < H.l_k[b]; j++)
+            cols.push_back({zbase[o] + zoff[o][b] + j, U + (int64_t)j * H.l_m[b], sc});
+        }
+      }
+      // rows padded to 16 bytes (2 doubles / 4 floats) with zero entries on column 0 of w
+      const int pad = 16 / (int)sizeof(T);
+      for (size_t c0 = 0; c0 < cols.size(); c0 += kPanelMaxCols) {
+        const int L = (int)std::min<size_t>(kPanelMaxCols, cols.size() - c0);
+        const int Lp = (L + pad - 1) / pad * pad;
+        PTask t{};
+        t.poff = (int64_t)pdata.size();
+        t.widx = (int64_t)widx.size();
+        t.m = m;
+        t.L = Lp;
+        t.yp = yp;
+        t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, (int64_t)ndense - (int64_t)c0));
+        for (int j = 0; j < Lp; j++) {
+          widx.push_back(j < L ? (int)cols[c0 + j].wi : 0);
+          if (scaled) cscale.push_back(j < L ? (float)cols[c0 + j].scale : 1.0f);
+        }
+        for (int r = 0; r < m; r++)
+          for (int j = 0; j < Lp; j++) pdata.push_back(j < L ? (T)(cols[c0 + j].src[r] / cols[c0 + j].scale) : (T)0);
+        for (int r = 0; r < m; r++) fin[(size_t)o * n + r0 + r].push_back(yp + r);
+        yp += m;
+        pt.push_back(t);
+      }
+    }
+  }
+  std::vector<int> fin_first((size_t)n * nops + 1, 0), fin_idx;
+  for (size_t g = 0; g < fin.size(); g++) {
+    fin_first[g] = (int)fin_idx.size();
+    fin_idx.insert(fin_idx.end(), fin[g].begin(), fin[g].end());
+  }
+  fin_first[fin.size()] = (int)fin_idx.size();
+  if (widx.size() > (size_t)INT32_MAX || zp < 0 || yp < 0) return DFN_ERR_UNSUPPORTED;
+  DFN_TRY(d.perm.upload(H0.perm));
+  DFN_TRY(d.vtasks.upload(vt));
+  DFN_TRY(d.vdata.upload(vdata));
+  DFN_TRY(d.zred.upload(zr));
+  DFN_TRY(d.zpart.alloc(sizeof(double) * std::max(zp, 1)));
+  DFN_TRY(d.ptasks.upload(pt));
+  DFN_TRY(d.pdata.upload(pdata));
+  DFN_TRY(d.widx.upload(widx));
+  DFN_TRY(d.cscale.upload(cscale));
+  DFN_TRY(d.w.alloc(sizeof(double) * wsize));
+  DFN_TRY(d.ypart.alloc(sizeof(double) * std::max(yp, 1)));
+  DFN_TRY(d.fin_first.upload(fin_first));
+  DFN_TRY(d.fin_idx.upload(fin_idx));
+  d.nvt = (int)vt.size();
+  d.nzr = (int)zr.size();
+  d.npt = (int)pt.size();
+  d.S_total = S_total;
+  // bytes: factors + x read + y written, per operator (x shared in Mode II)
+  d.alg_bytes = (double)sizeof(T) * (double)S_total + 8.0 * n + 8.0 * n * nops;
+  d.panel_bytes = (double)sizeof(T) * (double)panel_entries + 4.0 * (double)cscale.size() + 8.0 * (double)yp;
+  return DFN_OK;
+}
+
+}  // namespace dfn

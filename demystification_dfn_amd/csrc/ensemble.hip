@@ -1,0 +1,642 @@
+// Ensemble engine: many independent fault networks advanced together on one GPU.
+//
+// The networks are concatenated element-wise (network b owns elements [base_b,
+// base_b + N_b)); their stress-transfer operators become one block-diagonal
+// operator whose two-phase task lists cover every network, so a stage of the
+// Runge-Kutta step is one H-matvec launch sequence and one stage kernel for the
+// whole ensemble.  Each network keeps its own clock, trial step, step size and
+// error norm: the stage kernels read the per-network step size, reduce the
+// scaled error per network, and skip networks that have already accepted their
+// step (masked rejection: only networks whose attempt was rejected redo stages
+// 2..6, with their own reduced step).  compute_P runs as one launch over every
+// (network, stage) system.  The per-attempt launch sequences read all their
+// parameters from device memory and are captured once as HIP graphs and
+// replayed.  Every network follows the operation sequence of the single-network
+// engine (engine.hip), so each one's trajectory is the one it would have alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "dfn_device.hpp"
+#include "dfn_host.hpp"
+#include "engine_common.hpp"
+
+namespace dfn {
+
+namespace {
+
+constexpr int kEnsSlots = 6;
+
+// Concatenates per-network operators into one block-diagonal operator over the
+// concatenated element order (H positions and elements of network b offset by
+// base[b]).
+void concat_ops(const std::vector<HostOperator> &ops, const std::vector<int> &base, HostOperator &out) {
+  out = HostOperator();
+  int64_t nd = 0, dd = 0, nl = 0, lu = 0, lv = 0;
+  for (const HostOperator &H : ops) {
+    out.n += H.n;
+    out.nbclus += H.nbclus;
+    out.nbblclus += H.nbblclus;
+    nd += (int64_t)H.d_m.size();
+    dd += (int64_t)H.d_data.size();
+    nl += (int64_t)H.l_m.size();
+    lu += (int64_t)H.l_u.size();
+    lv += (int64_t)H.l_v.size();
+  }
+  out.perm.reserve(out.n);
+  out.d_r0.reserve(nd); out.d_c0.reserve(nd); out.d_m.reserve(nd); out.d_n.reserve(nd); out.d_off.reserve(nd);
+  out.d_data.reserve(dd);
+  out.l_r0.reserve(nl); out.l_c0.reserve(nl); out.l_m.reserve(nl); out.l_n.reserve(nl); out.l_k.reserve(nl);
+  out.l_uoff.reserve(nl); out.l_voff.reserve(nl);
+  out.l_u.reserve(lu);
+  out.l_v.reserve(lv);
+  for (size_t b = 0; b < ops.size(); b++) {
+    const HostOperator &H = ops[b];
+    const int o = base[b];
+    for (int p : H.perm) out.perm.push_back(p + o);
+    const int64_t doff = (int64_t)out.d_data.size(), uoff = (int64_t)out.l_u.size(), voff = (int64_t)out.l_v.size();
+    for (size_t i = 0; i < H.d_m.size(); i++) {
+      out.d_r0.push_back(H.d_r0[i] + o); out.d_c0.push_back(H.d_c0[i] + o);
+      out.d_m.push_back(H.d_m[i]); out.d_n.push_back(H.d_n[i]); out.d_off.push_back(H.d_off[i] + doff);
+    }
+    out.d_data.insert(out.d_data.end(), H.d_data.begin(), H.d_data.end());
+    for (size_t i = 0; i < H.l_m.size(); i++) {
+      out.l_r0.push_back(H.l_r0[i] + o); out.l_c0.push_back(H.l_c0[i] + o);
+      out.l_m.push_back(H.l_m[i]); out.l_n.push_back(H.l_n[i]); out.l_k.push_back(H.l_k[i]);
+      out.l_uoff.push_back(H.l_uoff[i] + uoff); out.l_voff.push_back(H.l_voff[i] + voff);
+    }
+    out.l_u.insert(out.l_u.end(), H.l_u.begin(), H.l_u.end());
+    out.l_v.insert(out.l_v.end(), H.l_v.begin(), H.l_v.end());
+    for (size_t i = 0; i < H.leaf_r0.size(); i++) {
+      out.leaf_r0.push_back(H.leaf_r0[i] + o);
+      out.leaf_m.push_back(H.leaf_m[i]);
+    }
+  }
+}
+
+template <class T>
+void append(std::vector<T> &dst, const std::vector<T> &src) {
+  dst.insert(dst.end(), src.begin(), src.end());
+}
+
+}  // namespace
+
+struct EnsEngine {
+  int B = 0, Ntot = 0, max_n = 0, mode = DFN_MODE_II, nops = 2;
+  double kscale = 0.0;
+  std::vector<HostProblem> hp;
+  std::vector<int> base, xbase;
+  std::vector<HostOperator> ops[2];  // per network until the layout is built
+  bool built = false;
+  int precision = DFN_PREC_FP64;
+  bool use_graph = true;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<DevOps> ops64, ops32;
+  HostOperator cat[2];  // concatenated operators (kept for the FP32 layout)
+
+  // concatenated problem data
+  DMem a, b, Dc, f0, V0, nld, sld, fake, net, ds, perm, permx, src_elem, src_Q, src_tb, src_te, cross, xrow_start,
+      xrow_ent, cgnets;
+  // state and stage buffers
+  DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot;
+  // per-network parameters read by the captured launches
+  DMem d_h, d_ttry, d_tol, d_eta, d_active, status, cg_work, cg_res, sys_all, sys_stage;
+  // pinned host mirrors
+  double *h_h = nullptr, *h_ttry = nullptr;
+  int *h_active = nullptr, *h_cgres = nullptr;
+  StatusWord *h_status = nullptr;
+  CgNet *h_nets = nullptr;
+  std::vector<CgNet> nets;
+  // per-network clocks and counters
+  std::vector<double> time, t_try;
+  std::vector<dfn_run_stats> st;
+  hipGraphExec_t g_try = nullptr, g_retry = nullptr;
+  const DevOps *graph_ops = nullptr;
+
+  double *kk(int i) const { return kbuf.as<double>() + (size_t)i * 3 * Ntot; }
+  double *cg_vec(int slot, int which) const { return cg_work.as<double>() + ((size_t)slot * 12 + which) * Ntot; }
+
+  ~EnsEngine() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
+    if (h_h) (void)hipHostFree(h_h);
+    if (h_ttry) (void)hipHostFree(h_ttry);
+    if (h_active) (void)hipHostFree(h_active);
+    if (h_cgres) (void)hipHostFree(h_cgres);
+    if (h_status) (void)hipHostFree(h_status);
+    if (h_nets) (void)hipHostFree(h_nets);
+    ops64.reset();
+    ops32.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void drop_graphs() {
+    if (g_try) (void)hipGraphExecDestroy(g_try);
+    if (g_retry) (void)hipGraphExecDestroy(g_retry);
+    g_try = g_retry = nullptr;
+    graph_ops = nullptr;
+  }
+
+  int init(const dfn_problem *probs, int nnet) {
+    B = nnet;
+    hp.resize(B);
+    for (int i = 0; i < B; i++) DFN_TRY(setup_problem(&probs[i], hp[i]));
+    mode = hp[0].mode;
+    nops = mode == DFN_MODE_II ? 2 : 1;
+    const double pi = 4.0 * std::atan(1.0);
+    auto scale_of = [&](const HostProblem &p) {
+      return p.mode == DFN_MODE_III ? p.mu / (2.0 * pi) : p.mu / pi * (1.0 - p.cs * p.cs / (p.cp * p.cp));
+    };
+    kscale = scale_of(hp[0]);
+    for (const HostProblem &p : hp) {
+      // one block-diagonal operator with one output scale; the step batches the
+      // compute_P solves of a step, which needs a permeability that does not evolve
+      if (p.mode != mode || p.kernel != DFN_KERNEL_HMATRIX || scale_of(p) != kscale || p.coupling == 1)
+        return DFN_ERR_UNSUPPORTED;
+    }
+    base.resize(B);
+    xbase.resize(B);
+    std::vector<double> ca, cb, cDc, cf0, cV0, cnld, csld, cds, cperm, cpx, cQ, ctb, cte, cP;
+    std::vector<uint8_t> cfake;
+    std::vector<int> cnet;
+    std::vector<int64_t> csrc, ccross;
+    std::vector<int> xstart, xent;
+    nets.resize(B);
+    int off = 0, xo = 0, so = 0, co = 0;
+    for (int i = 0; i < B; i++) {
+      const HostProblem &p = hp[i];
+      const int N = p.N;
+      base[i] = off;
+      xbase[i] = xo;
+      append(ca, p.a); append(cb, p.b); append(cDc, p.Dc); append(cf0, p.f0); append(cV0, p.V0);
+      append(cnld, p.nld); append(csld, p.sld); append(cfake, p.is_fake); append(cds, p.ds);
+      append(cperm, p.perm0); append(cP, p.P0);
+      for (int k = 0; k < N; k++) cnet.push_back(i);
+      // permeability_x (initialisation.f90:252-266), N + 1 entries per network
+      std::vector<double> px(N + 1, 0.0);
+      for (int k = 1; k < N; k++) px[k] = 2.0 * p.perm0[k - 1] * p.perm0[k] / (p.perm0[k - 1] + p.perm0[k]);
+      append(cpx, px);
+      append(csrc, p.src_elem); append(cQ, p.src_Q); append(ctb, p.src_tb); append(cte, p.src_te);
+      append(ccross, p.cross);
+      // crossing rows: entry indices global, crossing ids network-local
+      std::vector<std::vector<std::pair<int, int>>> rows(N);
+      for (int c = 0; c < p.ncross(); c++) {
+        const int64_t n1 = p.cross[6 * c + 2], n2 = p.cross[6 * c + 3];
+        const int64_t e[4] = {n1 - 2, n1 - 1, n2 - 2, n2 - 1};
+        for (int z = 0; z < 4; z++) rows[e[z]].push_back({c, z});
+      }
+      for (int k = 0; k < N; k++) {
+        xstart.push_back((int)(xent.size() / 2));
+        for (auto [c, z] : rows[k]) { xent.push_back(c); xent.push_back(z); }
+      }
+      CgNet &c = nets[i];
+      c.n = N; c.base = off; c.xbase = xo; c.src0 = so; c.nsrc = (int)p.src_elem.size();
+      c.cross0 = co; c.ncross = p.ncross(); c.pad = 0;
+      c.cfac = p.por * p.visc * (p.fcomp + p.rcomp);
+      c.time = 0.0; c.tol = p.tol; c.por = p.por; c.beta = p.fcomp + p.rcomp;
+      off += N;
+      xo += N + 1;
+      so += c.nsrc;
+      co += c.ncross;
+      max_n = std::max(max_n, N);
+    }
+    Ntot = off;
+    xstart.push_back((int)(xent.size() / 2));
+    if (xent.empty()) xent.assign(2, 0);
+    // state in the ensemble layout [V | psi | sigmaN], each over all networks
+    std::vector<double> y3((size_t)3 * Ntot);
+    for (int i = 0; i < B; i++)
+      for (int c = 0; c < 3; c++)
+        std::copy(hp[i].Y0.begin() + (size_t)c * hp[i].N, hp[i].Y0.begin() + (size_t)(c + 1) * hp[i].N,
+                  y3.begin() + (size_t)c * Ntot + base[i]);
+    DFN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    DFN_TRY(a.upload(ca)); DFN_TRY(b.upload(cb)); DFN_TRY(Dc.upload(cDc)); DFN_TRY(f0.upload(cf0));
+    DFN_TRY(V0.upload(cV0)); DFN_TRY(nld.upload(cnld)); DFN_TRY(sld.upload(csld)); DFN_TRY(fake.upload(cfake));
+    DFN_TRY(net.upload(cnet)); DFN_TRY(ds.upload(cds)); DFN_TRY(perm.upload(cperm)); DFN_TRY(permx.upload(cpx));
+    DFN_TRY(src_elem.upload(csrc)); DFN_TRY(src_Q.upload(cQ)); DFN_TRY(src_tb.upload(ctb));
+    DFN_TRY(src_te.upload(cte)); DFN_TRY(cross.upload(ccross)); DFN_TRY(xrow_start.upload(xstart));
+    DFN_TRY(xrow_ent.upload(xent)); DFN_TRY(cgnets.upload(nets));
+    const size_t n3 = (size_t)3 * Ntot * sizeof(double);
+    DFN_TRY(Y.upload(y3));
+    DFN_TRY(Ynew.alloc(n3)); DFN_TRY(yscal.alloc(n3)); DFN_TRY(kbuf.alloc(6 * n3));
+    DFN_TRY(ytA.alloc(n3)); DFN_TRY(ytB.alloc(n3));
+    DFN_TRY(sT.alloc(sizeof(double) * Ntot)); DFN_TRY(sN.alloc(sizeof(double) * Ntot));
+    DFN_HIP(hipMemset(sN.p, 0, sizeof(double) * Ntot));
+    DFN_TRY(P.upload(cP));
+    DFN_TRY(Pdot.alloc(sizeof(double) * Ntot));
+    DFN_HIP(hipMemset(Pdot.p, 0, sizeof(double) * Ntot));
+    DFN_TRY(cg_work.alloc(sizeof(double) * (size_t)kEnsSlots * 12 * Ntot));
+    DFN_TRY(cg_res.alloc(sizeof(int) * 2 * kEnsSlots * B));
+    DFN_TRY(d_h.alloc(sizeof(double) * B)); DFN_TRY(d_ttry.alloc(sizeof(double) * B));
+    DFN_TRY(d_active.alloc(sizeof(int) * B)); DFN_TRY(status.alloc(sizeof(StatusWord) * B));
+    std::vector<double> tol(B), eta(B);
+    for (int i = 0; i < B; i++) { tol[i] = hp[i].tol; eta[i] = hp[i].mu / (2.0 * hp[i].cs); }
+    DFN_TRY(d_tol.upload(tol));
+    DFN_TRY(d_eta.upload(eta));
+    // (network, slot) systems: all six slots, and slots 1..5 for retried attempts
+    std::vector<CgSys> all((size_t)B * kEnsSlots), stage((size_t)B * (kEnsSlots - 1));
+    for (int i = 0; i < B; i++)
+      for (int sl = 0; sl < kEnsSlots; sl++) {
+        CgSys c{};
+        c.dt = 0.0;
+        c.diag = cg_vec(sl, 0) + base[i]; c.A1 = cg_vec(sl, 1) + base[i]; c.A2 = cg_vec(sl, 2) + base[i];
+        c.A3 = cg_vec(sl, 3) + base[i]; c.b = cg_vec(sl, 4) + base[i]; c.x = cg_vec(sl, 5) + base[i];
+        c.r = cg_vec(sl, 6) + base[i]; c.p = cg_vec(sl, 7) + base[i]; c.Ap = cg_vec(sl, 8) + base[i];
+        c.Pt = cg_vec(sl, 10) + base[i]; c.Pd = cg_vec(sl, 11) + base[i];
+        c.result = cg_res.as<int>() + 2 * ((size_t)i * kEnsSlots + sl);
+        c.net = i;
+        all[(size_t)i * kEnsSlots + sl] = c;
+        if (sl > 0) stage[(size_t)i * (kEnsSlots - 1) + sl - 1] = c;
+      }
+    DFN_TRY(sys_all.upload(all));
+    DFN_TRY(sys_stage.upload(stage));
+    DFN_HIP(hipHostMalloc((void **)&h_h, sizeof(double) * B, hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_ttry, sizeof(double) * B, hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_active, sizeof(int) * B, hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_cgres, sizeof(int) * 2 * kEnsSlots * B, hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_status, sizeof(StatusWord) * B, hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_nets, sizeof(CgNet) * B, hipHostMallocDefault));
+    time.assign(B, 0.0);
+    t_try.assign(B, 100.0);
+    st.assign(B, dfn_run_stats{});
+    return DFN_OK;
+  }
+
+  int build_hmat() {
+    const char *hb = std::getenv("DFN_HBUILD");
+    const bool on_host = hb && std::string(hb) == "host";
+    for (int o = 0; o < nops; o++) ops[o].assign(B, HostOperator());
+    for (int i = 0; i < B; i++) {
+      // N then T, as the single-network build
+      for (int o = nops - 1; o >= 0; o--) {
+        const int op = o == 1 ? DFN_OP_N : DFN_OP_T;
+        DFN_TRY(on_host ? build_operator(hp[i], op, ops[o][i]) : build_operator_device(hp[i], op, ops[o][i], stream));
+      }
+    }
+    for (int o = 0; o < nops; o++) {
+      concat_ops(ops[o], base, cat[o]);
+      ops[o].clear();
+      ops[o].shrink_to_fit();
+    }
+    built = true;
+    return relayout();
+  }
+
+  int relayout() {
+    drop_graphs();
+    ops64.reset();
+    ops32.reset();
+    if (!built) return DFN_OK;
+    const HostOperator *list[2] = {&cat[0], &cat[1]};
+    ops64.reset(new DevOps);
+    DFN_TRY(layout_ops<double>(list, nops, *ops64));
+    if (precision == DFN_PREC_FP32) {
+      ops32.reset(new DevOps);
+      DFN_TRY(layout_ops<float>(list, nops, *ops32));
+    }
+    return DFN_OK;
+  }
+
+  const DevOps *active_ops() const { return (precision == DFN_PREC_FP32 && ops32) ? ops32.get() : ops64.get(); }
+
+  void apply(const double *dV) {
+    const DevOps *d = active_ops();
+    double *w = d->w.as<double>();
+    launch_gather(dV, d->perm.as<int>(), w, Ntot, stream);
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream);
+    launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
+    launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
+                 d->ypart.as<double>(), stream);
+    launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), Ntot,
+                    d->nops, kscale, sT.as<double>(), d->nops > 1 ? sN.as<double>() : sT.as<double>(), stream);
+  }
+
+  EnsStageArgs stage_args(const double *ytemp, int slot, double *ynext) {
+    EnsStageArgs E{};
+    StageArgs &s = E.a;
+    s.n = Ntot;
+    s.p = RhsParams{a.as<double>(), b.as<double>(), Dc.as<double>(), f0.as<double>(), V0.as<double>(),
+                    nld.as<double>(), sld.as<double>(), fake.as<uint8_t>(), 0.0, mode == DFN_MODE_III ? 1 : 0};
+    s.y = Y.as<double>();
+    s.ytemp = ytemp;
+    s.sT = sT.as<double>();
+    s.sN = mode == DFN_MODE_II ? sN.as<double>() : nullptr;
+    s.Pt = cg_vec(slot, 10);
+    s.Pd = cg_vec(slot, 11);
+    for (int i = 0; i < 6; i++) s.k[i] = kk(i);
+    s.ynext = ynext;
+    s.yscal = yscal.as<double>();
+    E.net = net.as<int>();
+    E.active = d_active.as<int>();
+    E.h = d_h.as<double>();
+    E.t_try = d_ttry.as<double>();
+    E.tol = d_tol.as<double>();
+    E.eta = d_eta.as<double>();
+    E.status = status.as<StatusWord>();
+    return E;
+  }
+
+  CgArgs cg_args(const CgSys *sys) {
+    CgArgs A{};
+    A.n = max_n;
+    A.sys = sys;
+    A.perm_x = permx.as<double>();
+    A.perm_x_diag = permx.as<double>();
+    A.perm = perm.as<double>();
+    A.perm_diag = perm.as<double>();
+    A.ds = ds.as<double>();
+    A.P = P.as<double>();
+    A.src_elem = src_elem.as<int64_t>();
+    A.src_Q = src_Q.as<double>();
+    A.src_tb = src_tb.as<double>();
+    A.src_te = src_te.as<double>();
+    A.cross = cross.as<int64_t>();
+    A.nets = cgnets.as<CgNet>();
+    A.active = d_active.as<int>();
+    return A;
+  }
+
+  void cg_launch(bool stages_only) {
+    CgSys *sys = stages_only ? sys_stage.as<CgSys>() : sys_all.as<CgSys>();
+    const int nslot = stages_only ? kEnsSlots - 1 : kEnsSlots;
+    launch_ens_cg_dt(sys, B, nslot, d_ttry.as<double>(), d_h.as<double>(), stages_only ? 1 : 0, stream);
+    const CgArgs A = cg_args(sys);
+    if (max_n <= kCgLdsMax) {
+      CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
+      launch_cg_lds_multi(A, X, B * nslot, max_n, stream);
+    } else {
+      launch_cg(A, B * nslot, stream);
+    }
+  }
+
+  // stages 2..6 of rkck (ode_bs.f90:199-224) for the active networks
+  void attempt() {
+    launch_ens_clear_status(status.as<StatusWord>(), d_active.as<int>(), B, stream);
+    double *in = ytA.as<double>(), *out = ytB.as<double>();
+    for (int s = 2; s <= 6; s++) {
+      apply(in);
+      const EnsStageArgs E = stage_args(in, s - 1, s == 6 ? Ynew.as<double>() : out);
+      launch_ens_stage(s, E, stream);
+      std::swap(in, out);
+    }
+  }
+
+  // first attempt of a step: compute_P of all six slots, k1 and yscal, stages 2..6
+  void seq_try() {
+    cg_launch(false);
+    apply(Y.as<double>());
+    launch_ens_stage(1, stage_args(Y.as<double>(), 0, ytA.as<double>()), stream);
+    attempt();
+  }
+
+  // retried attempt with reduced steps: compute_P of slots 1..5, restart, stages 2..6
+  void seq_retry() {
+    cg_launch(true);
+    launch_ens_restart_stage2(stage_args(Y.as<double>(), 0, ytA.as<double>()), stream);
+    attempt();
+  }
+
+  int capture(void (EnsEngine::*fn)(), hipGraphExec_t *exec) {
+    hipGraph_t g = nullptr;
+    DFN_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    (this->*fn)();
+    DFN_HIP(hipStreamEndCapture(stream, &g));
+    const hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return DFN_ERR_HIP;
+    return DFN_OK;
+  }
+
+  int run(bool retry) {
+    if (use_graph) {
+      if (graph_ops != active_ops()) {
+        drop_graphs();
+        DFN_TRY(capture(&EnsEngine::seq_try, &g_try));
+        DFN_TRY(capture(&EnsEngine::seq_retry, &g_retry));
+        graph_ops = active_ops();
+      }
+      DFN_HIP(hipGraphLaunch(retry ? g_retry : g_try, stream));
+    } else {
+      if (retry) seq_retry();
+      else seq_try();
+      DFN_HIP(hipGetLastError());
+    }
+    return DFN_OK;
+  }
+
+  int upload_params() {
+    for (int i = 0; i < B; i++) h_nets[i] = nets[i];
+    DFN_HIP(hipMemcpyAsync(cgnets.p, h_nets, sizeof(CgNet) * B, hipMemcpyHostToDevice, stream));
+    DFN_HIP(hipMemcpyAsync(d_h.p, h_h, sizeof(double) * B, hipMemcpyHostToDevice, stream));
+    DFN_HIP(hipMemcpyAsync(d_ttry.p, h_ttry, sizeof(double) * B, hipMemcpyHostToDevice, stream));
+    DFN_HIP(hipMemcpyAsync(d_active.p, h_active, sizeof(int) * B, hipMemcpyHostToDevice, stream));
+    return DFN_OK;
+  }
+
+  int readback() {
+    DFN_HIP(hipMemcpyAsync(h_status, status.p, sizeof(StatusWord) * B, hipMemcpyDeviceToHost, stream));
+    DFN_HIP(hipMemcpyAsync(h_cgres, cg_res.p, sizeof(int) * 2 * kEnsSlots * B, hipMemcpyDeviceToHost, stream));
+    DFN_HIP(hipStreamSynchronize(stream));
+    return DFN_OK;
+  }
+
+  // one iteration of src/solver.f90:59-198 for every network
+  int step() {
+    const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = (double)1.89e-4f;
+    std::vector<double> h(B), hdid(B), hnext(B);
+    std::vector<int> todo(B, 1);
+    for (int i = 0; i < B; i++) {
+      const HostProblem &p = hp[i];
+      if (t_try[i] > 10.0)  // :64-77
+        for (size_t s = 0; s < p.src_tb.size(); s++)
+          if ((time[i] < p.src_tb[s]) && (time[i] + t_try[i] > p.src_tb[s])) t_try[i] = 10.0;
+      nets[i].time = time[i];
+    }
+    // first attempt (with compute_P(t_try) halving, :81-92)
+    std::vector<int> act(B, 1);
+    for (;;) {
+      for (int i = 0; i < B; i++) {
+        h_ttry[i] = t_try[i];
+        h_h[i] = t_try[i];
+        h_active[i] = act[i];
+      }
+      DFN_TRY(upload_params());
+      DFN_TRY(run(false));
+      DFN_TRY(readback());
+      bool again = false;
+      for (int i = 0; i < B; i++) {
+        if (!act[i]) continue;
+        const int *r0 = h_cgres + 2 * ((size_t)i * kEnsSlots);
+        st[i].cg_iterations += r0[1];
+        st[i].cg_solves += 1;
+        if (r0[0]) {  // compute_P(t_try) failed: halve and redo this network
+          t_try[i] = t_try[i] / 2.0;
+          st[i].cg_halvings++;
+          again = true;
+        } else {
+          act[i] = 0;
+        }
+      }
+      if (!again) break;  // otherwise act holds the networks whose solve failed
+    }
+    // rkqs (ode_bs.f90:125-174) per network on the attempts just made
+    for (int i = 0; i < B; i++) {
+      h[i] = t_try[i];
+      st[i].rhs_evaluations += 1;
+    }
+    bool any = true;
+    while (any) {
+      any = false;
+      for (int i = 0; i < B; i++) {
+        if (!todo[i]) { h_active[i] = 0; continue; }
+        const int *r = h_cgres + 2 * ((size_t)i * kEnsSlots);
+        int fail = 0;
+        for (int sl = 1; sl < kEnsSlots; sl++) {
+          st[i].cg_iterations += r[2 * sl + 1];
+          if (r[2 * sl]) fail = 1;
+        }
+        st[i].cg_solves += 5;
+        st[i].rhs_evaluations += 5;
+        if (fail) return DFN_ERR_CG;
+        double em;
+        std::memcpy(&em, &h_status[i].errmax_bits, sizeof(double));
+        const double errmax = em / hp[i].tol;
+        if (errmax <= 1.0) {
+          hdid[i] = h[i];
+          hnext[i] = errmax > ERRCON ? SAFETY * h[i] * std::pow(errmax, PGROW) : 5.0 * h[i];
+          todo[i] = 0;
+          h_active[i] = 0;
+          continue;
+        }
+        st[i].rejected_attempts++;
+        const double htemp = SAFETY * h[i] * std::pow(errmax, PSHRNK);
+        h[i] = std::copysign(std::fmax(std::fabs(htemp), 0.1 * std::fabs(h[i])), h[i]);
+        if (0.0 + h[i] == 0.0) return DFN_ERR_UNDERFLOW;
+        h_active[i] = 1;
+        h_h[i] = h[i];
+        any = true;
+      }
+      if (!any) break;
+      DFN_TRY(upload_params());
+      DFN_TRY(run(true));
+      DFN_TRY(readback());
+    }
+    // commit (:174-177): the accepted state and compute_P(t_did) of slot 4
+    DFN_HIP(hipMemcpyAsync(Y.p, Ynew.p, (size_t)3 * Ntot * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    DFN_HIP(hipMemcpyAsync(P.p, cg_vec(4, 10), sizeof(double) * Ntot, hipMemcpyDeviceToDevice, stream));
+    DFN_HIP(hipMemcpyAsync(Pdot.p, cg_vec(4, 11), sizeof(double) * Ntot, hipMemcpyDeviceToDevice, stream));
+    bool tensile = false;
+    for (int i = 0; i < B; i++) {
+      t_try[i] = hnext[i];
+      if (t_try[i] > 24 * 60 * 60) t_try[i] = 24 * 60 * 60;
+      time[i] = time[i] + hdid[i];
+      st[i].accepted_steps++;
+      if (h_status[i].tensile) tensile = true;
+    }
+    return tensile ? DFN_ERR_TENSILE : DFN_OK;
+  }
+};
+
+}  // namespace dfn
+
+using dfn::EnsEngine;
+
+struct dfn_ensemble_s {
+  EnsEngine e;
+};
+
+#define API extern "C" __attribute__((visibility("default")))
+
+API int dfn_ensemble_create(const dfn_problem *problems, int32_t nnet, dfn_ensemble *out) {
+  if (!out || !problems || nnet < 1) return DFN_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return DFN_ERR_HIP;
+  dfn_ensemble h = new (std::nothrow) dfn_ensemble_s;
+  if (!h) return DFN_ERR_NOMEM;
+  const int st = h->e.init(problems, nnet);
+  if (st != DFN_OK) { delete h; return st; }
+  *out = h;
+  return DFN_OK;
+}
+
+API int dfn_ensemble_destroy(dfn_ensemble h) {
+  if (!h) return DFN_ERR_ARG;
+  delete h;
+  return DFN_OK;
+}
+
+API int dfn_ensemble_build_hmat(dfn_ensemble h) {
+  if (!h) return DFN_ERR_ARG;
+  return h->e.build_hmat();
+}
+
+API int dfn_ensemble_set_precision(dfn_ensemble h, int32_t prec) {
+  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32)) return DFN_ERR_ARG;
+  h->e.precision = prec;
+  if (prec == DFN_PREC_FP32 && !h->e.ops32) return h->e.relayout();
+  h->e.drop_graphs();
+  return DFN_OK;
+}
+
+API int dfn_ensemble_set_graph(dfn_ensemble h, int32_t use_graph) {
+  if (!h) return DFN_ERR_ARG;
+  h->e.use_graph = use_graph != 0;
+  return DFN_OK;
+}
+
+API int dfn_ensemble_solve(dfn_ensemble h, int64_t nsteps) {
+  if (!h || nsteps < 0) return DFN_ERR_ARG;
+  if (!h->e.active_ops()) return DFN_ERR_STATE;
+  for (int64_t s = 0; s < nsteps; s++) DFN_TRY(h->e.step());
+  return DFN_OK;
+}
+
+API int dfn_ensemble_get_state(dfn_ensemble h, int32_t netw, double *Yh, double *Ph, double *Pdh, double *time) {
+  if (!h || netw < 0 || netw >= h->e.B) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  const size_t N = e.hp[netw].N, o = e.base[netw], T = e.Ntot;
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  if (Yh)
+    for (int c = 0; c < 3; c++)
+      DFN_HIP(hipMemcpy(Yh + c * N, e.Y.as<double>() + c * T + o, N * sizeof(double), hipMemcpyDeviceToHost));
+  if (Ph) DFN_HIP(hipMemcpy(Ph, e.P.as<double>() + o, N * sizeof(double), hipMemcpyDeviceToHost));
+  if (Pdh) DFN_HIP(hipMemcpy(Pdh, e.Pdot.as<double>() + o, N * sizeof(double), hipMemcpyDeviceToHost));
+  if (time) *time = e.time[netw];
+  return DFN_OK;
+}
+
+API int dfn_ensemble_get_stats(dfn_ensemble h, int32_t netw, dfn_run_stats *s) {
+  if (!h || !s || netw < 0 || netw >= h->e.B) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  *s = e.st[netw];
+  s->time = e.time[netw];
+  s->t_try = e.t_try[netw];
+  const size_t N = e.hp[netw].N;
+  std::vector<double> V(N);
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  DFN_HIP(hipMemcpy(V.data(), e.Y.as<double>() + e.base[netw], N * sizeof(double), hipMemcpyDeviceToHost));
+  s->max_V = *std::max_element(V.begin(), V.end());
+  return DFN_OK;
+}
+
+API int dfn_ensemble_info(dfn_ensemble h, int64_t *info) {
+  if (!h || !info) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  const dfn::DevOps *d = e.active_ops();
+  info[0] = e.B;
+  info[1] = e.Ntot;
+  info[2] = e.max_n;
+  info[3] = d ? d->S_total : 0;
+  info[4] = d ? (int64_t)d->alg_bytes : 0;
+  info[5] = e.use_graph ? 1 : 0;
+  return DFN_OK;
+}

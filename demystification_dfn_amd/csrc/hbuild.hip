@@ -21,7 +21,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -38,6 +41,17 @@ namespace {
 constexpr int kAcaRows = 64;    // rows per tile (one wave, one row per lane)
 constexpr int kAcaCols = 256;   // columns per tile, in chunks of 64
 constexpr int kAcaChunk = 64;
+
+// phase timings on stderr when DFN_VERBOSE is set
+struct PhaseClock {
+  bool on = std::getenv("DFN_VERBOSE") != nullptr;
+  std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+  void lap(const char *what) {
+    const auto now = std::chrono::steady_clock::now();
+    if (on) std::fprintf(stderr, "[dfn build] %-28s %8.3f s\n", what, std::chrono::duration<double>(now - t).count());
+    t = now;
+  }
+};
 
 struct GeomDev {
   const double *ex, *ey, *nlx, *nly, *nrx, *nry, *tx, *ty, *nx, *ny;
@@ -332,7 +346,8 @@ struct DBuf {
   } while (0)
 
 template <int MODE>
-int run_device_build(const HostProblem &hp, const HostPartition &P, HostOperator &out, hipStream_t s) {
+int run_device_build(const HostProblem &hp, const HostPartition &P, HostOperator &out, hipStream_t s,
+                     PhaseClock &clk) {
   const int n = P.n;
   DBuf geo[10];
   const std::vector<double> *gv[10] = {&P.g.ex, &P.g.ey, &P.g.nlx, &P.g.nly, &P.g.nrx,
@@ -356,6 +371,7 @@ int run_device_build(const HostProblem &hp, const HostPartition &P, HostOperator
     HB_HIP(hipMemcpyAsync(out.d_data.data(), dd.p, sizeof(double) * out.d_data.size(), hipMemcpyDeviceToHost, s));
     HB_HIP(hipStreamSynchronize(s));
   }
+  clk.lap("dense leaves (GPU + copy)");
 
   // admissible leaves
   const int nl = (int)out.l_m.size();
@@ -420,6 +436,7 @@ int run_device_build(const HostProblem &hp, const HostPartition &P, HostOperator
       active.swap(next);
       if (pass > mr + 1) return DFN_ERR_STATE;  // every leaf stops by max_rank passes
     }
+    clk.lap("ACA passes (GPU)");
     // the first rank columns of every leaf's factors, packed, back to the host
     std::vector<int64_t> soa(nl), sob(nl), doa(nl), dob(nl), ca(nl), cb(nl);
     int64_t pa = 0, pb2 = 0;
@@ -444,11 +461,14 @@ int run_device_build(const HostProblem &hp, const HostPartition &P, HostOperator
     HB_HIP(hipMemcpyAsync(hA.data(), pkA.p, sizeof(double) * pa, hipMemcpyDeviceToHost, s));
     HB_HIP(hipMemcpyAsync(hB.data(), pkB.p, sizeof(double) * pb2, hipMemcpyDeviceToHost, s));
     HB_HIP(hipStreamSynchronize(s));
+    clk.lap("factors back to host");
     // recompression on the host
     recompress_all(hA.data(), doa.data(), hB.data(), dob.data(), out.l_m.data(), out.l_n.data(), nl, hp.eps_aca,
                    rank, Af, Bf);
+    clk.lap("recompression (host)");
   }
   pack_lowrank(Af, Bf, rank, out);
+  clk.lap("pack");
   return DFN_OK;
 }
 
@@ -456,13 +476,15 @@ int run_device_build(const HostProblem &hp, const HostPartition &P, HostOperator
 
 int build_operator_device(const HostProblem &hp, int op, HostOperator &out, void *stream) {
   if (hp.max_rank > kMaxRank || hp.max_rank < 1) return build_operator(hp, op, out);
+  PhaseClock clk;
   HostPartition P;
   DFN_TRY_HOST(build_partition(hp, op, P));
   init_operator(P, out);
+  clk.lap("partition (host)");
   hipStream_t s = (hipStream_t)stream;
-  if (P.mode3) return run_device_build<0>(hp, P, out, s);
-  if (!P.normal) return run_device_build<1>(hp, P, out, s);
-  return run_device_build<2>(hp, P, out, s);
+  if (P.mode3) return run_device_build<0>(hp, P, out, s, clk);
+  if (!P.normal) return run_device_build<1>(hp, P, out, s, clk);
+  return run_device_build<2>(hp, P, out, s, clk);
 }
 
 }  // namespace dfn

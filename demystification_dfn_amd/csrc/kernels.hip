@@ -415,56 +415,61 @@ __device__ __forceinline__ void rate_state(const RhsParams &p, int k, double V, 
   if (p.fake[k]) { dV = 0.0; dpsi = 0.0; dsig = 0.0; }
 }
 
-// Stage s of rkck: k_s = f(ytemp_s); then the next stage input (or, at s = 6, the
-// 5th-order solution, the embedded error and its scaled max norm).  At s = 1 the
-// kernel evaluates k1 at x = t_try and forms yscal (src/solver.f90:101-102).
+// Stage s of rkck at element k: k_s = f(ytemp_s); then the next stage input (or,
+// at s = 6, the 5th-order solution, and the scaled embedded error into emax).  At
+// s = 1 it evaluates k1 at x = t_try and forms yscal (src/solver.f90:101-102).
+template <int S>
+__device__ __forceinline__ void stage_body(const StageArgs &A, int k, double h, double t_try, double tol, double eta,
+                                           double &emax, int &tensile) {
+  const int n = A.n;
+  const double *yt = A.ytemp;
+  double d[3];
+  RhsParams p = A.p;
+  p.eta_rad = eta;
+  rate_state(p, k, yt[k], yt[n + k], yt[2 * n + k], A.sT[k], A.sN ? A.sN[k] : 0.0, A.Pt[k], A.Pd[k], d[0], d[1],
+             d[2]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const int i = c * n + k;
+    const double y = A.y[i];
+    if (S == 0) {
+      A.k[0][i] = d[c];
+    } else if (S == 1) {
+      A.k[0][i] = d[c];
+      A.yscal[i] = fabs(y) + fabs(d[c] * t_try) + tol;
+      A.ynext[i] = y + CK::B21 * h * d[c];
+    } else if (S == 2) {
+      A.k[1][i] = d[c];
+      A.ynext[i] = y + h * (CK::B31 * A.k[0][i] + CK::B32 * d[c]);
+    } else if (S == 3) {
+      A.k[2][i] = d[c];
+      A.ynext[i] = y + h * (CK::B41 * A.k[0][i] + CK::B42 * A.k[1][i] + CK::B43 * d[c]);
+    } else if (S == 4) {
+      A.k[3][i] = d[c];
+      A.ynext[i] = y + h * (CK::B51 * A.k[0][i] + CK::B52 * A.k[1][i] + CK::B53 * A.k[2][i] + CK::B54 * d[c]);
+    } else if (S == 5) {
+      A.k[4][i] = d[c];
+      A.ynext[i] = y + h * (CK::B61 * A.k[0][i] + CK::B62 * A.k[1][i] + CK::B63 * A.k[2][i] + CK::B64 * A.k[3][i] +
+                            CK::B65 * d[c]);
+    } else {
+      const double k1 = A.k[0][i], k3 = A.k[2][i], k4 = A.k[3][i], k5 = A.k[4][i];
+      const double yo = y + h * (CK::C1 * k1 + CK::C3 * k3 + CK::C4 * k4 + CK::C6 * d[c]);
+      const double ye = h * (CK::DC1 * k1 + CK::DC3 * k3 + CK::DC4 * k4 + CK::DC5 * k5 + CK::DC6 * d[c]);
+      A.ynext[i] = yo;
+      double e = fabs(ye / A.yscal[i]);
+      if (!(e == e)) e = INFINITY;
+      emax = fmax(emax, e);
+      if (c == 2 && yo > 0) tensile = 1;
+    }
+  }
+}
+
 template <int S>
 __global__ void __launch_bounds__(256) stage_kernel(StageArgs A) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = A.n;
   double emax = 0.0;
   int tensile = 0;
-  if (k < n) {
-    const double *yt = A.ytemp;
-    double d[3];
-    rate_state(A.p, k, yt[k], yt[n + k], yt[2 * n + k], A.sT[k], A.sN ? A.sN[k] : 0.0, A.Pt[k], A.Pd[k], d[0], d[1],
-               d[2]);
-    const double h = A.h;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const int i = c * n + k;
-      const double y = A.y[i];
-      if (S == 0) {
-        A.k[0][i] = d[c];
-      } else if (S == 1) {
-        A.k[0][i] = d[c];
-        A.yscal[i] = fabs(y) + fabs(d[c] * A.t_try) + A.tol;
-        A.ynext[i] = y + CK::B21 * h * d[c];
-      } else if (S == 2) {
-        A.k[1][i] = d[c];
-        A.ynext[i] = y + h * (CK::B31 * A.k[0][i] + CK::B32 * d[c]);
-      } else if (S == 3) {
-        A.k[2][i] = d[c];
-        A.ynext[i] = y + h * (CK::B41 * A.k[0][i] + CK::B42 * A.k[1][i] + CK::B43 * d[c]);
-      } else if (S == 4) {
-        A.k[3][i] = d[c];
-        A.ynext[i] = y + h * (CK::B51 * A.k[0][i] + CK::B52 * A.k[1][i] + CK::B53 * A.k[2][i] + CK::B54 * d[c]);
-      } else if (S == 5) {
-        A.k[4][i] = d[c];
-        A.ynext[i] = y + h * (CK::B61 * A.k[0][i] + CK::B62 * A.k[1][i] + CK::B63 * A.k[2][i] +
-                              CK::B64 * A.k[3][i] + CK::B65 * d[c]);
-      } else {
-        const double k1 = A.k[0][i], k3 = A.k[2][i], k4 = A.k[3][i], k5 = A.k[4][i];
-        const double yo = y + h * (CK::C1 * k1 + CK::C3 * k3 + CK::C4 * k4 + CK::C6 * d[c]);
-        const double ye = h * (CK::DC1 * k1 + CK::DC3 * k3 + CK::DC4 * k4 + CK::DC5 * k5 + CK::DC6 * d[c]);
-        A.ynext[i] = yo;
-        double e = fabs(ye / A.yscal[i]);
-        if (!(e == e)) e = INFINITY;
-        emax = fmax(emax, e);
-        if (c == 2 && yo > 0) tensile = 1;
-      }
-    }
-  }
+  if (k < A.n) stage_body<S>(A, k, A.h, A.t_try, A.tol, A.p.eta_rad, emax, tensile);
   if (S == 6) {
     __shared__ double red[4];
     __shared__ int tred[4];
@@ -479,6 +484,65 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs A) {
       if (tred[0] | tred[1] | tred[2] | tred[3]) atomicOr(&A.status->tensile, 1);
     }
   }
+}
+
+// The same stage over an ensemble: each element takes its network's step size,
+// trial step, tolerance and damping; elements of networks outside the attempt
+// are left untouched.  At s = 6 the error norm is reduced per network: a wave
+// whose elements all belong to one network reduces first, a wave straddling two
+// networks updates per element.
+template <int S>
+__global__ void __launch_bounds__(256) ens_stage_kernel(EnsStageArgs E) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = k < E.a.n ? E.net[k] : -1;
+  const bool on = nb >= 0 && E.active[nb];
+  double emax = 0.0;
+  int tensile = 0;
+  if (on) stage_body<S>(E.a, k, E.h[nb], E.t_try[nb], E.tol[nb], E.eta[nb], emax, tensile);
+  if (S == 6) {
+    const int first = __shfl(nb, 0, 64);
+    const bool uniform = __all(nb == first);
+    if (uniform) {
+      const double m = wave_max(emax);
+      const int tf = __any(tensile) ? 1 : 0;
+      if ((threadIdx.x & 63) == 0 && first >= 0 && E.active[first]) {
+        atomicMax(reinterpret_cast<unsigned long long *>(&E.status[first].errmax_bits), __double_as_longlong(m));
+        if (tf) atomicOr(&E.status[first].tensile, 1);
+      }
+    } else if (on) {
+      atomicMax(reinterpret_cast<unsigned long long *>(&E.status[nb].errmax_bits), __double_as_longlong(emax));
+      if (tensile) atomicOr(&E.status[nb].tensile, 1);
+    }
+  }
+}
+
+// ytemp2 = y + B21*h*k1 with each active network's h (ode_bs.f90:213)
+__global__ void __launch_bounds__(256) ens_restart_stage2_kernel(EnsStageArgs E) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E.a.n) return;
+  const int nb = E.net[k];
+  if (!E.active[nb]) return;
+  const double h = E.h[nb];
+  const int n = E.a.n;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const int i = c * n + k;
+    E.a.ynext[i] = E.a.y[i] + CK::B21 * h * E.a.k[0][i];
+  }
+}
+
+__global__ void __launch_bounds__(256) ens_clear_status_kernel(StatusWord *st, const int *active, int nnet) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nnet && (!active || active[b])) { st[b].errmax_bits = 0ull; st[b].tensile = 0; }
+}
+
+__global__ void __launch_bounds__(256) ens_cg_dt_kernel(CgSys *sys, int nnet, int nslot, const double *t_try,
+                                                        const double *h, int first_slot) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nnet * nslot) return;
+  const int b = q / nslot, slot = first_slot + q % nslot;
+  const double A[6] = {0.0, CK::A2, CK::A3, CK::A4, CK::A5, CK::A6};
+  sys[q].dt = slot == 0 ? t_try[b] : 0.0 + A[slot] * h[b];
 }
 
 // ytemp2 = y + B21*h*k1 for a retried step size (ode_bs.f90:213)
@@ -514,6 +578,36 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 }
 
 __device__ __forceinline__ double hmean(double pa, double pb) { return 2.0 * pa * pb / (pa + pb); }
+
+// Multi-network launches: restricts the arguments to the system's network
+// (offset arrays, its sources, crossings and constants); false when the network
+// sits out this launch.
+__device__ __forceinline__ bool cg_view(CgArgs &A, CrossRows &X, const CgSys &S) {
+  if (!A.nets) return true;
+  if (A.active && !A.active[S.net]) return false;
+  const CgNet c = A.nets[S.net];
+  A.n = c.n;
+  A.ds += c.base;
+  A.P += c.base;
+  A.perm += c.base;
+  A.perm_diag += c.base;
+  A.perm_x += c.xbase;
+  A.perm_x_diag += c.xbase;
+  A.cfac = c.cfac;
+  A.time = c.time;
+  A.tol = c.tol;
+  A.por = c.por;
+  A.beta = c.beta;
+  A.src_elem += c.src0;
+  A.src_Q += c.src0;
+  A.src_tb += c.src0;
+  A.src_te += c.src0;
+  A.nsrc = c.nsrc;
+  A.cross += 6 * (int64_t)c.cross0;
+  A.ncross = c.ncross;
+  if (X.start) X.start += c.base;
+  return true;
+}
 
 // M v for system s: rows (A1 v_{k-1} + A2 v_k + A3 v_{k+1}) / diag_k plus the crossing terms.
 __device__ void cg_matvec(const CgArgs &A, const CgSys &S, double dt, const double *v, double *out) {
@@ -556,6 +650,8 @@ __device__ void cg_matvec(const CgArgs &A, const CgSys &S, double dt, const doub
 __global__ void __launch_bounds__(1024) cg_kernel(CgArgs A) {
   __shared__ double red[16];
   const CgSys S = A.sys[blockIdx.x];
+  CrossRows nox{nullptr, nullptr};
+  if (!cg_view(A, nox, S)) return;
   const double dt = S.dt;
   const int n = A.n;
   const double c = A.cfac;
@@ -951,6 +1047,7 @@ __global__ void __launch_bounds__(kCgLdsThreads) cg_lds_kernel(CgArgs A, CrossRo
   extern __shared__ double lds_p[];  // [n]
   __shared__ double red1[kCgLdsThreads / 64], red2[kCgLdsThreads / 64], red3[kCgLdsThreads / 64];
   const CgSys S = A.sys[blockIdx.x];
+  if (!cg_view(A, X, S)) return;
   const int n = A.n, tid = threadIdx.x, wid = tid >> 6, nw = kCgLdsThreads / 64;
   const double dt = S.dt, c = A.cfac;
   const double *ds = A.ds, *kx = A.perm_x, *kxd = A.perm_x_diag, *P = A.P;
@@ -1171,6 +1268,32 @@ void launch_stage(int stage, const StageArgs &a, hipStream_t s) {
   }
 }
 
+void launch_ens_stage(int stage, const EnsStageArgs &a, hipStream_t s) {
+  const dim3 g(nblk(a.a.n, 256)), b(256);
+  switch (stage) {
+    case 1: hipLaunchKernelGGL(ens_stage_kernel<1>, g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL(ens_stage_kernel<2>, g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL(ens_stage_kernel<3>, g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL(ens_stage_kernel<4>, g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL(ens_stage_kernel<5>, g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL(ens_stage_kernel<6>, g, b, 0, s, a); break;
+  }
+}
+
+void launch_ens_restart_stage2(const EnsStageArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(ens_restart_stage2_kernel, dim3(nblk(a.a.n, 256)), dim3(256), 0, s, a);
+}
+
+void launch_ens_clear_status(StatusWord *st, const int *active, int nnet, hipStream_t s) {
+  hipLaunchKernelGGL(ens_clear_status_kernel, dim3(nblk(nnet, 256)), dim3(256), 0, s, st, active, nnet);
+}
+
+void launch_ens_cg_dt(CgSys *sys, int nnet, int nslot, const double *t_try, const double *h, int first_slot,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(ens_cg_dt_kernel, dim3(nblk((int64_t)nnet * nslot, 256)), dim3(256), 0, s, sys, nnet, nslot,
+                     t_try, h, first_slot);
+}
+
 void launch_restart_stage2(const double *y, const double *k1, double h, int n3, double *yt, hipStream_t s) {
   hipLaunchKernelGGL(restart_stage2_kernel, dim3(nblk(n3, 256)), dim3(256), 0, s, y, k1, h, n3, yt);
 }
@@ -1211,17 +1334,26 @@ bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *part
          hipSuccess;
 }
 
-bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s) {
-  const size_t sh = sizeof(double) * (size_t)a.n;
-  if (a.n <= kCgLdsThreads * 2)
+static bool cg_lds_dispatch(const CgArgs &a, const CrossRows &xr, int nsys, int max_n, hipStream_t s) {
+  const size_t sh = sizeof(double) * (size_t)max_n;
+  if (max_n <= kCgLdsThreads * 2)
     hipLaunchKernelGGL(cg_lds_kernel<2>, dim3(nsys), dim3(kCgLdsThreads), sh, s, a, xr);
-  else if (a.n <= kCgLdsThreads * 4)
+  else if (max_n <= kCgLdsThreads * 4)
     hipLaunchKernelGGL(cg_lds_kernel<4>, dim3(nsys), dim3(kCgLdsThreads), sh, s, a, xr);
-  else if (a.n <= kCgLdsThreads * 8)
+  else if (max_n <= kCgLdsThreads * 8)
     hipLaunchKernelGGL(cg_lds_kernel<8>, dim3(nsys), dim3(kCgLdsThreads), sh, s, a, xr);
   else
     return false;
   return true;
+}
+
+bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s) {
+  return cg_lds_dispatch(a, xr, nsys, a.n, s);
+}
+
+// every system's network has at most max_n elements; the kernel is picked by max_n
+void launch_cg_lds_multi(const CgArgs &a, const CrossRows &xr, int nsys, int max_n, hipStream_t s) {
+  (void)cg_lds_dispatch(a, xr, nsys, max_n, s);
 }
 
 void launch_perm_update(const PermArgs &a, hipStream_t s) {

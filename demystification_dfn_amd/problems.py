@@ -229,3 +229,17 @@ def network_small(n_faults=12, seed=1, box=800.0, lmin=40.0, lmax=300.0, alpha=1
     """Small seeded Mode II network for parity tests (a few hundred elements)."""
     return fault_network(n_faults=n_faults, box=box, lmin=lmin, lmax=lmax, alpha=alpha, ds=ds,
                          seed=seed, name=kw.pop("name", f"network_small_{n_faults}_seed{seed}"), **kw)
+
+
+def network_ensemble(n_networks=64, faults=(50, 400), seed=7, **kw):
+    """Independent seeded networks for the ensemble engine: network i has a fault
+    count drawn uniformly in `faults`, its own geometry seed and therefore its own
+    friction draws (a per fault, weak/strong b), and otherwise the parameters of
+    fault_network (defaults: the 80k network's length law on [50 m, 2 km] in a
+    7 km box, so 50-400 faults carry ~5k-40k elements)."""
+    rng = np.random.default_rng(seed)
+    counts = rng.integers(faults[0], faults[1] + 1, n_networks)
+    seeds = rng.integers(0, 2**31 - 1, n_networks)
+    kw.setdefault("alpha", 0.1)
+    return [fault_network(n_faults=int(c), seed=int(s), name=f"ensemble{seed}_net{i}_{int(c)}faults", **kw)
+            for i, (c, s) in enumerate(zip(counts, seeds))]

@@ -177,6 +177,37 @@ int dfn_bench_stream(int64_t bytes, int32_t reps, double *gbps);
 /* Device/library description string (NUL-terminated, at most len bytes). */
 int dfn_device_info(char *buf, int32_t len);
 
+/* ---- ensembles of independent networks ------------------------------------
+ * No single counterpart in the reference, which runs one network per process
+ * (src/main.f90); each network of an ensemble follows the step of
+ * src/solver.f90:59-198 exactly as a dfn_handle would, with its own clock,
+ * trial step and accept/reject decisions, while the stages of all networks run
+ * as shared launches on one GPU.  All networks must use the H-matrix kernel,
+ * the same fracture mode and elastic constants (one operator scale), and
+ * permeability_coupling != 1. */
+typedef struct dfn_ensemble_s *dfn_ensemble;
+
+/* problems[0..nnet-1] as for dfn_create; the arrays may be freed after the call. */
+int dfn_ensemble_create(const dfn_problem *problems, int32_t nnet, dfn_ensemble *out);
+int dfn_ensemble_destroy(dfn_ensemble e);
+/* Builds every network's operators (as dfn_build_hmat) and the block-diagonal
+ * device layout of the whole ensemble. */
+int dfn_ensemble_build_hmat(dfn_ensemble e);
+/* DFN_PREC_FP64 / DFN_PREC_FP32 operator storage, as dfn_set_precision. */
+int dfn_ensemble_set_precision(dfn_ensemble e, int32_t precision);
+/* 1 (default): the per-attempt launch sequences are captured as HIP graphs and
+ * replayed; 0: the same launches are issued one by one. */
+int dfn_ensemble_set_graph(dfn_ensemble e, int32_t use_graph);
+/* Every network advances nsteps accepted steps. */
+int dfn_ensemble_solve(dfn_ensemble e, int64_t nsteps);
+/* State of network `net` in its own layout: Y = [V | theta | sigmaN] (3*N), P, P_dot
+ * (N each), time.  NULL pointers are skipped. */
+int dfn_ensemble_get_state(dfn_ensemble e, int32_t net, double *Y, double *P, double *P_dot, double *time);
+int dfn_ensemble_get_stats(dfn_ensemble e, int32_t net, dfn_run_stats *stats);
+/* info[0..5] = networks, total elements, largest network, matrix entries streamed
+ * per stress evaluation, algorithmic bytes per stress evaluation, graphs on. */
+int dfn_ensemble_info(dfn_ensemble e, int64_t *info);
+
 #ifdef __cplusplus
 }
 #endif
