@@ -137,6 +137,80 @@ def mixed_precision(sim, p, args, kb64):
                 note="continues from the FP64 run's final state; operators FP32, state FP64")
 
 
+def run_ensemble(args):
+    """Ensemble workload: independent seeded networks advanced together by the
+    ensemble engine (shared stage launches, per-network clocks and step sizes,
+    HIP-graph replay).  Throughput is the aggregate simulated time over all
+    networks per wall hour; the same networks run one handle at a time, and the
+    ensemble with graphs off, are timed beside it."""
+    from demystification_dfn_amd import _abi, problems
+    from demystification_dfn_amd.device import DeviceEnsemble, DeviceSimulation
+    probs = problems.network_ensemble(args.networks, faults=(5, 36), box=2500.0, seed=args.seed)
+    Ns = [p.N for p in probs]
+    desc = dict(workload=f"ensemble{len(probs)}_{sum(Ns) // 1000}k", networks=len(probs), faults_per_network="5-36",
+                box_m=2500, length_law="p(L)~L^-0.1 on [50 m, 2 km]", element_m=10, fracture_mode="II",
+                kernel="hmatrix", eta=2.0, eps_aca=1e-6, leaf=64, elements=sum(Ns), min_elements=min(Ns),
+                max_elements=max(Ns), seed=args.seed,
+                friction="per network: a~U[0.008,0.012] per fault, b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa")
+    log(f"[bench] ensemble of {len(probs)} networks, {sum(Ns)} elements (max {max(Ns)})")
+    ens = DeviceEnsemble(probs)
+    t0 = time.perf_counter()
+    ens.initiate_hmat()
+    setup_s = time.perf_counter() - t0
+    info = ens.info()
+    log(f"[bench] ensemble setup {setup_s:.1f} s {info}")
+
+    def timed(steps):
+        t_before = [ens.stats(i)["time"] for i in range(len(probs))]
+        w0 = time.perf_counter()
+        ens.solve(steps)
+        wall = time.perf_counter() - w0
+        dt = sum(ens.stats(i)["time"] - t_before[i] for i in range(len(probs)))
+        return dt, wall
+
+    ens.solve(args.warmup)
+    dt, wall = timed(args.steps)
+    value = (dt / YEAR) / (wall / 3600.0)
+    log(f"[bench] ensemble {args.steps} steps in {wall:.2f} s")
+    ens.set_graph(False)
+    dt_e, wall_e = timed(max(args.steps // 3, 1))
+    ens.set_graph(True)
+    ens.set_precision(_abi.PREC_FP32)
+    dt32, wall32 = timed(args.steps)
+    ens.set_precision(_abi.PREC_FP64)
+    rej = sum(ens.stats(i)["rejected_attempts"] for i in range(len(probs)))
+    acc = sum(ens.stats(i)["accepted_steps"] for i in range(len(probs)))
+    # the same networks one handle at a time (a sample of them, same step count)
+    nsamp = min(len(probs), args.ensemble_single_sample)
+    dt_s, wall_s = 0.0, 0.0
+    for p in probs[:nsamp]:
+        d = DeviceSimulation(p)
+        d.initiate_hmat()
+        d.solve(args.warmup)
+        s0 = d.stats()["time"]
+        w0 = time.perf_counter()
+        d.solve(args.steps)
+        wall_s += time.perf_counter() - w0
+        dt_s += d.stats()["time"] - s0
+        d.close()
+    single = (dt_s / YEAR) / (wall_s / 3600.0)
+    out = dict(
+        metric="simulated network-yr / wall hr, " + str(len(probs)) + "-network ensemble", value=value,
+        unit="simulated network-yr / wall hr", n_gpus=1, steps=args.steps, warmup=args.warmup,
+        ms_per_step=1e3 * wall / args.steps, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64",
+        data="synthetic (seeded generator)", config=desc,
+        ensemble=dict(setup_s=setup_s, S_entries=info["S_entries"], alg_bytes_per_stress_eval=info["alg_bytes"],
+                      accepted_steps=acc, rejected_attempts=rej,
+                      eager_no_graph=dict(value=(dt_e / YEAR) / (wall_e / 3600.0),
+                                          ms_per_step=1e3 * wall_e / max(args.steps // 3, 1)),
+                      fp32_operators=dict(value=(dt32 / YEAR) / (wall32 / 3600.0), ms_per_step=1e3 * wall32 / args.steps),
+                      one_handle_at_a_time=dict(value=single, networks_sampled=nsamp,
+                                                note="same networks, one dfn_handle each, run in turn"),
+                      speedup_vs_one_at_a_time=value / single),
+        roofline=None, cpu_baseline=None)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,16 +224,20 @@ def main():
                     help="steps of the FP32-operator (mixed precision) run on the same workload; 0 skips it")
     ap.add_argument("--mixed-vectors", type=int, default=200,
                     help="random log-uniform slip-rate vectors for the FP32-vs-FP64 stress-rate error")
+    ap.add_argument("--networks", type=int, default=64, help="networks of the ensemble workload")
+    ap.add_argument("--ensemble-single-sample", type=int, default=16,
+                    help="ensemble networks also run one handle at a time, for the batching comparison")
     ap.add_argument("--copy-probe-gb", type=float, default=2.0,
                     help="size of the copy kernel whose rate is reported as the achievable HBM bandwidth")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this path has no multi-GPU form (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: every workload here runs on a single GPU", file=sys.stderr)
         sys.exit(2)
 
     from demystification_dfn_amd.device import DeviceSimulation, bench_stream, device_info
     log("[bench]", device_info())
+    if args.workload == "ensemble":
+        return run_ensemble(args)
     p, desc = make_workload(args.workload, args.seed)
     log(f"[bench] workload {desc['workload']} N={p.N}")
     sim = DeviceSimulation(p)
