@@ -137,6 +137,52 @@ def mixed_precision(sim, p, args, kb64):
                 note="continues from the FP64 run's final state; operators FP32, state FP64")
 
 
+def learned_step(args, probs):
+    """Step-size control on the ensemble: pairs recorded from an rkqs-controlled
+    run train the proposal network on the device; then the same networks, from
+    the same initial state, are stepped to a common final time under rkqs, the
+    PI controller and the learned controller.  Reports steps, rejected-attempt
+    fraction and aggregate simulated time per wall hour for each."""
+    from demystification_dfn_amd import _abi
+    from demystification_dfn_amd.device import DeviceEnsemble, DeviceMlp
+    rec = DeviceEnsemble(probs)
+    rec.initiate_hmat()
+    rec.set_recording(True)
+    rec.solve(args.learned_steps)
+    X, T = rec.take_records()
+    t_end = min(rec.stats(i)["time"] for i in range(len(probs)))
+    rec.close()
+    mlp = DeviceMlp()
+    out = dict(pairs=int(T.shape[0]), t_end_s=t_end)
+    if T.shape[0] >= 16:
+        norm = np.concatenate([X.mean(0), X.std(0) + 1e-3, [T.mean()], [T.std() + 1e-3]])
+        p0, _ = mlp.get_params()
+        mlp.set_params(p0, norm)
+        l0, _ = mlp.loss_grad(X, T)
+        w0 = time.perf_counter()
+        lf = mlp.train(X, T, epochs=args.learned_epochs, batch=min(256, T.shape[0]), lr=1e-3, seed=1)
+        out["training"] = dict(epochs=args.learned_epochs, loss_before=l0, loss_after=lf,
+                               seconds=time.perf_counter() - w0)
+    for key, mode in (("rkqs", _abi.STEP_RKQS), ("pi", _abi.STEP_PI), ("learned", _abi.STEP_LEARNED)):
+        if mode == _abi.STEP_LEARNED and "training" not in out:
+            continue
+        ens = DeviceEnsemble(probs)
+        ens.initiate_hmat()
+        ens.set_controller(mode, mlp if mode == _abi.STEP_LEARNED else None)
+        w0 = time.perf_counter()
+        ens.solve_until(t_end, max_steps=100000)
+        wall = time.perf_counter() - w0
+        acc = sum(ens.stats(i)["accepted_steps"] for i in range(len(probs)))
+        rej = sum(ens.stats(i)["rejected_attempts"] for i in range(len(probs)))
+        out[key] = dict(value=(len(probs) * t_end / YEAR) / (wall / 3600.0), unit="simulated network-yr / wall hr",
+                        accepted_steps=acc, rejected_attempts=rej, rejected_fraction=rej / max(acc + rej, 1),
+                        wall_s=wall)
+        ens.close()
+    out["note"] = ("pairs exclude steps whose successor the one-day cap set; all controllers start from t=0 and "
+                   "land on t_end; every step is accepted or rejected by the Cash-Karp test")
+    return out
+
+
 def run_ensemble(args):
     """Ensemble workload: independent seeded networks advanced together by the
     ensemble engine (shared stage launches, per-network clocks and step sizes,
@@ -208,6 +254,9 @@ def run_ensemble(args):
                                                 note="same networks, one dfn_handle each, run in turn"),
                       speedup_vs_one_at_a_time=value / single),
         roofline=None, cpu_baseline=None)
+    if args.learned_steps > 0:
+        log("[bench] learned step-size control")
+        out["step_control"] = learned_step(args, probs)
     print(json.dumps(out), flush=True)
 
 
@@ -227,6 +276,9 @@ def main():
     ap.add_argument("--networks", type=int, default=64, help="networks of the ensemble workload")
     ap.add_argument("--ensemble-single-sample", type=int, default=16,
                     help="ensemble networks also run one handle at a time, for the batching comparison")
+    ap.add_argument("--learned-steps", type=int, default=300,
+                    help="ensemble workload: steps of the recording run for the step-size network; 0 skips it")
+    ap.add_argument("--learned-epochs", type=int, default=30)
     ap.add_argument("--copy-probe-gb", type=float, default=2.0,
                     help="size of the copy kernel whose rate is reported as the achievable HBM bandwidth")
     args = ap.parse_args()

@@ -34,10 +34,12 @@ MODE_II, MODE_III = 2, 3
 KERNEL_DIRECT, KERNEL_HMATRIX = 0, 1
 OP_T, OP_N = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
+STEP_RKQS, STEP_PI, STEP_LEARNED = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
 
 
 class DfnProblem(C.Structure):
@@ -108,6 +110,14 @@ SIGNATURES = {
     "bench_kernel": [_H, C.c_int32, _dp],
     "bench_stream": [C.c_int64, C.c_int32, _dp],
     "device_info": [C.c_char_p, C.c_int32],
+    # step-size proposal network
+    "mlp_create": [C.POINTER(C.c_void_p)],
+    "mlp_destroy": [_H],
+    "mlp_set_params": [_H, _fp, _fp],
+    "mlp_get_params": [_H, _fp, _fp],
+    "mlp_forward": [_H, _fp, C.c_int64, _fp],
+    "mlp_loss_grad": [_H, _fp, _fp, C.c_int64, _dp, _fp],
+    "mlp_train": [_H, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _dp],
     # ensembles of independent networks
     "ensemble_create": [C.POINTER(DfnProblem), C.c_int32, C.POINTER(C.c_void_p)],
     "ensemble_destroy": [_H],
@@ -115,9 +125,13 @@ SIGNATURES = {
     "ensemble_set_precision": [_H, C.c_int32],
     "ensemble_set_graph": [_H, C.c_int32],
     "ensemble_solve": [_H, C.c_int64],
+    "ensemble_solve_until": [_H, C.c_double, C.c_int64],
     "ensemble_get_state": [_H, C.c_int32, _dp, _dp, _dp, C.POINTER(C.c_double)],
     "ensemble_get_stats": [_H, C.c_int32, C.POINTER(DfnRunStats)],
     "ensemble_info": [_H, _i64p],
+    "ensemble_set_controller": [_H, C.c_int32, _H],
+    "ensemble_set_recording": [_H, C.c_int32],
+    "ensemble_take_records": [_H, _fp, _fp, C.c_int64, _i64p],
 }
 
 # entry points every build of the device library must export (checked by the CPU tests)
@@ -150,6 +164,18 @@ def i64ptr(a):
         return None
     assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_i64p)
+
+
+def fptr(a):
+    """Pointer to a C-contiguous float32 numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_fp)
+
+
+MLP_INPUTS = 8
+MLP_PARAMS = 4801
 
 
 def i32ptr(a):

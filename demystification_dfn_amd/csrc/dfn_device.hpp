@@ -156,6 +156,9 @@ struct EnsStageArgs {
 void launch_ens_stage(int stage, const EnsStageArgs &a, hipStream_t s);
 void launch_ens_restart_stage2(const EnsStageArgs &a, hipStream_t s);
 void launch_ens_clear_status(StatusWord *st, const int *active, int nnet, hipStream_t s);
+// per-network max |V| over real elements (bits of non-negative doubles, cleared first)
+void launch_ens_vmax(const double *V, const int *net, const uint8_t *fake, int n, unsigned long long *vmax, int nnet,
+                     hipStream_t s);
 // dt of every (network, slot) system of an ensemble compute_P launch: slot 0 and,
 // when with_stages, slots 1..5 (x + A_i h with x = 0), from t_try (slot 0) and h
 void launch_ens_cg_dt(CgSys *sys, int nnet, int nslot, const double *t_try, const double *h, int first_slot,

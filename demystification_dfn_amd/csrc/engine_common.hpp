@@ -255,4 +255,11 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   return DFN_OK;
 }
 
+// step-size proposal network (mlp.hip): forward pass of n feature rows already on
+// the device, normalised outputs into dY; the output's physical value
+struct Mlp;
+int mlp_forward_device(Mlp *m, const float *dX, int n, float *dY, hipStream_t s);
+float mlp_output_scale(const Mlp *m, float y);
+Mlp *mlp_of(dfn_mlp h);
+
 }  // namespace dfn

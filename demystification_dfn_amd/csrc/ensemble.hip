@@ -121,6 +121,20 @@ struct EnsEngine {
   hipGraphExec_t g_try = nullptr, g_retry = nullptr;
   const DevOps *graph_ops = nullptr;
 
+  // step-size control: 0 = rkqs's proposal (the reference), 1 = PI controller,
+  // 2 = the proposal network with the PI controller as fallback
+  int controller = 0;
+  Mlp *mlp = nullptr;
+  bool recording = false;
+  std::vector<float> rec_X, rec_T;  // (features, log10 accepted-step ratio) pairs
+  std::vector<double> prev_hdid, prev_err, prev_vmax, pend_href;
+  std::vector<int> fallback, learned_last, has_pend, pend_skip;
+  std::vector<float> pend_feat;
+  DMem vmax_bits, feat_d, out_d;
+  float *h_feat = nullptr, *h_out = nullptr;
+  unsigned long long *h_vmax = nullptr;
+  static constexpr int kFallbackSteps = 3;
+
   double *kk(int i) const { return kbuf.as<double>() + (size_t)i * 3 * Ntot; }
   double *cg_vec(int slot, int which) const { return cg_work.as<double>() + ((size_t)slot * 12 + which) * Ntot; }
 
@@ -133,6 +147,9 @@ struct EnsEngine {
     if (h_cgres) (void)hipHostFree(h_cgres);
     if (h_status) (void)hipHostFree(h_status);
     if (h_nets) (void)hipHostFree(h_nets);
+    if (h_feat) (void)hipHostFree(h_feat);
+    if (h_out) (void)hipHostFree(h_out);
+    if (h_vmax) (void)hipHostFree(h_vmax);
     ops64.reset();
     ops32.reset();
     if (stream) (void)hipStreamDestroy(stream);
@@ -267,6 +284,101 @@ struct EnsEngine {
     time.assign(B, 0.0);
     t_try.assign(B, 100.0);
     st.assign(B, dfn_run_stats{});
+    DFN_TRY(vmax_bits.alloc(sizeof(unsigned long long) * B));
+    DFN_TRY(feat_d.alloc(sizeof(float) * DFN_MLP_INPUTS * B));
+    DFN_TRY(out_d.alloc(sizeof(float) * B));
+    DFN_HIP(hipHostMalloc((void **)&h_feat, sizeof(float) * DFN_MLP_INPUTS * B, hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_out, sizeof(float) * B, hipHostMallocDefault));
+    DFN_HIP(hipHostMalloc((void **)&h_vmax, sizeof(unsigned long long) * B, hipHostMallocDefault));
+    prev_hdid.assign(B, 0.0);
+    prev_err.assign(B, 0.0);
+    prev_vmax.assign(B, 0.0);
+    pend_href.assign(B, 0.0);
+    fallback.assign(B, 0);
+    learned_last.assign(B, 0);
+    has_pend.assign(B, 0);
+    pend_skip.assign(B, 0);
+    pend_feat.assign((size_t)DFN_MLP_INPUTS * B, 0.0f);
+    return DFN_OK;
+  }
+
+  // PI step-size controller (Gustafsson): exponents 0.7/5 on the current and 0.4/5
+  // on the previous scaled error, growth within [0.2, 5] per step
+  static double pi_next(double h, double err, double err_prev) {
+    const double e = std::fmax(err, 1e-10), ep = std::fmax(err_prev, 1e-10);
+    double fac = 0.9 * std::pow(e, -0.7 / 5.0) * std::pow(ep, 0.4 / 5.0);
+    fac = std::fmin(5.0, std::fmax(0.2, fac));
+    return h * fac;
+  }
+
+  // the proposal network's features of network i after a step
+  void features(int i, double hdid, double href, double err, int nrej, double vmax, float *f) const {
+    const double tiny = 1e-30;
+    f[0] = (float)std::log10(std::fmax(hdid, tiny));
+    f[1] = (float)std::log10(std::fmax(href, tiny));
+    f[2] = (float)std::log10(std::fmax(err, 1e-10));
+    f[3] = (float)nrej;
+    f[4] = (float)std::log10(std::fmax(vmax, tiny));
+    f[5] = prev_hdid[i] > 0.0 ? (float)std::log10(hdid / prev_hdid[i]) : 0.0f;
+    f[6] = prev_hdid[i] > 0.0 ? (float)std::log10(std::fmax(prev_err[i], 1e-10)) : f[2];
+    f[7] = prev_vmax[i] > 0.0 ? (float)std::log10(std::fmax(vmax, tiny) / prev_vmax[i]) : 0.0f;
+  }
+
+  // next trial steps from this step's outcome (t_try before the one-day cap)
+  int control(const std::vector<double> &hdid, const std::vector<double> &href, const std::vector<double> &err,
+              const std::vector<int> &nrej, const std::vector<int> &guarded, const std::vector<int> &stepping,
+              const std::vector<int> &clipped) {
+    if (controller == 0 && !recording) {
+      for (int i = 0; i < B; i++)
+        if (stepping[i]) t_try[i] = href[i];
+      return DFN_OK;
+    }
+    launch_ens_vmax(Y.as<double>(), net.as<int>(), fake.as<uint8_t>(), Ntot, vmax_bits.as<unsigned long long>(), B,
+                    stream);
+    DFN_HIP(hipMemcpyAsync(h_vmax, vmax_bits.p, sizeof(unsigned long long) * B, hipMemcpyDeviceToHost, stream));
+    DFN_HIP(hipStreamSynchronize(stream));
+    std::vector<int> use_mlp(B, 0);
+    for (int i = 0; i < B; i++) {
+      if (!stepping[i]) continue;
+      double vmax;
+      std::memcpy(&vmax, &h_vmax[i], sizeof(double));
+      float *f = h_feat + (size_t)i * DFN_MLP_INPUTS;
+      features(i, hdid[i], href[i], err[i], nrej[i], vmax, f);
+      if (recording) {
+        if (has_pend[i] && !pend_skip[i] && !guarded[i] && !clipped[i]) {
+          rec_X.insert(rec_X.end(), pend_feat.begin() + (size_t)i * DFN_MLP_INPUTS,
+                       pend_feat.begin() + (size_t)(i + 1) * DFN_MLP_INPUTS);
+          rec_T.push_back((float)std::log10(hdid[i] / pend_href[i]));
+        }
+        std::copy(f, f + DFN_MLP_INPUTS, pend_feat.begin() + (size_t)i * DFN_MLP_INPUTS);
+        pend_href[i] = href[i];
+        // the cap or the landing on a final time, not the controller, sets the next step
+        pend_skip[i] = href[i] > 24 * 60 * 60 || clipped[i];
+        has_pend[i] = 1;
+      }
+      if (controller == 2 && learned_last[i] && nrej[i] > 0) fallback[i] = kFallbackSteps;
+      if (controller == 0) t_try[i] = href[i];
+      else if (controller == 1 || fallback[i] > 0) t_try[i] = pi_next(hdid[i], err[i], prev_err[i]);
+      else use_mlp[i] = 1;
+      learned_last[i] = use_mlp[i];
+      if (controller == 2 && !use_mlp[i] && fallback[i] > 0) fallback[i]--;
+      prev_hdid[i] = hdid[i];
+      prev_err[i] = err[i];
+      prev_vmax[i] = std::fmax(vmax, 1e-30);
+    }
+    if (controller == 2) {
+      if (!mlp) return DFN_ERR_STATE;
+      DFN_HIP(hipMemcpyAsync(feat_d.p, h_feat, sizeof(float) * DFN_MLP_INPUTS * B, hipMemcpyHostToDevice, stream));
+      DFN_TRY(mlp_forward_device(mlp, feat_d.as<float>(), B, out_d.as<float>(), stream));
+      DFN_HIP(hipMemcpyAsync(h_out, out_d.p, sizeof(float) * B, hipMemcpyDeviceToHost, stream));
+      DFN_HIP(hipStreamSynchronize(stream));
+      for (int i = 0; i < B; i++)
+        if (use_mlp[i]) {
+          // log10 of the ratio to rkqs's proposal, kept within [1/10, 2]
+          const double y = std::fmin(0.30103, std::fmax(-1.0, (double)mlp_output_scale(mlp, h_out[i])));
+          t_try[i] = href[i] * std::pow(10.0, y);
+        }
+    }
     return DFN_OK;
   }
 
@@ -449,19 +561,25 @@ struct EnsEngine {
   }
 
   // one iteration of src/solver.f90:59-198 for every network
-  int step() {
+  // One step of every network with part[i] != 0 (all when part is null); with
+  // t_end > 0 a participating network's trial step is clipped so that it lands
+  // on t_end.  Networks sitting out keep their state: no launch writes them.
+  int step(const std::vector<int> *part = nullptr, double t_end = 0.0) {
     const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = (double)1.89e-4f;
-    std::vector<double> h(B), hdid(B), hnext(B);
-    std::vector<int> todo(B, 1);
+    std::vector<double> h(B), hdid(B), hnext(B), err_acc(B, 0.0);
+    std::vector<int> todo(B, 1), nrej(B, 0), guarded(B, 0), clipped(B, 0);
+    if (part) todo = *part;
     for (int i = 0; i < B; i++) {
+      if (!todo[i]) continue;
       const HostProblem &p = hp[i];
       if (t_try[i] > 10.0)  // :64-77
         for (size_t s = 0; s < p.src_tb.size(); s++)
-          if ((time[i] < p.src_tb[s]) && (time[i] + t_try[i] > p.src_tb[s])) t_try[i] = 10.0;
+          if ((time[i] < p.src_tb[s]) && (time[i] + t_try[i] > p.src_tb[s])) { t_try[i] = 10.0; guarded[i] = 1; }
+      if (t_end > 0.0 && time[i] + t_try[i] >= t_end) { t_try[i] = t_end - time[i]; clipped[i] = 1; }
       nets[i].time = time[i];
     }
     // first attempt (with compute_P(t_try) halving, :81-92)
-    std::vector<int> act(B, 1);
+    std::vector<int> act = todo;
     for (;;) {
       for (int i = 0; i < B; i++) {
         h_ttry[i] = t_try[i];
@@ -479,6 +597,7 @@ struct EnsEngine {
         st[i].cg_solves += 1;
         if (r0[0]) {  // compute_P(t_try) failed: halve and redo this network
           t_try[i] = t_try[i] / 2.0;
+          clipped[i] = 0;
           st[i].cg_halvings++;
           again = true;
         } else {
@@ -489,9 +608,11 @@ struct EnsEngine {
     }
     // rkqs (ode_bs.f90:125-174) per network on the attempts just made
     for (int i = 0; i < B; i++) {
+      if (!todo[i]) continue;
       h[i] = t_try[i];
       st[i].rhs_evaluations += 1;
     }
+    const std::vector<int> stepping = todo;
     bool any = true;
     while (any) {
       any = false;
@@ -510,6 +631,7 @@ struct EnsEngine {
         std::memcpy(&em, &h_status[i].errmax_bits, sizeof(double));
         const double errmax = em / hp[i].tol;
         if (errmax <= 1.0) {
+          err_acc[i] = errmax;
           hdid[i] = h[i];
           hnext[i] = errmax > ERRCON ? SAFETY * h[i] * std::pow(errmax, PGROW) : 5.0 * h[i];
           todo[i] = 0;
@@ -517,6 +639,8 @@ struct EnsEngine {
           continue;
         }
         st[i].rejected_attempts++;
+        nrej[i]++;
+        clipped[i] = 0;  // the retried step stops short of t_end
         const double htemp = SAFETY * h[i] * std::pow(errmax, PSHRNK);
         h[i] = std::copysign(std::fmax(std::fabs(htemp), 0.1 * std::fabs(h[i])), h[i]);
         if (0.0 + h[i] == 0.0) return DFN_ERR_UNDERFLOW;
@@ -535,11 +659,16 @@ struct EnsEngine {
     DFN_HIP(hipMemcpyAsync(Pdot.p, cg_vec(4, 11), sizeof(double) * Ntot, hipMemcpyDeviceToDevice, stream));
     bool tensile = false;
     for (int i = 0; i < B; i++) {
-      t_try[i] = hnext[i];
-      if (t_try[i] > 24 * 60 * 60) t_try[i] = 24 * 60 * 60;
-      time[i] = time[i] + hdid[i];
+      if (!stepping[i]) continue;
+      time[i] = clipped[i] ? t_end : time[i] + hdid[i];
       st[i].accepted_steps++;
       if (h_status[i].tensile) tensile = true;
+    }
+    const std::vector<double> t_keep = t_try;
+    DFN_TRY(control(hdid, hnext, err_acc, nrej, guarded, stepping, clipped));
+    for (int i = 0; i < B; i++) {
+      if (!stepping[i]) t_try[i] = t_keep[i];
+      if (t_try[i] > 24 * 60 * 60) t_try[i] = 24 * 60 * 60;
     }
     return tensile ? DFN_ERR_TENSILE : DFN_OK;
   }
@@ -600,6 +729,23 @@ API int dfn_ensemble_solve(dfn_ensemble h, int64_t nsteps) {
   return DFN_OK;
 }
 
+API int dfn_ensemble_solve_until(dfn_ensemble h, double t_end, int64_t max_steps) {
+  if (!h || !(t_end > 0.0) || max_steps < 0) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  if (!e.active_ops()) return DFN_ERR_STATE;
+  for (int64_t s = 0; s < max_steps; s++) {
+    std::vector<int> part(e.B, 0);
+    bool any = false;
+    for (int i = 0; i < e.B; i++)
+      if (e.time[i] < t_end) { part[i] = 1; any = true; }
+    if (!any) return DFN_OK;
+    DFN_TRY(e.step(&part, t_end));
+  }
+  for (int i = 0; i < e.B; i++)
+    if (e.time[i] < t_end) return DFN_ERR_STATE;  // max_steps ran out first
+  return DFN_OK;
+}
+
 API int dfn_ensemble_get_state(dfn_ensemble h, int32_t netw, double *Yh, double *Ph, double *Pdh, double *time) {
   if (!h || netw < 0 || netw >= h->e.B) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
@@ -625,6 +771,39 @@ API int dfn_ensemble_get_stats(dfn_ensemble h, int32_t netw, dfn_run_stats *s) {
   DFN_HIP(hipStreamSynchronize(e.stream));
   DFN_HIP(hipMemcpy(V.data(), e.Y.as<double>() + e.base[netw], N * sizeof(double), hipMemcpyDeviceToHost));
   s->max_V = *std::max_element(V.begin(), V.end());
+  return DFN_OK;
+}
+
+API int dfn_ensemble_set_controller(dfn_ensemble h, int32_t mode, dfn_mlp m) {
+  if (!h || mode < DFN_STEP_RKQS || mode > DFN_STEP_LEARNED) return DFN_ERR_ARG;
+  if (mode == DFN_STEP_LEARNED && !m) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  e.controller = mode;
+  e.mlp = mode == DFN_STEP_LEARNED ? dfn::mlp_of(m) : nullptr;
+  std::fill(e.fallback.begin(), e.fallback.end(), 0);
+  std::fill(e.learned_last.begin(), e.learned_last.end(), 0);
+  return DFN_OK;
+}
+
+API int dfn_ensemble_set_recording(dfn_ensemble h, int32_t on) {
+  if (!h) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  e.recording = on != 0;
+  std::fill(e.has_pend.begin(), e.has_pend.end(), 0);
+  return DFN_OK;
+}
+
+API int dfn_ensemble_take_records(dfn_ensemble h, float *X, float *T, int64_t cap, int64_t *count) {
+  if (!h || !count || cap < 0) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  const int64_t avail = (int64_t)e.rec_T.size();
+  if (!X || !T) { *count = avail; return DFN_OK; }
+  const int64_t n = std::min(cap, avail);
+  std::copy(e.rec_X.begin(), e.rec_X.begin() + n * DFN_MLP_INPUTS, X);
+  std::copy(e.rec_T.begin(), e.rec_T.begin() + n, T);
+  e.rec_X.erase(e.rec_X.begin(), e.rec_X.begin() + n * DFN_MLP_INPUTS);
+  e.rec_T.erase(e.rec_T.begin(), e.rec_T.begin() + n);
+  *count = n;
   return DFN_OK;
 }
 

@@ -531,6 +531,27 @@ __global__ void __launch_bounds__(256) ens_restart_stage2_kernel(EnsStageArgs E)
   }
 }
 
+// max |V| over each network's real elements, as the bits of a non-negative double
+__global__ void __launch_bounds__(256) ens_vmax_kernel(const double *__restrict__ V, const int *__restrict__ net,
+                                                       const uint8_t *__restrict__ fake, int n,
+                                                       unsigned long long *__restrict__ vmax) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = k < n ? net[k] : -1;
+  const double v = (k < n && !fake[k]) ? fabs(V[k]) : 0.0;
+  const int first = __shfl(nb, 0, 64);
+  if (__all(nb == first)) {
+    const double m = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && first >= 0) atomicMax(&vmax[first], (unsigned long long)__double_as_longlong(m));
+  } else if (nb >= 0) {
+    atomicMax(&vmax[nb], (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+__global__ void __launch_bounds__(256) fill_u64_kernel(unsigned long long *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 __global__ void __launch_bounds__(256) ens_clear_status_kernel(StatusWord *st, const int *active, int nnet) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nnet && (!active || active[b])) { st[b].errmax_bits = 0ull; st[b].tensile = 0; }
@@ -1282,6 +1303,12 @@ void launch_ens_stage(int stage, const EnsStageArgs &a, hipStream_t s) {
 
 void launch_ens_restart_stage2(const EnsStageArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(ens_restart_stage2_kernel, dim3(nblk(a.a.n, 256)), dim3(256), 0, s, a);
+}
+
+void launch_ens_vmax(const double *V, const int *net, const uint8_t *fake, int n, unsigned long long *vmax, int nnet,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nnet, 256)), dim3(256), 0, s, vmax, nnet);
+  hipLaunchKernelGGL(ens_vmax_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, V, net, fake, n, vmax);
 }
 
 void launch_ens_clear_status(StatusWord *st, const int *active, int nnet, hipStream_t s) {

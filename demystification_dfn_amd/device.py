@@ -95,6 +95,10 @@ class DeviceEnsemble:
     def solve(self, nsteps):
         _abi.check(self._fns["ensemble_solve"](self._h, int(nsteps)), "dfn_ensemble_solve")
 
+    def solve_until(self, t_end, max_steps=1_000_000):
+        _abi.check(self._fns["ensemble_solve_until"](self._h, float(t_end), int(max_steps)),
+                   "dfn_ensemble_solve_until")
+
     def get_state(self, net):
         N = self.problems[net].N
         Y, P, Pd, t = np.zeros(3 * N), np.zeros(N), np.zeros(N), C.c_double()
@@ -107,8 +111,82 @@ class DeviceEnsemble:
         _abi.check(self._fns["ensemble_get_stats"](self._h, int(net), C.byref(s)), "dfn_ensemble_get_stats")
         return s.as_dict()
 
+    def set_controller(self, mode, mlp=None):
+        h = mlp._h if mlp is not None else None
+        _abi.check(self._fns["ensemble_set_controller"](self._h, int(mode), h), "dfn_ensemble_set_controller")
+        self._mlp = mlp  # the network must outlive its use here
+
+    def set_recording(self, on):
+        _abi.check(self._fns["ensemble_set_recording"](self._h, 1 if on else 0), "dfn_ensemble_set_recording")
+
+    def take_records(self):
+        n = C.c_int64()
+        _abi.check(self._fns["ensemble_take_records"](self._h, None, None, 0, C.byref(n)), "dfn_ensemble_take_records")
+        X = np.zeros((n.value, _abi.MLP_INPUTS), np.float32)
+        T = np.zeros(n.value, np.float32)
+        if n.value:
+            _abi.check(self._fns["ensemble_take_records"](self._h, _abi.fptr(X), _abi.fptr(T), n.value, C.byref(n)),
+                       "dfn_ensemble_take_records")
+        return X[:n.value], T[:n.value]
+
     def info(self):
         v = np.zeros(6, dtype=np.int64)
         _abi.check(self._fns["ensemble_info"](self._h, _abi.i64ptr(v)), "dfn_ensemble_info")
         return dict(zip(("networks", "elements", "max_elements", "S_entries", "alg_bytes", "graph"),
                         (int(x) for x in v)))
+
+
+class DeviceMlp:
+    """The step-size proposal network on the device (dfn_mlp_*): 8 -> 64 -> 64 -> 1,
+    GELU, BF16 weights, FP32 accumulation, Adam training."""
+
+    def __init__(self):
+        self._fns = load()
+        h = C.c_void_p()
+        _abi.check(self._fns["mlp_create"](C.byref(h)), "dfn_mlp_create")
+        self._h = h
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            self._fns["mlp_destroy"](self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params, norm=None):
+        p = np.ascontiguousarray(params, np.float32)
+        nm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        _abi.check(self._fns["mlp_set_params"](self._h, _abi.fptr(p), _abi.fptr(nm)), "dfn_mlp_set_params")
+
+    def get_params(self):
+        p = np.zeros(_abi.MLP_PARAMS, np.float32)
+        nm = np.zeros(2 * _abi.MLP_INPUTS + 2, np.float32)
+        _abi.check(self._fns["mlp_get_params"](self._h, _abi.fptr(p), _abi.fptr(nm)), "dfn_mlp_get_params")
+        return p, nm
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros(X.shape[0], np.float32)
+        _abi.check(self._fns["mlp_forward"](self._h, _abi.fptr(X), X.shape[0], _abi.fptr(Y)), "dfn_mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(_abi.MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        _abi.check(self._fns["mlp_loss_grad"](self._h, _abi.fptr(X), _abi.fptr(T), X.shape[0], C.byref(loss),
+                                              _abi.fptr(g)), "dfn_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, X, T, epochs=10, batch=1024, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = C.c_double()
+        _abi.check(self._fns["mlp_train"](self._h, _abi.fptr(X), _abi.fptr(T), X.shape[0], int(epochs), int(batch),
+                                          float(lr), int(seed), C.byref(loss)), "dfn_mlp_train")
+        return loss.value

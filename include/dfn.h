@@ -177,6 +177,32 @@ int dfn_bench_stream(int64_t bytes, int32_t reps, double *gbps);
 /* Device/library description string (NUL-terminated, at most len bytes). */
 int dfn_device_info(char *buf, int32_t len);
 
+/* ---- step-size proposal network ------------------------------------------
+ * No counterpart in the reference (its step size comes from rkqs alone,
+ * src/ode_solver/ode_bs.f90:125-174).  An 8 -> 64 -> 64 -> 1 perceptron with
+ * GELU hidden layers, BF16 weights and FP32 accumulation.  Parameters (FP32
+ * master copy, DFN_MLP_PARAMS values): W1 (64 x 8, row-major), b1 (64), W2
+ * (64 x 64), b2 (64), w3 (64), b3.  norm (18 values): feature means (8), feature
+ * scales (8), output mean, output scale; the network sees (x - mean) / scale and
+ * its output y maps to mean_y + scale_y * y.  Loss: mean of (y - (t - mean_y) /
+ * scale_y)^2. */
+#define DFN_MLP_INPUTS 8
+#define DFN_MLP_PARAMS 4801
+typedef struct dfn_mlp_s *dfn_mlp;
+int dfn_mlp_create(dfn_mlp *out);  /* He-initialised with a fixed seed, identity normalisation */
+int dfn_mlp_destroy(dfn_mlp m);
+int dfn_mlp_set_params(dfn_mlp m, const float *params, const float *norm); /* norm may be NULL; resets Adam */
+int dfn_mlp_get_params(dfn_mlp m, float *params, float *norm);
+/* Y[i] = network(X[i*8 .. i*8+7]) for host arrays, n samples */
+int dfn_mlp_forward(dfn_mlp m, const float *X, int64_t n, float *Y);
+/* loss and its gradient (DFN_MLP_PARAMS values, with respect to the BF16 weights
+ * the forward pass uses) over the n samples (X, targets T) */
+int dfn_mlp_loss_grad(dfn_mlp m, const float *X, const float *T, int64_t n, double *loss, float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8) over `epochs` shuffled passes of whole
+ * mini-batches of `batch` samples; *final_loss (may be NULL) = loss over all n */
+int dfn_mlp_train(dfn_mlp m, const float *X, const float *T, int64_t n, int32_t epochs, int32_t batch, float lr,
+                  uint64_t seed, double *final_loss);
+
 /* ---- ensembles of independent networks ------------------------------------
  * No single counterpart in the reference, which runs one network per process
  * (src/main.f90); each network of an ensemble follows the step of
@@ -200,10 +226,37 @@ int dfn_ensemble_set_precision(dfn_ensemble e, int32_t precision);
 int dfn_ensemble_set_graph(dfn_ensemble e, int32_t use_graph);
 /* Every network advances nsteps accepted steps. */
 int dfn_ensemble_solve(dfn_ensemble e, int64_t nsteps);
+/* Steps every network until its time reaches t_end exactly: a network's trial
+ * step is clipped to land on t_end (a retried attempt stops short and the
+ * network steps again), and networks already at t_end sit out.  Fails with
+ * DFN_ERR_STATE if max_steps ensemble steps do not bring every network there. */
+int dfn_ensemble_solve_until(dfn_ensemble e, double t_end, int64_t max_steps);
 /* State of network `net` in its own layout: Y = [V | theta | sigmaN] (3*N), P, P_dot
  * (N each), time.  NULL pointers are skipped. */
 int dfn_ensemble_get_state(dfn_ensemble e, int32_t net, double *Y, double *P, double *P_dot, double *time);
 int dfn_ensemble_get_stats(dfn_ensemble e, int32_t net, dfn_run_stats *stats);
+/* Step-size control of every network after an accepted step:
+ * DFN_STEP_RKQS (default) rkqs's proposal (src/ode_solver/ode_bs.f90:165-169),
+ * the reference's; DFN_STEP_PI a PI controller (exponents 0.7/5 and 0.4/5 on the
+ * current and previous scaled error, growth within [0.2, 5]); DFN_STEP_LEARNED
+ * rkqs's proposal times 10^y, y from the proposal network `m` (clamped to
+ * [-1, log10 2]), with the PI controller for 3 steps after a learned proposal
+ * was rejected.  Every step is still accepted or rejected by the Cash-Karp error
+ * test, and the one-day cap and the injection guard of src/solver.f90 apply. */
+#define DFN_STEP_RKQS 0
+#define DFN_STEP_PI 1
+#define DFN_STEP_LEARNED 2
+int dfn_ensemble_set_controller(dfn_ensemble e, int32_t mode, dfn_mlp m);
+/* Records (features, target) pairs for the proposal network after every step:
+ * features of network i after step n (log10 accepted step, log10 rkqs proposal,
+ * log10 scaled error, rejected attempts, log10 max |V|, log10 step growth,
+ * log10 previous scaled error, log10 max |V| growth) and target log10(accepted
+ * step n+1 / rkqs proposal after step n); steps set by the one-day cap or the
+ * injection guard are left out. */
+int dfn_ensemble_set_recording(dfn_ensemble e, int32_t on);
+/* Moves up to cap recorded pairs (X: 8 floats each, T: 1) out of the ensemble;
+ * with X or T NULL, *count = pairs available. */
+int dfn_ensemble_take_records(dfn_ensemble e, float *X, float *T, int64_t cap, int64_t *count);
 /* info[0..5] = networks, total elements, largest network, matrix entries streamed
  * per stress evaluation, algorithmic bytes per stress evaluation, graphs on. */
 int dfn_ensemble_info(dfn_ensemble e, int64_t *info);
