@@ -844,6 +844,44 @@ __device__ __forceinline__ double coop_mv(const CgArgs &A, const CrossRows &X, i
 
 constexpr int kCgMaxSys = 8;
 
+// Sum of the wps workgroup partials sp[4 p + off] of one system, one per thread
+// then a fixed tree: every workgroup of the system gets the same double.
+__device__ __forceinline__ double partial_sum(const double *sp, int wps, int off, double *red) {
+  double v = 0.0;
+  for (int q = threadIdx.x; q < wps; q += blockDim.x) v += sp[4 * q + off];
+  return block_sum(v, red);
+}
+
+// Per-system max of the residual-norm partials (slot 1) of all nsys * wps
+// workgroups into out[0..kCgMaxSys); max is exact, so any order gives the same
+// value.  red holds kCgMaxSys * (blockDim.x / 64) doubles.
+__device__ __forceinline__ void system_max(const double *partial, int nsys, int wps, double *out, double *red) {
+  double lm[kCgMaxSys];
+#pragma unroll
+  for (int s = 0; s < kCgMaxSys; s++) lm[s] = 0.0;
+  for (int q = threadIdx.x; q < nsys * wps; q += blockDim.x) {
+    const int sq = q / wps;
+    const double v = partial[4 * (int64_t)q + 1];
+#pragma unroll
+    for (int s = 0; s < kCgMaxSys; s++)
+      if (s == sq) lm[s] = fmax(lm[s], v);
+  }
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kCgMaxSys; s++) {
+    const double m = wave_max(lm[s]);
+    if ((threadIdx.x & 63) == 0) red[s * nw + wid] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x < kCgMaxSys) {
+    double m = 0.0;
+    for (int w = 0; w < nw; w++) m = fmax(m, red[threadIdx.x * nw + w]);
+    out[threadIdx.x] = m;
+  }
+  __syncthreads();
+}
+
 // Per iteration: A p (neighbours of p_i formed from r_{i-1} and p_{i-1} in global
 // memory), grid barrier on p.Ap, update of x and r (r and p_i published), grid
 // barrier on r.r and max|r|, new direction in registers.  S.r and S.p / S.Ap hold
@@ -923,21 +961,17 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   if (threadIdx.x == 0) { mypart[0] = part_rr; mypart[1] = part_em; }
   grid.sync();
   // every workgroup tracks every system's state from the shared partials
+  __shared__ double s_em[kCgMaxSys], s_red[kCgMaxSys * 4];
   bool sdone[kCgMaxSys];
   int sit[kCgMaxSys];
+  system_max(partial, nsys, wps, s_em, s_red);
 #pragma unroll
   for (int s2 = 0; s2 < kCgMaxSys; s2++) {
     sit[s2] = 0;
-    sdone[s2] = true;
-    if (s2 < nsys) {
-      double em2 = 0.0;
-      for (int pp = 0; pp < wps; pp++) em2 = fmax(em2, partial[4 * (int64_t)(s2 * wps + pp) + 1]);
-      sdone[s2] = !(em2 / A.tol >= 1.0);
-    }
+    sdone[s2] = s2 < nsys ? !(s_em[s2] / A.tol >= 1.0) : true;
   }
   const double *sp = partial + 4 * (int64_t)(sys * wps);
-  double rr = 0.0;
-  for (int pp = 0; pp < wps; pp++) rr += sp[4 * pp];
+  double rr = partial_sum(sp, wps, 0, red);
   bool done = false, fail = false;
   int it = 0;
   double beta_prev = 0.0;
@@ -968,9 +1002,8 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
     if (threadIdx.x == 0) mypart[2] = part_pap;
     grid.sync();
     double part_r1 = 0.0, part_m = 0.0;
+    const double pAp = partial_sum(sp, wps, 2, red);
     if (!done) {
-      double pAp = 0.0;
-      for (int pp = 0; pp < wps; pp++) pAp += sp[4 * pp + 2];
       const double alpha = rr / pAp;
 #pragma unroll
       for (int q = 0; q < kCgEPT; q++) {
@@ -989,18 +1022,16 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
     part_m = block_max(part_m, red);
     if (threadIdx.x == 0) { mypart[0] = part_r1; mypart[1] = part_m; }
     grid.sync();
+    system_max(partial, nsys, wps, s_em, s_red);
 #pragma unroll
     for (int s2 = 0; s2 < kCgMaxSys; s2++) {
       if (s2 < nsys && !sdone[s2]) {
         sit[s2]++;
-        double em2 = 0.0;
-        for (int pp = 0; pp < wps; pp++) em2 = fmax(em2, partial[4 * (int64_t)(s2 * wps + pp) + 1]);
-        sdone[s2] = (sit[s2] > kCgMaxIter) || !(em2 / A.tol >= 1.0);
+        sdone[s2] = (sit[s2] > kCgMaxIter) || !(s_em[s2] / A.tol >= 1.0);
       }
     }
+    const double r1r1 = partial_sum(sp, wps, 0, red);
     if (!done) {
-      double r1r1 = 0.0;
-      for (int pp = 0; pp < wps; pp++) r1r1 += sp[4 * pp];
       const double beta = r1r1 / rr;
 #pragma unroll
       for (int q = 0; q < kCgEPT; q++) p[q] = r[q] + beta * p[q];
