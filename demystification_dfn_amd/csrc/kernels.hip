@@ -12,7 +12,6 @@
 //
 // Kernels whose results feed the ODE state are compiled without floating-point
 // contraction so that every +,-,*,/ rounds as in the reference's expressions.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -547,6 +546,10 @@ __global__ void __launch_bounds__(256) ens_vmax_kernel(const double *__restrict_
   }
 }
 
+__global__ void __launch_bounds__(64) fill_u32_kernel(unsigned *p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
 __global__ void __launch_bounds__(256) fill_u64_kernel(unsigned long long *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -842,7 +845,6 @@ __device__ __forceinline__ double coop_mv(const CgArgs &A, const CrossRows &X, i
   return r;
 }
 
-constexpr int kCgMaxSys = 8;
 
 // Sum of the wps workgroup partials sp[4 p + off] of one system, one per thread
 // then a fixed tree: every workgroup of the system gets the same double.
@@ -852,45 +854,45 @@ __device__ __forceinline__ double partial_sum(const double *sp, int wps, int off
   return block_sum(v, red);
 }
 
-// Per-system max of the residual-norm partials (slot 1) of all nsys * wps
-// workgroups into out[0..kCgMaxSys); max is exact, so any order gives the same
-// value.  red holds kCgMaxSys * (blockDim.x / 64) doubles.
-__device__ __forceinline__ void system_max(const double *partial, int nsys, int wps, double *out, double *red) {
-  double lm[kCgMaxSys];
-#pragma unroll
-  for (int s = 0; s < kCgMaxSys; s++) lm[s] = 0.0;
-  for (int q = threadIdx.x; q < nsys * wps; q += blockDim.x) {
-    const int sq = q / wps;
-    const double v = partial[4 * (int64_t)q + 1];
-#pragma unroll
-    for (int s = 0; s < kCgMaxSys; s++)
-      if (s == sq) lm[s] = fmax(lm[s], v);
-  }
-  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < kCgMaxSys; s++) {
-    const double m = wave_max(lm[s]);
-    if ((threadIdx.x & 63) == 0) red[s * nw + wid] = m;
-  }
-  __syncthreads();
-  if (threadIdx.x < kCgMaxSys) {
-    double m = 0.0;
-    for (int w = 0; w < nw; w++) m = fmax(m, red[threadIdx.x * nw + w]);
-    out[threadIdx.x] = m;
-  }
-  __syncthreads();
+__device__ __forceinline__ double partial_max(const double *sp, int wps, int off, double *red) {
+  double v = 0.0;
+  for (int q = threadIdx.x; q < wps; q += blockDim.x) v = fmax(v, sp[4 * q + off]);
+  return block_max(v, red);
 }
 
+// Barrier among the nblk workgroups of one system of a cooperative launch (all
+// of them are resident, so the wait ends): release, arrive on the system's
+// counter, spin until all have arrived at this barrier, acquire.  The counter
+// only grows (target += nblk per barrier).  A wait that outlasts ~2^20 polls
+// gives up and reports through timed_out, so a fault can never hang the device.
+__device__ __forceinline__ void sys_barrier(unsigned *bar, int nblk, unsigned &target, bool &timed_out) {
+  __shared__ int s_late;
+  __syncthreads();
+  target += (unsigned)nblk;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target && spins <= (1 << 20)) {
+      __builtin_amdgcn_s_sleep(2);
+      spins++;
+    }
+    __threadfence();
+    s_late = spins > (1 << 20);
+  }
+  __syncthreads();
+  timed_out = timed_out || s_late;
+}
+
+
 // Per iteration: A p (neighbours of p_i formed from r_{i-1} and p_{i-1} in global
-// memory), grid barrier on p.Ap, update of x and r (r and p_i published), grid
-// barrier on r.r and max|r|, new direction in registers.  S.r and S.p / S.Ap hold
-// the published residual and the two alternating copies of the direction.
+// memory), a barrier among the system's workgroups on p.Ap, update of x and r (r
+// and p_i published), a second barrier on r.r and max|r|, new direction in
+// registers.  S.r and S.p / S.Ap hold the published residual and the two
+// alternating copies of the direction.  Systems synchronise only within
+// themselves: a system that converges early leaves the device to the others.
 __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int wps, double *partial, int *sysflag) {
-  namespace cgr = cooperative_groups;
-  cgr::grid_group grid = cgr::this_grid();
   __shared__ double red[8];
-  const int nsys = gridDim.x / wps;
   const int sys = blockIdx.x / wps, part = blockIdx.x - sys * wps;
   const CgSys S = A.sys[sys];
   const int n = A.n;
@@ -956,90 +958,67 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
     }
   }
   double *mypart = partial + 4 * (int64_t)(sys * wps + part);
+  const double *sp = partial + 4 * (int64_t)(sys * wps);
+  // this system's barrier counter (zeroed before the launch) and its target
+  unsigned *bar = reinterpret_cast<unsigned *>(sysflag) + sys;
+  unsigned target = 0;
+  bool timed_out = false;
   part_rr = block_sum(part_rr, red);
   part_em = block_max(part_em, red);
   if (threadIdx.x == 0) { mypart[0] = part_rr; mypart[1] = part_em; }
-  grid.sync();
-  // every workgroup tracks every system's state from the shared partials
-  __shared__ double s_em[kCgMaxSys], s_red[kCgMaxSys * 4];
-  bool sdone[kCgMaxSys];
-  int sit[kCgMaxSys];
-  system_max(partial, nsys, wps, s_em, s_red);
-#pragma unroll
-  for (int s2 = 0; s2 < kCgMaxSys; s2++) {
-    sit[s2] = 0;
-    sdone[s2] = s2 < nsys ? !(s_em[s2] / A.tol >= 1.0) : true;
-  }
-  const double *sp = partial + 4 * (int64_t)(sys * wps);
+  sys_barrier(bar, wps, target, timed_out);
   double rr = partial_sum(sp, wps, 0, red);
-  bool done = false, fail = false;
+  double em = partial_max(sp, wps, 1, red);
+  bool fail = false;
   int it = 0;
   double beta_prev = 0.0;
-  for (;;) {
-    bool all_done = true;
-#pragma unroll
-    for (int s2 = 0; s2 < kCgMaxSys; s2++) all_done = all_done && sdone[s2];
-    if (all_done) break;
-#pragma unroll
-    for (int s2 = 0; s2 < kCgMaxSys; s2++)
-      if (s2 == sys) done = sdone[s2];
+  // every workgroup of the system reads the same partials in the same order, so
+  // they all take the same decisions and meet the same barriers
+  while (em / A.tol >= 1.0 && !timed_out) {
     const double *pprev = pbuf[(it + 1) & 1];  // p_{i-1}
     double *pcur = pbuf[it & 1];               // receives p_i
     double part_pap = 0.0;
-    if (!done) {
-      // p_1 = r_0 is read as is; later p_i = r_{i-1} + beta_{i-1} p_{i-1}
-      const NbVec vp{S.r, it == 0 ? nullptr : pprev, beta_prev};
+    // p_1 = r_0 is read as is; later p_i = r_{i-1} + beta_{i-1} p_{i-1}
+    const NbVec vp{S.r, it == 0 ? nullptr : pprev, beta_prev};
 #pragma unroll
-      for (int q = 0; q < kCgEPT; q++) {
-        const int k = base + q * 256;
-        if (k < n) {
-          ap[q] = coop_mv(A, X, k, xs[q], xe[q], p[q], vp, a1[q], a2[q], a3[q], dg[q], dt, dq);
-          part_pap += p[q] * ap[q];
-        }
+    for (int q = 0; q < kCgEPT; q++) {
+      const int k = base + q * 256;
+      if (k < n) {
+        ap[q] = coop_mv(A, X, k, xs[q], xe[q], p[q], vp, a1[q], a2[q], a3[q], dg[q], dt, dq);
+        part_pap += p[q] * ap[q];
       }
     }
     part_pap = block_sum(part_pap, red);
     if (threadIdx.x == 0) mypart[2] = part_pap;
-    grid.sync();
-    double part_r1 = 0.0, part_m = 0.0;
+    sys_barrier(bar, wps, target, timed_out);
     const double pAp = partial_sum(sp, wps, 2, red);
-    if (!done) {
-      const double alpha = rr / pAp;
+    const double alpha = rr / pAp;
+    double part_r1 = 0.0, part_m = 0.0;
 #pragma unroll
-      for (int q = 0; q < kCgEPT; q++) {
-        const int k = base + q * 256;
-        if (k < n) {
-          x[q] = x[q] + alpha * p[q];
-          r[q] = r[q] - alpha * ap[q];
-          S.r[k] = r[q];
-          pcur[k] = p[q];
-          part_r1 += r[q] * r[q];
-          part_m = fmax(part_m, fabs(r[q] / 1.0));
-        }
+    for (int q = 0; q < kCgEPT; q++) {
+      const int k = base + q * 256;
+      if (k < n) {
+        x[q] = x[q] + alpha * p[q];
+        r[q] = r[q] - alpha * ap[q];
+        S.r[k] = r[q];
+        pcur[k] = p[q];
+        part_r1 += r[q] * r[q];
+        part_m = fmax(part_m, fabs(r[q] / 1.0));
       }
     }
     part_r1 = block_sum(part_r1, red);
     part_m = block_max(part_m, red);
     if (threadIdx.x == 0) { mypart[0] = part_r1; mypart[1] = part_m; }
-    grid.sync();
-    system_max(partial, nsys, wps, s_em, s_red);
-#pragma unroll
-    for (int s2 = 0; s2 < kCgMaxSys; s2++) {
-      if (s2 < nsys && !sdone[s2]) {
-        sit[s2]++;
-        sdone[s2] = (sit[s2] > kCgMaxIter) || !(s_em[s2] / A.tol >= 1.0);
-      }
-    }
+    sys_barrier(bar, wps, target, timed_out);
     const double r1r1 = partial_sum(sp, wps, 0, red);
-    if (!done) {
-      const double beta = r1r1 / rr;
+    em = partial_max(sp, wps, 1, red);
+    const double beta = r1r1 / rr;
 #pragma unroll
-      for (int q = 0; q < kCgEPT; q++) p[q] = r[q] + beta * p[q];
-      rr = r1r1;
-      beta_prev = beta;
-      it++;
-      if (it > kCgMaxIter) fail = true;
-    }
+    for (int q = 0; q < kCgEPT; q++) p[q] = r[q] + beta * p[q];
+    rr = r1r1;
+    beta_prev = beta;
+    it++;
+    if (it > kCgMaxIter) { fail = true; break; }
   }
   int bad = 0;
   if (!fail) {
@@ -1056,14 +1035,14 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   }
   const double b2 = block_max(bad ? 1.0 : 0.0, red);
   if (threadIdx.x == 0) mypart[3] = b2;
-  grid.sync();
+  sys_barrier(bar, wps, target, timed_out);
   if (part == 0 && threadIdx.x == 0) {
     int anybad = 0;
     for (int pp = 0; pp < wps; pp++) anybad |= (sp[4 * pp + 3] > 0.0);
-    S.result[0] = (fail || anybad) ? 1 : 0;
+    // a barrier that never completed reports as a failed solve (2)
+    S.result[0] = timed_out ? 2 : ((fail || anybad) ? 1 : 0);
     S.result[1] = it;
   }
-  (void)sysflag;
 }
 
 // compute_P for systems that fit one workgroup (n <= kCgLdsThreads * EPT): one
@@ -1387,6 +1366,7 @@ bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *part
   if (nsys * wps > max_blocks) return false;
   CgArgs A = a;
   CrossRows X = xr;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(sysflag), nsys);
   void *args[] = {&A, &X, &wps, &partial, &sysflag};
   return hipLaunchCooperativeKernel((const void *)cg_coop_kernel, dim3(nsys * wps), dim3(256), args, 0, s) ==
          hipSuccess;
