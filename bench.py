@@ -200,6 +200,8 @@ def run_ensemble(args):
                 friction="per network: a~U[0.008,0.012] per fault, b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa")
     log(f"[bench] ensemble of {len(probs)} networks, {sum(Ns)} elements (max {max(Ns)})")
     ens = DeviceEnsemble(probs)
+    if args.no_graph:
+        ens.set_graph(False)
     t0 = time.perf_counter()
     ens.initiate_hmat()
     setup_s = time.perf_counter() - t0
@@ -220,7 +222,7 @@ def run_ensemble(args):
     log(f"[bench] ensemble {args.steps} steps in {wall:.2f} s")
     ens.set_graph(False)
     dt_e, wall_e = timed(max(args.steps // 3, 1))
-    ens.set_graph(True)
+    ens.set_graph(not args.no_graph)
     ens.set_precision(_abi.PREC_FP32)
     dt32, wall32 = timed(args.steps)
     ens.set_precision(_abi.PREC_FP64)
@@ -276,6 +278,8 @@ def main():
     ap.add_argument("--networks", type=int, default=64, help="networks of the ensemble workload")
     ap.add_argument("--ensemble-single-sample", type=int, default=16,
                     help="ensemble networks also run one handle at a time, for the batching comparison")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="ensemble workload: issue the launches one by one instead of replaying HIP graphs")
     ap.add_argument("--learned-steps", type=int, default=300,
                     help="ensemble workload: steps of the recording run for the step-size network; 0 skips it")
     ap.add_argument("--learned-epochs", type=int, default=30)
