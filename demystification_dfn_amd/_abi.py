@@ -107,6 +107,9 @@ SIGNATURES = {
     "compute_P": [_H, C.c_double, _dp, _dp, _i32p],
     "rkqs": [_H, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
     "solve": [_H, C.c_int64, C.c_double, C.POINTER(DfnRunStats)],
+    "monitor": [_H, C.c_int32],
+    "monitor_take": [_H, C.c_int64, _dp, _dp, _i64p],
+    "moment": [_H, _dp, _dp],
     "bench_kernel": [_H, C.c_int32, _dp],
     "bench_stream": [C.c_int64, C.c_int32, _dp],
     "device_info": [C.c_char_p, C.c_int32],

@@ -48,6 +48,12 @@ struct Engine {
   int last_pd_slot = -1;
   double time = 0, t_try = 100.0;
   dfn_run_stats st{};
+  // event monitor: per accepted step the time and each fault's max |V|, and the
+  // slip integral (trapezoid rule) per element
+  bool monitor = false;
+  DMem slip, fault_of, fvmax;
+  unsigned long long *h_fvmax = nullptr;  // pinned [nfault]
+  std::vector<double> mon_t, mon_v;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   double *k(int i) const { return kbuf.as<double>() + (size_t)i * 3 * hp.N; }
@@ -61,6 +67,7 @@ struct Engine {
     if (stream) (void)hipStreamSynchronize(stream);
     if (h_status) (void)hipHostFree(h_status);
     if (h_cgres) (void)hipHostFree(h_cgres);
+    if (h_fvmax) (void)hipHostFree(h_fvmax);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     ops64.reset();
@@ -425,9 +432,45 @@ struct Engine {
       }
     }
     last_pd_slot = -1;
+    const double t_before = time;
     time = time + t_did;  // :177
     st.accepted_steps++;
+    // cumtrapz integrates over the differences of the recorded times
+    if (monitor) DFN_TRY(monitor_step(time - t_before));
     if (h_status->tensile) return DFN_ERR_TENSILE;  // check_errors :412-420 on the accepted state
+    return DFN_OK;
+  }
+
+  int monitor_start() {
+    const int N = hp.N, nf = hp.nfault;
+    std::vector<int> fo(N, -1);
+    for (int f = 0; f < nf; f++)
+      for (int64_t e = 0; e < hp.fnel[f]; e++) fo[hp.map1[f] - 1 + e] = f;
+    DFN_TRY(fault_of.upload(fo));
+    std::vector<double> z(N, 0.0);
+    DFN_TRY(slip.upload(z));
+    DFN_TRY(fvmax.alloc(sizeof(unsigned long long) * nf));
+    if (!h_fvmax) DFN_HIP(hipHostMalloc((void **)&h_fvmax, sizeof(unsigned long long) * nf, hipHostMallocDefault));
+    mon_t.clear();
+    mon_v.clear();
+    monitor = true;
+    return DFN_OK;
+  }
+
+  // after an accepted step over the time difference h: Ynew holds the step-start
+  // state, Y the new one
+  int monitor_step(double h) {
+    const int nf = hp.nfault;
+    launch_monitor(Ynew.as<double>(), Y.as<double>(), h, fault_of.as<int>(), hp.N, slip.as<double>(),
+                   fvmax.as<unsigned long long>(), nf, stream);
+    DFN_HIP(hipMemcpyAsync(h_fvmax, fvmax.p, sizeof(unsigned long long) * nf, hipMemcpyDeviceToHost, stream));
+    DFN_HIP(hipStreamSynchronize(stream));
+    mon_t.push_back(time);
+    for (int f = 0; f < nf; f++) {
+      double v;
+      std::memcpy(&v, &h_fvmax[f], sizeof(double));
+      mon_v.push_back(v);
+    }
     return DFN_OK;
   }
 
@@ -793,6 +836,48 @@ API int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *s
     if (st == DFN_OK) st = s2;
   }
   return st;
+}
+
+API int dfn_monitor(dfn_handle h, int32_t on) {
+  if (!h) return DFN_ERR_ARG;
+  if (!on) { h->e.monitor = false; return DFN_OK; }
+  return h->e.monitor_start();
+}
+
+API int dfn_monitor_take(dfn_handle h, int64_t cap, double *times, double *maxV, int64_t *count) {
+  if (!h || !count || cap < 0) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const int nf = e.hp.nfault;
+  const int64_t avail = (int64_t)e.mon_t.size();
+  if (!times || !maxV) { *count = avail; return DFN_OK; }
+  const int64_t n = std::min(cap, avail);
+  std::copy(e.mon_t.begin(), e.mon_t.begin() + n, times);
+  std::copy(e.mon_v.begin(), e.mon_v.begin() + n * nf, maxV);
+  e.mon_t.erase(e.mon_t.begin(), e.mon_t.begin() + n);
+  e.mon_v.erase(e.mon_v.begin(), e.mon_v.begin() + n * nf);
+  *count = n;
+  return DFN_OK;
+}
+
+API int dfn_moment(dfn_handle h, double *M0, double *slip) {
+  if (!h) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  if (!e.monitor && e.slip.p == nullptr) return DFN_ERR_STATE;
+  const int N = e.hp.N;
+  std::vector<double> s(N);
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  DFN_HIP(hipMemcpy(s.data(), e.slip.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+  if (slip) std::copy(s.begin(), s.end(), slip);
+  if (M0)
+    for (int f = 0; f < e.hp.nfault; f++) {  // wrapper/network.m:474-484: sum(mu * slip .* ds)
+      double m = 0.0;
+      for (int64_t q = 0; q < e.hp.fnel[f]; q++) {
+        const int64_t k = e.hp.map1[f] - 1 + q;
+        m = m + e.hp.mu * s[k] * e.hp.ds[k];
+      }
+      M0[f] = m;
+    }
+  return DFN_OK;
 }
 
 API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {

@@ -546,6 +546,28 @@ __global__ void __launch_bounds__(256) ens_vmax_kernel(const double *__restrict_
   }
 }
 
+// After an accepted step of length h: slip += (V_old + V_new) h / 2 per element
+// (the trapezoid rule of a cumulative time integral) and max |V_new| per fault
+// (bits of non-negative doubles; fake elements, fault -1, are skipped).
+__global__ void __launch_bounds__(256) monitor_kernel(const double *__restrict__ Vold, const double *__restrict__ Vnew,
+                                                      double h, const int *__restrict__ fault_of, int n,
+                                                      double *__restrict__ slip, unsigned long long *__restrict__ fvmax) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = k < n ? fault_of[k] : -1;
+  double v = 0.0;
+  if (f >= 0) {
+    slip[k] = slip[k] + 0.5 * (Vold[k] + Vnew[k]) * h;
+    v = fabs(Vnew[k]);
+  }
+  const int first = __shfl(f, 0, 64);
+  if (__all(f == first)) {
+    const double m = wave_max(v);
+    if ((threadIdx.x & 63) == 0 && first >= 0) atomicMax(&fvmax[first], (unsigned long long)__double_as_longlong(m));
+  } else if (f >= 0) {
+    atomicMax(&fvmax[f], (unsigned long long)__double_as_longlong(v));
+  }
+}
+
 __global__ void __launch_bounds__(64) fill_u32_kernel(unsigned *p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
 }
@@ -1319,6 +1341,12 @@ void launch_ens_vmax(const double *V, const int *net, const uint8_t *fake, int n
                      hipStream_t s) {
   hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nnet, 256)), dim3(256), 0, s, vmax, nnet);
   hipLaunchKernelGGL(ens_vmax_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, V, net, fake, n, vmax);
+}
+
+void launch_monitor(const double *Vold, const double *Vnew, double h, const int *fault_of, int n, double *slip,
+                    unsigned long long *fvmax, int nfault, hipStream_t s) {
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nfault, 256)), dim3(256), 0, s, fvmax, nfault);
+  hipLaunchKernelGGL(monitor_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, Vold, Vnew, h, fault_of, n, slip, fvmax);
 }
 
 void launch_ens_clear_status(StatusWord *st, const int *active, int nnet, hipStream_t s) {

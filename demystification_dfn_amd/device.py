@@ -45,6 +45,29 @@ class DeviceSimulation(Simulation):
     def __init__(self, problem):
         super().__init__(load(), problem, tag="dfn")
 
+    # -- event monitor (dfn_monitor, dfn_monitor_take, dfn_moment) --------------
+    def monitor(self, on=True):
+        _abi.check(self._fns["monitor"](self._h, 1 if on else 0), "dfn_monitor")
+
+    def monitor_take(self):
+        """(times, maxV[records, faults]) recorded since the last call."""
+        n = C.c_int64()
+        _abi.check(self._fns["monitor_take"](self._h, 0, None, None, C.byref(n)), "dfn_monitor_take")
+        nf = self.problem.nb_fault
+        t = np.zeros(n.value)
+        v = np.zeros(n.value * nf)
+        if n.value:
+            _abi.check(self._fns["monitor_take"](self._h, n.value, _abi.dptr(t), _abi.dptr(v), C.byref(n)),
+                       "dfn_monitor_take")
+        return t, v.reshape(-1, nf)
+
+    def moment(self):
+        """(M0 per fault, slip per element)."""
+        m = np.zeros(self.problem.nb_fault)
+        s = np.zeros(self.N)
+        _abi.check(self._fns["moment"](self._h, _abi.dptr(m), _abi.dptr(s)), "dfn_moment")
+        return m, s
+
     def bench_kernel(self, reps=20):
         ms = np.zeros(4)
         _abi.check(self._fns["bench_kernel"](self._h, int(reps), _abi.dptr(ms)), "dfn_bench_kernel")

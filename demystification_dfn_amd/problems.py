@@ -243,3 +243,24 @@ def network_ensemble(n_networks=64, faults=(50, 400), seed=7, **kw):
     kw.setdefault("alpha", 0.1)
     return [fault_network(n_faults=int(c), seed=int(s), name=f"ensemble{seed}_net{i}_{int(c)}faults", **kw)
             for i, (c, s) in enumerate(zip(counts, seeds))]
+
+
+def loaded_faults(n_faults=2, n=200, length=2000.0, spacing=600.0, a=0.010, b=0.015, Dc=5e-4, sigmaN=-30e6,
+                  tau0=18.0e6, V_init=1e-9, tau_dot=0.1, tol=1e-6, f0=0.6, V0=1e-6, name=None):
+    """Parallel straight velocity-weakening faults (Mode III, dense kernel) under a
+    uniform shear stress rate tau_dot (Sigma32_dot): an earthquake cycle with events
+    within a few simulated years, for the event catalog and moment.  No injection."""
+    ds = length / n
+    faults = []
+    psi = psi_from_stress(tau0, a, b, f0, V0, V_init, Dc, sigmaN)
+    for i in range(n_faults):
+        x = np.linspace(-length / 2, length / 2, n + 1) + 0.37 * ds * i
+        node = np.stack([x, np.full(n + 1, spacing * i)], axis=1)
+        one = np.ones(n)
+        faults.append(dict(node=node, a=a * one, b=b * one, Dc=Dc * one, f0=f0 * one, V0=V0 * one,
+                           sigmaN=sigmaN * one, V=V_init * (1.0 + 0.1 * i) * one, theta=psi * one, P=0 * one,
+                           permeability=1e-15 * one, Lk=one, Tk=one, Snk=one, kmin=0 * one, kmax=0 * one))
+    nels, node, arrays = concatenate_faults(faults)
+    return Problem(name=name or f"loaded_faults_{n_faults}x{n}", fracture_mode=_abi.MODE_III,
+                   static_kernel=_abi.KERNEL_DIRECT, fault_nb_element=nels, node=node, arrays=arrays, mu=32.04e9,
+                   cs=3.464e3, cp=1.0, tol_solver=tol, Sigma32_dot=tau_dot, meta=dict(length=length, ds=ds))

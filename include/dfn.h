@@ -162,6 +162,23 @@ int dfn_rkqs(dfn_handle h, double *y, const double *dydx, double *x, double htry
  * no time limit).  The loop state (time, t_try, counters) persists between calls. */
 int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats);
 
+/* ---- event catalog and moment ------------------------------------------------
+ * The inputs of the reference's post-processing, gathered on the device as the
+ * solver runs (the reference writes V every stride_time steps and post-processes
+ * in MATLAB).  With the monitor on, every accepted step of dfn_solve records the
+ * time and, per fault, max |V| over the fault's elements (the series that
+ * wrapper/microseismicity.m:447-472 thresholds into an event catalog), and adds
+ * (V_old + V_new) h / 2 to each element's slip (cumtrapz of V over the accepted
+ * steps, wrapper/network.m:474-476).  Turning it on zeroes the slip and drops
+ * unread records. */
+int dfn_monitor(dfn_handle h, int32_t on);
+/* Moves up to cap records out: times[i] and maxV[i*nb_fault + f]; with times or maxV
+ * NULL, *count = records available. */
+int dfn_monitor_take(dfn_handle h, int64_t cap, double *times, double *maxV, int64_t *count);
+/* Seismic moment per fault M0[f] = sum over its elements of mu * slip * ds
+ * (wrapper/network.m:478-483) and, if slip is not NULL, the slip of every element. */
+int dfn_moment(dfn_handle h, double *M0, double *slip);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Time `reps` applications of the stress-transfer kernel on device-resident input
  * (the handle's current V) with HIP events on the handle's stream.  ms[0] = mean ms
