@@ -110,6 +110,12 @@ SIGNATURES = {
     "monitor": [_H, C.c_int32],
     "monitor_take": [_H, C.c_int64, _dp, _dp, _i64p],
     "moment": [_H, _dp, _dp],
+    "h5_open": [C.c_char_p, C.POINTER(C.c_void_p)],
+    "h5_close": [_H],
+    "h5_info": [_H, C.c_char_p, _i32p, _i32p, _i64p],
+    "h5_read_f64": [_H, C.c_char_p, _dp, C.c_int64],
+    "h5_read_i64": [_H, C.c_char_p, _i64p, C.c_int64],
+    "h5_read_string": [_H, C.c_char_p, C.c_char_p, C.c_int32],
     "bench_kernel": [_H, C.c_int32, _dp],
     "bench_stream": [C.c_int64, C.c_int32, _dp],
     "device_info": [C.c_char_p, C.c_int32],

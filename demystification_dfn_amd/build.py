@@ -31,9 +31,10 @@ def _run(cmd):
 
 def build(verbose=True):
     objs = []
-    o = os.path.join(CSRC, "host_setup.o")
-    _run([CXX] + HOST_FLAGS + ["-c", "host_setup.cpp", "-o", o])
-    objs.append(o)
+    for src in ("host_setup.cpp", "h5min.cpp"):
+        o = os.path.join(CSRC, src.replace(".cpp", ".o"))
+        _run([CXX] + HOST_FLAGS + ["-c", src, "-o", o])
+        objs.append(o)
     for src in ("kernels.hip", "engine.hip", "hbuild.hip", "ensemble.hip", "mlp.hip"):
         o = os.path.join(CSRC, src.replace(".hip", ".o"))
         _run([HIPCC] + HIP_FLAGS + ["-c", src, "-o", o])

@@ -179,6 +179,24 @@ int dfn_monitor_take(dfn_handle h, int64_t cap, double *times, double *maxV, int
  * (wrapper/network.m:478-483) and, if slip is not NULL, the slip of every element. */
 int dfn_moment(dfn_handle h, double *M0, double *slip);
 
+/* ---- reading the reference's HDF5 input files -------------------------------
+ * The reference reads its configuration through libhdf5 (src/configuration.f90:
+ * 70-379).  This is a reader, without libhdf5, for the subset its input files use:
+ * superblock version 0, symbol-table groups, version-1 object headers,
+ * contiguous or compact integer, floating-point and string (fixed or variable
+ * length) datasets.  Paths are like "fault1/a"; other layouts return
+ * DFN_ERR_UNSUPPORTED, a missing path DFN_ERR_ARG. */
+typedef struct dfn_h5_s *dfn_h5;
+int dfn_h5_open(const char *path, dfn_h5 *out);
+int dfn_h5_close(dfn_h5 f);
+/* type_class: 0 integer, 1 floating point, 2 string; dims holds up to 8 extents */
+int dfn_h5_info(dfn_h5 f, const char *path, int32_t *type_class, int32_t *rank, int64_t *dims);
+/* n must equal the dataset's element count; integers and floats convert */
+int dfn_h5_read_f64(dfn_h5 f, const char *path, double *buf, int64_t n);
+int dfn_h5_read_i64(dfn_h5 f, const char *path, int64_t *buf, int64_t n);
+/* the (first) string of a string dataset, NUL-terminated, at most cap bytes */
+int dfn_h5_read_string(dfn_h5 f, const char *path, char *buf, int32_t cap);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Time `reps` applications of the stress-transfer kernel on device-resident input
  * (the handle's current V) with HIP events on the handle's stream.  ms[0] = mean ms
