@@ -1,4 +1,6 @@
 """Problems the golden fixtures are generated on (shared by make_golden.py and the tests)."""
+import os
+
 import numpy as np
 
 from demystification_dfn_amd import problems
@@ -59,3 +61,36 @@ def grid_network(ds=2.0):
     nels, node, arr = problems.concatenate_faults(faults)
     return problems.Problem("grid_network", 2, 0, nels, node, arr, tol_solver=1e-6,
                             sources=[(1, 75, 2e-4, 0.0, 1e6)])
+
+
+SCALARS = ("mu", "cs", "cp", "Sigma11_dot", "Sigma12_dot", "Sigma22_dot", "Sigma31_dot", "Sigma32_dot", "porosity",
+           "fluid_comp", "rock_comp", "dyn_viscosity", "fluid_density", "tol_solver", "eta", "eps_aca")
+
+
+def problem_arrays(p):
+    """A Problem as plain arrays (for a fixture file)."""
+    out = {"in_node": p.node, "in_fnel": np.asarray(p.fault_nb_element, np.int64),
+           "in_ints": np.array([p.fracture_mode, p.static_kernel, p.permeability_coupling, p.nleaf, p.max_rank]),
+           "in_scalars": np.array([float(getattr(p, k)) for k in SCALARS]),
+           "in_sources": np.array(p.sources, np.float64).reshape(-1, 5)}
+    for k, v in p.arrays.items():
+        out["in_" + k] = v
+    return out
+
+
+def problem_from_arrays(z, name):
+    """Inverse of problem_arrays."""
+    arrays = {k[3:]: np.asarray(z[k]) for k in z.files if k.startswith("in_") and k[3:] in problems.PER_ELEMENT}
+    ints = z["in_ints"]
+    sc = dict(zip(SCALARS, (float(x) for x in z["in_scalars"])))
+    src = [(int(r[0]), int(r[1]), float(r[2]), float(r[3]), float(r[4])) for r in z["in_sources"]]
+    return problems.Problem(name, int(ints[0]), int(ints[1]), np.asarray(z["in_fnel"]), np.asarray(z["in_node"]),
+                            arrays, permeability_coupling=int(ints[2]), nleaf=int(ints[3]), max_rank=int(ints[4]),
+                            sources=src, **sc)
+
+
+def reference_network():
+    """The reference's problems/network_rate_strengthening/config (10 Mode II faults,
+    1029 elements), as read by the library's HDF5 reader into the fixture."""
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "network_rate_strengthening.npz"))
+    return problem_from_arrays(z, "network_rate_strengthening")

@@ -59,6 +59,16 @@ def main():
     g = O.geometry(osim)
     t = O.RefSimulation.trees(g["elem"], q.nleaf, q.eta)
     np.savez_compressed(os.path.join(HERE, "trees_network20.npz"), elem=g["elem"], **t)
+    # the reference's own network input file, read with the library's HDF5 reader,
+    # run by the reference's Fortran with the dense Mode II kernel
+    cfg = "/root/reference/problems/network_rate_strengthening/config"
+    if os.path.exists(cfg):
+        from demystification_dfn_amd.config_io import read_config
+        p3 = read_config(cfg)
+        p3.static_kernel = 0
+        out = cases.problem_arrays(p3)
+        out.update(run_case(p3, [1.0, 1000.0, 20000.0], [10.0, 5000.0, 86400.0], 20, 2))
+        np.savez_compressed(os.path.join(HERE, "network_rate_strengthening.npz"), **out)
     print("written")
 
 

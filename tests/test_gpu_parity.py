@@ -259,3 +259,51 @@ def test_single_fault_4000_trajectory():
     for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
         assert close_elementwise(Yd[sl], Yo[sl], 1e-8, 1e-300)
     assert np.max(np.abs(Pd - Po)) <= 1e-9 * max(np.abs(Po).max(), 1.0) + 10 * p.tol_solver
+
+
+# ---------------------------------------------------------------------------
+def test_reference_input_file_network_against_reference_outputs():
+    """The reference's own problems/network_rate_strengthening/config (10 Mode II
+    faults, 1029 elements, read by the library's HDF5 reader into the fixture) with
+    the dense Mode II kernel: the device's kernel, friction law and 20 solver
+    steps against the outputs of the reference's compiled Fortran on it."""
+    import os
+    p = cases.reference_network()
+    g = dict(np.load(os.path.join(os.path.dirname(cases.__file__), "network_rate_strengthening.npz")))
+    d = dev(p)
+    aT, aN = d.kernel(g["kernel_in"])
+    assert rel(aT, g["kernel_T"]) <= 1e-12 and rel(aN, g["kernel_N"]) <= 1e-12
+    Y0 = p.initial_Y()
+    for x, ref in zip(g["rhs_x"], g["rhs"]):
+        assert close_elementwise(d.friction_law(x, Y0), ref, 1e-9, 1e-300 + np.abs(ref).max() * 1e-12)
+    s = d.solve(int(g["solve_steps"]))
+    Y, P, _, t = d.get_state()
+    assert abs(t - float(g["solve_time"])) <= 1e-8 * float(g["solve_time"])
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Y[sl][real], g["solve_Y"][sl][real], 1e-8, 1e-300)
+
+
+def test_reference_input_file_network_hmatrix_on_identical_block_lists():
+    """The same reference network with the H-matrix kernel at the reference's
+    default eta / ACA / leaf (3, 1e-8, 100): device vs oracle on the same block
+    lists, matvec and 15 solver steps."""
+    p = cases.reference_network()
+    p.static_kernel = _abi.KERNEL_HMATRIX
+    d, o = dev(p), OracleSimulation(p)
+    d.initiate_hmat()
+    for op in (0, 1):
+        o.hmat_import(op, d.hmat_export(op))
+    x = np.random.default_rng(5).standard_normal(p.N)
+    for op in (0, 1):
+        assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
+    sd, so = d.solve(15), o.solve(15)
+    assert sd["rejected_attempts"] == so["rejected_attempts"]
+    Yd, Yo = d.get_state()[0], o.get_state()[0]
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)

@@ -15,7 +15,8 @@ def _load(golden_dir, name):
 
 
 @pytest.mark.parametrize("fixture,make", [("single_fault_300.npz", cases.single_fault_small),
-                                          ("crossing_network.npz", cases.crossing_network)])
+                                          ("crossing_network.npz", cases.crossing_network),
+                                          ("network_rate_strengthening.npz", cases.reference_network)])
 def test_oracle_matches_reference_fixtures(golden_dir, fixture, make):
     g = _load(golden_dir, fixture)
     prob = make()
