@@ -72,6 +72,20 @@ __device__ __forceinline__ double wave_sum_full(double sum) {
   return (lane_value(sum, 0) + lane_value(sum, 16)) + (lane_value(sum, 32) + lane_value(sum, 48));
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_move_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_sum_full(float sum) {
+  sum += dpp_move_f<kDppXor1>(sum);
+  sum += dpp_move_f<kDppXor2>(sum);
+  sum += dpp_move_f<kDppHalfMirror>(sum);
+  sum += dpp_move_f<kDppMirror>(sum);
+  const auto at = [](float v, int l) { return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l)); };
+  return (at(sum, 0) + at(sum, 16)) + (at(sum, 32) + at(sum, 48));
+}
+
 // ---------------------------------------------------------------------------
 // H-matvec
 // ---------------------------------------------------------------------------
@@ -83,8 +97,8 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 
 // Phase 1: one wave per task.  The body is specialised on the rank K (a
 // wave-uniform switch), so the K column loads of a row strip are all in flight
-// together; lanes stride the rows, K partial sums stay in registers and the K
-// wave reductions run interleaved.
+// together; lanes stride the rows, K partial sums stay in registers and are
+// reduced across the wave with DPP lane moves.
 template <int K, typename TV>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
                                          double *__restrict__ zpart, double *__restrict__ wz, int lane) {
@@ -99,19 +113,17 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
     const TV xi = (TV)x[i];
     TV v[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) v[j] = V[i + j * ld];
+    for (int j = 0; j < K; j++) v[j] = __builtin_nontemporal_load(V + i + j * ld);  // streamed once
 #pragma unroll
     for (int j = 0; j < K; j++) acc[j] += v[j] * xi;
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int j = 0; j < K; j++) acc[j] += __shfl_xor(acc[j], off, 64);
-  }
+  // all 64 lanes are active here: DPP wave totals, lane j keeps column j's
   TV mine = (TV)0;
 #pragma unroll
-  for (int j = 0; j < K; j++)
-    if (lane == j) mine = acc[j];
+  for (int j = 0; j < K; j++) {
+    const TV t = wave_sum_full(acc[j]);
+    if (lane == j) mine = t;
+  }
   if (lane < K) {
     const double v = (double)mine * task.scale;
     if (task.wout >= 0) wz[task.wout + lane] = v;
@@ -155,6 +167,8 @@ __global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ z
   w[r.zout + j] = s;
 }
 
+typedef double dv2 __attribute__((ext_vector_type(2)));
+
 // Phase 2: one workgroup per panel task.  w entries of the panel's columns are
 // gathered into LDS once; each wave then streams two whole rows at a time
 // (row-major panel, rows padded to a multiple of 2 doubles / 4 floats so every
@@ -169,13 +183,17 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int L2 = task.L >> 1;
-  const double2 *P = reinterpret_cast<const double2 *>(pdata + task.poff);
+  const dv2 *P = reinterpret_cast<const dv2 *>(pdata + task.poff);
   int r = wave * 2;
   for (; r + 1 < task.m; r += 8) {
-    const double2 *row0 = P + (int64_t)r * L2, *row1 = row0 + L2;
+    const dv2 *row0 = P + (int64_t)r * L2, *row1 = row0 + L2;
     double a0 = 0.0, a1 = 0.0;
+    // the panel is streamed once per application: non-temporal loads keep it
+    // from displacing the gathered vectors in the caches
+#pragma unroll 4
     for (int j = lane; j < L2; j += 64) {
-      const double2 wv = wl2[j], v0 = row0[j], v1 = row1[j];
+      const double2 wv = wl2[j];
+      const dv2 v0 = __builtin_nontemporal_load(row0 + j), v1 = __builtin_nontemporal_load(row1 + j);
       a0 += v0.x * wv.x + v0.y * wv.y;
       a1 += v1.x * wv.x + v1.y * wv.y;
     }
@@ -187,10 +205,11 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
     }
   }
   if (r < task.m) {
-    const double2 *row0 = P + (int64_t)r * L2;
+    const dv2 *row0 = P + (int64_t)r * L2;
     double a0 = 0.0;
     for (int j = lane; j < L2; j += 64) {
-      const double2 wv = wl2[j], v0 = row0[j];
+      const double2 wv = wl2[j];
+      const dv2 v0 = __builtin_nontemporal_load(row0 + j);
       a0 += v0.x * wv.x + v0.y * wv.y;
     }
     a0 = wave_sum(a0);
@@ -208,7 +227,9 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
   s = t;
 }
 
-__device__ __forceinline__ void acc4(float4 v, float4 wv, int j, int nd, float &s, float &c, float &a) {
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void acc4(fv4 v, float4 wv, int j, int nd, float &s, float &c, float &a) {
   if (j + 3 < nd) {
     kahan_add(s, c, v.x * wv.x);
     kahan_add(s, c, v.y * wv.y);
@@ -237,13 +258,15 @@ __global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict_
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int L4 = task.L >> 2, nd = task.nd;
-  const float4 *P = reinterpret_cast<const float4 *>(pdata + task.poff);
+  const fv4 *P = reinterpret_cast<const fv4 *>(pdata + task.poff);
   for (int r = wave * 2; r < task.m; r += 8) {
     const bool two = r + 1 < task.m;
-    const float4 *row0 = P + (int64_t)r * L4, *row1 = row0 + (two ? L4 : 0);
+    const fv4 *row0 = P + (int64_t)r * L4, *row1 = row0 + (two ? L4 : 0);
     float s0 = 0.0f, c0 = 0.0f, a0 = 0.0f, s1 = 0.0f, c1 = 0.0f, a1 = 0.0f;
+#pragma unroll 2
     for (int j = lane; j < L4; j += 64) {
-      const float4 wv = wl4[j], v0 = row0[j], v1 = row1[j];
+      const float4 wv = wl4[j];
+      const fv4 v0 = __builtin_nontemporal_load(row0 + j), v1 = __builtin_nontemporal_load(row1 + j);
       acc4(v0, wv, 4 * j, nd, s0, c0, a0);
       acc4(v1, wv, 4 * j, nd, s1, c1, a1);
     }
@@ -275,7 +298,6 @@ __global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ f
 // ---------------------------------------------------------------------------
 // Direct kernels (src/kernel_calculation/directModeIII.f:7-74, directModeII.f90:7-107)
 // ---------------------------------------------------------------------------
-// c_l: weights of the node dislocations: l=0: -s_1, 1..N-1: -(s_{l+1}-s_l), N: +s_N
 // Dense direct kernels (src/kernel_calculation/directModeIII.f:7-74,
 // directModeII.f90:7-107).  Both sum over the N+1 nodes rather than the N
 // elements: the reference adds each element's contribution as (left node term)
