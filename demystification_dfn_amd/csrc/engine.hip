@@ -55,6 +55,10 @@ struct Engine {
   unsigned long long *h_fvmax = nullptr;  // pinned [nfault]
   std::vector<double> mon_t, mon_v;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the step's compute_P batch can run on a second stream (default priority)
+  // beside the k1 stress evaluation, which does not depend on it
+  hipStream_t cg_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   double *k(int i) const { return kbuf.as<double>() + (size_t)i * 3 * hp.N; }
   double *cg_ptr(int slot, int which) const {  // 12 vectors of N per slot
@@ -70,6 +74,12 @@ struct Engine {
     if (h_fvmax) (void)hipHostFree(h_fvmax);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (cg_stream) {
+      (void)hipStreamSynchronize(cg_stream);
+      (void)hipStreamDestroy(cg_stream);
+    }
     ops64.reset();
     ops32.reset();
     if (stream) (void)hipStreamDestroy(stream);
@@ -85,6 +95,9 @@ struct Engine {
     DFN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     DFN_HIP(hipEventCreate(&ev0));
     DFN_HIP(hipEventCreate(&ev1));
+    DFN_HIP(hipStreamCreateWithFlags(&cg_stream, hipStreamNonBlocking));
+    DFN_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    DFN_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     const int N = hp.N;
     DFN_TRY(node.upload(hp.node));
     DFN_TRY(elem.upload(hp.elem));
@@ -212,7 +225,10 @@ struct Engine {
   }
 
   // compute_P for the time steps dts[0..nsys) into CG slots slot0.. (no host sync)
-  int cg_launch(const double *dts, int nsys, int slot0) {
+  int cg_launch(const double *dts, int nsys, int slot0) { return cg_launch_on(dts, nsys, slot0, stream); }
+
+  // compute_P for nsys time steps into CG slots slot0.., issued on stream cs
+  int cg_launch_on(const double *dts, int nsys, int slot0, hipStream_t cs) {
     const int N = hp.N;
     std::vector<CgSys> sys(nsys);
     for (int s = 0; s < nsys; s++) {
@@ -225,15 +241,15 @@ struct Engine {
       c.result = cg_res.as<int>() + 2 * sl;
     }
     DFN_HIP(hipMemcpyAsync(cg_sys.as<CgSys>() + slot0, sys.data(), sizeof(CgSys) * nsys, hipMemcpyHostToDevice,
-                           stream));
+                           cs));
     // permeability update (fluid_diffusion.f90:67-75); diag keeps the values before it
     const double *px_diag = permx_cur.as<double>(), *p_diag = perm_cur.as<double>();
     const double *px = px_diag, *pp = p_diag;
     if (hp.coupling == 1) {
-      DFN_HIP(hipMemcpyAsync(perm_next.p, perm_cur.p, sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+      DFN_HIP(hipMemcpyAsync(perm_next.p, perm_cur.p, sizeof(double) * N, hipMemcpyDeviceToDevice, cs));
       PermArgs pa{N, 1, perm_next.as<double>(), permx_next.as<double>(), perm_star.as<double>(), kmin.as<double>(),
                   Y.as<double>() + 2 * N, P.as<double>(), Snk.as<double>()};
-      launch_perm_update(pa, stream);
+      launch_perm_update(pa, cs);
       px = permx_next.as<double>();
       pp = perm_next.as<double>();
     }
@@ -261,14 +277,14 @@ struct Engine {
     bool launched = false;
     if (N <= kCgLdsMax && cg_path <= 1) {
       CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
-      launched = launch_cg_lds(A, X, nsys, stream);
+      launched = launch_cg_lds(A, X, nsys, cs);
     }
     if (!launched && coop_ok && cg_path != 3) {
       CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
-      launched = launch_cg_coop(A, X, nsys, cg_partial.as<double>() + 0, cg_flags.as<int>(), stream);
+      launched = launch_cg_coop(A, X, nsys, cg_partial.as<double>() + 0, cg_flags.as<int>(), cs);
       if (!launched) coop_ok = false;  // grid too large to be co-resident: one workgroup per system
     }
-    if (!launched) launch_cg(A, nsys, stream);
+    if (!launched) launch_cg(A, nsys, cs);
     if (hp.coupling == 1) {  // the updated permeability becomes the state (swap buffers)
       std::swap(perm_cur.p, perm_next.p);
       std::swap(permx_cur.p, permx_next.p);
@@ -388,24 +404,38 @@ struct Engine {
     // step depends on dt alone, so the five stage solves of a first attempt with
     // h = t_try (x + A_i h, x = 0) run in the same launch.
     const bool batch = hp.coupling != 1;
-    auto launch_try = [&]() -> int {
-      if (!batch) return cg_launch(&t_try, 1, 0);
+    auto launch_try = [&](hipStream_t cs) -> int {
+      if (!batch) return cg_launch_on(&t_try, 1, 0, cs);
       const double dts[6] = {t_try, 0.0 + 0.2 * t_try, 0.0 + 0.3 * t_try, 0.0 + 0.6 * t_try, 0.0 + 1.0 * t_try,
                              0.0 + 0.875 * t_try};
-      return cg_launch(dts, 6, 0);
+      return cg_launch_on(dts, 6, 0, cs);
     };
-    DFN_TRY(launch_try());
+    // With the one-workgroup-per-system CG (six workgroups) the first batch runs
+    // on the side stream beside the k1 stress evaluation, which needs only Y; the
+    // cooperative CG spans the device and slows a concurrent matvec, so it stays
+    // on the main stream.
+    if (N <= kCgLdsMax && cg_path <= 1) {
+      DFN_HIP(hipEventRecord(ev_fork, stream));
+      DFN_HIP(hipStreamWaitEvent(cg_stream, ev_fork, 0));
+      DFN_TRY(launch_try(cg_stream));
+      DFN_HIP(hipEventRecord(ev_join, cg_stream));
+      DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
+      DFN_HIP(hipStreamWaitEvent(stream, ev_join, 0));
+    } else {
+      DFN_TRY(launch_try(stream));
+      DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
+    }
     DFN_TRY(cg_collect(0, 1, &fail));
     while (fail) {
       t_try = t_try / 2.0;
       st.cg_halvings++;
-      DFN_TRY(launch_try());
+      DFN_TRY(launch_try(stream));
       DFN_TRY(cg_collect(0, 1, &fail));
     }
     if (batch) st.cg_solves -= 5;  // counted when the stages use them
-    // k1 = f(t_try, Y) and yscal (:101-102); compute_P(t_try) of the RHS equals the
-    // call above unless the permeability evolves between calls
-    DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
+    // k1 = f(t_try, Y) and yscal (:101-102): the stress rates are those evaluated
+    // above; compute_P(t_try) of the RHS equals the call above unless the
+    // permeability evolves between calls
     if (hp.coupling == 1) {
       DFN_TRY(cg_launch(&t_try, 1, 0));
       DFN_TRY(cg_collect(0, 1, &fail));
