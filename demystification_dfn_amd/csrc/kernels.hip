@@ -113,7 +113,7 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
     const TV xi = (TV)x[i];
     TV v[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) v[j] = __builtin_nontemporal_load(V + i + j * ld);  // streamed once
+    for (int j = 0; j < K; j++) v[j] = V[i + j * ld];
 #pragma unroll
     for (int j = 0; j < K; j++) acc[j] += v[j] * xi;
   }
