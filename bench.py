@@ -44,6 +44,13 @@ def make_workload(name, seed):
                     element_m=10, fracture_mode="II", kernel="hmatrix", eta=2.0, eps_aca=1e-6, leaf=64,
                     friction="aging, a~U[0.008,0.012], b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa",
                     pulse="10 MPa on one seeded element", seed=seed)
+    elif name == "network80k_connected":
+        p = problems.network_80k(seed=seed, snap=True, name=f"network800_80k_connected_seed{seed}")
+        desc = dict(workload="network800_80k_connected", n_faults=800, box_m=7000,
+                    length_law="p(L)~L^-0.1 on [50 m, 2 km]", element_m=10, fracture_mode="II", kernel="hmatrix",
+                    eta=2.0, eps_aca=1e-6, leaf=64, crossings=p.meta["snapped_crossings"],
+                    note="the 80k network with every fault intersection snapped to a shared node (hydraulic "
+                         "crossings)", seed=seed)
     elif name == "single_fault":
         p = problems.single_fault(n=4000, length=40e3, static_kernel=_abi.KERNEL_DIRECT, tol=1e-8)
         desc = dict(workload="single_fault_4000", fracture_mode="III", kernel="direct", length_m=40e3,

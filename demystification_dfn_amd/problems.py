@@ -157,6 +157,41 @@ def single_fault(n=4000, length=40e3, static_kernel=_abi.KERNEL_DIRECT, mode=_ab
                    meta=dict(length=length, ds=ds), **hpar)
 
 
+def snap_intersections(nodes):
+    """Moves, for every pair of intersecting straight faults (node arrays, in
+    order), the nearest interior node of each onto the intersection point; a node
+    takes at most one crossing.  Returns the number of crossings made."""
+    nf = len(nodes)
+    p0 = np.array([n[0] for n in nodes])
+    p1 = np.array([n[-1] for n in nodes])
+    d = p1 - p0
+    used = [set() for _ in range(nf)]
+    made = 0
+    for i in range(nf):
+        # segment i against all later segments j: p0_i + s d_i = p0_j + t d_j, s, t in (0, 1)
+        dj = d[i + 1:]
+        den = d[i, 0] * dj[:, 1] - d[i, 1] * dj[:, 0]
+        ok = np.abs(den) > 1e-12
+        q = p0[i + 1:] - p0[i]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sv = (q[:, 0] * dj[:, 1] - q[:, 1] * dj[:, 0]) / den
+            tv = (q[:, 0] * d[i, 1] - q[:, 1] * d[i, 0]) / den
+        hit = ok & (sv > 0) & (sv < 1) & (tv > 0) & (tv < 1)
+        for jj in np.nonzero(hit)[0]:
+            j = i + 1 + int(jj)
+            x = p0[i] + sv[jj] * d[i]
+            ni, nj = nodes[i].shape[0] - 1, nodes[j].shape[0] - 1
+            ki, kj = int(round(sv[jj] * ni)), int(round(tv[jj] * nj))
+            if not (1 <= ki <= ni - 1 and 1 <= kj <= nj - 1) or ki in used[i] or kj in used[j]:
+                continue
+            nodes[i][ki] = x
+            nodes[j][kj] = x
+            used[i].add(ki)
+            used[j].add(kj)
+            made += 1
+    return made
+
+
 def power_law_lengths(rng, n, lmin, lmax, alpha):
     """Inverse-CDF sample of p(L) ~ L^-alpha on [lmin, lmax]."""
     u = rng.random(n)
@@ -170,7 +205,7 @@ def fault_network(n_faults=800, box=7000.0, lmin=50.0, lmax=2000.0, alpha=2.5, d
                   mode=_abi.MODE_II, static_kernel=_abi.KERNEL_HMATRIX, a_range=(0.008, 0.012),
                   b_weak=0.01, b_strong=0.006, frac_weak=0.7, Dc=1e-3, sigmaN=-30e6, f0=0.6, V0=1e-6,
                   V_init=1e-12, permeability=1e-15, pulse=10e6, tol=1e-6, eta=2.0, eps_aca=1e-6,
-                  nleaf=64, max_rank=20, name=None):
+                  nleaf=64, max_rank=20, snap=False, name=None):
     """Seeded random network of straight faults in a box.
 
     Lengths follow p(L) ~ L^-alpha on [lmin, lmax], strikes are uniform in
@@ -179,6 +214,12 @@ def fault_network(n_faults=800, box=7000.0, lmin=50.0, lmax=2000.0, alpha=2.5, d
     faults, else b_strong.  No tectonic loading and no injection source: a
     pore-pressure pulse of `pulse` Pa sits on one seeded element.  Material and
     hydraulic values as wrapper_python/LSBB_noloading.py:141-200.
+
+    The reference connects faults hydraulically only where two faults share a node
+    (distance below 0.01 min ds, src/initialisation.f90:279-344), which random
+    straight faults never do.  With snap=True, every intersection of two faults
+    moves the nearest interior node of each onto the intersection point (at most
+    one crossing per node), so intersecting faults become hydraulic crossings.
     """
     rng = np.random.default_rng(seed)
     lengths = power_law_lengths(rng, n_faults, lmin, lmax, alpha)
@@ -187,11 +228,16 @@ def fault_network(n_faults=800, box=7000.0, lmin=50.0, lmax=2000.0, alpha=2.5, d
     avals = rng.uniform(a_range[0], a_range[1], n_faults)
     weak = rng.random(n_faults) < frac_weak
     faults = []
+    nodes = []
     for i in range(n_faults):
         nel = max(1, int(round(lengths[i] / ds)))
         s = np.linspace(-lengths[i] / 2, lengths[i] / 2, nel + 1)
         d = np.array([np.cos(strikes[i]), np.sin(strikes[i])])
-        node = centres[i][None, :] + s[:, None] * d[None, :]
+        nodes.append(centres[i][None, :] + s[:, None] * d[None, :])
+    crossings = snap_intersections(nodes) if snap else 0
+    for i in range(n_faults):
+        node = nodes[i]
+        nel = node.shape[0] - 1
         one = np.ones(nel)
         faults.append(dict(node=node, a=avals[i] * one, b=(b_weak if weak[i] else b_strong) * one,
                            Dc=Dc * one, f0=f0 * one, V0=V0 * one, sigmaN=sigmaN * one, V=V_init * one,
@@ -208,7 +254,8 @@ def fault_network(n_faults=800, box=7000.0, lmin=50.0, lmax=2000.0, alpha=2.5, d
                    mu=32.04e9, cs=3.464e3, cp=5000.0, tol_solver=tol, eta=eta, eps_aca=eps_aca,
                    nleaf=nleaf, max_rank=max_rank,
                    meta=dict(seed=seed, n_faults=n_faults, box=box, lmin=lmin, lmax=lmax, alpha=alpha,
-                             ds=ds, pulse_element=k_pulse, weak_fraction=float(np.mean(weak))))
+                             ds=ds, pulse_element=k_pulse, weak_fraction=float(np.mean(weak)),
+                             snapped_crossings=crossings))
     return prob
 
 

@@ -94,3 +94,12 @@ def reference_network():
     1029 elements), as read by the library's HDF5 reader into the fixture."""
     z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "network_rate_strengthening.npz"))
     return problem_from_arrays(z, "network_rate_strengthening")
+
+
+def connected_network(static_kernel=0):
+    """40 straight Mode II faults in 600 m x 600 m with every intersection snapped
+    to a shared node (problems.snap_intersections): 944 elements, 52 hydraulic
+    crossings, so the pulse diffuses from fault to fault."""
+    return problems.network_small(n_faults=40, seed=3, box=600.0, lmin=100.0, lmax=400.0, alpha=0.5, snap=True,
+                                  static_kernel=static_kernel, eta=2.0, eps_aca=1e-6, nleaf=32,
+                                  name="connected_network_40")

@@ -56,7 +56,8 @@ CG_PATHS = ["auto", "coop", "plain"]
 
 
 @pytest.mark.parametrize("cg_path", CG_PATHS)
-@pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network, cases.grid_network])
+@pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network, cases.grid_network,
+                                  cases.connected_network])
 def test_compute_P(make, cg_path, monkeypatch):
     monkeypatch.setenv("DFN_CG_PATH", cg_path)
     p = make()
@@ -73,7 +74,7 @@ def test_compute_P(make, cg_path, monkeypatch):
 
 @pytest.mark.parametrize("cg_path", CG_PATHS)
 @pytest.mark.parametrize("make,nsteps", [(cases.single_fault_small, 20), (cases.crossing_network, 30),
-                                          (cases.grid_network, 4)])
+                                          (cases.grid_network, 4), (cases.connected_network, 12)])
 def test_solver_trajectory(make, nsteps, cg_path, monkeypatch):
     monkeypatch.setenv("DFN_CG_PATH", cg_path)
     p = make()
@@ -307,3 +308,23 @@ def test_reference_input_file_network_hmatrix_on_identical_block_lists():
     real[p.fake_indices()] = False
     for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
         assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)
+
+
+def test_connected_network_hmatrix_trajectory():
+    """The snapped network (52 crossings) with the H-matrix kernel on identical
+    block lists: 12 steps with rejections and crossing-coupled diffusion."""
+    p = cases.connected_network(static_kernel=_abi.KERNEL_HMATRIX)
+    d, o = dev(p), OracleSimulation(p, threads=8)
+    d.initiate_hmat()
+    for op in (0, 1):
+        o.hmat_import(op, d.hmat_export(op))
+    sd, so = d.solve(12), o.solve(12)
+    assert sd["rejected_attempts"] == so["rejected_attempts"] > 0
+    Yd, Pd = d.get_state()[:2]
+    Yo, Po = o.get_state()[:2]
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)
+    assert np.max(np.abs(Pd - Po)) <= 1e-9 * np.abs(Po).max() + 10 * p.tol_solver
