@@ -118,6 +118,7 @@ constexpr int kCgLdsThreads = 512;          // threads of the one-workgroup-per-
 constexpr int kCgLdsMax = kCgLdsThreads * 8;  // largest system it takes (elements)
 constexpr int kCgEPT = 8;                   // elements per thread in the cooperative CG
 constexpr int kCgSpan = kCgEPT * 256;       // elements per workgroup
+constexpr int kCgCoopMaxEntries = 896;      // crossing entries per workgroup span the cooperative CG stages in LDS
 
 struct PermArgs {
   int n, coupling;
@@ -169,9 +170,12 @@ void launch_ens_cg_dt(CgSys *sys, int nnet, int nslot, const double *t_try, cons
 void launch_cg_lds_multi(const CgArgs &a, const CrossRows &xr, int nsys, int max_n, hipStream_t s);
 void launch_restart_stage2(const double *y, const double *k1, double h, int n3, double *yt, hipStream_t s);
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
-// cooperative multi-workgroup CG; returns false if the grid cannot be co-resident
 bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s);
-bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *partial, int *sysflag, hipStream_t s);
+// cooperative multi-workgroup CG; cap = the most crossing entries of any span of
+// kCgSpan rows.  Returns false if a span holds more than kCgCoopMaxEntries or one
+// system's workgroups cannot be co-resident.
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, double *partial, int *sysflag,
+                    hipStream_t s);
 void launch_perm_update(const PermArgs &a, hipStream_t s);
 void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 

@@ -35,6 +35,7 @@ struct Engine {
   DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot, scratch3, scratch3b;
   DMem cg_work, cg_sys, cg_res, cg_partial, cg_flags, xrow_start, xrow_ent;
   bool coop_ok = true;
+  int span_entries = 0;  // most crossing entries in one span of kCgSpan rows (cooperative CG)
   // compute_P kernel choice: 0 = by size (one workgroup per system up to
   // kCgLdsMax elements, else cooperative, else the plain one-workgroup kernel);
   // 1/2/3 = prefer the LDS / cooperative / plain kernel.  Set from the
@@ -165,6 +166,8 @@ struct Engine {
         for (auto [c, z] : rows[k]) { ent.push_back(c); ent.push_back(z); }
       }
       start[N] = (int)(ent.size() / 2);
+      for (int s0 = 0; s0 < N; s0 += kCgSpan)
+        span_entries = std::max(span_entries, start[std::min(s0 + kCgSpan, N)] - start[s0]);
       if (ent.empty()) ent.assign(2, 0);
       DFN_TRY(xrow_start.upload(start));
       DFN_TRY(xrow_ent.upload(ent));
@@ -281,7 +284,7 @@ struct Engine {
     }
     if (!launched && coop_ok && cg_path != 3) {
       CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
-      launched = launch_cg_coop(A, X, nsys, cg_partial.as<double>() + 0, cg_flags.as<int>(), cs);
+      launched = launch_cg_coop(A, X, nsys, span_entries, cg_partial.as<double>() + 0, cg_flags.as<int>(), cs);
       if (!launched) coop_ok = false;  // grid too large to be co-resident: one workgroup per system
     }
     if (!launched) launch_cg(A, nsys, cs);

@@ -905,50 +905,78 @@ __device__ __forceinline__ double partial_max(const double *sp, int wps, int off
 }
 
 // Barrier among the nblk workgroups of one system of a cooperative launch (all
-// of them are resident, so the wait ends): release, arrive on the system's
-// counter, spin until all have arrived at this barrier, acquire.  The counter
-// only grows (target += nblk per barrier).  A wait that outlasts ~2^20 polls
-// gives up and reports through timed_out, so a fault can never hang the device.
+// of them are resident, so the wait ends): every thread releases its writes at
+// device scope, one thread arrives on the system's counter and polls it with
+// relaxed loads until all have arrived at this barrier, then every thread
+// acquires (an acquiring load per poll would invalidate the caches on every
+// poll).  The counter only grows (target += nblk per barrier).  A wait that
+// outlasts ~2^20 polls gives up and reports through timed_out, so a fault can
+// never hang the device.
 __device__ __forceinline__ void sys_barrier(unsigned *bar, int nblk, unsigned &target, bool &timed_out) {
   __shared__ int s_late;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   target += (unsigned)nblk;
   if (threadIdx.x == 0) {
-    __threadfence();
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     int spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target && spins <= (1 << 20)) {
-      __builtin_amdgcn_s_sleep(2);
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target && spins <= (1 << 20)) {
+      __builtin_amdgcn_s_sleep(1);
       spins++;
     }
-    __threadfence();
     s_late = spins > (1 << 20);
   }
   __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   timed_out = timed_out || s_late;
 }
 
 
-// Per iteration: A p (neighbours of p_i formed from r_{i-1} and p_{i-1} in global
-// memory), a barrier among the system's workgroups on p.Ap, update of x and r (r
-// and p_i published), a second barrier on r.r and max|r|, new direction in
-// registers.  S.r and S.p / S.Ap hold the published residual and the two
-// alternating copies of the direction.  Systems synchronise only within
-// themselves: a system that converges early leaves the device to the others.
-__global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int wps, double *partial, int *sysflag) {
+// The crossing entries of one workgroup's span of rows, staged in LDS at the
+// start of a cooperative compute_P.  Entry j (in row order) contributes
+// (c1 v(ea) + c2 v(eb) + c3 v(ec)) / dv to its row (src/fluid_diffusion.f90:221-291);
+// term[j] holds that contribution for the current direction.
+struct SpanEntries {
+  double *c1, *c2, *c3, *dv, *term;
+  int *ea, *eb, *ec;
+  __device__ SpanEntries(double *base, int cap)
+      : c1(base), c2(base + cap), c3(base + 2 * cap), dv(base + 3 * cap), term(base + 4 * cap),
+        ea(reinterpret_cast<int *>(base + 5 * cap)), eb(ea + cap), ec(ea + 2 * cap) {}
+};
+
+// Per iteration: the direction p_i of the span goes to LDS, with the two halo
+// values of the neighbouring spans formed from r_{i-1} and p_{i-1} in global
+// memory; the span's crossing terms are evaluated in parallel (operands from
+// LDS inside the span, from global memory outside it); then A p row by row, a
+// barrier among the system's workgroups on p.Ap, the update of x and r (r and
+// p_i published for the elements other workgroups read: the span ends and the
+// crossing elements), and a second barrier on r.r and max|r|.  S.r and S.p /
+// S.Ap hold the published residual and the two alternating copies of the
+// direction.  Systems synchronise only within themselves: a system that
+// converges early leaves the device to the others.  Dynamic LDS:
+// (kCgSpan + 2) doubles for p, then 5 doubles and 3 ints per crossing entry of
+// the span (cap entries; the launcher checks that every span fits).
+__global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int wps, int cap, double *partial,
+                                                      int *sysflag) {
+  extern __shared__ double coop_lds[];
   __shared__ double red[8];
   const int sys = blockIdx.x / wps, part = blockIdx.x - sys * wps;
   const CgSys S = A.sys[sys];
   const int n = A.n;
   const double dt = S.dt, c = A.cfac;
   const double *ds = A.ds, *kx = A.perm_x, *kxd = A.perm_x_diag, *P = A.P;
-  const int base = part * kCgSpan + threadIdx.x;
+  const int span0 = part * kCgSpan, span1 = min(span0 + kCgSpan, n);
+  const int base = span0 + threadIdx.x;
   const int kq = n > 2 ? n : 2;
   const double dq = c * (ds[kq - 2] * ds[kq - 2]);
   const double d1 = c * (ds[0] * ds[0]);
   double *pbuf[2] = {S.p, S.Ap};
+  double *lp = coop_lds;  // lp[1 + k - span0] = p at element k; lp[0] and lp[1 + span1 - span0] are the halo
+  const SpanEntries E(coop_lds + kCgSpan + 2, cap);
+  const int t_lo = X.start[span0], nent = X.start[span1] - t_lo;
   double a1[kCgEPT], a2[kCgEPT], a3[kCgEPT], dg[kCgEPT], x[kCgEPT], r[kCgEPT], p[kCgEPT], ap[kCgEPT];
   int xs[kCgEPT], xe[kCgEPT];  // this element's range of crossing entries
+  unsigned shared = 0;         // bit q: element q is read by other workgroups
   double part_rr = 0.0, part_em = 0.0;
 #pragma unroll
   for (int q = 0; q < kCgEPT; q++) {
@@ -958,6 +986,7 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
     if (k < n) {
       xs[q] = X.start[k];
       xe[q] = X.start[k + 1];
+      if (k == span0 || k == span1 - 1 || xs[q] < xe[q]) shared |= 1u << q;
       // compute_diag (:308-405) with the permeability in effect before this call
       double d;
       if (k == 0) d = 1.0 + kxd[1] * dt / (c * (ds[0] * ds[0]));
@@ -972,6 +1001,22 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
         d = d + kl * dt / d1 + kr * dt / d1;
       }
       dg[q] = d;
+      // the crossing entries of this row with this call's coefficients (:221-291)
+      for (int t = xs[q]; t < xe[q]; t++) {
+        const int cidx = X.ent[2 * t], z = X.ent[2 * t + 1];
+        const int64_t n1 = A.cross[6 * cidx + 2], n2 = A.cross[6 * cidx + 3];
+        int64_t ea, eb, ec, pl0, pl1, pr0, pr1;
+        cross_ends(z, n1, n2, ea, eb, ec, pl0, pl1, pr0, pr1);
+        const double kl = hmean(A.perm[pl0], A.perm[pl1]), kr = hmean(A.perm[pr0], A.perm[pr1]);
+        const int j = t - t_lo;
+        E.c1[j] = -(kl * dt / dq);
+        E.c3[j] = -(kr * dt / dq);
+        E.c2[j] = kl * dt / dq + kr * dt / dq;
+        E.dv[j] = z == 1 ? (double)n1 : d;
+        E.ea[j] = (int)ea;
+        E.eb[j] = (int)eb;
+        E.ec[j] = (int)ec;
+      }
       // compute_MatrixVector coefficients (:175-204)
       if (k == 0) {
         a2[q] = 1.0 + kx[1] * dt / (c * (ds[0] * ds[0]));
@@ -996,7 +1041,7 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       const double mv = coop_mv(A, X, k, xs[q], xe[q], P[k], vP, a1[q], a2[q], a3[q], d, dt, dq);
       r[q] = b - mv;
       p[q] = r[q];
-      S.r[k] = r[q];
+      if (shared & (1u << q)) S.r[k] = r[q];
       part_rr += r[q] * r[q];
       part_em = fmax(part_em, fabs(r[q] / 1.0));
     }
@@ -1021,14 +1066,38 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   while (em / A.tol >= 1.0 && !timed_out) {
     const double *pprev = pbuf[(it + 1) & 1];  // p_{i-1}
     double *pcur = pbuf[it & 1];               // receives p_i
-    double part_pap = 0.0;
     // p_1 = r_0 is read as is; later p_i = r_{i-1} + beta_{i-1} p_{i-1}
     const NbVec vp{S.r, it == 0 ? nullptr : pprev, beta_prev};
 #pragma unroll
     for (int q = 0; q < kCgEPT; q++) {
       const int k = base + q * 256;
+      if (k < n) lp[1 + k - span0] = p[q];
+    }
+    if (threadIdx.x == 0 && span0 > 0) lp[0] = vp(span0 - 1);
+    if (threadIdx.x == blockDim.x - 1 && span1 < n) lp[1 + span1 - span0] = vp(span1);
+    __syncthreads();  // p of the span and its halo are in LDS
+    for (int j = threadIdx.x; j < nent; j += blockDim.x) {
+      const int ea = E.ea[j], eb = E.eb[j], ec = E.ec[j];
+      const double va = (ea >= span0 && ea < span1) ? lp[1 + ea - span0] : vp(ea);
+      const double vb = (eb >= span0 && eb < span1) ? lp[1 + eb - span0] : vp(eb);
+      const double vc = (ec >= span0 && ec < span1) ? lp[1 + ec - span0] : vp(ec);
+      const double num = E.c1[j] * va + E.c2[j] * vb + E.c3[j] * vc;
+      E.term[j] = num / E.dv[j];
+    }
+    __syncthreads();  // the crossing terms are in LDS
+    double part_pap = 0.0;
+#pragma unroll
+    for (int q = 0; q < kCgEPT; q++) {
+      const int k = base + q * 256;
       if (k < n) {
-        ap[q] = coop_mv(A, X, k, xs[q], xe[q], p[q], vp, a1[q], a2[q], a3[q], dg[q], dt, dq);
+        const int l = 1 + k - span0;
+        double v;
+        if (k == 0) v = a2[q] * p[q] + a3[q] * lp[l + 1];
+        else if (k == n - 1) v = a1[q] * lp[l - 1] + a2[q] * p[q];
+        else v = a1[q] * lp[l - 1] + a2[q] * p[q] + a3[q] * lp[l + 1];
+        v = v / dg[q];
+        for (int t = xs[q]; t < xe[q]; t++) v = v + E.term[t - t_lo];
+        ap[q] = v;
         part_pap += p[q] * ap[q];
       }
     }
@@ -1044,8 +1113,10 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       if (k < n) {
         x[q] = x[q] + alpha * p[q];
         r[q] = r[q] - alpha * ap[q];
-        S.r[k] = r[q];
-        pcur[k] = p[q];
+        if (shared & (1u << q)) {
+          S.r[k] = r[q];
+          pcur[k] = p[q];
+        }
         part_r1 += r[q] * r[q];
         part_m = fmax(part_m, fabs(r[q] / 1.0));
       }
@@ -1400,26 +1471,45 @@ void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s) {
   hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
 }
 
-bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, double *partial, int *sysflag, hipStream_t s) {
+size_t cg_coop_lds_bytes(int cap) {
+  return sizeof(double) * (kCgSpan + 2) + (size_t)(cap > 1 ? cap : 1) * (5 * sizeof(double) + 3 * sizeof(int));
+}
+
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, double *partial, int *sysflag,
+                    hipStream_t s) {
+  if (cap > kCgCoopMaxEntries) return false;
+  const size_t sh = cg_coop_lds_bytes(cap);
+  static size_t cached_sh = 0;
   static int max_blocks = -1;
-  if (max_blocks < 0) {
+  if (max_blocks < 0 || sh != cached_sh) {
     int per_cu = 0, dev = 0, cus = 0, coop = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_coop_kernel, 256, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_coop_kernel, 256, sh) != hipSuccess)
       max_blocks = 0;
     else
       max_blocks = per_cu * cus;
+    cached_sh = sh;
   }
   int wps = (a.n + kCgSpan - 1) / kCgSpan;
-  if (nsys * wps > max_blocks) return false;
-  CgArgs A = a;
-  CrossRows X = xr;
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(sysflag), nsys);
-  void *args[] = {&A, &X, &wps, &partial, &sysflag};
-  return hipLaunchCooperativeKernel((const void *)cg_coop_kernel, dim3(nsys * wps), dim3(256), args, 0, s) ==
-         hipSuccess;
+  if (wps > max_blocks) return false;
+  // as many systems per cooperative launch as can be resident together
+  const int per_launch = max_blocks / wps;
+  for (int s0 = 0; s0 < nsys; s0 += per_launch) {
+    const int ns = nsys - s0 < per_launch ? nsys - s0 : per_launch;
+    CgArgs A = a;
+    A.sys = a.sys + s0;
+    CrossRows X = xr;
+    double *part = partial + 4 * (size_t)s0 * wps;
+    int *flags = sysflag + s0;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(flags), ns);
+    void *args[] = {&A, &X, &wps, &cap, &part, &flags};
+    if (hipLaunchCooperativeKernel((const void *)cg_coop_kernel, dim3(ns * wps), dim3(256), args, sh, s) !=
+        hipSuccess)
+      return false;
+  }
+  return true;
 }
 
 static bool cg_lds_dispatch(const CgArgs &a, const CrossRows &xr, int nsys, int max_n, hipStream_t s) {

@@ -103,3 +103,12 @@ def connected_network(static_kernel=0):
     return problems.network_small(n_faults=40, seed=3, box=600.0, lmin=100.0, lmax=400.0, alpha=0.5, snap=True,
                                   static_kernel=static_kernel, eta=2.0, eps_aca=1e-6, nleaf=32,
                                   name="connected_network_40")
+
+
+def connected_network_spans(static_kernel=0):
+    """120 snapped Mode II faults in 2 km x 2 km: 4855 elements, 143 hydraulic
+    crossings, so the cooperative compute_P splits each system over three
+    workgroup spans of 2048 rows with crossing terms between spans."""
+    return problems.fault_network(n_faults=120, box=2000.0, lmin=100.0, lmax=800.0, alpha=0.5, seed=3, eta=2.0,
+                                  eps_aca=1e-6, nleaf=32, tol=1e-8, snap=True, static_kernel=static_kernel,
+                                  name="connected_network_120")

@@ -57,7 +57,7 @@ CG_PATHS = ["auto", "coop", "plain"]
 
 @pytest.mark.parametrize("cg_path", CG_PATHS)
 @pytest.mark.parametrize("make", [cases.single_fault_small, cases.crossing_network, cases.grid_network,
-                                  cases.connected_network])
+                                  cases.connected_network, cases.connected_network_spans])
 def test_compute_P(make, cg_path, monkeypatch):
     monkeypatch.setenv("DFN_CG_PATH", cg_path)
     p = make()
@@ -320,6 +320,28 @@ def test_connected_network_hmatrix_trajectory():
         o.hmat_import(op, d.hmat_export(op))
     sd, so = d.solve(12), o.solve(12)
     assert sd["rejected_attempts"] == so["rejected_attempts"] > 0
+    Yd, Pd = d.get_state()[:2]
+    Yo, Po = o.get_state()[:2]
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)
+    assert np.max(np.abs(Pd - Po)) <= 1e-9 * np.abs(Po).max() + 10 * p.tol_solver
+
+
+def test_connected_network_spans_trajectory():
+    """4855 elements, 143 crossings: the cooperative compute_P (three workgroup
+    spans per system, crossing operands in other spans read from global memory)
+    inside 6 H-matrix steps on identical block lists."""
+    p = cases.connected_network_spans(static_kernel=_abi.KERNEL_HMATRIX)
+    d, o = dev(p), OracleSimulation(p, threads=8)
+    d.initiate_hmat()
+    for op in (0, 1):
+        o.hmat_import(op, d.hmat_export(op))
+    sd, so = d.solve(6), o.solve(6)
+    assert sd["rejected_attempts"] == so["rejected_attempts"]
+    assert sd["cg_iterations"] > 0
     Yd, Pd = d.get_state()[:2]
     Yo, Po = o.get_state()[:2]
     N = p.N
