@@ -119,6 +119,8 @@ constexpr int kCgLdsMax = kCgLdsThreads * 8;  // largest system it takes (elemen
 constexpr int kCgEPT = 8;                   // elements per thread in the cooperative CG
 constexpr int kCgSpan = kCgEPT * 256;       // elements per workgroup
 constexpr int kCgCoopMaxEntries = 896;      // crossing entries per workgroup span the cooperative CG stages in LDS
+constexpr int kCgRecordWords = 8;           // message words per workgroup of the cooperative CG
+constexpr unsigned kCgMaxSeq = 1u << 24;    // launch sequence numbers of the cooperative CG are below this
 
 struct PermArgs {
   int n, coupling;
@@ -172,10 +174,13 @@ void launch_restart_stage2(const double *y, const double *k1, double h, int n3, 
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
 bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s);
 // cooperative multi-workgroup CG; cap = the most crossing entries of any span of
-// kCgSpan rows.  Returns false if a span holds more than kCgCoopMaxEntries or one
-// system's workgroups cannot be co-resident.
-bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, double *partial, int *sysflag,
-                    hipStream_t s);
+// kCgSpan rows; seq in [1, 2^24) differs from the seq of every earlier launch on
+// the same records and direction buffers since they were last cleared; records
+// holds kCgRecordWords words per workgroup of each system.  Returns false if a
+// span holds more than kCgCoopMaxEntries or one system's workgroups cannot be
+// co-resident.
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, unsigned seq,
+                    unsigned long long *records, hipStream_t s);
 void launch_perm_update(const PermArgs &a, hipStream_t s);
 void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 

@@ -33,9 +33,10 @@ struct Engine {
   DMem perm_cur, perm_next, permx_cur, permx_next;
   // state and stage buffers
   DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot, scratch3, scratch3b;
-  DMem cg_work, cg_sys, cg_res, cg_partial, cg_flags, xrow_start, xrow_ent;
+  DMem cg_work, cg_sys, cg_res, cg_records, xrow_start, xrow_ent;
   bool coop_ok = true;
   int span_entries = 0;  // most crossing entries in one span of kCgSpan rows (cooperative CG)
+  unsigned cg_seq = 0;   // sequence number of the last cooperative CG launch
   // compute_P kernel choice: 0 = by size (one workgroup per system up to
   // kCgLdsMax elements, else cooperative, else the plain one-workgroup kernel);
   // 1/2/3 = prefer the LDS / cooperative / plain kernel.  Set from the
@@ -150,8 +151,10 @@ struct Engine {
     DFN_TRY(cg_sys.alloc(sizeof(CgSys) * kSlots));
     DFN_TRY(status.alloc(sizeof(StatusWord)));
     const int wps = (N + kCgSpan - 1) / kCgSpan;
-    DFN_TRY(cg_partial.alloc(sizeof(double) * 4 * (size_t)kSlots * wps));
-    DFN_TRY(cg_flags.alloc(sizeof(int) * kSlots));
+    DFN_TRY(cg_records.alloc(sizeof(unsigned long long) * kCgRecordWords * (size_t)kSlots * wps));
+    // the cooperative CG's message words start with epoch 0, which no launch uses
+    DFN_HIP(hipMemset(cg_records.p, 0, sizeof(unsigned long long) * kCgRecordWords * (size_t)kSlots * wps));
+    DFN_HIP(hipMemset(cg_work.p, 0, sizeof(double) * (size_t)kSlots * 12 * N));
     // crossing rows of the stencil, per element in crossing order
     {
       std::vector<std::vector<std::pair<int, int>>> rows(N);
@@ -284,7 +287,14 @@ struct Engine {
     }
     if (!launched && coop_ok && cg_path != 3) {
       CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
-      launched = launch_cg_coop(A, X, nsys, span_entries, cg_partial.as<double>() + 0, cg_flags.as<int>(), cs);
+      if (++cg_seq >= kCgMaxSeq) {  // epochs would repeat: clear every message word first
+        const int wps = (N + kCgSpan - 1) / kCgSpan;
+        DFN_HIP(hipMemsetAsync(cg_records.p, 0, sizeof(unsigned long long) * kCgRecordWords * (size_t)kSlots * wps,
+                               cs));
+        DFN_HIP(hipMemsetAsync(cg_work.p, 0, sizeof(double) * (size_t)kSlots * 12 * N, cs));
+        cg_seq = 1;
+      }
+      launched = launch_cg_coop(A, X, nsys, span_entries, cg_seq, cg_records.as<unsigned long long>(), cs);
       if (!launched) coop_ok = false;  // grid too large to be co-resident: one workgroup per system
     }
     if (!launched) launch_cg(A, nsys, cs);

@@ -890,47 +890,74 @@ __device__ __forceinline__ double coop_mv(const CgArgs &A, const CrossRows &X, i
 }
 
 
-// Sum of the wps workgroup partials sp[4 p + off] of one system, one per thread
-// then a fixed tree: every workgroup of the system gets the same double.
-__device__ __forceinline__ double partial_sum(const double *sp, int wps, int off, double *red) {
+// Messages between the workgroups of one system of a cooperative compute_P.  A
+// double travels as two 64-bit words, each holding 32 bits of it beside the
+// 32-bit epoch of the message; a reader takes the value once both words carry
+// the epoch it waits for.  Every word is written and read with relaxed
+// device-scope atomics, so each is seen whole and eventually, and no fence or
+// grid barrier is needed: each word's epoch vouches for its own half.  A wait
+// that outlasts kCgSpins polls gives up and sets `late`, so a fault can never
+// hang the device.
+constexpr int kCgSpins = 1 << 18;
+
+__device__ __forceinline__ void ll_send(unsigned long long *w0, unsigned long long *w1, double v, unsigned epoch) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long e = (unsigned long long)epoch << 32;
+  __hip_atomic_store(w0, e | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(w1, e | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double ll_recv(unsigned long long *w0, unsigned long long *w1, unsigned epoch,
+                                          bool &late) {
+  for (int spins = 0;; spins++) {
+    const unsigned long long a = __hip_atomic_load(w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long b = __hip_atomic_load(w1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned)(a >> 32) == epoch && (unsigned)(b >> 32) == epoch)
+      return __longlong_as_double((long long)((b << 32) | (a & 0xffffffffull)));
+    if (spins >= kCgSpins) { late = true; return 0.0; }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Workgroup records of a system: 8 words per workgroup, [0,1] the p.Ap partial
+// (or the final overflow flag), [2,3] r.r, [4,5] max|r|.
+constexpr int kCgRec = kCgRecordWords;
+
+// Sum (or max) over the wps workgroups' messages at word `off` of their
+// records, one per thread, then a fixed tree: every workgroup of the system
+// gets the same double.
+__device__ __forceinline__ double ll_sum(unsigned long long *rec, int wps, int off, unsigned epoch, double *red,
+                                         bool &late) {
   double v = 0.0;
-  for (int q = threadIdx.x; q < wps; q += blockDim.x) v += sp[4 * q + off];
+  for (int q = threadIdx.x; q < wps; q += blockDim.x)
+    v += ll_recv(rec + kCgRec * q + off, rec + kCgRec * q + off + 1, epoch, late);
   return block_sum(v, red);
 }
 
-__device__ __forceinline__ double partial_max(const double *sp, int wps, int off, double *red) {
-  double v = 0.0;
-  for (int q = threadIdx.x; q < wps; q += blockDim.x) v = fmax(v, sp[4 * q + off]);
-  return block_max(v, red);
+// Sum and max over the workgroup in one pass (red holds 16 doubles).
+__device__ __forceinline__ void block_sum_max(double &s, double &m, double *red) {
+  wave_sum_max_full(s, m);
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) { red[wid] = s; red[8 + wid] = m; }
+  __syncthreads();
+  double a = 0.0, b = 0.0;
+  for (int q = 0; q < nw; q++) { a += red[q]; b = fmax(b, red[8 + q]); }
+  s = a;
+  m = b;
 }
 
-// Barrier among the nblk workgroups of one system of a cooperative launch (all
-// of them are resident, so the wait ends): every thread releases its writes at
-// device scope, one thread arrives on the system's counter and polls it with
-// relaxed loads until all have arrived at this barrier, then every thread
-// acquires (an acquiring load per poll would invalidate the caches on every
-// poll).  The counter only grows (target += nblk per barrier).  A wait that
-// outlasts ~2^20 polls gives up and reports through timed_out, so a fault can
-// never hang the device.
-__device__ __forceinline__ void sys_barrier(unsigned *bar, int nblk, unsigned &target, bool &timed_out) {
-  __shared__ int s_late;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  target += (unsigned)nblk;
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    int spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target && spins <= (1 << 20)) {
-      __builtin_amdgcn_s_sleep(1);
-      spins++;
-    }
-    s_late = spins > (1 << 20);
+// The sum of the messages at word off_s and the max of those at off_m.
+__device__ __forceinline__ void ll_sum_max(unsigned long long *rec, int wps, int off_s, int off_m, unsigned epoch,
+                                           double *red, bool &late, double &s, double &m) {
+  s = 0.0;
+  m = 0.0;
+  for (int q = threadIdx.x; q < wps; q += blockDim.x) {
+    s += ll_recv(rec + kCgRec * q + off_s, rec + kCgRec * q + off_s + 1, epoch, late);
+    m = fmax(m, ll_recv(rec + kCgRec * q + off_m, rec + kCgRec * q + off_m + 1, epoch, late));
   }
-  __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  timed_out = timed_out || s_late;
+  block_sum_max(s, m, red);
 }
-
 
 // The crossing entries of one workgroup's span of rows, staged in LDS at the
 // start of a cooperative compute_P.  Entry j (in row order) contributes
@@ -944,22 +971,26 @@ struct SpanEntries {
         ea(reinterpret_cast<int *>(base + 5 * cap)), eb(ea + cap), ec(ea + 2 * cap) {}
 };
 
-// Per iteration: the direction p_i of the span goes to LDS, with the two halo
-// values of the neighbouring spans formed from r_{i-1} and p_{i-1} in global
-// memory; the span's crossing terms are evaluated in parallel (operands from
-// LDS inside the span, from global memory outside it); then A p row by row, a
-// barrier among the system's workgroups on p.Ap, the update of x and r (r and
-// p_i published for the elements other workgroups read: the span ends and the
-// crossing elements), and a second barrier on r.r and max|r|.  S.r and S.p /
-// S.Ap hold the published residual and the two alternating copies of the
-// direction.  Systems synchronise only within themselves: a system that
-// converges early leaves the device to the others.  Dynamic LDS:
-// (kCgSpan + 2) doubles for p, then 5 doubles and 3 ints per crossing entry of
-// the span (cap entries; the launcher checks that every span fits).
-__global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int wps, int cap, double *partial,
-                                                      int *sysflag) {
+// Cooperative compute_P: every system (time step dt) is spread over `wps`
+// workgroups of 256 threads, each owning a span of kCgSpan rows, kCgEPT per
+// thread held in registers (x, r, p, A p and the stencil coefficients).  Per
+// iteration: the span's direction goes to LDS, with the halo values of the
+// neighbouring spans and the crossing operands outside the span received as
+// messages; the span's crossing terms are evaluated in parallel; A p row by row;
+// the p.Ap partials are exchanged as messages and summed in a fixed order, so
+// every workgroup of a system computes the same alpha; x and r are updated and
+// the r.r and max|r| partials exchanged the same way (same beta and stopping
+// decision everywhere); the new direction is sent for the elements other
+// workgroups read (the span ends and the crossing elements).  Systems never
+// wait for each other: one that converges early leaves the device to the
+// others.  Message epochs are seq * 256 + a step number, so no word left by an
+// earlier launch can match.  Dynamic LDS: (kCgSpan + 2) doubles for p, then 5
+// doubles and 3 ints per crossing entry of the span (cap entries; the launcher
+// checks that every span fits).
+__global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int wps, int cap, unsigned seq,
+                                                      unsigned long long *records) {
   extern __shared__ double coop_lds[];
-  __shared__ double red[8];
+  __shared__ double red[16];
   const int sys = blockIdx.x / wps, part = blockIdx.x - sys * wps;
   const CgSys S = A.sys[sys];
   const int n = A.n;
@@ -970,7 +1001,12 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   const int kq = n > 2 ? n : 2;
   const double dq = c * (ds[kq - 2] * ds[kq - 2]);
   const double d1 = c * (ds[0] * ds[0]);
-  double *pbuf[2] = {S.p, S.Ap};
+  // direction messages, one per element: the two words of element e are w0[e], w1[e]
+  unsigned long long *w0 = reinterpret_cast<unsigned long long *>(S.p);
+  unsigned long long *w1 = reinterpret_cast<unsigned long long *>(S.Ap);
+  unsigned long long *rec = records + (size_t)kCgRec * sys * wps;
+  unsigned long long *myrec = rec + (size_t)kCgRec * part;
+  const unsigned ep0 = seq << 8;
   double *lp = coop_lds;  // lp[1 + k - span0] = p at element k; lp[0] and lp[1 + span1 - span0] are the halo
   const SpanEntries E(coop_lds + kCgSpan + 2, cap);
   const int t_lo = X.start[span0], nent = X.start[span1] - t_lo;
@@ -1041,46 +1077,39 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       const double mv = coop_mv(A, X, k, xs[q], xe[q], P[k], vP, a1[q], a2[q], a3[q], d, dt, dq);
       r[q] = b - mv;
       p[q] = r[q];
-      if (shared & (1u << q)) S.r[k] = r[q];
+      if (shared & (1u << q)) ll_send(w0 + k, w1 + k, p[q], ep0 + 1);  // p_1 = r_0
       part_rr += r[q] * r[q];
       part_em = fmax(part_em, fabs(r[q] / 1.0));
     }
   }
-  double *mypart = partial + 4 * (int64_t)(sys * wps + part);
-  const double *sp = partial + 4 * (int64_t)(sys * wps);
-  // this system's barrier counter (zeroed before the launch) and its target
-  unsigned *bar = reinterpret_cast<unsigned *>(sysflag) + sys;
-  unsigned target = 0;
-  bool timed_out = false;
-  part_rr = block_sum(part_rr, red);
-  part_em = block_max(part_em, red);
-  if (threadIdx.x == 0) { mypart[0] = part_rr; mypart[1] = part_em; }
-  sys_barrier(bar, wps, target, timed_out);
-  double rr = partial_sum(sp, wps, 0, red);
-  double em = partial_max(sp, wps, 1, red);
+  bool late = false;
+  block_sum_max(part_rr, part_em, red);
+  if (threadIdx.x == 0) {
+    ll_send(myrec + 2, myrec + 3, part_rr, ep0);
+    ll_send(myrec + 4, myrec + 5, part_em, ep0);
+  }
+  double rr, em;
+  ll_sum_max(rec, wps, 2, 4, ep0, red, late, rr, em);
+  bool timed_out = __syncthreads_or(late);
   bool fail = false;
   int it = 0;
-  double beta_prev = 0.0;
-  // every workgroup of the system reads the same partials in the same order, so
-  // they all take the same decisions and meet the same barriers
+  // every workgroup of the system receives the same partials and sums them in the
+  // same order, so they all take the same decisions
   while (em / A.tol >= 1.0 && !timed_out) {
-    const double *pprev = pbuf[(it + 1) & 1];  // p_{i-1}
-    double *pcur = pbuf[it & 1];               // receives p_i
-    // p_1 = r_0 is read as is; later p_i = r_{i-1} + beta_{i-1} p_{i-1}
-    const NbVec vp{S.r, it == 0 ? nullptr : pprev, beta_prev};
+    const unsigned ep = ep0 + (unsigned)it + 1;  // p_{it+1} and this iteration's partials
 #pragma unroll
     for (int q = 0; q < kCgEPT; q++) {
       const int k = base + q * 256;
       if (k < n) lp[1 + k - span0] = p[q];
     }
-    if (threadIdx.x == 0 && span0 > 0) lp[0] = vp(span0 - 1);
-    if (threadIdx.x == blockDim.x - 1 && span1 < n) lp[1 + span1 - span0] = vp(span1);
+    if (threadIdx.x == 0 && span0 > 0) lp[0] = ll_recv(w0 + span0 - 1, w1 + span0 - 1, ep, late);
+    if (threadIdx.x == blockDim.x - 1 && span1 < n) lp[1 + span1 - span0] = ll_recv(w0 + span1, w1 + span1, ep, late);
     __syncthreads();  // p of the span and its halo are in LDS
     for (int j = threadIdx.x; j < nent; j += blockDim.x) {
       const int ea = E.ea[j], eb = E.eb[j], ec = E.ec[j];
-      const double va = (ea >= span0 && ea < span1) ? lp[1 + ea - span0] : vp(ea);
-      const double vb = (eb >= span0 && eb < span1) ? lp[1 + eb - span0] : vp(eb);
-      const double vc = (ec >= span0 && ec < span1) ? lp[1 + ec - span0] : vp(ec);
+      const double va = (ea >= span0 && ea < span1) ? lp[1 + ea - span0] : ll_recv(w0 + ea, w1 + ea, ep, late);
+      const double vb = (eb >= span0 && eb < span1) ? lp[1 + eb - span0] : ll_recv(w0 + eb, w1 + eb, ep, late);
+      const double vc = (ec >= span0 && ec < span1) ? lp[1 + ec - span0] : ll_recv(w0 + ec, w1 + ec, ep, late);
       const double num = E.c1[j] * va + E.c2[j] * vb + E.c3[j] * vc;
       E.term[j] = num / E.dv[j];
     }
@@ -1102,9 +1131,8 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       }
     }
     part_pap = block_sum(part_pap, red);
-    if (threadIdx.x == 0) mypart[2] = part_pap;
-    sys_barrier(bar, wps, target, timed_out);
-    const double pAp = partial_sum(sp, wps, 2, red);
+    if (threadIdx.x == 0) ll_send(myrec + 0, myrec + 1, part_pap, ep);
+    const double pAp = ll_sum(rec, wps, 0, ep, red, late);
     const double alpha = rr / pAp;
     double part_r1 = 0.0, part_m = 0.0;
 #pragma unroll
@@ -1113,26 +1141,27 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       if (k < n) {
         x[q] = x[q] + alpha * p[q];
         r[q] = r[q] - alpha * ap[q];
-        if (shared & (1u << q)) {
-          S.r[k] = r[q];
-          pcur[k] = p[q];
-        }
         part_r1 += r[q] * r[q];
         part_m = fmax(part_m, fabs(r[q] / 1.0));
       }
     }
-    part_r1 = block_sum(part_r1, red);
-    part_m = block_max(part_m, red);
-    if (threadIdx.x == 0) { mypart[0] = part_r1; mypart[1] = part_m; }
-    sys_barrier(bar, wps, target, timed_out);
-    const double r1r1 = partial_sum(sp, wps, 0, red);
-    em = partial_max(sp, wps, 1, red);
+    block_sum_max(part_r1, part_m, red);
+    if (threadIdx.x == 0) {
+      ll_send(myrec + 2, myrec + 3, part_r1, ep);
+      ll_send(myrec + 4, myrec + 5, part_m, ep);
+    }
+    double r1r1;
+    ll_sum_max(rec, wps, 2, 4, ep, red, late, r1r1, em);
     const double beta = r1r1 / rr;
 #pragma unroll
-    for (int q = 0; q < kCgEPT; q++) p[q] = r[q] + beta * p[q];
+    for (int q = 0; q < kCgEPT; q++) {
+      const int k = base + q * 256;
+      p[q] = r[q] + beta * p[q];
+      if (k < n && (shared & (1u << q))) ll_send(w0 + k, w1 + k, p[q], ep + 1);
+    }
     rr = r1r1;
-    beta_prev = beta;
     it++;
+    timed_out = __syncthreads_or(late);
     if (it > kCgMaxIter) { fail = true; break; }
   }
   int bad = 0;
@@ -1148,15 +1177,18 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       }
     }
   }
+  // the overflow flags go to the system's first workgroup, which reports
   const double b2 = block_max(bad ? 1.0 : 0.0, red);
-  if (threadIdx.x == 0) mypart[3] = b2;
-  sys_barrier(bar, wps, target, timed_out);
-  if (part == 0 && threadIdx.x == 0) {
-    int anybad = 0;
-    for (int pp = 0; pp < wps; pp++) anybad |= (sp[4 * pp + 3] > 0.0);
-    // a barrier that never completed reports as a failed solve (2)
-    S.result[0] = timed_out ? 2 : ((fail || anybad) ? 1 : 0);
-    S.result[1] = it;
+  if (threadIdx.x == 0) ll_send(myrec + 0, myrec + 1, b2, ep0 + 255);
+  if (part == 0) {
+    double nbad, anybad;
+    ll_sum_max(rec, wps, 0, 0, ep0 + 255, red, late, nbad, anybad);
+    timed_out = __syncthreads_or(late) || timed_out;
+    // a message that never arrived reports as a failed solve (2)
+    if (threadIdx.x == 0) {
+      S.result[0] = timed_out ? 2 : ((fail || anybad > 0.0) ? 1 : 0);
+      S.result[1] = it;
+    }
   }
 }
 
@@ -1475,8 +1507,8 @@ size_t cg_coop_lds_bytes(int cap) {
   return sizeof(double) * (kCgSpan + 2) + (size_t)(cap > 1 ? cap : 1) * (5 * sizeof(double) + 3 * sizeof(int));
 }
 
-bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, double *partial, int *sysflag,
-                    hipStream_t s) {
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, unsigned seq,
+                    unsigned long long *records, hipStream_t s) {
   if (cap > kCgCoopMaxEntries) return false;
   const size_t sh = cg_coop_lds_bytes(cap);
   static size_t cached_sh = 0;
@@ -1501,10 +1533,8 @@ bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, dou
     CgArgs A = a;
     A.sys = a.sys + s0;
     CrossRows X = xr;
-    double *part = partial + 4 * (size_t)s0 * wps;
-    int *flags = sysflag + s0;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned *>(flags), ns);
-    void *args[] = {&A, &X, &wps, &cap, &part, &flags};
+    unsigned long long *rec = records + (size_t)kCgRecordWords * s0 * wps;
+    void *args[] = {&A, &X, &wps, &cap, &seq, &rec};
     if (hipLaunchCooperativeKernel((const void *)cg_coop_kernel, dim3(ns * wps), dim3(256), args, sh, s) !=
         hipSuccess)
       return false;
