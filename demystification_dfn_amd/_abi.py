@@ -116,6 +116,9 @@ SIGNATURES = {
     "h5_read_f64": [_H, C.c_char_p, _dp, C.c_int64],
     "h5_read_i64": [_H, C.c_char_p, _i64p, C.c_int64],
     "h5_read_string": [_H, C.c_char_p, C.c_char_p, C.c_int32],
+    "h5_write_f64": [C.c_char_p, C.c_int32, C.POINTER(C.c_char_p), _i32p, _i64p, C.POINTER(_dp)],
+    "set_output": [_H, C.c_char_p, C.c_int64, C.c_int64],
+    "flush_output": [_H],
     "bench_kernel": [_H, C.c_int32, _dp],
     "bench_stream": [C.c_int64, C.c_int32, _dp],
     "device_info": [C.c_char_p, C.c_int32],

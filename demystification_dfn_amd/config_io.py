@@ -76,6 +76,35 @@ class H5File:
         return buf.value.decode()
 
 
+def write_h5(path, datasets):
+    """Writes {name: FP64 array of rank 1 or 2} as the root group's datasets of a
+    new HDF5 file (dfn_h5_write_f64, at most 8 datasets)."""
+    from .device import load
+    f = load()
+    names = list(datasets)
+    arrs = [np.ascontiguousarray(datasets[k], np.float64) for k in names]
+    n = len(names)
+    cnames = (C.c_char_p * n)(*[k.encode() for k in names])
+    ranks = np.array([a.ndim for a in arrs], np.int32)
+    dims = np.zeros(2 * n, np.int64)
+    for i, a in enumerate(arrs):
+        k = min(a.ndim, 2)  # the library rejects other ranks
+        dims[2 * i:2 * i + k] = a.shape[:k]
+    ptrs = (C.POINTER(C.c_double) * n)(*[_abi.dptr(a) for a in arrs])
+    _abi.check(f["h5_write_f64"](os.fsencode(path), n, cnames, _abi.i32ptr(ranks), _abi.i64ptr(dims), ptrs),
+               f"dfn_h5_write_f64({path})")
+
+
+OUTPUT_FIELDS = ("V", "theta", "P", "tractionTel", "tractionNel", "permeability", "time")
+
+
+def read_output(path):
+    """{name: array} of a solver output file outputNNNNNNN (src/solver.f90:227-317):
+    the element fields have shape (N, saves), time has shape (saves,)."""
+    with H5File(path) as f:
+        return {k: f.f64(k) for k in OUTPUT_FIELDS}
+
+
 def read_config(path, **hmatrix):
     """Problem from a reference configuration file; hmatrix may override eta,
     eps_aca, nleaf and max_rank."""

@@ -160,6 +160,8 @@ void launch_ens_stage(int stage, const EnsStageArgs &a, hipStream_t s);
 void launch_ens_restart_stage2(const EnsStageArgs &a, hipStream_t s);
 void launch_ens_clear_status(StatusWord *st, const int *active, int nnet, hipStream_t s);
 // event monitor: slip += (V_old + V_new) h / 2 and per-fault max |V_new| (cleared first)
+void launch_save_output(const double *Y, const double *P, const double *sT, const double *perm, double *buf, int n,
+                        int64_t ld, int64_t col, hipStream_t s);
 void launch_monitor(const double *Vold, const double *Vnew, double h, const int *fault_of, int n, double *slip,
                     unsigned long long *fvmax, int nfault, hipStream_t s);
 // per-network max |V| over real elements (bits of non-negative doubles, cleared first)

@@ -480,6 +480,7 @@ struct Engine {
     st.accepted_steps++;
     // cumtrapz integrates over the differences of the recorded times
     if (monitor) DFN_TRY(monitor_step(time - t_before));
+    DFN_TRY(output_step());                         // save_data / write_data (:180-193)
     if (h_status->tensile) return DFN_ERR_TENSILE;  // check_errors :412-420 on the accepted state
     return DFN_OK;
   }
@@ -502,6 +503,73 @@ struct Engine {
 
   // after an accepted step over the time difference h: Ynew holds the step-start
   // state, Y the new one
+  // solver output (save_data / write_data, src/solver.f90:211-317)
+  std::string out_dir;
+  int64_t out_stride = 0, out_freq = 0, out_count = 0;
+  std::vector<double> out_time;
+  DMem out_buf;  // six (N x out_freq) fields, see launch_save_output
+
+  int output_set(const char *dir, int64_t stride, int64_t freq) {
+    out_dir.clear();
+    out_count = 0;
+    out_time.clear();
+    if (!dir) return DFN_OK;
+    if (stride < 1 || freq < 1) return DFN_ERR_ARG;
+    const size_t bytes = sizeof(double) * 6 * (size_t)hp.N * (size_t)freq;
+    if (out_buf.bytes < bytes) DFN_TRY(out_buf.alloc(bytes));
+    out_dir = dir;
+    out_stride = stride;
+    out_freq = freq;
+    return DFN_OK;
+  }
+
+  // after accepted step it_count = st.accepted_steps
+  int output_step() {
+    if (out_dir.empty()) return DFN_OK;
+    const int64_t it_count = st.accepted_steps;
+    if ((it_count - 1) % out_stride != 0) return DFN_OK;
+    launch_save_output(Y.as<double>(), P.as<double>(), sT.as<double>(), perm_cur.as<double>(), out_buf.as<double>(),
+                       hp.N, out_freq, out_count, stream);
+    out_time.push_back(time);
+    out_count++;
+    if (out_count == out_freq) return output_write();
+    return DFN_OK;
+  }
+
+  // dir/outputNNNNNNN with the out_count saves held, then none held
+  int output_write() {
+    if (out_dir.empty() || out_count == 0) return DFN_OK;
+    const int64_t N = hp.N, F = out_freq, C = out_count;
+    std::vector<double> h((size_t)(6 * N * F));
+    DFN_HIP(hipMemcpyAsync(h.data(), out_buf.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+    DFN_HIP(hipStreamSynchronize(stream));
+    std::vector<std::vector<double>> fld(6, std::vector<double>((size_t)(N * C)));
+    for (int f = 0; f < 6; f++)
+      for (int64_t e = 0; e < N; e++)
+        std::memcpy(&fld[f][(size_t)(e * C)], &h[(size_t)(f * N * F + e * F)], sizeof(double) * (size_t)C);
+    // theta = Dc / V0 * exp(psi) (:214)
+    for (int64_t e = 0; e < N; e++)
+      for (int64_t c = 0; c < C; c++) {
+        double &v = fld[1][(size_t)(e * C + c)];
+        v = hp.Dc[e] / hp.V0[e] * std::exp(v);
+      }
+    char name[32];
+    std::snprintf(name, sizeof name, "/output%07lld", (long long)st.accepted_steps);
+    const std::string path = out_dir + name;
+    const char *names[7] = {"V", "theta", "P", "tractionTel", "tractionNel", "permeability", "time"};
+    const int32_t ranks[7] = {2, 2, 2, 2, 2, 2, 1};
+    int64_t dims[14];
+    for (int i = 0; i < 6; i++) { dims[2 * i] = N; dims[2 * i + 1] = C; }
+    dims[12] = C;
+    dims[13] = 0;
+    const double *data[7] = {fld[0].data(), fld[1].data(), fld[2].data(), fld[3].data(), fld[4].data(),
+                             fld[5].data(), out_time.data()};
+    const int rc = dfn_h5_write_f64(path.c_str(), 7, names, ranks, dims, data);
+    out_count = 0;
+    out_time.clear();
+    return rc;
+  }
+
   int monitor_step(double h) {
     const int nf = hp.nfault;
     launch_monitor(Ynew.as<double>(), Y.as<double>(), h, fault_of.as<int>(), hp.N, slip.as<double>(),
@@ -879,6 +947,16 @@ API int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *s
     if (st == DFN_OK) st = s2;
   }
   return st;
+}
+
+API int dfn_set_output(dfn_handle h, const char *dir, int64_t stride_time, int64_t freq_writing_file) {
+  if (!h) return DFN_ERR_ARG;
+  return h->e.output_set(dir, stride_time, freq_writing_file);
+}
+
+API int dfn_flush_output(dfn_handle h) {
+  if (!h) return DFN_ERR_ARG;
+  return h->e.output_write();
 }
 
 API int dfn_monitor(dfn_handle h, int32_t on) {

@@ -10,6 +10,12 @@
 // This reader implements exactly that subset, from the published HDF5 file-format
 // specification; anything else (chunked or filtered storage, version-2 object
 // headers, other superblock versions) is reported as unsupported.
+//
+// The writer produces the reference's output files (src/solver.f90:227-317):
+// FP64 datasets of rank 1 or 2 in the root group, in the same subset (superblock
+// 0, one symbol-table node under a version-1 B-tree, a local heap of names,
+// version-1 object headers with dataspace, IEEE little-endian FP64 datatype,
+// fill-value and contiguous layout messages), encoded as libhdf5 encodes them.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -290,7 +296,175 @@ struct H5 {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Writer
+struct Writer {
+  std::vector<uint8_t> b;
+  static constexpr uint64_t kLeafK = 4, kInternalK = 16;  // symbol-table node and group B-tree ranks
+
+  uint64_t alloc(uint64_t n) {  // 8-byte aligned block of n zero bytes
+    const uint64_t off = (b.size() + 7) & ~7ull;
+    b.resize(off + n, 0);
+    return off;
+  }
+  void put(uint64_t off, uint64_t v, int n) {
+    for (int i = 0; i < n; i++) b[off + i] = (uint8_t)(v >> (8 * i));
+  }
+  // one object-header message: type, size (a multiple of 8), flags
+  static void message(std::vector<uint8_t> &m, int type, const std::vector<uint8_t> &body, int flags) {
+    const size_t sz = (body.size() + 7) & ~size_t(7);
+    const size_t at = m.size();
+    m.resize(at + 8 + sz, 0);
+    m[at] = (uint8_t)type;
+    m[at + 1] = (uint8_t)(type >> 8);
+    m[at + 2] = (uint8_t)sz;
+    m[at + 3] = (uint8_t)(sz >> 8);
+    m[at + 4] = (uint8_t)flags;
+    std::memcpy(&m[at + 8], body.data(), body.size());
+  }
+  static void le(std::vector<uint8_t> &v, uint64_t x, int n) {
+    for (int i = 0; i < n; i++) v.push_back((uint8_t)(x >> (8 * i)));
+  }
+  // version-1 object header holding the given messages
+  uint64_t object_header(const std::vector<uint8_t> &msgs, int nmsg) {
+    const uint64_t oh = alloc(16 + msgs.size());
+    b[oh] = 1;                      // version
+    put(oh + 2, (uint64_t)nmsg, 2);
+    put(oh + 4, 1, 4);              // reference count
+    put(oh + 8, msgs.size(), 4);    // bytes of message data
+    std::memcpy(&b[oh + 16], msgs.data(), msgs.size());
+    return oh;
+  }
+};
+
 }  // namespace
+
+// FP64 datasets (rank 1 or 2, C order) as the root group's members; names must be
+// distinct.  Returns DFN_ERR_ARG for bad arguments, DFN_ERR_UNSUPPORTED for more
+// datasets than one symbol-table node holds (8) or a name over 255 bytes.
+static int h5_write_f64(const char *path, int n, const char *const *names, const int32_t *ranks,
+                        const int64_t *dims, const double *const *data) {
+  if (!path || n < 1 || !names || !ranks || !dims || !data) return DFN_ERR_ARG;
+  if (n > (int)(2 * Writer::kLeafK)) return DFN_ERR_UNSUPPORTED;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) {
+    if (!names[i] || !data[i] || ranks[i] < 1 || ranks[i] > 2) return DFN_ERR_ARG;
+    const size_t len = strnlen(names[i], 256);
+    if (len == 0 || len > 255) return DFN_ERR_UNSUPPORTED;
+    for (int r = 0; r < ranks[i]; r++)
+      if (dims[2 * i + r] < 0) return DFN_ERR_ARG;
+    for (int j = 0; j < i; j++)
+      if (std::strcmp(names[i], names[j]) == 0) return DFN_ERR_ARG;
+    order[i] = i;
+  }
+  // symbol-table entries are kept in name order
+  std::sort(order.begin(), order.end(), [&](int x, int y) { return std::strcmp(names[x], names[y]) < 0; });
+  Writer w;
+  w.alloc(96);  // superblock and root symbol-table entry
+  // root group: symbol-table message -> B-tree + local heap
+  std::vector<uint8_t> rm, body;
+  Writer::le(body, 0, 8);  // B-tree address, patched below
+  Writer::le(body, 0, 8);  // heap address, patched below
+  Writer::message(rm, 0x11, body, 0);
+  const uint64_t root = w.object_header(rm, 1);
+  const uint64_t tree = w.alloc(24 + (2 * Writer::kInternalK + 1) * 8 + 2 * Writer::kInternalK * 8);
+  // local heap: "" at offset 0, then the names (each padded to 8 bytes), then one free block
+  std::vector<uint64_t> name_off(n);
+  uint64_t hsize = 8;
+  for (int i : order) {
+    name_off[i] = hsize;
+    hsize += (std::strlen(names[i]) + 1 + 7) & ~size_t(7);
+  }
+  const uint64_t free_off = hsize;
+  hsize += 16;
+  const uint64_t heap = w.alloc(32);
+  const uint64_t hdata = w.alloc(hsize);
+  std::memcpy(&w.b[heap], "HEAP", 4);
+  w.put(heap + 8, hsize, 8);
+  w.put(heap + 16, free_off, 8);
+  w.put(heap + 24, hdata, 8);
+  for (int i : order) std::memcpy(&w.b[hdata + name_off[i]], names[i], std::strlen(names[i]));
+  w.put(hdata + free_off, 1, 8);    // end of the free list
+  w.put(hdata + free_off + 8, 16, 8);
+  // symbol-table node
+  const uint64_t snod = w.alloc(8 + 2 * Writer::kLeafK * 40);
+  std::memcpy(&w.b[snod], "SNOD", 4);
+  w.b[snod + 4] = 1;
+  w.put(snod + 6, (uint64_t)n, 2);
+  // B-tree: one leaf node, keys = heap offsets ("" and the last name), child = the node
+  std::memcpy(&w.b[tree], "TREE", 4);
+  w.put(tree + 6, 1, 2);
+  w.put(tree + 8, kUndef, 8);
+  w.put(tree + 16, kUndef, 8);
+  w.put(tree + 24, 0, 8);
+  w.put(tree + 32, snod, 8);
+  w.put(tree + 40, name_off[order[n - 1]], 8);
+  w.put(root + 16 + 8, tree, 8);
+  w.put(root + 16 + 16, heap, 8);
+  // datasets
+  std::vector<uint64_t> oh(n), raw(n), nbytes(n);
+  for (int i = 0; i < n; i++) {
+    const int rank = ranks[i];
+    uint64_t cnt = 1;
+    for (int r = 0; r < rank; r++) cnt *= (uint64_t)dims[2 * i + r];
+    nbytes[i] = cnt * 8;
+    std::vector<uint8_t> m, d;
+    // dataspace, version 1, with maximum dimensions equal to the dimensions
+    d = {1, (uint8_t)rank, 1, 0, 0, 0, 0, 0};
+    for (int r = 0; r < rank; r++) Writer::le(d, (uint64_t)dims[2 * i + r], 8);
+    for (int r = 0; r < rank; r++) Writer::le(d, (uint64_t)dims[2 * i + r], 8);
+    Writer::message(m, 0x01, d, 0);
+    // datatype: floating point, version 1, little endian, implied mantissa MSB,
+    // sign bit 63, 8 bytes, precision 64, exponent at 52 (11 bits), mantissa at 0
+    // (52 bits), bias 1023
+    d = {0x11, 0x20, 0x3f, 0x00, 8, 0, 0, 0, 0, 0, 64, 0, 52, 11, 0, 52, 0xff, 0x03, 0, 0};
+    Writer::message(m, 0x03, d, 1);
+    // fill value, version 2: late allocation, written if set, defined with no value
+    d = {2, 2, 2, 1, 0, 0, 0, 0};
+    Writer::message(m, 0x05, d, 1);
+    // contiguous layout, version 3 (address and size patched below)
+    d = {3, 1};
+    Writer::le(d, 0, 8);
+    Writer::le(d, nbytes[i], 8);
+    Writer::message(m, 0x08, d, 0);
+    oh[i] = w.object_header(m, 4);
+  }
+  for (int i = 0; i < n; i++) {
+    raw[i] = w.alloc(nbytes[i]);
+    if (nbytes[i]) std::memcpy(&w.b[raw[i]], data[i], nbytes[i]);
+    // layout message is the fourth: 16 (header) + 3 earlier messages
+    const int rank = ranks[i];
+    const uint64_t lay = oh[i] + 16 + (8 + 8 + 16ull * rank) + (8 + 24) + (8 + 8);
+    w.put(lay + 8 + 2, nbytes[i] ? raw[i] : kUndef, 8);
+  }
+  for (int s = 0; s < n; s++) {
+    const int i = order[s];
+    const uint64_t ent = snod + 8 + 40ull * s;
+    w.put(ent, name_off[i], 8);
+    w.put(ent + 8, oh[i], 8);
+  }
+  // superblock version 0
+  static const uint8_t sig[8] = {0x89, 'H', 'D', 'F', '\r', '\n', 0x1a, '\n'};
+  std::memcpy(&w.b[0], sig, 8);
+  w.b[13] = 8;  // size of offsets
+  w.b[14] = 8;  // size of lengths
+  w.put(16, Writer::kLeafK, 2);
+  w.put(18, Writer::kInternalK, 2);
+  w.put(24, 0, 8);            // base address
+  w.put(32, kUndef, 8);       // free-space info
+  w.put(40, w.b.size(), 8);   // end of file
+  w.put(48, kUndef, 8);       // driver information
+  w.put(56, 0, 8);            // root entry: name offset
+  w.put(64, root, 8);         //   object header
+  w.put(72, 1, 4);            //   cache type 1: B-tree and heap addresses follow
+  w.put(80, tree, 8);
+  w.put(88, heap, 8);
+  FILE *f = std::fopen(path, "wb");
+  if (!f) return DFN_ERR_ARG;
+  const size_t put = std::fwrite(w.b.data(), 1, w.b.size(), f);
+  const int closed = std::fclose(f);
+  return (put == w.b.size() && closed == 0) ? DFN_OK : DFN_ERR_ARG;
+}
 
 struct dfn_h5_s {
   H5 f;
@@ -365,4 +539,9 @@ API int dfn_h5_read_string(dfn_h5 h, const char *path, char *buf, int32_t cap) {
   std::memcpy(buf, s.data(), n);
   buf[n] = 0;
   return DFN_OK;
+}
+
+API int dfn_h5_write_f64(const char *path, int32_t n, const char *const *names, const int32_t *ranks,
+                         const int64_t *dims, const double *const *data) {
+  return h5_write_f64(path, n, names, ranks, dims, data);
 }

@@ -1468,6 +1468,28 @@ void launch_ens_vmax(const double *V, const int *net, const uint8_t *fake, int n
   hipLaunchKernelGGL(ens_vmax_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, V, net, fake, n, vmax);
 }
 
+// One save of the solver output (save_data, src/solver.f90:211-222): column col
+// of six (n x ld) row-major fields in buf: V, log state, P, the last shear stress
+// rate, sigma_N and the permeability of every element.
+__global__ void __launch_bounds__(256) save_output_kernel(const double *Y, const double *P, const double *sT,
+                                                          const double *perm, double *buf, int n, int64_t ld,
+                                                          int64_t col) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int64_t o = (int64_t)e * ld + col, fs = (int64_t)n * ld;
+  buf[o] = Y[e];
+  buf[fs + o] = Y[n + e];
+  buf[2 * fs + o] = P[e];
+  buf[3 * fs + o] = sT[e];
+  buf[4 * fs + o] = Y[2 * n + e];
+  buf[5 * fs + o] = perm[e];
+}
+
+void launch_save_output(const double *Y, const double *P, const double *sT, const double *perm, double *buf, int n,
+                        int64_t ld, int64_t col, hipStream_t s) {
+  hipLaunchKernelGGL(save_output_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, Y, P, sT, perm, buf, n, ld, col);
+}
+
 void launch_monitor(const double *Vold, const double *Vnew, double h, const int *fault_of, int n, double *slip,
                     unsigned long long *fvmax, int nfault, hipStream_t s) {
   hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nfault, 256)), dim3(256), 0, s, fvmax, nfault);

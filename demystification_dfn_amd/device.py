@@ -45,6 +45,15 @@ class DeviceSimulation(Simulation):
     def __init__(self, problem):
         super().__init__(load(), problem, tag="dfn")
 
+    # -- solver output files (dfn_set_output, dfn_flush_output) ----------------
+    def set_output(self, directory, stride_time, freq_writing_file):
+        """Solver output files (dfn_set_output); directory None stops it."""
+        d = os.fsencode(directory) if directory is not None else None
+        _abi.check(self._fns["set_output"](self._h, d, int(stride_time), int(freq_writing_file)), "dfn_set_output")
+
+    def flush_output(self):
+        _abi.check(self._fns["flush_output"](self._h), "dfn_flush_output")
+
     # -- event monitor (dfn_monitor, dfn_monitor_take, dfn_moment) --------------
     def monitor(self, on=True):
         _abi.check(self._fns["monitor"](self._h, 1 if on else 0), "dfn_monitor")

@@ -196,6 +196,28 @@ int dfn_h5_read_f64(dfn_h5 f, const char *path, double *buf, int64_t n);
 int dfn_h5_read_i64(dfn_h5 f, const char *path, int64_t *buf, int64_t n);
 /* the (first) string of a string dataset, NUL-terminated, at most cap bytes */
 int dfn_h5_read_string(dfn_h5 f, const char *path, char *buf, int32_t cap);
+/* Writes a new file at path whose root group holds n FP64 datasets (n <= 8,
+ * distinct names, rank 1 or 2, C order; dims holds two extents per dataset, the
+ * second unused at rank 1), in the subset above and with the header messages
+ * libhdf5 writes for them: the format of the reference's output files
+ * (write_data, src/solver.f90:227-317). */
+int dfn_h5_write_f64(const char *path, int32_t n, const char *const *names, const int32_t *ranks,
+                     const int64_t *dims, const double *const *data);
+
+/* ---- solver output ----------------------------------------------------------
+ * save_data / write_data of src/solver.f90:180-193, 211-317.  With an output
+ * directory set, dfn_solve saves, after each accepted step it_count (the handle's
+ * accepted-step count) with mod(it_count - 1, stride_time) == 0, V, theta =
+ * Dc/V0 exp(psi), P, the last shear stress rate of the kernel (SigmaT_dot),
+ * sigma_N, the permeability and the time, on the device; once freq_writing_file
+ * saves are held it writes dir/outputNNNNNNN (NNNNNNN = it_count, 7 digits) with
+ * datasets V, theta, P, tractionTel, tractionNel, permeability of C-order shape
+ * (N, freq_writing_file) and time (freq_writing_file), as the reference does.
+ * dir = NULL ends the output and drops unwritten saves. */
+int dfn_set_output(dfn_handle h, const char *dir, int64_t stride_time, int64_t freq_writing_file);
+/* The write_data after the loop (src/solver.f90:202): a file with the saves held,
+ * shape (N, saves); nothing when none are held. */
+int dfn_flush_output(dfn_handle h);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Time `reps` applications of the stress-transfer kernel on device-resident input

@@ -1526,6 +1526,16 @@ API int dfno_debug_geometry(dfn_handle hh, double *elem, double *ds, double *tan
   return DFN_OK;
 }
 
+/* SigmaT_dot as the last friction-law evaluation left it (the stress rate the
+ * reference saves as tractionTel, src/solver.f90:218; written by the kernel call
+ * of src/friction/RateStateAgingModeII.f90:31 / ModeIII.f90:34). */
+API int dfno_debug_stress_rate(dfn_handle hh, double *sT) {
+  if (!hh || !sT) return DFN_ERR_ARG;
+  oracle_t *o = &((ohandle *)hh)->o;
+  memcpy(sT, o->sT, 8 * (size_t)o->N);
+  return DFN_OK;
+}
+
 API int dfno_debug_crossings(dfn_handle hh, int64_t *cross, int32_t cap, int32_t *count) {
   if (!hh || !count) return DFN_ERR_ARG;
   oracle_t *o = &((ohandle *)hh)->o;

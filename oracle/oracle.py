@@ -65,6 +65,7 @@ def _bind_debug():
     _lib.dfno_debug_geometry.argtypes = [C.c_void_p] + [_dp] * 6
     _lib.dfno_debug_crossings.argtypes = [C.c_void_p, _i64p, C.c_int32, _i32p]
     _lib.dfno_eval_matrix.argtypes = [C.c_void_p, C.c_int32, _dp]
+    _lib.dfno_debug_stress_rate.argtypes = [C.c_void_p, _dp]
     _lib.dfno_hmat_tree.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _i32p, C.c_int32, _i32p, _i32p,
                                     _i64p, C.c_int32]
 
@@ -76,6 +77,14 @@ def geometry(sim):
                sld=np.zeros(N), nld=np.zeros(N))
     rc = _lib.dfno_debug_geometry(sim._h, *(_p(out[k]) for k in ("elem", "ds", "tang", "nrm", "sld", "nld")))
     assert rc == 0
+    return out
+
+
+def stress_rate(sim):
+    """The shear stress rate of the oracle's last friction-law evaluation."""
+    _bind_debug()
+    out = np.zeros(sim.N)
+    assert _lib.dfno_debug_stress_rate(sim._h, _p(out)) == 0
     return out
 
 
