@@ -110,24 +110,29 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   std::vector<T> vdata;
   int zp = 0;
   int64_t S_total = 0;
+  // each block's V column-major with its leading dimension padded (zero rows) to a
+  // multiple of the 16-byte vector, so a lane loads whole vectors of one column
+  constexpr int kVec = 16 / (int)sizeof(T);
   for (int o = 0; o < nops; o++) {
     const HostOperator &H = *ops[o];
-    const int64_t vbase = (int64_t)vdata.size();
-    vdata.resize(vdata.size() + H.l_v.size());
     for (size_t b = 0; b < H.l_k.size(); b++) {
       const int k = H.l_k[b], nb = H.l_n[b];
       if (k > kMaxRank) return DFN_ERR_UNSUPPORTED;
+      const int ldp = (nb + kVec - 1) / kVec * kVec;
+      const int64_t voff = (int64_t)vdata.size();
+      vdata.resize(vdata.size() + (size_t)ldp * k, (T)0);
       const double *vb = H.l_v.data() + H.l_voff[b];
       const size_t cnt = (size_t)nb * k;
       const double vs = scaled ? pow2_scale(vb, cnt) : 1.0;
-      for (size_t i = 0; i < cnt; i++) vdata[vbase + H.l_voff[b] + i] = (T)(vb[i] / vs);
+      for (int j = 0; j < k; j++)
+        for (int i = 0; i < nb; i++) vdata[voff + (int64_t)j * ldp + i] = (T)(vb[(size_t)j * nb + i] / vs);
       const int wz = (int)(zbase[o] + zoff[o][b]);
       const bool single = nb <= kVChunkRows;  // one chunk: z goes straight into w
       ZRed r{zp, 0, k, wz};
       for (int r0 = 0; r0 < nb; r0 += kVChunkRows) {
         VTask t{};
-        t.voff = vbase + H.l_voff[b];
-        t.ld = nb;
+        t.voff = voff;
+        t.ld = ldp;
         t.row0 = r0;
         t.len = std::min(kVChunkRows, nb - r0);
         t.k = k;

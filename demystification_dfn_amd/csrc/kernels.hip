@@ -97,11 +97,14 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 
 // Phase 1: one wave per task.  The body is specialised on the rank K (a
 // wave-uniform switch), so the K column loads of a row strip are all in flight
-// together; lanes stride the rows, K partial sums stay in registers and are
-// reduced across the wave with DPP lane moves.
+// together; each lane takes a 16-byte vector of consecutive rows (2 FP64 / 4 FP32;
+// the leading dimension is padded with zero rows to a whole vector), K partial
+// sums stay in registers and are reduced across the wave with DPP lane moves.
 template <int K, typename TV>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
                                          double *__restrict__ zpart, double *__restrict__ wz, int lane) {
+  constexpr int kVec = 16 / (int)sizeof(TV);
+  typedef TV tvec __attribute__((ext_vector_type(kVec)));
   // FP64 storage accumulates in FP64; FP32 storage in FP32 (TV)
   TV acc[K];
 #pragma unroll
@@ -109,13 +112,17 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
   const TV *V = vdata + task.voff + task.row0;
   const double *x = w + task.xcol + task.row0;
   const int64_t ld = task.ld;
-  for (int i = lane; i < task.len; i += 64) {
-    const TV xi = (TV)x[i];
-    TV v[K];
+  for (int i = lane * kVec; i < task.len; i += 64 * kVec) {
+    TV xi[kVec];
 #pragma unroll
-    for (int j = 0; j < K; j++) v[j] = V[i + j * ld];
+    for (int u = 0; u < kVec; u++) xi[u] = i + u < task.len ? (TV)x[i + u] : (TV)0;  // padding rows hold zeros
+    tvec v[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) acc[j] += v[j] * xi;
+    for (int j = 0; j < K; j++) v[j] = *reinterpret_cast<const tvec *>(V + i + j * ld);
+#pragma unroll
+    for (int j = 0; j < K; j++)
+#pragma unroll
+      for (int u = 0; u < kVec; u++) acc[j] += v[j][u] * xi[u];
   }
   // all 64 lanes are active here: DPP wave totals, lane j keeps column j's
   TV mine = (TV)0;
