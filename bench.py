@@ -198,10 +198,12 @@ def run_ensemble(args):
     ensemble with graphs off, are timed beside it."""
     from demystification_dfn_amd import _abi, problems
     from demystification_dfn_amd.device import DeviceEnsemble, DeviceSimulation
-    probs = problems.network_ensemble(args.networks, faults=(5, 36), box=2500.0, seed=args.seed)
+    lo, hi = (int(v) for v in args.ensemble_faults.split(","))
+    probs = problems.network_ensemble(args.networks, faults=(lo, hi), box=args.ensemble_box, seed=args.seed)
     Ns = [p.N for p in probs]
-    desc = dict(workload=f"ensemble{len(probs)}_{sum(Ns) // 1000}k", networks=len(probs), faults_per_network="5-36",
-                box_m=2500, length_law="p(L)~L^-0.1 on [50 m, 2 km]", element_m=10, fracture_mode="II",
+    desc = dict(workload=f"ensemble{len(probs)}_{sum(Ns) // 1000}k", networks=len(probs),
+                faults_per_network=f"{lo}-{hi}", box_m=args.ensemble_box,
+                length_law="p(L)~L^-0.1 on [50 m, 2 km]", element_m=10, fracture_mode="II",
                 kernel="hmatrix", eta=2.0, eps_aca=1e-6, leaf=64, elements=sum(Ns), min_elements=min(Ns),
                 max_elements=max(Ns), seed=args.seed,
                 friction="per network: a~U[0.008,0.012] per fault, b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa")
@@ -280,9 +282,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed-steps", type=int, default=300,
                     help="steps of the FP32-operator (mixed precision) run on the same workload; 0 skips it")
-    ap.add_argument("--mixed-vectors", type=int, default=200,
+    ap.add_argument("--mixed-vectors", type=int, default=1000,
                     help="random log-uniform slip-rate vectors for the FP32-vs-FP64 stress-rate error")
     ap.add_argument("--networks", type=int, default=64, help="networks of the ensemble workload")
+    ap.add_argument("--ensemble-faults", default="5,36",
+                    help="fault-count range per network of the ensemble workload (50,400 with --ensemble-box 7000 "
+                         "is the 1.5M-element ensemble)")
+    ap.add_argument("--ensemble-box", type=float, default=2500.0, help="box size (m) of the ensemble's networks")
     ap.add_argument("--ensemble-single-sample", type=int, default=16,
                     help="ensemble networks also run one handle at a time, for the batching comparison")
     ap.add_argument("--no-graph", action="store_true",
@@ -363,6 +369,10 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] cpu baseline on {threads} threads")
         out["cpu_baseline"] = cpu_baseline(sim, p, args.cpu_seconds, threads)
+        if out["cpu_baseline"]["kind"] == "port" and threads > 1:
+            # the reference as built runs on one thread
+            log("[bench] cpu baseline on 1 thread")
+            out["cpu_baseline_1thread"] = cpu_baseline(sim, p, args.cpu_seconds / 2, 1)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

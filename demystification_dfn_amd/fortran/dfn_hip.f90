@@ -41,7 +41,7 @@ type, bind(C), public :: dfn_run_stats
 end type
 
 public :: dfn_create, dfn_destroy, dfn_build_hmat, dfn_set_state, dfn_get_state, dfn_kernel, dfn_friction_law
-public :: dfn_compute_P, dfn_rkqs, dfn_solve, dfn_get_stats
+public :: dfn_compute_P, dfn_rkqs, dfn_solve, dfn_get_stats, dfn_set_output, dfn_flush_output
 public :: dfn_hip_setup, dfn_hip_close, hip_kernel, ode_RateStateAgingHIP, compute_P_hip, rkqs_hip, dfn_hip_handle
 
 interface
@@ -108,6 +108,18 @@ interface
     integer(c_int64_t), value :: nsteps
     real(c_double), value :: t_final
     type(dfn_run_stats), intent(out) :: s
+  end function
+  ! save_data / write_data of src/solver.f90:180-202, 211-317 inside dfn_solve;
+  ! dir is a NUL-terminated directory name (c_null_char)
+  integer(c_int) function dfn_set_output(h, dir, stride_time, freq_writing_file) bind(C, name='dfn_set_output')
+    import :: c_ptr, c_int, c_char, c_int64_t
+    type(c_ptr), value :: h
+    character(kind=c_char), intent(in) :: dir(*)
+    integer(c_int64_t), value :: stride_time, freq_writing_file
+  end function
+  integer(c_int) function dfn_flush_output(h) bind(C, name='dfn_flush_output')
+    import :: c_ptr, c_int
+    type(c_ptr), value :: h
   end function
 end interface
 

@@ -3,11 +3,16 @@
 ! t_try with step halving, the friction law at t_try for the error scale, rkqs,
 ! the 1-day cap and the pressure commit.  Reads a problem written by
 ! demystification_dfn_amd.fortran_io.write_problem, writes time, Y and P.
+! With a third argument (a directory) and fourth/fifth (stride_time,
+! freq_writing_file) it runs the loop in one dfn_solve call instead, with the
+! library writing the output files there (dfn_set_output / dfn_flush_output).
 program drive_solver
 use iso_c_binding
 use dfn_hip
 implicit none
-character(len=512) :: fin, fout
+character(len=512) :: fin, fout, odir, arg
+integer(c_int64_t) :: stride_time, freq_writing_file
+type(dfn_run_stats) :: rs
 integer(c_int64_t) :: hdr(6)
 integer(c_int64_t), allocatable :: fnel(:), sfault(:), sidx(:)
 real(c_double), allocatable :: node(:,:), arr(:,:), sQ(:), stb(:), ste(:)
@@ -43,6 +48,21 @@ if (ier /= 0) then
 end if
 allocate(Y(3*N), Y_dot(3*N), yscal(3*N), P(N), P_dot(N), P_temp(N))
 st = dfn_get_state(dfn_hip_handle, Y, P, P_dot, time)
+if (command_argument_count() >= 5) then
+  call get_command_argument(3, odir)
+  call get_command_argument(4, arg)
+  read(arg, *) stride_time
+  call get_command_argument(5, arg)
+  read(arg, *) freq_writing_file
+  st = dfn_set_output(dfn_hip_handle, trim(odir)//c_null_char, stride_time, freq_writing_file)
+  if (st /= 0) stop 2
+  st = dfn_solve(dfn_hip_handle, nsteps, 0._c_double, rs)
+  if (st /= 0) stop 3
+  st = dfn_flush_output(dfn_hip_handle)
+  if (st /= 0) stop 4
+  st = dfn_get_state(dfn_hip_handle, Y, P, P_dot, time)
+  nsteps = 0
+end if
 t_try = 100._c_double
 do it = 1, int(nsteps)
   if (t_try > 10._c_double) then
