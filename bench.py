@@ -250,7 +250,7 @@ def run_ensemble(args):
         wall_s += time.perf_counter() - w0
         dt_s += d.stats()["time"] - s0
         d.close()
-    single = (dt_s / YEAR) / (wall_s / 3600.0)
+    single = (dt_s / YEAR) / (wall_s / 3600.0) if wall_s > 0 else None
     out = dict(
         metric="simulated network-yr / wall hr, " + str(len(probs)) + "-network ensemble", value=value,
         unit="simulated network-yr / wall hr", n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -263,7 +263,7 @@ def run_ensemble(args):
                       fp32_operators=dict(value=(dt32 / YEAR) / (wall32 / 3600.0), ms_per_step=1e3 * wall32 / args.steps),
                       one_handle_at_a_time=dict(value=single, networks_sampled=nsamp,
                                                 note="same networks, one dfn_handle each, run in turn"),
-                      speedup_vs_one_at_a_time=value / single),
+                      speedup_vs_one_at_a_time=value / single if single else None),
         roofline=None, cpu_baseline=None)
     if args.learned_steps > 0:
         log("[bench] learned step-size control")

@@ -142,7 +142,9 @@ template <typename TV>
 __global__ void __launch_bounds__(256) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
                                                           const TV *__restrict__ vdata, double *__restrict__ w,
                                                           double *__restrict__ zpart) {
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the task index is uniform across the wave: scalar loads of the task and a
+  // scalar branch on its rank
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (t >= ntasks) return;
   const VTask task = tasks[t];
