@@ -107,6 +107,7 @@ SIGNATURES = {
     "compute_P": [_H, C.c_double, _dp, _dp, _i32p],
     "rkqs": [_H, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
     "solve": [_H, C.c_int64, C.c_double, C.POINTER(DfnRunStats)],
+    "set_formulation": [_H, C.c_int32],
     "monitor": [_H, C.c_int32],
     "monitor_take": [_H, C.c_int64, _dp, _dp, _i64p],
     "moment": [_H, _dp, _dp],
@@ -185,6 +186,9 @@ def fptr(a):
     assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_fp)
 
+
+FORM_REFERENCE = 0
+FORM_NEWTON_V = 1
 
 MLP_INPUTS = 8
 MLP_PARAMS = 4801

@@ -56,9 +56,23 @@ struct StatusWord {
   int pad;
 };
 
+// Newton-V form (an extension with no reference counterpart): the state holds
+// slip instead of slip rate, and each evaluation solves the stress balance
+// tau = eta V - (sigma + P) a asinh(V / (2 V0) exp((f0 + b psi) / a)) for V,
+// with tau = tau_ref + shear_loading_dot t + K slip and
+// sigma = sig_ref + normal_loading_dot t (+ K_N slip in Mode II), t measured
+// from the switch to this form.  on = 0: the reference form.
+struct NvArgs {
+  const double *tau_ref, *sig_ref;
+  double *Vg;  // last solved slip rate per element (the next solve's first guess)
+  double t;    // time of this evaluation since the switch
+  int on;
+};
+
 struct StageArgs {
   int n;
   RhsParams p;
+  NvArgs nv;
   const double *y;      // step-start state (3N)
   const double *ytemp;  // this stage's input state (3N)
   const double *sT, *sN, *Pt, *Pd;
@@ -146,6 +160,14 @@ void launch_direct3(const double *node, const double *slip, double *part, int n,
 void launch_direct2(const double *node, const double *elem, const double *nrm, const double *slip, double *part, int n,
                     double scale, double *sT, double *sN, hipStream_t s);
 void launch_stage(int stage, const StageArgs &a, hipStream_t s);
+// Newton-V form: from the reference-form state Y = [V | psi | sigma_N] and P, the
+// balance's reference stress and normal stress, first guesses, and Y's slip and
+// third blocks set to zero (nv_enter); the reference-form state of a Newton-V
+// state whose stress transfer K slip is in sT / sN (nv_state, t since the switch)
+void launch_nv_enter(const RhsParams &p, double *Y, const double *P, int n, double *tau_ref, double *sig_ref,
+                     double *Vg, hipStream_t s);
+void launch_nv_state(const RhsParams &p, const NvArgs &nv, const double *Y, const double *P, const double *sT,
+                     const double *sN, int n, double *out, hipStream_t s);
 
 // Ensemble of networks concatenated element-wise (network of element k: net[k]):
 // the stage kernels with per-network step sizes, masks and error norms.

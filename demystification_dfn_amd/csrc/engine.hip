@@ -319,13 +319,72 @@ struct Engine {
     return DFN_OK;
   }
 
+  RhsParams rhs_params() {
+    return RhsParams{a.as<double>(), b.as<double>(), Dc.as<double>(), f0.as<double>(), V0.as<double>(),
+                     nld.as<double>(), sld.as<double>(), fake.as<uint8_t>(), hp.mu / (2.0 * hp.cs),
+                     hp.mode == DFN_MODE_III ? 1 : 0};
+  }
+
+  // Newton-V form (an extension; DFN_FORM_NEWTON_V): Y holds [slip since the switch |
+  // psi | 0]; the balance's reference stresses, the last solved slip rates and the
+  // switch time define the stress at every evaluation
+  int form = DFN_FORM_REFERENCE;
+  DMem nv_tau, nv_sig, nv_Vg;
+  double nv_t0 = 0.0;
+
+  NvArgs nv_args(double x) {
+    return NvArgs{nv_tau.as<double>(), nv_sig.as<double>(), nv_Vg.as<double>(), (time - nv_t0) + x, 1};
+  }
+
+  int nv_enter() {
+    const int N = hp.N;
+    if (!nv_tau.p) {
+      DFN_TRY(nv_tau.alloc(sizeof(double) * N));
+      DFN_TRY(nv_sig.alloc(sizeof(double) * N));
+      DFN_TRY(nv_Vg.alloc(sizeof(double) * N));
+    }
+    launch_nv_enter(rhs_params(), Y.as<double>(), P.as<double>(), N, nv_tau.as<double>(), nv_sig.as<double>(),
+                    nv_Vg.as<double>(), stream);
+    nv_t0 = time;
+    DFN_HIP(hipStreamSynchronize(stream));
+    return DFN_OK;
+  }
+
+  // the reference-form state [V | psi | sigma_N] of the Newton-V state into out (3N)
+  int nv_state(double *out) {
+    DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
+    launch_nv_state(rhs_params(), nv_args(0.0), Y.as<double>(), P.as<double>(), sT.as<double>(), sN.as<double>(),
+                    hp.N, out, stream);
+    DFN_HIP(hipStreamSynchronize(stream));
+    return DFN_OK;
+  }
+
+  int set_formulation(int f) {
+    if (f != DFN_FORM_REFERENCE && f != DFN_FORM_NEWTON_V) return DFN_ERR_ARG;
+    if (f == form) return DFN_OK;
+    if (monitor || !out_dir.empty()) return DFN_ERR_STATE;
+    if (hp.kernel == DFN_KERNEL_HMATRIX && !active_ops()) return DFN_ERR_STATE;
+    DFN_TRY(sync_pdot());
+    if (f == DFN_FORM_NEWTON_V) {
+      DFN_TRY(nv_enter());
+    } else {
+      DFN_TRY(nv_state(scratch3.as<double>()));
+      DFN_HIP(hipMemcpy(Y.p, scratch3.p, sizeof(double) * 3 * (size_t)hp.N, hipMemcpyDeviceToDevice));
+    }
+    form = f;
+    return DFN_OK;
+  }
+
   StageArgs stage_args(const double *ytemp, int slot, double *ynext, double h) {
     const int N = hp.N;
     StageArgs s{};
     s.n = N;
-    s.p = RhsParams{a.as<double>(), b.as<double>(), Dc.as<double>(), f0.as<double>(), V0.as<double>(),
-                    nld.as<double>(), sld.as<double>(), fake.as<uint8_t>(), hp.mu / (2.0 * hp.cs),
-                    hp.mode == DFN_MODE_III ? 1 : 0};
+    s.p = rhs_params();
+    if (form == DFN_FORM_NEWTON_V) {
+      // the abscissa of this evaluation: k1 at x = t_try (h), stage s at A_s h
+      static const double Ai[5] = {0.2, 0.3, 0.6, 1.0, 0.875};
+      s.nv = nv_args(slot == 0 ? h : Ai[slot - 1] * h);
+    }
     s.y = Y.as<double>();
     s.ytemp = ytemp;
     s.sT = sT.as<double>();
@@ -811,10 +870,22 @@ API int dfn_set_state(dfn_handle h, const double *Yh, const double *Ph, double t
   if (!h) return DFN_ERR_ARG;
   Engine &e = h->e;
   const size_t N = e.hp.N;
+  if (e.form == DFN_FORM_NEWTON_V && Yh) {  // a reference-form state: leave and re-enter the Newton-V form
+    DFN_TRY(e.set_formulation(DFN_FORM_REFERENCE));
+    DFN_HIP(hipMemcpy(e.Y.p, Yh, 3 * N * sizeof(double), hipMemcpyHostToDevice));
+    if (Ph) DFN_HIP(hipMemcpy(e.P.p, Ph, N * sizeof(double), hipMemcpyHostToDevice));
+    e.time = time;
+    return e.set_formulation(DFN_FORM_NEWTON_V);
+  }
   if (Yh) DFN_HIP(hipMemcpy(e.Y.p, Yh, 3 * N * sizeof(double), hipMemcpyHostToDevice));
   if (Ph) DFN_HIP(hipMemcpy(e.P.p, Ph, N * sizeof(double), hipMemcpyHostToDevice));
   e.time = time;
   return DFN_OK;
+}
+
+API int dfn_set_formulation(dfn_handle h, int32_t form) {
+  if (!h) return DFN_ERR_ARG;
+  return h->e.set_formulation(form);
 }
 
 API int dfn_get_state(dfn_handle h, double *Yh, double *Ph, double *Pdh, double *time) {
@@ -823,7 +894,12 @@ API int dfn_get_state(dfn_handle h, double *Yh, double *Ph, double *Pdh, double 
   const size_t N = e.hp.N;
   DFN_TRY(e.sync_pdot());
   DFN_HIP(hipStreamSynchronize(e.stream));
-  if (Yh) DFN_HIP(hipMemcpy(Yh, e.Y.p, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+  if (Yh && e.form == DFN_FORM_NEWTON_V) {
+    DFN_TRY(e.nv_state(e.scratch3.as<double>()));
+    DFN_HIP(hipMemcpy(Yh, e.scratch3.p, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+  } else if (Yh) {
+    DFN_HIP(hipMemcpy(Yh, e.Y.p, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+  }
   if (Ph) DFN_HIP(hipMemcpy(Ph, e.P.p, N * sizeof(double), hipMemcpyDeviceToHost));
   if (Pdh) DFN_HIP(hipMemcpy(Pdh, e.Pdot.p, N * sizeof(double), hipMemcpyDeviceToHost));
   if (time) *time = e.time;
@@ -838,7 +914,12 @@ API int dfn_get_stats(dfn_handle h, dfn_run_stats *s) {
   s->t_try = e.t_try;
   std::vector<double> V(e.hp.N);
   DFN_HIP(hipStreamSynchronize(e.stream));
-  DFN_HIP(hipMemcpy(V.data(), e.Y.p, V.size() * sizeof(double), hipMemcpyDeviceToHost));
+  const void *Vd = e.Y.p;
+  if (e.form == DFN_FORM_NEWTON_V) {
+    DFN_TRY(e.nv_state(e.scratch3.as<double>()));
+    Vd = e.scratch3.p;
+  }
+  DFN_HIP(hipMemcpy(V.data(), Vd, V.size() * sizeof(double), hipMemcpyDeviceToHost));
   s->max_V = *std::max_element(V.begin(), V.end());
   return DFN_OK;
 }
@@ -892,6 +973,7 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
 API int dfn_friction_law(dfn_handle h, double x, const double *Yh, double *Ydh) {
   if (!h || !Yh || !Ydh) return DFN_ERR_ARG;
   Engine &e = h->e;
+  if (e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
   const size_t n3 = 3 * (size_t)e.hp.N;
   DFN_HIP(hipMemcpyAsync(e.scratch3.p, Yh, n3 * sizeof(double), hipMemcpyHostToDevice, e.stream));
   DFN_TRY(e.friction_law_dev(x, e.scratch3.as<double>(), e.scratch3b.as<double>()));
@@ -920,6 +1002,7 @@ API int dfn_rkqs(dfn_handle h, double *y, const double *dydx, double *x, double 
                  const double *yscal, double *hdid, double *hnext) {
   if (!h || !y || !dydx || !x || !yscal || !hdid || !hnext) return DFN_ERR_ARG;
   Engine &e = h->e;
+  if (e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
   const size_t n3 = 3 * (size_t)e.hp.N;
   // run on a copy of the handle's state slot: Y <- y
   DFN_HIP(hipMemcpyAsync(e.Y.p, y, n3 * sizeof(double), hipMemcpyHostToDevice, e.stream));
@@ -938,7 +1021,10 @@ API int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *s
   if (e.hp.kernel == DFN_KERNEL_HMATRIX && !e.active_ops()) return DFN_ERR_STATE;
   int st = DFN_OK;
   for (int64_t i = 0; i < nsteps; i++) {
-    if (t_final > 0 && e.time >= t_final) break;
+    if (t_final > 0) {  // land on t_final: the trial step never passes it
+      if (e.time >= t_final) break;
+      if (e.time + e.t_try > t_final) e.t_try = t_final - e.time;
+    }
     st = e.step();
     if (st != DFN_OK) break;
   }
@@ -951,6 +1037,7 @@ API int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *s
 
 API int dfn_set_output(dfn_handle h, const char *dir, int64_t stride_time, int64_t freq_writing_file) {
   if (!h) return DFN_ERR_ARG;
+  if (dir && h->e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
   return h->e.output_set(dir, stride_time, freq_writing_file);
 }
 
@@ -962,6 +1049,7 @@ API int dfn_flush_output(dfn_handle h) {
 API int dfn_monitor(dfn_handle h, int32_t on) {
   if (!h) return DFN_ERR_ARG;
   if (!on) { h->e.monitor = false; return DFN_OK; }
+  if (h->e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
   return h->e.monitor_start();
 }
 

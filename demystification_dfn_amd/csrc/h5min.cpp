@@ -320,7 +320,7 @@ struct Writer {
     m[at + 2] = (uint8_t)sz;
     m[at + 3] = (uint8_t)(sz >> 8);
     m[at + 4] = (uint8_t)flags;
-    std::memcpy(&m[at + 8], body.data(), body.size());
+    if (!body.empty()) std::copy(body.begin(), body.end(), m.begin() + (long)(at + 8));
   }
   static void le(std::vector<uint8_t> &v, uint64_t x, int n) {
     for (int i = 0; i < n; i++) v.push_back((uint8_t)(x >> (8 * i)));

@@ -445,6 +445,94 @@ __device__ __forceinline__ void rate_state(const RhsParams &p, int k, double V, 
   if (p.fake[k]) { dV = 0.0; dpsi = 0.0; dsig = 0.0; }
 }
 
+// Slip rate from the quasi-dynamic stress balance whose time derivative the
+// reference integrates (src/friction/RateStateAgingModeII.f90:61-66):
+// g(V) = eta V - s a asinh(V C / (2 V0)) - tau = 0, s = sigma + P < 0 (compressive),
+// C = exp((f0 + b psi) / a).  g is odd in (V, tau) and increasing in V, so the root
+// has the sign of tau; |V| is found by Newton steps in ln V (the friction term is
+// logarithmic in V over most of its range), kept inside the bracket [lo, hi] with
+// g(lo) <= 0 <= g(hi), starting from the last solution.  Without compressive
+// normal stress the balance has no friction term: V = tau / eta.
+__device__ double nv_solve_V(double tau, double s, double a, double C, double V0, double eta, double Vg) {
+  if (tau == 0.0) return 0.0;
+  const double t = fabs(tau);
+  if (!(s < 0.0)) return tau / eta;
+  double lo = 0.0, hi = t / eta;
+  double V = fabs(Vg);
+  if (!(V > lo && V < hi)) V = 0.5 * hi;
+  for (int it = 0; it < 200; it++) {
+    const double B = V / (2.0 * V0);
+    const double g = eta * V - s * a * asinh(B * C) - t;
+    if (g == 0.0) break;
+    if (g > 0.0) hi = V;
+    else lo = V;
+    const double D = 1.0 / sqrt(1.0 / (C * C) + B * B);  // (C / 2V0) / sqrt(1 + (B C)^2) = D / 2V0
+    const double gp = eta - s * a * D / (2.0 * V0);
+    double Vn = V * exp(-g / (gp * V));
+    if (!(Vn > lo && Vn < hi)) Vn = lo > 0.0 ? sqrt(lo * hi) : 0.5 * (lo + hi);
+    const bool done = fabs(Vn - V) <= 1e-14 * Vn;
+    V = Vn;
+    if (done) break;
+  }
+  return tau < 0.0 ? -V : V;
+}
+
+// Newton-V right-hand side at element k: [dslip, dpsi, 0] and the normal stress
+// of this evaluation (for the tensile check).  The aging law is the reference's
+// (RateStateAgingModeII.f90:69-70).
+__device__ __forceinline__ void rate_state_nv(const RhsParams &p, const NvArgs &nv, int k, double psi, double sT,
+                                              double sN, double Pt, double &dd, double &dpsi, double &dz,
+                                              double &sig) {
+  const double a = p.a[k], b = p.b[k], Dc = p.Dc[k], V0 = p.V0[k];
+  sig = nv.sig_ref[k] + p.nld[k] * nv.t + (p.mode3 ? 0.0 : sN);
+  const double tau = nv.tau_ref[k] + p.sld[k] * nv.t + sT;
+  const double C = exp((p.f0[k] + b * psi) / a);
+  const double V = nv_solve_V(tau, sig + Pt, a, C, V0, p.eta_rad, nv.Vg[k]);
+  dd = V;
+  dpsi = V0 / Dc * exp(-psi) - fabs(V) / Dc;
+  dz = 0.0;
+  if (p.fake[k]) { dd = 0.0; dpsi = 0.0; sig = -1.0; }
+  else nv.Vg[k] = V;
+}
+
+__global__ void __launch_bounds__(256) nv_enter_kernel(RhsParams p, double *Y, const double *P, int n,
+                                                       double *tau_ref, double *sig_ref, double *Vg) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double V = Y[k], psi = Y[n + k], sig = Y[2 * n + k];
+  const double a = p.a[k], V0 = p.V0[k];
+  const double C = exp((p.f0[k] + p.b[k] * psi) / a);
+  tau_ref[k] = p.eta_rad * V - (sig + P[k]) * a * asinh(V / (2.0 * V0) * C);
+  sig_ref[k] = sig;
+  Vg[k] = V;
+  Y[k] = 0.0;
+  Y[2 * n + k] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) nv_state_kernel(RhsParams p, NvArgs nv, const double *Y, const double *P,
+                                                       const double *sT, const double *sN, int n, double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double psi = Y[n + k];
+  const double a = p.a[k], V0 = p.V0[k];
+  const double sig = nv.sig_ref[k] + p.nld[k] * nv.t + (p.mode3 ? 0.0 : sN[k]);
+  const double tau = nv.tau_ref[k] + p.sld[k] * nv.t + sT[k];
+  const double C = exp((p.f0[k] + p.b[k] * psi) / a);
+  out[k] = p.fake[k] ? nv.Vg[k] : nv_solve_V(tau, sig + P[k], a, C, V0, p.eta_rad, nv.Vg[k]);
+  out[n + k] = psi;
+  out[2 * n + k] = sig;
+}
+
+void launch_nv_enter(const RhsParams &p, double *Y, const double *P, int n, double *tau_ref, double *sig_ref,
+                     double *Vg, hipStream_t s) {
+  hipLaunchKernelGGL(nv_enter_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, Y, P, n, tau_ref, sig_ref, Vg);
+}
+
+void launch_nv_state(const RhsParams &p, const NvArgs &nv, const double *Y, const double *P, const double *sT,
+                     const double *sN, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(nv_state_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, nv, Y, P, sT, sN, n, out);
+}
+
 // Stage s of rkck at element k: k_s = f(ytemp_s); then the next stage input (or,
 // at s = 6, the 5th-order solution, and the scaled embedded error into emax).  At
 // s = 1 it evaluates k1 at x = t_try and forms yscal (src/solver.f90:101-102).
@@ -456,8 +544,12 @@ __device__ __forceinline__ void stage_body(const StageArgs &A, int k, double h, 
   double d[3];
   RhsParams p = A.p;
   p.eta_rad = eta;
-  rate_state(p, k, yt[k], yt[n + k], yt[2 * n + k], A.sT[k], A.sN ? A.sN[k] : 0.0, A.Pt[k], A.Pd[k], d[0], d[1],
-             d[2]);
+  double sig_nv = -1.0;
+  if (A.nv.on)
+    rate_state_nv(p, A.nv, k, yt[n + k], A.sT[k], A.sN ? A.sN[k] : 0.0, A.Pt[k], d[0], d[1], d[2], sig_nv);
+  else
+    rate_state(p, k, yt[k], yt[n + k], yt[2 * n + k], A.sT[k], A.sN ? A.sN[k] : 0.0, A.Pt[k], A.Pd[k], d[0], d[1],
+               d[2]);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const int i = c * n + k;
@@ -489,7 +581,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &A, int k, double h, 
       double e = fabs(ye / A.yscal[i]);
       if (!(e == e)) e = INFINITY;
       emax = fmax(emax, e);
-      if (c == 2 && yo > 0) tensile = 1;
+      if (c == 2 && (yo > 0 || sig_nv > 0)) tensile = 1;  // Newton-V: the last stage's normal stress
     }
   }
 }

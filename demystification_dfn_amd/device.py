@@ -45,6 +45,10 @@ class DeviceSimulation(Simulation):
     def __init__(self, problem):
         super().__init__(load(), problem, tag="dfn")
 
+    def set_formulation(self, form):
+        """dfn_set_formulation: _abi.FORM_REFERENCE or _abi.FORM_NEWTON_V."""
+        _abi.check(self._fns["set_formulation"](self._h, int(form)), "dfn_set_formulation")
+
     # -- solver output files (dfn_set_output, dfn_flush_output) ----------------
     def set_output(self, directory, stride_time, freq_writing_file):
         """Solver output files (dfn_set_output); directory None stops it."""

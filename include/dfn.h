@@ -158,9 +158,28 @@ int dfn_rkqs(dfn_handle h, double *y, const double *dydx, double *x, double htry
              double eps, const double *yscal, double *hdid, double *hnext);
 
 /* The solver loop of src/solver.f90:59-198 on the handle's device-resident state:
- * runs until nsteps more accepted steps are done, or time >= t_final (t_final <= 0:
- * no time limit).  The loop state (time, t_try, counters) persists between calls. */
+ * runs until nsteps more accepted steps are done, or until time reaches t_final
+ * (t_final <= 0: no time limit; the reference stops on max_it only).  With a time
+ * limit, a trial step that would pass t_final is shortened to end on it, so the
+ * run lands on t_final exactly.  The loop state (time, t_try, counters) persists
+ * between calls. */
 int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats);
+
+/* ---- Newton-V form ------------------------------------------------------------
+ * No counterpart in the reference, which integrates the slip rate V through its
+ * time derivative (src/friction/RateStateAgingModeII.f90:61-66).  The Newton-V
+ * form integrates slip and log state instead and solves, at every evaluation and
+ * element, the stress balance whose derivative that is,
+ *   tau_ref + shear_loading_dot t + K slip = eta V - (sigma_N + P) a asinh(V C / (2 V0)),
+ * for V (C = exp((f0 + b psi) / a); sigma_N = sigma_ref + normal_loading_dot t, plus
+ * K_N slip in Mode II; t and slip counted from the switch, whose state fixes
+ * tau_ref and sigma_ref).  Same step controller and pressure solves; the k1
+ * evaluation is at x = t_try as in the reference.  dfn_get_state returns the state
+ * in the reference layout [V | psi | sigma_N]; dfn_set_state takes one.  Not with
+ * the event monitor, solver output, dfn_friction_law or dfn_rkqs (DFN_ERR_STATE). */
+#define DFN_FORM_REFERENCE 0
+#define DFN_FORM_NEWTON_V 1
+int dfn_set_formulation(dfn_handle h, int32_t form);
 
 /* ---- event catalog and moment ------------------------------------------------
  * The inputs of the reference's post-processing, gathered on the device as the

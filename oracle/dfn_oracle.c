@@ -1478,7 +1478,10 @@ API int dfno_solve(dfn_handle hh, int64_t nsteps, double t_final, dfn_run_stats 
   double *Yd = xcalloc((size_t)n, 8), *yscal = xcalloc((size_t)n, 8), *Pt = xcalloc((size_t)N, 8);
   int st = DFN_OK;
   for (int64_t it = 0; it < nsteps; it++) {
-    if (t_final > 0 && o->time >= t_final) break;
+    if (t_final > 0) { /* not in the reference (it stops on max_it): land on t_final */
+      if (o->time >= t_final) break;
+      if (o->time + o->t_try > t_final) o->t_try = t_final - o->time;
+    }
     if (o->t_try > 10.0) { /* :64-77 */
       for (int s = 0; s < o->nsrc; s++)
         if ((o->time < o->tb[s]) && (o->time + o->t_try > o->tb[s])) o->t_try = 10.0;
