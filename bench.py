@@ -126,6 +126,7 @@ def mixed_precision(sim, p, args, kb64):
             big = np.abs(y64) > 1e-3 * np.abs(y64).max()
             sig.append(np.max(np.abs(y32[big] - y64[big]) / np.abs(y64[big])))
     sim.set_precision(_abi.PREC_FP32)
+    tune32 = sim.tuning()
     kb = sim.bench_kernel(reps=20)
     s0 = sim.stats()
     w0 = time.perf_counter()
@@ -141,7 +142,7 @@ def mixed_precision(sim, p, args, kb64):
                 panel_GB_s=kb["dominant_bytes"] / (kb["ms_dominant"] * 1e-3) / 1e9,
                 stress_rate_rel_err=dict(vectors=args.mixed_vectors, norm_max=float(np.max(errs)),
                                          norm_median=float(np.median(errs)), significant_rows_max=float(np.max(sig))),
-                note="continues from the FP64 run's final state; operators FP32, state FP64")
+                autotune=tune32, note="continues from the FP64 run's final state; operators FP32, state FP64")
 
 
 def learned_step(args, probs):
@@ -320,7 +321,7 @@ def main():
         hm = dict(S_entries=int(sum(s["S_dense"] + s["S_lowrank_U"] + s["S_lowrank_V"] for s in st)),
                   max_rank=int(max(s["max_rank"] for s in st)),
                   lowrank_blocks=int(sum(s["nb_lowrank"] for s in st)),
-                  dense_blocks=int(sum(s["nb_dense"] for s in st)))
+                  dense_blocks=int(sum(s["nb_dense"] for s in st)), autotune=sim.tuning())
     log(f"[bench] H-matrix setup {setup_s:.1f} s {hm}")
     sim.solve(args.warmup)
     s0 = sim.stats()

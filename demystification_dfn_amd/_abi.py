@@ -108,6 +108,7 @@ SIGNATURES = {
     "rkqs": [_H, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
     "solve": [_H, C.c_int64, C.c_double, C.POINTER(DfnRunStats)],
     "set_formulation": [_H, C.c_int32],
+    "tuning": [_H, _i32p, _dp],
     "monitor": [_H, C.c_int32],
     "monitor_take": [_H, C.c_int64, _dp, _dp, _i64p],
     "moment": [_H, _dp, _dp],

@@ -221,10 +221,10 @@ struct Engine {
     if (!d) return DFN_ERR_STATE;
     double *w = d->w.as<double>();
     launch_gather(d_V, d->perm.as<int>(), w, N, stream);
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream);
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves);
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
-                 d->ypart.as<double>(), stream);
+                 d->ypart.as<double>(), stream, d->panel_rows);
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                     d->nops, kernel_scale(), d_sT, d->nops > 1 ? d_sN : d_sT, stream);
     return DFN_OK;
@@ -776,6 +776,18 @@ API int dfn_set_precision(dfn_handle h, int32_t prec) {
   return DFN_OK;
 }
 
+API int dfn_tuning(dfn_handle h, int32_t *choice, double *ms) {
+  if (!h || !choice || !ms) return DFN_ERR_ARG;
+  const dfn::DevOps *d = h->e.active_ops();
+  if (h->e.hp.kernel != DFN_KERNEL_HMATRIX || !d) return DFN_ERR_STATE;
+  choice[0] = d->vtx_waves;
+  choice[1] = d->vtx_order;
+  choice[2] = d->panel_rows;
+  ms[0] = d->tune_ms_default;
+  ms[1] = d->tune_ms;
+  return DFN_OK;
+}
+
 API int dfn_hmat_stats(dfn_handle h, int32_t op, int64_t *stats) {
   if (!h || !stats || op < 0 || op > 1) return DFN_ERR_ARG;
   const dfn::HostOperator &H = h->e.hop[op];
@@ -955,10 +967,11 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
   DFN_HIP(hipMemcpyAsync(dV, xo.data(), N * sizeof(double), hipMemcpyHostToDevice, e.stream));
   double *w = d->w.as<double>();
   dfn::launch_gather(dV, d->perm.as<int>(), w, N, e.stream);
-  dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), e.stream);
+  dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), e.stream,
+                  d->vtx_waves);
   dfn::launch_zreduce(d->zred.as<dfn::ZRed>(), d->nzr, d->zpart.as<double>(), w, e.stream);
   dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(),
-                    d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream);
+                    d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream, d->panel_rows);
   dfn::launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                   d->nops, 1.0, dT, d->nops > 1 ? dN : dT, e.stream);
   DFN_HIP(hipMemcpyAsync(yT.data(), op == 0 ? dT : dN, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
@@ -1112,7 +1125,7 @@ API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
     DFN_HIP(hipEventRecord(e.ev0, e.stream));
     for (int r = 0; r < reps; r++)
       dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(),
-                        d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream);
+                        d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream, d->panel_rows);
     DFN_HIP(hipEventRecord(e.ev1, e.stream));
     DFN_HIP(hipEventSynchronize(e.ev1));
     DFN_HIP(hipEventElapsedTime(&t, e.ev0, e.ev1));

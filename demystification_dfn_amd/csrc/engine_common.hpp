@@ -7,8 +7,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "dfn_device.hpp"
@@ -72,7 +74,69 @@ struct DevOps {
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
   int64_t S_total = 0;   // matrix entries streamed per application
+  // launch shapes picked by autotune (none changes a sum): phase-1 tasks per
+  // workgroup, phase-1 task order (0 as built, 1 grouped by rank, largest first),
+  // FP64 panel rows per wave step; and the matvec time (ms) of the default and of
+  // the chosen shapes
+  int vtx_waves = 4, vtx_order = 0, panel_rows = 2;
+  double tune_ms_default = 0, tune_ms = 0;
 };
+
+// Autotuner of the stress-evaluation launch shapes, run once per operator layout on
+// the operator itself: phase-1 tasks (waves) per workgroup {2, 4, 8}, the phase-1
+// task order (as built, or grouped by rank bucket with the largest ranks first, so a
+// workgroup's waves share one code path and the long tasks start early) and, for
+// FP64 panels, 2 or 4 rows per wave step.  Every candidate computes every sum in the
+// same order, so the choice never changes a result.  Each candidate is timed over
+// phase 1, the z reduction and phase 2 (HIP events, 2 warm-up + 8 timed
+// applications); a candidate replaces the incumbent only if it is 1 % faster.
+inline int autotune(DevOps &d, const std::vector<VTask> &vt) {
+  hipEvent_t e0, e1;
+  DFN_HIP(hipEventCreate(&e0));
+  DFN_HIP(hipEventCreate(&e1));
+  double *w = d.w.as<double>();
+  DFN_HIP(hipMemset(w, 0, d.w.bytes));
+  auto measure = [&](int waves, int rows, double &ms) -> int {
+    for (int rep = 0; rep < 10; rep++) {
+      if (rep == 2) DFN_HIP(hipEventRecord(e0, nullptr));
+      launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.fp32, w, d.zpart.as<double>(), nullptr, waves);
+      launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, nullptr);
+      launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.fp32, d.widx.as<int>(), d.cscale.as<float>(), w,
+                   d.ypart.as<double>(), nullptr, rows);
+    }
+    DFN_HIP(hipEventRecord(e1, nullptr));
+    DFN_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    DFN_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms = t / 8.0;
+    return DFN_OK;
+  };
+  double best = 0, t = 0;
+  DFN_TRY(measure(4, 2, best));
+  d.tune_ms_default = best;
+  int waves = 4, rows = 2, order = 0;
+  for (int cand : {2, 8}) {
+    DFN_TRY(measure(cand, rows, t));
+    if (t < 0.99 * best) { best = t; waves = cand; }
+  }
+  std::vector<VTask> grouped(vt);
+  std::stable_sort(grouped.begin(), grouped.end(), [](const VTask &x, const VTask &y) { return x.k > y.k; });
+  DFN_TRY(d.vtasks.upload(grouped));
+  DFN_TRY(measure(waves, rows, t));
+  if (t < 0.99 * best) { best = t; order = 1; }
+  else DFN_TRY(d.vtasks.upload(vt));
+  if (!d.fp32) {
+    DFN_TRY(measure(waves, 4, t));
+    if (t < 0.99 * best) { best = t; rows = 4; }
+  }
+  d.vtx_waves = waves;
+  d.vtx_order = order;
+  d.panel_rows = rows;
+  d.tune_ms = best;
+  DFN_HIP(hipEventDestroy(e0));
+  DFN_HIP(hipEventDestroy(e1));
+  return DFN_OK;
+}
 
 // power-of-two scale 2^e >= max|v| (1 for an all-zero block): dividing by it is exact
 inline double pow2_scale(const double *v, size_t n) {
@@ -257,6 +321,8 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   // bytes: factors + x read + y written, per operator (x shared in Mode II)
   d.alg_bytes = (double)sizeof(T) * (double)S_total + 8.0 * n + 8.0 * n * nops;
   d.panel_bytes = (double)sizeof(T) * (double)panel_entries + 4.0 * (double)cscale.size() + 8.0 * (double)yp;
+  const char *at = std::getenv("DFN_AUTOTUNE");
+  if (!(at && std::string(at) == "0")) DFN_TRY(autotune(d, vt));
   return DFN_OK;
 }
 

@@ -423,10 +423,10 @@ struct EnsEngine {
     const DevOps *d = active_ops();
     double *w = d->w.as<double>();
     launch_gather(dV, d->perm.as<int>(), w, Ntot, stream);
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream);
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves);
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
-                 d->ypart.as<double>(), stream);
+                 d->ypart.as<double>(), stream, d->panel_rows);
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), Ntot,
                     d->nops, kscale, sT.as<double>(), d->nops > 1 ? sN.as<double>() : sT.as<double>(), stream);
   }

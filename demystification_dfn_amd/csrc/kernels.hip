@@ -139,12 +139,12 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
 }
 
 template <typename TV>
-__global__ void __launch_bounds__(256) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
+__global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
                                                           const TV *__restrict__ vdata, double *__restrict__ w,
                                                           double *__restrict__ zpart) {
-  // the task index is uniform across the wave: scalar loads of the task and a
-  // scalar branch on its rank
-  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  // one task per wave, blockDim.x / 64 waves per workgroup; the task index is
+  // uniform across the wave: scalar loads of the task and a scalar branch on its rank
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (t >= ntasks) return;
   const VTask task = tasks[t];
@@ -182,6 +182,7 @@ typedef double dv2 __attribute__((ext_vector_type(2)));
 // gathered into LDS once; each wave then streams two whole rows at a time
 // (row-major panel, rows padded to a multiple of 2 doubles / 4 floats so every
 // lane issues one 16-byte load per row: 1 KiB per wave instruction).
+template <int R>
 __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict__ tasks, const double *__restrict__ pdata,
                                                         const int *__restrict__ widx, const double *__restrict__ w,
                                                         double *__restrict__ ypart) {
@@ -193,27 +194,31 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int L2 = task.L >> 1;
   const dv2 *P = reinterpret_cast<const dv2 *>(pdata + task.poff);
-  int r = wave * 2;
-  for (; r + 1 < task.m; r += 8) {
-    const dv2 *row0 = P + (int64_t)r * L2, *row1 = row0 + L2;
-    double a0 = 0.0, a1 = 0.0;
+  // R rows per wave step (each row's sum is the same whatever R is)
+  int r = wave * R;
+  for (; r + R - 1 < task.m; r += 4 * R) {
+    const dv2 *row0 = P + (int64_t)r * L2;
+    double a[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) a[q] = 0.0;
     // the panel is streamed once per application: non-temporal loads keep it
     // from displacing the gathered vectors in the caches
 #pragma unroll 4
     for (int j = lane; j < L2; j += 64) {
       const double2 wv = wl2[j];
-      const dv2 v0 = __builtin_nontemporal_load(row0 + j), v1 = __builtin_nontemporal_load(row1 + j);
-      a0 += v0.x * wv.x + v0.y * wv.y;
-      a1 += v1.x * wv.x + v1.y * wv.y;
+      dv2 v[R];
+#pragma unroll
+      for (int q = 0; q < R; q++) v[q] = __builtin_nontemporal_load(row0 + (int64_t)q * L2 + j);
+#pragma unroll
+      for (int q = 0; q < R; q++) a[q] += v[q].x * wv.x + v[q].y * wv.y;
     }
-    a0 = wave_sum(a0);
-    a1 = wave_sum(a1);
-    if (lane == 0) {
-      ypart[task.yp + r] = a0;
-      ypart[task.yp + r + 1] = a1;
-    }
+#pragma unroll
+    for (int q = 0; q < R; q++) a[q] = wave_sum(a[q]);
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < R; q++) ypart[task.yp + r + q] = a[q];
   }
-  if (r < task.m) {
+  for (; r < task.m; r++) {  // fewer than R rows left for this wave
     const dv2 *row0 = P + (int64_t)r * L2;
     double a0 = 0.0;
     for (int j = lane; j < L2; j += 64) {
@@ -1486,14 +1491,16 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
   hipLaunchKernelGGL(gather_perm_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, x, perm, xh, n);
 }
 
-void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s) {
+void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
+                int waves) {
   if (nt <= 0) return;
+  if (waves != 2 && waves != 8) waves = 4;
   if (fp32)
-    hipLaunchKernelGGL(lowrank_vtx_kernel<float>, dim3(nblk(nt, 4)), dim3(256), 0, s, t, nt, (const float *)vdata, w,
-                       zpart);
+    hipLaunchKernelGGL(lowrank_vtx_kernel<float>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+                       (const float *)vdata, w, zpart);
   else
-    hipLaunchKernelGGL(lowrank_vtx_kernel<double>, dim3(nblk(nt, 4)), dim3(256), 0, s, t, nt, (const double *)vdata,
-                       w, zpart);
+    hipLaunchKernelGGL(lowrank_vtx_kernel<double>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+                       (const double *)vdata, w, zpart);
 }
 
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s) {
@@ -1502,12 +1509,14 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
 }
 
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
-                  const double *w, double *ypart, hipStream_t s) {
+                  const double *w, double *ypart, hipStream_t s, int rows) {
   if (nt <= 0) return;
   if (fp32)
     hipLaunchKernelGGL(panel_kernel_f32, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart);
+  else if (rows == 4)
+    hipLaunchKernelGGL(panel_kernel_f64<4>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
   else
-    hipLaunchKernelGGL(panel_kernel_f64, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
+    hipLaunchKernelGGL(panel_kernel_f64<2>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
 }
 
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,

@@ -132,6 +132,29 @@ def test_hmatvec_on_identical_block_lists(mode):
         assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
 
 
+@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_FP32])
+def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
+    """The autotuner only picks launch shapes that keep every sum in the same order:
+    the stress rates of the tuned layout equal those of the default shapes bit for bit."""
+    p = _hnet(2, n_faults=40, seed=8)
+    out = []
+    for tune in ("1", "0"):
+        monkeypatch.setenv("DFN_AUTOTUNE", tune)
+        d = dev(p)
+        d.initiate_hmat()
+        if prec != _abi.PREC_FP64:
+            d.set_precision(prec)
+        t = d.tuning()
+        print(f"[gpu] autotune={tune} {t}", flush=True)
+        assert t["vtx_waves"] in (2, 4, 8) and t["vtx_order"] in (0, 1) and t["panel_rows"] in (2, 4)
+        if tune == "0":
+            assert (t["vtx_waves"], t["vtx_order"], t["panel_rows"]) == (4, 0, 2)
+        V = 10.0 ** np.random.default_rng(4).uniform(-12, 0, p.N)
+        out.append(d.kernel(V))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("builder", ["device", "host"])
 @pytest.mark.parametrize("mode", [2, 3])
 def test_device_built_operator_matches_oracle_construction(mode, builder, monkeypatch):
