@@ -208,6 +208,8 @@ bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s
 // co-resident.
 bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, unsigned seq,
                     unsigned long long *records, hipStream_t s);
+// workgroups of cg_coop_kernel that can be resident together (0: none / unsupported)
+int cg_coop_capacity(int cap);
 void launch_perm_update(const PermArgs &a, hipStream_t s);
 void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s);
 

@@ -46,7 +46,22 @@ struct Engine {
   DMem status;
   StatusWord *h_status = nullptr;  // pinned
   int *h_cgres = nullptr;          // pinned [kSlots*2]
-  static constexpr int kSlots = 6;
+  // CG systems: slots 0-5 are compute_P(t_try) and the five stage abscissae of a
+  // first attempt; slots 6-11 the same for t_try / 2, solved beside them so that a
+  // failed t_try needs no new launch (the reference halves t_try and solves again)
+  static constexpr int kSlots = 12;
+  int slot_base = 0;  // slot set the current attempt's stages read
+  // twelve systems fit one launch: the one-workgroup kernels always, the cooperative
+  // kernel when all their workgroups can be resident together
+  int coop_capacity = -1;
+  bool spec_fits() {
+    const int N = hp.N;
+    if (N <= kCgLdsMax && cg_path <= 1) return true;
+    if (!coop_ok || cg_path == 3) return true;  // plain kernel: one workgroup per system
+    if (coop_capacity < 0) coop_capacity = cg_coop_capacity(span_entries);
+    const int wps = (N + kCgSpan - 1) / kCgSpan;
+    return 12 * wps <= coop_capacity;
+  }
   int last_pd_slot = -1;
   double time = 0, t_try = 100.0;
   dfn_run_stats st{};
@@ -389,8 +404,8 @@ struct Engine {
     s.ytemp = ytemp;
     s.sT = sT.as<double>();
     s.sN = hp.mode == DFN_MODE_II ? sN.as<double>() : nullptr;
-    s.Pt = Pt(slot);
-    s.Pd = Pd(slot);
+    s.Pt = Pt(slot_base + slot);
+    s.Pd = Pd(slot_base + slot);
     for (int i = 0; i < 6; i++) s.k[i] = k(i);
     s.ynext = ynext;
     s.yscal = yscal.as<double>();
@@ -403,6 +418,7 @@ struct Engine {
 
   // friction_law(x, Y) for a device Y into device Ydot (uses CG slot 0)
   int friction_law_dev(double x, const double *dY, double *dYd) {
+    slot_base = 0;
     DFN_TRY(apply_kernel(dY, sT.as<double>(), sN.as<double>()));
     DFN_TRY(cg_launch(&x, 1, 0));
     int fail = 0;
@@ -431,7 +447,10 @@ struct Engine {
       static const double Ai[5] = {0.2, 0.3, 0.6, 1.0, 0.875};
       double dts[5];
       for (int s = 0; s < 5; s++) dts[s] = x + Ai[s] * h;
-      if (!stages_ready) DFN_TRY(cg_launch(dts, 5, 1));
+      if (!stages_ready) {
+        slot_base = 0;
+        DFN_TRY(cg_launch(dts, 5, 1));
+      }
       else st.cg_solves += 5;
       stages_ready = false;
       DFN_HIP(hipMemsetAsync(status.p, 0, sizeof(StatusWord), stream));
@@ -445,7 +464,7 @@ struct Engine {
       }
       DFN_HIP(hipMemcpyAsync(h_status, status.p, sizeof(StatusWord), hipMemcpyDeviceToHost, stream));
       int fail = 0;
-      DFN_TRY(cg_collect(1, 5, &fail));
+      DFN_TRY(cg_collect(slot_base + 1, 5, &fail));
       if (fail) return DFN_ERR_CG;
       double em;
       std::memcpy(&em, &h_status->errmax_bits, sizeof(double));
@@ -476,11 +495,17 @@ struct Engine {
     // step depends on dt alone, so the five stage solves of a first attempt with
     // h = t_try (x + A_i h, x = 0) run in the same launch.
     const bool batch = hp.coupling != 1;
+    // the solves for t_try / 2 run in the same launch when they fit beside the others
+    const bool spec = batch && spec_fits();
     auto launch_try = [&](hipStream_t cs) -> int {
       if (!batch) return cg_launch_on(&t_try, 1, 0, cs);
-      const double dts[6] = {t_try, 0.0 + 0.2 * t_try, 0.0 + 0.3 * t_try, 0.0 + 0.6 * t_try, 0.0 + 1.0 * t_try,
-                             0.0 + 0.875 * t_try};
-      return cg_launch_on(dts, 6, 0, cs);
+      double dts[12];
+      for (int g = 0; g < 2; g++) {
+        const double h = g == 0 ? t_try : t_try / 2.0;
+        const double row[6] = {h, 0.0 + 0.2 * h, 0.0 + 0.3 * h, 0.0 + 0.6 * h, 0.0 + 1.0 * h, 0.0 + 0.875 * h};
+        for (int s = 0; s < 6; s++) dts[6 * g + s] = row[s];
+      }
+      return cg_launch_on(dts, spec ? 12 : 6, 0, cs);
     };
     // With the one-workgroup-per-system CG (six workgroups) the first batch runs
     // on the side stream beside the k1 stress evaluation, which needs only Y; the
@@ -497,14 +522,21 @@ struct Engine {
       DFN_TRY(launch_try(stream));
       DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
     }
+    slot_base = 0;
     DFN_TRY(cg_collect(0, 1, &fail));
-    while (fail) {
+    while (fail) {  // :86-90: halve t_try until compute_P converges
       t_try = t_try / 2.0;
       st.cg_halvings++;
+      if (spec && slot_base == 0) {  // compute_P(t_try) already ran in slots 6-11
+        slot_base = 6;
+        DFN_TRY(cg_collect(6, 1, &fail));
+        continue;
+      }
+      slot_base = 0;
       DFN_TRY(launch_try(stream));
       DFN_TRY(cg_collect(0, 1, &fail));
     }
-    if (batch) st.cg_solves -= 5;  // counted when the stages use them
+    if (batch) st.cg_solves -= spec ? 11 : 5;  // the stages' solves are counted when the stages use them
     // k1 = f(t_try, Y) and yscal (:101-102): the stress rates are those evaluated
     // above; compute_P(t_try) of the RHS equals the call above unless the
     // permeability evolves between calls
@@ -526,8 +558,8 @@ struct Engine {
       DFN_TRY(cg_collect(5, 1, &fail));
     }
     if (!fail) {
-      DFN_HIP(hipMemcpyAsync(P.p, Pt(4), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
-      DFN_HIP(hipMemcpyAsync(Pdot.p, Pd(4), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+      DFN_HIP(hipMemcpyAsync(P.p, Pt(slot_base + 4), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
+      DFN_HIP(hipMemcpyAsync(Pdot.p, Pd(slot_base + 4), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
       if (hp.coupling == 1) {
         DFN_HIP(hipMemcpyAsync(P.p, Pt(5), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));
         DFN_HIP(hipMemcpyAsync(Pdot.p, Pd(5), sizeof(double) * N, hipMemcpyDeviceToDevice, stream));

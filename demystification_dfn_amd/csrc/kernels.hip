@@ -1639,6 +1639,18 @@ size_t cg_coop_lds_bytes(int cap) {
   return sizeof(double) * (kCgSpan + 2) + (size_t)(cap > 1 ? cap : 1) * (5 * sizeof(double) + 3 * sizeof(int));
 }
 
+int cg_coop_capacity(int cap) {
+  if (cap > kCgCoopMaxEntries) return 0;
+  int per_cu = 0, dev = 0, cus = 0, coop = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_coop_kernel, 256, cg_coop_lds_bytes(cap)) !=
+          hipSuccess)
+    return 0;
+  return per_cu * cus;
+}
+
 bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, unsigned seq,
                     unsigned long long *records, hipStream_t s) {
   if (cap > kCgCoopMaxEntries) return false;

@@ -93,6 +93,34 @@ def test_solver_trajectory(make, nsteps, cg_path, monkeypatch):
     assert close_elementwise(Yd[2 * N:][real], Yo[2 * N:][real], 1e-8, 1e-300)   # normal stress
 
 
+def _permeable_crossing_network():
+    """The crossing network with 100 times the permeability: compute_P needs more
+    than 200 iterations at some trial steps (13 halvings over 30 steps)."""
+    p = cases.crossing_network()
+    p.arrays["permeability"] = p.arrays["permeability"] * 100.0
+    return p
+
+
+@pytest.mark.parametrize("make", [_permeable_crossing_network])
+def test_step_halvings_match_the_oracle(make):
+    """Trial steps whose compute_P fails: the device solves compute_P(t_try / 2)
+    beside compute_P(t_try) and uses it when t_try fails; the halving sequence,
+    step counts and states must be those of the oracle, which halves and solves
+    again as the reference does (src/solver.f90:81-92)."""
+    p = make()
+    d, o = dev(p), OracleSimulation(p)
+    sd, so = d.solve(30), o.solve(30)
+    print(f"[gpu] halvings device {sd['cg_halvings']} oracle {so['cg_halvings']}", flush=True)
+    assert sd["cg_halvings"] == so["cg_halvings"] > 0
+    assert sd["accepted_steps"] == so["accepted_steps"] and sd["rejected_attempts"] == so["rejected_attempts"]
+    Yd, Yo = d.get_state()[0], o.get_state()[0]
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)
+
+
 def test_rkqs_single_call():
     p = cases.single_fault_small()
     d, o = dev(p), OracleSimulation(p)
