@@ -1,7 +1,7 @@
 """Parity of the gfx950 library (through its C ABI) with the CPU oracle on the same
 seeded inputs.
 
-Bars (BASELINE north star): stress-transfer matvec relative error <= 1e-12 in
+Bars: stress-transfer matvec relative error <= 1e-12 in
 plain FP64; slip rate / log state within 1e-8 relative over interseismic steps.
 compute_P is a CG iteration whose sums are reduced in a different order on the
 device, so pressures are compared to the CG stopping tolerance (max residual of
