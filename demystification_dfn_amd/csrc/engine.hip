@@ -51,6 +51,7 @@ struct Engine {
   // failed t_try needs no new launch (the reference halves t_try and solves again)
   static constexpr int kSlots = 12;
   int slot_base = 0;  // slot set the current attempt's stages read
+  bool halved_last = false;  // the last step began by halving t_try
   // twelve systems fit one launch: the one-workgroup kernels always, the cooperative
   // kernel when all their workgroups can be resident together
   int coop_capacity = -1;
@@ -496,7 +497,9 @@ struct Engine {
     // h = t_try (x + A_i h, x = 0) run in the same launch.
     const bool batch = hp.coupling != 1;
     // the solves for t_try / 2 run in the same launch when they fit beside the others
-    const bool spec = batch && spec_fits();
+    // and the last step had to halve (otherwise they would only add work)
+    const bool spec = batch && halved_last && spec_fits();
+    const int64_t halvings0 = st.cg_halvings;
     auto launch_try = [&](hipStream_t cs) -> int {
       if (!batch) return cg_launch_on(&t_try, 1, 0, cs);
       double dts[12];
@@ -537,6 +540,7 @@ struct Engine {
       DFN_TRY(cg_collect(0, 1, &fail));
     }
     if (batch) st.cg_solves -= spec ? 11 : 5;  // the stages' solves are counted when the stages use them
+    halved_last = st.cg_halvings > halvings0;
     // k1 = f(t_try, Y) and yscal (:101-102): the stress rates are those evaluated
     // above; compute_P(t_try) of the RHS equals the call above unless the
     // permeability evolves between calls

@@ -99,6 +99,8 @@ struct EnsEngine {
   int precision = DFN_PREC_FP64;
   bool use_graph = true;
   hipStream_t stream = nullptr;
+  hipStream_t cg_stream = nullptr;  // the first attempt's pressure solves run beside the k1 stress evaluation
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::unique_ptr<DevOps> ops64, ops32;
   HostOperator cat[2];  // concatenated operators (kept for the FP32 layout)
 
@@ -152,6 +154,10 @@ struct EnsEngine {
     if (h_vmax) (void)hipHostFree(h_vmax);
     ops64.reset();
     ops32.reset();
+    if (cg_stream) (void)hipStreamSynchronize(cg_stream);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (cg_stream) (void)hipStreamDestroy(cg_stream);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -235,6 +241,9 @@ struct EnsEngine {
         std::copy(hp[i].Y0.begin() + (size_t)c * hp[i].N, hp[i].Y0.begin() + (size_t)(c + 1) * hp[i].N,
                   y3.begin() + (size_t)c * Ntot + base[i]);
     DFN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    DFN_HIP(hipStreamCreateWithFlags(&cg_stream, hipStreamNonBlocking));
+    DFN_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    DFN_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     DFN_TRY(a.upload(ca)); DFN_TRY(b.upload(cb)); DFN_TRY(Dc.upload(cDc)); DFN_TRY(f0.upload(cf0));
     DFN_TRY(V0.upload(cV0)); DFN_TRY(nld.upload(cnld)); DFN_TRY(sld.upload(csld)); DFN_TRY(fake.upload(cfake));
     DFN_TRY(net.upload(cnet)); DFN_TRY(ds.upload(cds)); DFN_TRY(perm.upload(cperm)); DFN_TRY(permx.upload(cpx));
@@ -476,16 +485,16 @@ struct EnsEngine {
     return A;
   }
 
-  void cg_launch(bool stages_only) {
+  void cg_launch(bool stages_only, hipStream_t cs) {
     CgSys *sys = stages_only ? sys_stage.as<CgSys>() : sys_all.as<CgSys>();
     const int nslot = stages_only ? kEnsSlots - 1 : kEnsSlots;
-    launch_ens_cg_dt(sys, B, nslot, d_ttry.as<double>(), d_h.as<double>(), stages_only ? 1 : 0, stream);
+    launch_ens_cg_dt(sys, B, nslot, d_ttry.as<double>(), d_h.as<double>(), stages_only ? 1 : 0, cs);
     const CgArgs A = cg_args(sys);
     if (max_n <= kCgLdsMax) {
       CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
-      launch_cg_lds_multi(A, X, B * nslot, max_n, stream);
+      launch_cg_lds_multi(A, X, B * nslot, max_n, cs);
     } else {
-      launch_cg(A, B * nslot, stream);
+      launch_cg(A, B * nslot, cs);
     }
   }
 
@@ -502,16 +511,21 @@ struct EnsEngine {
   }
 
   // first attempt of a step: compute_P of all six slots, k1 and yscal, stages 2..6
+  // (the pressure solves on the side stream, joined before stage 1, which reads them)
   void seq_try() {
-    cg_launch(false);
+    (void)hipEventRecord(ev_fork, stream);
+    (void)hipStreamWaitEvent(cg_stream, ev_fork, 0);
+    cg_launch(false, cg_stream);
+    (void)hipEventRecord(ev_join, cg_stream);
     apply(Y.as<double>());
+    (void)hipStreamWaitEvent(stream, ev_join, 0);
     launch_ens_stage(1, stage_args(Y.as<double>(), 0, ytA.as<double>()), stream);
     attempt();
   }
 
   // retried attempt with reduced steps: compute_P of slots 1..5, restart, stages 2..6
   void seq_retry() {
-    cg_launch(true);
+    cg_launch(true, stream);
     launch_ens_restart_stage2(stage_args(Y.as<double>(), 0, ytA.as<double>()), stream);
     attempt();
   }
