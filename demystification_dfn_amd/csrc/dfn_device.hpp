@@ -148,7 +148,7 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
 void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
                 int waves = 4);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
-// rows: FP64 panel rows per wave step, 2 or 4 (FP32: 2)
+// rows: panel rows per wave step, 2 or 4
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
                   const double *w, double *ypart, hipStream_t s, int rows = 2);
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,

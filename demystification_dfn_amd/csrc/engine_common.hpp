@@ -76,7 +76,7 @@ struct DevOps {
   int64_t S_total = 0;   // matrix entries streamed per application
   // launch shapes picked by autotune (none changes a sum): phase-1 tasks per
   // workgroup, phase-1 task order (0 as built, 1 grouped by rank, largest first),
-  // FP64 panel rows per wave step; and the matvec time (ms) of the default and of
+  // panel rows per wave step; and the matvec time (ms) of the default and of
   // the chosen shapes
   int vtx_waves = 4, vtx_order = 0, panel_rows = 2;
   double tune_ms_default = 0, tune_ms = 0;
@@ -85,11 +85,13 @@ struct DevOps {
 // Autotuner of the stress-evaluation launch shapes, run once per operator layout on
 // the operator itself: phase-1 tasks (waves) per workgroup {2, 4, 8}, the phase-1
 // task order (as built, or grouped by rank bucket with the largest ranks first, so a
-// workgroup's waves share one code path and the long tasks start early) and, for
-// FP64 panels, 2 or 4 rows per wave step.  Every candidate computes every sum in the
+// workgroup's waves share one code path and the long tasks start early) and 2 or 4
+// panel rows per wave step.  Every candidate computes every sum in the
 // same order, so the choice never changes a result.  Each candidate is timed over
 // phase 1, the z reduction and phase 2 (HIP events, 2 warm-up + 8 timed
 // applications); a candidate replaces the incumbent only if it is 1 % faster.
+// DFN_AUTOTUNE=alt takes every non-default shape without timing, so that a test can
+// check that the shapes agree.
 inline int autotune(DevOps &d, const std::vector<VTask> &vt) {
   hipEvent_t e0, e1;
   DFN_HIP(hipEventCreate(&e0));
@@ -111,6 +113,18 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt) {
     ms = t / 8.0;
     return DFN_OK;
   };
+  const char *mode = std::getenv("DFN_AUTOTUNE");
+  if (mode && std::string(mode) == "alt") {  // every non-default shape, untimed
+    std::vector<VTask> grouped(vt);
+    std::stable_sort(grouped.begin(), grouped.end(), [](const VTask &x, const VTask &y) { return x.k > y.k; });
+    DFN_TRY(d.vtasks.upload(grouped));
+    d.vtx_waves = 8;
+    d.vtx_order = 1;
+    d.panel_rows = 4;
+    DFN_HIP(hipEventDestroy(e0));
+    DFN_HIP(hipEventDestroy(e1));
+    return DFN_OK;
+  }
   double best = 0, t = 0;
   DFN_TRY(measure(4, 2, best));
   d.tune_ms_default = best;
@@ -125,10 +139,8 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt) {
   DFN_TRY(measure(waves, rows, t));
   if (t < 0.99 * best) { best = t; order = 1; }
   else DFN_TRY(d.vtasks.upload(vt));
-  if (!d.fp32) {
-    DFN_TRY(measure(waves, 4, t));
-    if (t < 0.99 * best) { best = t; rows = 4; }
-  }
+  DFN_TRY(measure(waves, 4, t));
+  if (t < 0.99 * best) { best = t; rows = 4; }
   d.vtx_waves = waves;
   d.vtx_order = order;
   d.panel_rows = rows;

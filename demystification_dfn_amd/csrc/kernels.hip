@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
 #pragma unroll
       for (int q = 0; q < R; q++) v[q] = __builtin_nontemporal_load(row0 + (int64_t)q * L2 + j);
 #pragma unroll
-      for (int q = 0; q < R; q++) a[q] += v[q].x * wv.x + v[q].y * wv.y;
+      for (int q = 0; q < R; q++) a[q] = fma(v[q].y, wv.y, fma(v[q].x, wv.x, a[q]));
     }
 #pragma unroll
     for (int q = 0; q < R; q++) a[q] = wave_sum(a[q]);
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
     for (int j = lane; j < L2; j += 64) {
       const double2 wv = wl2[j];
       const dv2 v0 = __builtin_nontemporal_load(row0 + j);
-      a0 += v0.x * wv.x + v0.y * wv.y;
+      a0 = fma(v0.y, wv.y, fma(v0.x, wv.x, a0));  // as the R-row loop
     }
     a0 = wave_sum(a0);
     if (lane == 0) ypart[task.yp + r] = a0;
@@ -234,7 +234,10 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
 // FP32 storage: entries are block / 2^e per block, the column scales 2^e fold into
 // the gathered input; FP32 accumulation, with a compensated (Kahan) sum over the
 // near-field columns j < nd of each row.  Lane partials are combined in FP64.
+// (contraction off: the compensation needs the rounded difference and sum, and every
+// launch shape of the panel must compute the same floats)
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
+#pragma clang fp contract(off)
   const float y = v - c;
   const float t = s + y;
   c = (t - s) - y;
@@ -244,13 +247,14 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
 typedef float fv4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void acc4(fv4 v, float4 wv, int j, int nd, float &s, float &c, float &a) {
+#pragma clang fp contract(off)
   if (j + 3 < nd) {
     kahan_add(s, c, v.x * wv.x);
     kahan_add(s, c, v.y * wv.y);
     kahan_add(s, c, v.z * wv.z);
     kahan_add(s, c, v.w * wv.w);
   } else if (j >= nd) {
-    a += v.x * wv.x + v.y * wv.y + v.z * wv.z + v.w * wv.w;
+    a = fmaf(v.w, wv.w, fmaf(v.z, wv.z, fmaf(v.y, wv.y, fmaf(v.x, wv.x, a))));
   } else {
     const float p[4] = {v.x * wv.x, v.y * wv.y, v.z * wv.z, v.w * wv.w};
 #pragma unroll
@@ -261,6 +265,7 @@ __device__ __forceinline__ void acc4(fv4 v, float4 wv, int j, int nd, float &s, 
   }
 }
 
+template <int R>
 __global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict__ tasks, const float *__restrict__ pdata,
                                                         const int *__restrict__ widx, const float *__restrict__ cscale,
                                                         const double *__restrict__ w, double *__restrict__ ypart) {
@@ -273,24 +278,30 @@ __global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict_
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int L4 = task.L >> 2, nd = task.nd;
   const fv4 *P = reinterpret_cast<const fv4 *>(pdata + task.poff);
-  for (int r = wave * 2; r < task.m; r += 8) {
-    const bool two = r + 1 < task.m;
-    const fv4 *row0 = P + (int64_t)r * L4, *row1 = row0 + (two ? L4 : 0);
-    float s0 = 0.0f, c0 = 0.0f, a0 = 0.0f, s1 = 0.0f, c1 = 0.0f, a1 = 0.0f;
+  // R rows per wave step (rows past the panel re-read its first row of the step and
+  // are not stored; each row's sum is the same whatever R is)
+  for (int r = wave * R; r < task.m; r += 4 * R) {
+    const fv4 *row[R];
+    float s[R], c[R], a[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      row[q] = P + (int64_t)(r + q < task.m ? r + q : r) * L4;
+      s[q] = 0.0f; c[q] = 0.0f; a[q] = 0.0f;
+    }
 #pragma unroll 2
     for (int j = lane; j < L4; j += 64) {
       const float4 wv = wl4[j];
-      const fv4 v0 = __builtin_nontemporal_load(row0 + j), v1 = __builtin_nontemporal_load(row1 + j);
-      acc4(v0, wv, 4 * j, nd, s0, c0, a0);
-      acc4(v1, wv, 4 * j, nd, s1, c1, a1);
+      fv4 v[R];
+#pragma unroll
+      for (int q = 0; q < R; q++) v[q] = __builtin_nontemporal_load(row[q] + j);
+#pragma unroll
+      for (int q = 0; q < R; q++) acc4(v[q], wv, 4 * j, nd, s[q], c[q], a[q]);
     }
-    double t0 = ((double)s0 - (double)c0) + (double)a0;
-    double t1 = ((double)s1 - (double)c1) + (double)a1;
-    t0 = wave_sum(t0);
-    t1 = wave_sum(t1);
-    if (lane == 0) {
-      ypart[task.yp + r] = t0;
-      if (two) ypart[task.yp + r + 1] = t1;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      double t = ((double)s[q] - (double)c[q]) + (double)a[q];
+      t = wave_sum(t);
+      if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
     }
   }
 }
@@ -1511,8 +1522,10 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
                   const double *w, double *ypart, hipStream_t s, int rows) {
   if (nt <= 0) return;
-  if (fp32)
-    hipLaunchKernelGGL(panel_kernel_f32, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart);
+  if (fp32 && rows == 4)
+    hipLaunchKernelGGL(panel_kernel_f32<4>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart);
+  else if (fp32)
+    hipLaunchKernelGGL(panel_kernel_f32<2>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart);
   else if (rows == 4)
     hipLaunchKernelGGL(panel_kernel_f64<4>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
   else

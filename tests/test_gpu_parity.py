@@ -163,10 +163,11 @@ def test_hmatvec_on_identical_block_lists(mode):
 @pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_FP32])
 def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
     """The autotuner only picks launch shapes that keep every sum in the same order:
-    the stress rates of the tuned layout equal those of the default shapes bit for bit."""
+    the stress rates of the tuned layout, of the default shapes and of every
+    alternative shape (DFN_AUTOTUNE=alt) are equal bit for bit."""
     p = _hnet(2, n_faults=40, seed=8)
     out = []
-    for tune in ("1", "0"):
+    for tune in ("1", "0", "alt"):
         monkeypatch.setenv("DFN_AUTOTUNE", tune)
         d = dev(p)
         d.initiate_hmat()
@@ -177,10 +178,13 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
         assert t["vtx_waves"] in (2, 4, 8) and t["vtx_order"] in (0, 1) and t["panel_rows"] in (2, 4)
         if tune == "0":
             assert (t["vtx_waves"], t["vtx_order"], t["panel_rows"]) == (4, 0, 2)
+        if tune == "alt":
+            assert (t["vtx_waves"], t["vtx_order"], t["panel_rows"]) == (8, 1, 4)
         V = 10.0 ** np.random.default_rng(4).uniform(-12, 0, p.N)
         out.append(d.kernel(V))
-    for a, b in zip(out[0], out[1]):
-        np.testing.assert_array_equal(a, b)
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("builder", ["device", "host"])
