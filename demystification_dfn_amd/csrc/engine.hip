@@ -819,6 +819,8 @@ API int dfn_tuning(dfn_handle h, int32_t *choice, double *ms) {
   choice[0] = d->vtx_waves;
   choice[1] = d->vtx_order;
   choice[2] = d->panel_rows;
+  choice[3] = d->panel_order;
+  choice[4] = d->panel_split;
   ms[0] = d->tune_ms_default;
   ms[1] = d->tune_ms;
   return DFN_OK;

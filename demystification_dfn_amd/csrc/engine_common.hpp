@@ -75,24 +75,45 @@ struct DevOps {
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
   int64_t S_total = 0;   // matrix entries streamed per application
   // launch shapes picked by autotune (none changes a sum): phase-1 tasks per
-  // workgroup, phase-1 task order (0 as built, 1 grouped by rank, largest first),
-  // panel rows per wave step; and the matvec time (ms) of the default and of
-  // the chosen shapes
-  int vtx_waves = 4, vtx_order = 0, panel_rows = 2;
+  // workgroup, phase-1 task order (0 as built, 1 grouped by rank, 2 largest tasks
+  // first), panel rows per wave step, panel task order (0 as built, 1 largest
+  // first) and rows per panel task (0: whole panels); and the matvec time (ms) of
+  // the default and of the chosen shapes
+  int vtx_waves = 4, vtx_order = 0, panel_rows = 2, panel_order = 0, panel_split = 0;
   double tune_ms_default = 0, tune_ms = 0;
 };
 
+// Phase-2 schedule: each panel task cut into tasks of at most `split` rows (0: not
+// cut; a row's sum is computed the same way whichever task holds it), the largest
+// tasks first, so that the last workgroups to start are short ones.
+inline std::vector<PTask> panel_schedule(const std::vector<PTask> &pt, int split) {
+  std::vector<PTask> out;
+  for (const PTask &t : pt)
+    for (int r0 = 0; r0 < t.m; r0 += (split > 0 ? split : t.m)) {
+      PTask s = t;
+      s.m = split > 0 ? std::min(split, t.m - r0) : t.m;
+      s.poff = t.poff + (int64_t)r0 * t.L;
+      s.yp = t.yp + r0;
+      out.push_back(s);
+    }
+  std::stable_sort(out.begin(), out.end(),
+                   [](const PTask &x, const PTask &y) { return (int64_t)x.m * x.L > (int64_t)y.m * y.L; });
+  return out;
+}
+
 // Autotuner of the stress-evaluation launch shapes, run once per operator layout on
-// the operator itself: phase-1 tasks (waves) per workgroup {2, 4, 8}, the phase-1
-// task order (as built, or grouped by rank bucket with the largest ranks first, so a
-// workgroup's waves share one code path and the long tasks start early) and 2 or 4
-// panel rows per wave step.  Every candidate computes every sum in the
-// same order, so the choice never changes a result.  Each candidate is timed over
-// phase 1, the z reduction and phase 2 (HIP events, 2 warm-up + 8 timed
-// applications); a candidate replaces the incumbent only if it is 1 % faster.
-// DFN_AUTOTUNE=alt takes every non-default shape without timing, so that a test can
-// check that the shapes agree.
-inline int autotune(DevOps &d, const std::vector<VTask> &vt) {
+// the operator itself: phase-1 tasks (waves) per workgroup {2, 4, 8}; the phase-1
+// task order (as built; grouped by rank bucket with the largest ranks first, so a
+// workgroup's waves share one code path; or the largest tasks, rows x rank, first);
+// 2 or 4 panel rows per wave step; the phase-2 schedule (as built, or largest
+// panels first, whole or cut into tasks of 32 or 16 rows).  Every candidate
+// computes every sum in the same order, so the choice never changes a result.
+// Each candidate is timed over phase 1, the z reduction and phase 2 (HIP events,
+// 2 warm-up + 8 timed applications); a candidate replaces the incumbent only if it
+// is 1 % faster.  DFN_AUTOTUNE=alt takes non-default shapes (8 waves, largest
+// phase-1 tasks first, 4 rows, 16-row panel tasks) without timing, so that a test
+// can check that the shapes agree.
+inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<PTask> &pt) {
   hipEvent_t e0, e1;
   DFN_HIP(hipEventCreate(&e0));
   DFN_HIP(hipEventCreate(&e1));
@@ -113,37 +134,55 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt) {
     ms = t / 8.0;
     return DFN_OK;
   };
+  auto vorder = [&](int order) -> int {
+    std::vector<VTask> v(vt);
+    if (order == 1)
+      std::stable_sort(v.begin(), v.end(), [](const VTask &x, const VTask &y) { return x.k > y.k; });
+    else if (order == 2)
+      std::stable_sort(v.begin(), v.end(),
+                       [](const VTask &x, const VTask &y) { return x.len * x.k > y.len * y.k; });
+    return d.vtasks.upload(v);
+  };
+  auto porder = [&](int order, int split) -> int {  // order 0: as built
+    const std::vector<PTask> p = order ? panel_schedule(pt, split) : pt;
+    d.npt = (int)p.size();
+    return d.ptasks.upload(p);
+  };
+  int waves = 4, order = 0, rows = 2, pord = 0, split = 0;
   const char *mode = std::getenv("DFN_AUTOTUNE");
-  if (mode && std::string(mode) == "alt") {  // every non-default shape, untimed
-    std::vector<VTask> grouped(vt);
-    std::stable_sort(grouped.begin(), grouped.end(), [](const VTask &x, const VTask &y) { return x.k > y.k; });
-    DFN_TRY(d.vtasks.upload(grouped));
-    d.vtx_waves = 8;
-    d.vtx_order = 1;
-    d.panel_rows = 4;
-    DFN_HIP(hipEventDestroy(e0));
-    DFN_HIP(hipEventDestroy(e1));
-    return DFN_OK;
-  }
+  const bool alt = mode && std::string(mode) == "alt";
   double best = 0, t = 0;
-  DFN_TRY(measure(4, 2, best));
-  d.tune_ms_default = best;
-  int waves = 4, rows = 2, order = 0;
-  for (int cand : {2, 8}) {
-    DFN_TRY(measure(cand, rows, t));
-    if (t < 0.99 * best) { best = t; waves = cand; }
+  if (alt) {
+    waves = 8; order = 2; rows = 4; pord = 1; split = 16;
+    DFN_TRY(vorder(order));
+    DFN_TRY(porder(pord, split));
+  } else {
+    DFN_TRY(measure(waves, rows, best));
+    d.tune_ms_default = best;
+    for (int cand : {2, 8}) {
+      DFN_TRY(measure(cand, rows, t));
+      if (t < 0.99 * best) { best = t; waves = cand; }
+    }
+    for (int cand : {1, 2}) {
+      DFN_TRY(vorder(cand));
+      DFN_TRY(measure(waves, rows, t));
+      if (t < 0.99 * best) { best = t; order = cand; }
+    }
+    DFN_TRY(vorder(order));
+    DFN_TRY(measure(waves, 4, t));
+    if (t < 0.99 * best) { best = t; rows = 4; }
+    for (int cand : {0, 32, 16}) {
+      DFN_TRY(porder(1, cand));
+      DFN_TRY(measure(waves, rows, t));
+      if (t < 0.99 * best) { best = t; pord = 1; split = cand; }
+    }
+    DFN_TRY(porder(pord, split));
   }
-  std::vector<VTask> grouped(vt);
-  std::stable_sort(grouped.begin(), grouped.end(), [](const VTask &x, const VTask &y) { return x.k > y.k; });
-  DFN_TRY(d.vtasks.upload(grouped));
-  DFN_TRY(measure(waves, rows, t));
-  if (t < 0.99 * best) { best = t; order = 1; }
-  else DFN_TRY(d.vtasks.upload(vt));
-  DFN_TRY(measure(waves, 4, t));
-  if (t < 0.99 * best) { best = t; rows = 4; }
   d.vtx_waves = waves;
   d.vtx_order = order;
   d.panel_rows = rows;
+  d.panel_order = pord;
+  d.panel_split = split;
   d.tune_ms = best;
   DFN_HIP(hipEventDestroy(e0));
   DFN_HIP(hipEventDestroy(e1));
@@ -334,7 +373,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   d.alg_bytes = (double)sizeof(T) * (double)S_total + 8.0 * n + 8.0 * n * nops;
   d.panel_bytes = (double)sizeof(T) * (double)panel_entries + 4.0 * (double)cscale.size() + 8.0 * (double)yp;
   const char *at = std::getenv("DFN_AUTOTUNE");
-  if (!(at && std::string(at) == "0")) DFN_TRY(autotune(d, vt));
+  if (!(at && std::string(at) == "0")) DFN_TRY(autotune(d, vt, pt));
   return DFN_OK;
 }
 

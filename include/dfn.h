@@ -167,9 +167,12 @@ int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats
 
 /* Launch shapes the autotuner picked for the active operator layout (no reference
  * counterpart): choice = {phase-1 tasks per workgroup (2, 4, 8), phase-1 task order
- * (0 as built, 1 grouped by rank bucket, largest ranks first), panel rows per wave
- * step (2, 4)}; ms = {matvec time of the default shapes, of the chosen ones}.
- * None of the choices changes a result.  DFN_AUTOTUNE=0 keeps the defaults. */
+ * (0 as built, 1 grouped by rank bucket, largest ranks first, 2 largest tasks
+ * first), panel rows per wave step (2, 4), panel task order (0 as built, 1 largest
+ * panels first), rows per panel task (0 whole panels, 32, 16)} (5 entries);
+ * ms = {matvec time of the default shapes, of the chosen ones}.  None of the
+ * choices changes a result.  DFN_AUTOTUNE=0 keeps the defaults, DFN_AUTOTUNE=alt
+ * takes a set of non-default shapes without timing. */
 int dfn_tuning(dfn_handle h, int32_t *choice, double *ms);
 
 /* ---- Newton-V form ------------------------------------------------------------

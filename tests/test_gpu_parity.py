@@ -166,6 +166,7 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
     the stress rates of the tuned layout, of the default shapes and of every
     alternative shape (DFN_AUTOTUNE=alt) are equal bit for bit."""
     p = _hnet(2, n_faults=40, seed=8)
+    keys = ("vtx_waves", "vtx_order", "panel_rows", "panel_order", "panel_split")
     out = []
     for tune in ("1", "0", "alt"):
         monkeypatch.setenv("DFN_AUTOTUNE", tune)
@@ -175,11 +176,12 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
             d.set_precision(prec)
         t = d.tuning()
         print(f"[gpu] autotune={tune} {t}", flush=True)
-        assert t["vtx_waves"] in (2, 4, 8) and t["vtx_order"] in (0, 1) and t["panel_rows"] in (2, 4)
+        assert t["vtx_waves"] in (2, 4, 8) and t["vtx_order"] in (0, 1, 2) and t["panel_rows"] in (2, 4)
+        assert t["panel_order"] in (0, 1) and t["panel_split"] in (0, 32, 16)
         if tune == "0":
-            assert (t["vtx_waves"], t["vtx_order"], t["panel_rows"]) == (4, 0, 2)
+            assert [t[k] for k in keys] == [4, 0, 2, 0, 0]
         if tune == "alt":
-            assert (t["vtx_waves"], t["vtx_order"], t["panel_rows"]) == (8, 1, 4)
+            assert [t[k] for k in keys] == [8, 2, 4, 1, 16]
         V = 10.0 ** np.random.default_rng(4).uniform(-12, 0, p.N)
         out.append(d.kernel(V))
     for other in out[1:]:
