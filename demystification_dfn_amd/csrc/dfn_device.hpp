@@ -14,9 +14,8 @@ constexpr int kCgMaxIter = 200;       // compute_P gives up after 200 iterations
 // Phase 1: out[j] = scale * sum_{i<len} V[row0+i, j] * w[xcol+row0+i], j < k.
 // FP32 storage holds V / scale with a power-of-two scale per block.
 struct VTask {
-  int64_t voff;   // offset of the block's V (column-major, leading dimension ld)
+  int64_t voff;   // offset of this chunk's V, stored in strips of 64 x (16-byte vector) rows
   double scale;
-  int32_t ld;     // rows of V (the block's column count)
   int32_t row0;   // first row of this chunk
   int32_t len;    // rows in this chunk
   int32_t k;      // rank

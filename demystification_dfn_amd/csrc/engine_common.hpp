@@ -225,31 +225,41 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   std::vector<T> vdata;
   int zp = 0;
   int64_t S_total = 0;
-  // each block's V column-major with its leading dimension padded (zero rows) to a
-  // multiple of the 16-byte vector, so a lane loads whole vectors of one column
+  // V of each phase-1 task (a chunk of at most kVChunkRows rows of one block) is
+  // stored strip by strip: a strip is the 64 x kVec rows one wave instruction covers,
+  // and holds the strip's rows of column 0, then of column 1, ... (the last strip's
+  // height padded with zero rows to a whole 16-byte vector).  The k loads of one
+  // wave step are then one contiguous run of memory, and a task reads its data
+  // front to back.
   constexpr int kVec = 16 / (int)sizeof(T);
+  constexpr int kStrip = 64 * kVec;
+  static_assert(kVChunkRows % kStrip == 0, "phase-1 chunks hold whole strips");
   for (int o = 0; o < nops; o++) {
     const HostOperator &H = *ops[o];
     for (size_t b = 0; b < H.l_k.size(); b++) {
       const int k = H.l_k[b], nb = H.l_n[b];
       if (k > kMaxRank) return DFN_ERR_UNSUPPORTED;
-      const int ldp = (nb + kVec - 1) / kVec * kVec;
-      const int64_t voff = (int64_t)vdata.size();
-      vdata.resize(vdata.size() + (size_t)ldp * k, (T)0);
       const double *vb = H.l_v.data() + H.l_voff[b];
       const size_t cnt = (size_t)nb * k;
       const double vs = scaled ? pow2_scale(vb, cnt) : 1.0;
-      for (int j = 0; j < k; j++)
-        for (int i = 0; i < nb; i++) vdata[voff + (int64_t)j * ldp + i] = (T)(vb[(size_t)j * nb + i] / vs);
       const int wz = (int)(zbase[o] + zoff[o][b]);
       const bool single = nb <= kVChunkRows;  // one chunk: z goes straight into w
       ZRed r{zp, 0, k, wz};
       for (int r0 = 0; r0 < nb; r0 += kVChunkRows) {
+        const int len = std::min(kVChunkRows, nb - r0);
+        const int64_t voff = (int64_t)vdata.size();
+        for (int s0 = 0; s0 < len; s0 += kStrip) {
+          const int h = std::min(kStrip, len - s0), hp = (h + kVec - 1) / kVec * kVec;
+          const size_t base = vdata.size();
+          vdata.resize(base + (size_t)hp * k, (T)0);
+          for (int j = 0; j < k; j++)
+            for (int i = 0; i < h; i++)
+              vdata[base + (size_t)j * hp + i] = (T)(vb[(size_t)j * nb + r0 + s0 + i] / vs);
+        }
         VTask t{};
         t.voff = voff;
-        t.ld = ldp;
         t.row0 = r0;
-        t.len = std::min(kVChunkRows, nb - r0);
+        t.len = len;
         t.k = k;
         t.xcol = H.l_c0[b];
         t.zp = single ? -1 : zp;

@@ -97,9 +97,11 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 
 // Phase 1: one wave per task.  The body is specialised on the rank K (a
 // wave-uniform switch), so the K column loads of a row strip are all in flight
-// together; each lane takes a 16-byte vector of consecutive rows (2 FP64 / 4 FP32;
-// the leading dimension is padded with zero rows to a whole vector), K partial
-// sums stay in registers and are reduced across the wave with DPP lane moves.
+// together; each lane takes a 16-byte vector of consecutive rows (2 FP64 / 4 FP32),
+// K partial sums stay in registers and are reduced across the wave with DPP lane
+// moves.  The task's V is stored strip by strip (64 vectors of rows, column after
+// column; the last strip's height padded with zero rows to a whole vector), so the
+// K loads of one step read one contiguous run.
 template <int K, typename TV>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
                                          double *__restrict__ zpart, double *__restrict__ wz, int lane) {
@@ -109,16 +111,21 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
   TV acc[K];
 #pragma unroll
   for (int j = 0; j < K; j++) acc[j] = (TV)0;
-  const TV *V = vdata + task.voff + task.row0;
+  constexpr int kStrip = 64 * kVec;
+  const TV *V = vdata + task.voff;
   const double *x = w + task.xcol + task.row0;
-  const int64_t ld = task.ld;
-  for (int i = lane * kVec; i < task.len; i += 64 * kVec) {
+  for (int s0 = 0; s0 < task.len; s0 += kStrip) {
+    const int i = s0 + lane * kVec;
+    const int h = min(kStrip, task.len - s0);
+    const int hp = (h + kVec - 1) / kVec * kVec;  // the strip's column stride
+    if (i >= task.len) break;
+    const TV *Vs = V + (int64_t)s0 * K + lane * kVec;  // the strips before hold kStrip rows each
     TV xi[kVec];
 #pragma unroll
     for (int u = 0; u < kVec; u++) xi[u] = i + u < task.len ? (TV)x[i + u] : (TV)0;  // padding rows hold zeros
     tvec v[K];
 #pragma unroll
-    for (int j = 0; j < K; j++) v[j] = *reinterpret_cast<const tvec *>(V + i + j * ld);
+    for (int j = 0; j < K; j++) v[j] = *reinterpret_cast<const tvec *>(Vs + j * hp);
 #pragma unroll
     for (int j = 0; j < K; j++)
 #pragma unroll
