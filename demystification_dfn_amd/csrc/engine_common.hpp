@@ -5,12 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dfn_device.hpp"
@@ -55,10 +57,14 @@ struct DMem {
     return DFN_OK;
   }
   template <class T>
-  int upload(const std::vector<T> &v) {
-    DFN_TRY(alloc(v.size() * sizeof(T)));
-    if (!v.empty()) DFN_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  int upload(const T *v, size_t count) {
+    DFN_TRY(alloc(count * sizeof(T)));
+    if (count) DFN_HIP(hipMemcpy(p, v, count * sizeof(T), hipMemcpyHostToDevice));
     return DFN_OK;
+  }
+  template <class T>
+  int upload(const std::vector<T> &v) {
+    return upload(v.data(), v.size());
   }
   template <class T>
   T *as() const { return static_cast<T *>(p); }
@@ -189,6 +195,21 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   return DFN_OK;
 }
 
+// f(i) for i in [0, n) on up to 16 host threads (items handed out one at a time,
+// so each item should be a block- or panel-sized piece of work)
+template <class F>
+void parallel_for(int64_t n, F f) {
+  const int nt = (int)std::min<int64_t>(n, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  std::atomic<int64_t> next{0};
+  auto work = [&] {
+    for (int64_t i; (i = next.fetch_add(1)) < n;) f(i);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+}
+
 // power-of-two scale 2^e >= max|v| (1 for an all-zero block): dividing by it is exact
 inline double pow2_scale(const double *v, size_t n) {
   double m = 0.0;
@@ -222,7 +243,6 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   // phase 1 tasks and V data
   std::vector<VTask> vt;
   std::vector<ZRed> zr;
-  std::vector<T> vdata;
   int zp = 0;
   int64_t S_total = 0;
   // V of each phase-1 task (a chunk of at most kVChunkRows rows of one block) is
@@ -234,28 +254,30 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   constexpr int kVec = 16 / (int)sizeof(T);
   constexpr int kStrip = 64 * kVec;
   static_assert(kVChunkRows % kStrip == 0, "phase-1 chunks hold whole strips");
+  // planned here, written by parallel_for below
+  struct VFill { const double *src; int nb, k, r0, len; double vs; int64_t voff; };
+  std::vector<VFill> vfill;
+  int64_t vsize = 0;
   for (int o = 0; o < nops; o++) {
     const HostOperator &H = *ops[o];
+    std::vector<double> vsc(H.l_k.size(), 1.0);
+    if (scaled)
+      parallel_for((int64_t)H.l_k.size(), [&](int64_t b) {
+        vsc[b] = pow2_scale(H.l_v.data() + H.l_voff[b], (size_t)H.l_n[b] * H.l_k[b]);
+      });
     for (size_t b = 0; b < H.l_k.size(); b++) {
       const int k = H.l_k[b], nb = H.l_n[b];
       if (k > kMaxRank) return DFN_ERR_UNSUPPORTED;
       const double *vb = H.l_v.data() + H.l_voff[b];
-      const size_t cnt = (size_t)nb * k;
-      const double vs = scaled ? pow2_scale(vb, cnt) : 1.0;
+      const double vs = vsc[b];
       const int wz = (int)(zbase[o] + zoff[o][b]);
       const bool single = nb <= kVChunkRows;  // one chunk: z goes straight into w
       ZRed r{zp, 0, k, wz};
       for (int r0 = 0; r0 < nb; r0 += kVChunkRows) {
         const int len = std::min(kVChunkRows, nb - r0);
-        const int64_t voff = (int64_t)vdata.size();
-        for (int s0 = 0; s0 < len; s0 += kStrip) {
-          const int h = std::min(kStrip, len - s0), hp = (h + kVec - 1) / kVec * kVec;
-          const size_t base = vdata.size();
-          vdata.resize(base + (size_t)hp * k, (T)0);
-          for (int j = 0; j < k; j++)
-            for (int i = 0; i < h; i++)
-              vdata[base + (size_t)j * hp + i] = (T)(vb[(size_t)j * nb + r0 + s0 + i] / vs);
-        }
+        const int64_t voff = vsize;
+        for (int s0 = 0; s0 < len; s0 += kStrip) vsize += (int64_t)((std::min(kStrip, len - s0) + kVec - 1) / kVec * kVec) * k;
+        vfill.push_back({vb, nb, k, r0, len, vs, voff});
         VTask t{};
         t.voff = voff;
         t.row0 = r0;
@@ -273,6 +295,19 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
     }
     S_total += (int64_t)H.l_v.size() + (int64_t)H.l_u.size() + (int64_t)H.d_data.size();
   }
+  std::unique_ptr<T[]> vdata(new T[std::max<int64_t>(vsize, 1)]);
+  parallel_for((int64_t)vfill.size(), [&](int64_t q) {
+    const VFill &f = vfill[q];
+    T *dst = vdata.get() + f.voff;
+    for (int s0 = 0; s0 < f.len; s0 += kStrip) {
+      const int h = std::min(kStrip, f.len - s0), hp = (h + kVec - 1) / kVec * kVec;
+      for (int j = 0; j < f.k; j++) {
+        const double *col = f.src + (size_t)j * f.nb + f.r0 + s0;
+        for (int i = 0; i < hp; i++) dst[(size_t)j * hp + i] = i < h ? (T)(col[i] / f.vs) : (T)0;
+      }
+      dst += (size_t)hp * f.k;
+    }
+  });
   // phase 2 panels: per operator and leaf row cluster, all covering blocks
   const HostOperator &H0 = *ops[0];
   const int nleaf = (int)H0.leaf_r0.size();
@@ -280,7 +315,6 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   for (int l = 0; l < nleaf; l++)
     for (int i = 0; i < H0.leaf_m[l]; i++) row_leaf[H0.leaf_r0[l] + i] = l;
   std::vector<PTask> pt;
-  std::vector<T> pdata;
   std::vector<int> widx;
   std::vector<float> cscale;
   std::vector<std::vector<double>> dscale(nops), uscale(nops);
@@ -291,13 +325,21 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
     dscale[o].assign(H.d_m.size(), 1.0);
     uscale[o].assign(H.l_m.size(), 1.0);
     if (scaled) {
-      for (size_t b = 0; b < H.d_m.size(); b++)
+      parallel_for((int64_t)H.d_m.size(), [&](int64_t b) {
         dscale[o][b] = pow2_scale(H.d_data.data() + H.d_off[b], (size_t)H.d_m[b] * H.d_n[b]);
-      for (size_t b = 0; b < H.l_m.size(); b++)
+      });
+      parallel_for((int64_t)H.l_m.size(), [&](int64_t b) {
         uscale[o][b] = pow2_scale(H.l_u.data() + H.l_uoff[b], (size_t)H.l_m[b] * H.l_k[b]);
+      });
     }
   }
-  pdata.reserve(reserve);
+  // columns of every panel: (w index, source pointer into the block, block scale);
+  // panels are planned here and written by parallel_for below
+  struct Col { int64_t wi; const double *src; double scale; };
+  struct PFill { int64_t poff, col0; int L, Lp, m; };
+  std::vector<Col> cols;
+  std::vector<PFill> pfill;
+  int64_t psize = 0;
   const int64_t panel_entries = reserve;  // dense + U entries, every one stored once
   std::vector<std::vector<int>> fin((size_t)n * nops);
   int yp = 0;
@@ -313,10 +355,8 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
         cover[l].push_back({1, (int)b});
     for (int l = 0; l < nleaf; l++) {
       const int r0 = H0.leaf_r0[l], m = H0.leaf_m[l];
-      // columns of this panel: (w index, source pointer into the block, block scale);
-      // near-field (dense) columns first
-      struct Col { int64_t wi; const double *src; double scale; };
-      std::vector<Col> cols;
+      // this panel's columns, near-field (dense) columns first
+      const int64_t cbase = (int64_t)cols.size();
       int ndense = 0;
       for (auto [kind, b] : cover[l]) {
         if (kind == 0) {
@@ -333,28 +373,39 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
       }
       // rows padded to 16 bytes (2 doubles / 4 floats) with zero entries on column 0 of w
       const int pad = 16 / (int)sizeof(T);
-      for (size_t c0 = 0; c0 < cols.size(); c0 += kPanelMaxCols) {
-        const int L = (int)std::min<size_t>(kPanelMaxCols, cols.size() - c0);
+      const int64_t ncols = (int64_t)cols.size() - cbase;
+      for (int64_t c0 = 0; c0 < ncols; c0 += kPanelMaxCols) {
+        const int L = (int)std::min<int64_t>(kPanelMaxCols, ncols - c0);
         const int Lp = (L + pad - 1) / pad * pad;
         PTask t{};
-        t.poff = (int64_t)pdata.size();
+        t.poff = psize;
         t.widx = (int64_t)widx.size();
         t.m = m;
         t.L = Lp;
         t.yp = yp;
-        t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, (int64_t)ndense - (int64_t)c0));
+        t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, (int64_t)ndense - c0));
         for (int j = 0; j < Lp; j++) {
-          widx.push_back(j < L ? (int)cols[c0 + j].wi : 0);
-          if (scaled) cscale.push_back(j < L ? (float)cols[c0 + j].scale : 1.0f);
+          widx.push_back(j < L ? (int)cols[cbase + c0 + j].wi : 0);
+          if (scaled) cscale.push_back(j < L ? (float)cols[cbase + c0 + j].scale : 1.0f);
         }
-        for (int r = 0; r < m; r++)
-          for (int j = 0; j < Lp; j++) pdata.push_back(j < L ? (T)(cols[c0 + j].src[r] / cols[c0 + j].scale) : (T)0);
+        pfill.push_back({psize, cbase + c0, L, Lp, m});
+        psize += (int64_t)m * Lp;
         for (int r = 0; r < m; r++) fin[(size_t)o * n + r0 + r].push_back(yp + r);
         yp += m;
         pt.push_back(t);
       }
     }
   }
+  // panel rows, zero entries padding each row to 16 bytes
+  std::unique_ptr<T[]> pdata(new T[std::max<int64_t>(psize, 1)]);
+  parallel_for((int64_t)pfill.size(), [&](int64_t q) {
+    const PFill &f = pfill[q];
+    const Col *c = cols.data() + f.col0;
+    for (int r = 0; r < f.m; r++) {
+      T *row = pdata.get() + f.poff + (int64_t)r * f.Lp;
+      for (int j = 0; j < f.Lp; j++) row[j] = j < f.L ? (T)(c[j].src[r] / c[j].scale) : (T)0;
+    }
+  });
   std::vector<int> fin_first((size_t)n * nops + 1, 0), fin_idx;
   for (size_t g = 0; g < fin.size(); g++) {
     fin_first[g] = (int)fin_idx.size();
@@ -364,11 +415,11 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   if (widx.size() > (size_t)INT32_MAX || zp < 0 || yp < 0) return DFN_ERR_UNSUPPORTED;
   DFN_TRY(d.perm.upload(H0.perm));
   DFN_TRY(d.vtasks.upload(vt));
-  DFN_TRY(d.vdata.upload(vdata));
+  DFN_TRY(d.vdata.upload(vdata.get(), (size_t)vsize));
   DFN_TRY(d.zred.upload(zr));
   DFN_TRY(d.zpart.alloc(sizeof(double) * std::max(zp, 1)));
   DFN_TRY(d.ptasks.upload(pt));
-  DFN_TRY(d.pdata.upload(pdata));
+  DFN_TRY(d.pdata.upload(pdata.get(), (size_t)psize));
   DFN_TRY(d.widx.upload(widx));
   DFN_TRY(d.cscale.upload(cscale));
   DFN_TRY(d.w.alloc(sizeof(double) * wsize));
