@@ -700,6 +700,7 @@ struct dfn_handle_s {
 };
 
 #define API extern "C" __attribute__((visibility("default")))
+#include "api_guard.hpp"
 
 static int device_ok() {
   int n = 0;
@@ -714,6 +715,7 @@ static int device_ok() {
 }
 
 API int dfn_bench_stream(int64_t bytes, int32_t reps, double *gbps) {
+  DFN_API_BEGIN
   if (bytes < 16 || reps < 1 || !gbps) return DFN_ERR_ARG;
   int st = device_ok();
   if (st) return st;
@@ -739,9 +741,11 @@ API int dfn_bench_stream(int64_t bytes, int32_t reps, double *gbps) {
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_device_info(char *buf, int32_t len) {
+  DFN_API_BEGIN
   if (!buf || len < 1) return DFN_ERR_ARG;
   int n = 0;
   std::string s;
@@ -758,9 +762,11 @@ API int dfn_device_info(char *buf, int32_t len) {
   std::strncpy(buf, s.c_str(), (size_t)len - 1);
   buf[len - 1] = 0;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_create(const dfn_problem *p, dfn_handle *out) {
+  DFN_API_BEGIN
   if (!out) return DFN_ERR_ARG;
   *out = nullptr;
   int st = device_ok();
@@ -772,15 +778,19 @@ API int dfn_create(const dfn_problem *p, dfn_handle *out) {
   if (st != DFN_OK) { delete h; return st; }
   *out = h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_destroy(dfn_handle h) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   delete h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_build_hmat(dfn_handle h) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   Engine &e = h->e;
   if (e.hp.kernel != DFN_KERNEL_HMATRIX) return DFN_OK;
@@ -803,16 +813,20 @@ API int dfn_build_hmat(dfn_handle h) {
     std::fprintf(stderr, "[dfn build] %-28s %8.3f s\n", "device layout + upload",
                  std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
   return st;
+  DFN_API_END
 }
 
 API int dfn_set_precision(dfn_handle h, int32_t prec) {
+  DFN_API_BEGIN
   if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32)) return DFN_ERR_ARG;
   h->e.precision = prec;
   if (prec == DFN_PREC_FP32 && !h->e.ops32) return h->e.relayout();
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_tuning(dfn_handle h, int32_t *choice, double *ms) {
+  DFN_API_BEGIN
   if (!h || !choice || !ms) return DFN_ERR_ARG;
   const dfn::DevOps *d = h->e.active_ops();
   if (h->e.hp.kernel != DFN_KERNEL_HMATRIX || !d) return DFN_ERR_STATE;
@@ -824,9 +838,11 @@ API int dfn_tuning(dfn_handle h, int32_t *choice, double *ms) {
   ms[0] = d->tune_ms_default;
   ms[1] = d->tune_ms;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_hmat_stats(dfn_handle h, int32_t op, int64_t *stats) {
+  DFN_API_BEGIN
   if (!h || !stats || op < 0 || op > 1) return DFN_ERR_ARG;
   const dfn::HostOperator &H = h->e.hop[op];
   if (!h->e.have_op[op]) return DFN_ERR_STATE;
@@ -836,11 +852,13 @@ API int dfn_hmat_stats(dfn_handle h, int32_t op, int64_t *stats) {
                          (int64_t)H.l_u.size(), (int64_t)H.l_v.size(), mr, H.nbclus, H.nbblclus, H.n, sr};
   std::memcpy(stats, s, sizeof(s));
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_hmat_export(dfn_handle h, int32_t op, int32_t *perm, int32_t *d_r0, int32_t *d_c0, int32_t *d_m,
                         int32_t *d_n, int64_t *d_off, double *d_data, int32_t *l_r0, int32_t *l_c0, int32_t *l_m,
                         int32_t *l_n, int32_t *l_k, int64_t *l_uoff, int64_t *l_voff, double *l_u, double *l_v) {
+  DFN_API_BEGIN
   if (!h || op < 0 || op > 1) return DFN_ERR_ARG;
   if (!h->e.have_op[op]) return DFN_ERR_STATE;
   const dfn::HostOperator &H = h->e.hop[op];
@@ -851,6 +869,7 @@ API int dfn_hmat_export(dfn_handle h, int32_t op, int32_t *perm, int32_t *d_r0, 
   cp(d_data, H.d_data); cp(l_r0, H.l_r0); cp(l_c0, H.l_c0); cp(l_m, H.l_m); cp(l_n, H.l_n); cp(l_k, H.l_k);
   cp(l_uoff, H.l_uoff); cp(l_voff, H.l_voff); cp(l_u, H.l_u); cp(l_v, H.l_v);
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_hmat_import(dfn_handle h, int32_t op, int32_t n, const int32_t *perm, int32_t nd, const int32_t *d_r0,
@@ -858,6 +877,7 @@ API int dfn_hmat_import(dfn_handle h, int32_t op, int32_t n, const int32_t *perm
                         const double *d_data, int32_t nl, const int32_t *l_r0, const int32_t *l_c0,
                         const int32_t *l_m, const int32_t *l_n, const int32_t *l_k, const int64_t *l_uoff,
                         const int64_t *l_voff, const double *l_u, const double *l_v) {
+  DFN_API_BEGIN
   if (!h || op < 0 || op > 1 || !perm || n != h->e.hp.N || nd < 0 || nl < 0) return DFN_ERR_ARG;
   if (op == DFN_OP_N && h->e.hp.mode != DFN_MODE_II) return DFN_ERR_ARG;
   dfn::HostOperator H;
@@ -914,9 +934,11 @@ API int dfn_hmat_import(dfn_handle h, int32_t op, int32_t n, const int32_t *perm
   e.hop[op] = std::move(H);
   e.have_op[op] = true;
   return e.relayout();
+  DFN_API_END
 }
 
 API int dfn_set_state(dfn_handle h, const double *Yh, const double *Ph, double time) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   Engine &e = h->e;
   const size_t N = e.hp.N;
@@ -931,14 +953,18 @@ API int dfn_set_state(dfn_handle h, const double *Yh, const double *Ph, double t
   if (Ph) DFN_HIP(hipMemcpy(e.P.p, Ph, N * sizeof(double), hipMemcpyHostToDevice));
   e.time = time;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_set_formulation(dfn_handle h, int32_t form) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   return h->e.set_formulation(form);
+  DFN_API_END
 }
 
 API int dfn_get_state(dfn_handle h, double *Yh, double *Ph, double *Pdh, double *time) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   Engine &e = h->e;
   const size_t N = e.hp.N;
@@ -954,9 +980,11 @@ API int dfn_get_state(dfn_handle h, double *Yh, double *Ph, double *Pdh, double 
   if (Pdh) DFN_HIP(hipMemcpy(Pdh, e.Pdot.p, N * sizeof(double), hipMemcpyDeviceToHost));
   if (time) *time = e.time;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_get_stats(dfn_handle h, dfn_run_stats *s) {
+  DFN_API_BEGIN
   if (!h || !s) return DFN_ERR_ARG;
   Engine &e = h->e;
   *s = e.st;
@@ -972,9 +1000,11 @@ API int dfn_get_stats(dfn_handle h, dfn_run_stats *s) {
   DFN_HIP(hipMemcpy(V.data(), Vd, V.size() * sizeof(double), hipMemcpyDeviceToHost));
   s->max_V = *std::max_element(V.begin(), V.end());
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_kernel(dfn_handle h, const double *slip, double *sTh, double *sNh) {
+  DFN_API_BEGIN
   if (!h || !slip || !sTh) return DFN_ERR_ARG;
   Engine &e = h->e;
   const size_t N = e.hp.N;
@@ -987,9 +1017,11 @@ API int dfn_kernel(dfn_handle h, const double *slip, double *sTh, double *sNh) {
     DFN_HIP(hipMemcpyAsync(sNh, e.sN.p, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
   DFN_HIP(hipStreamSynchronize(e.stream));
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const double *x, double *y) {
+  DFN_API_BEGIN
   if (!h || !x || !y || op < 0 || op > 1) return DFN_ERR_ARG;
   Engine &e = h->e;
   if (e.hp.kernel != DFN_KERNEL_HMATRIX || !e.active_ops()) return DFN_ERR_STATE;
@@ -1019,9 +1051,11 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
     y[i] = alpha * hx + (beta == 0.0 ? 0.0 : beta * y[i]);
   }
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_friction_law(dfn_handle h, double x, const double *Yh, double *Ydh) {
+  DFN_API_BEGIN
   if (!h || !Yh || !Ydh) return DFN_ERR_ARG;
   Engine &e = h->e;
   if (e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
@@ -1030,9 +1064,11 @@ API int dfn_friction_law(dfn_handle h, double x, const double *Yh, double *Ydh) 
   DFN_TRY(e.friction_law_dev(x, e.scratch3.as<double>(), e.scratch3b.as<double>()));
   DFN_HIP(hipMemcpy(Ydh, e.scratch3b.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_compute_P(dfn_handle h, double dt, double *Pth, double *Pdh, int32_t *ier) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   Engine &e = h->e;
   const size_t N = e.hp.N;
@@ -1047,10 +1083,12 @@ API int dfn_compute_P(dfn_handle h, double dt, double *Pth, double *Pdh, int32_t
     e.last_pd_slot = -1;  // the reference leaves P_dot untouched on failure
   }
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_rkqs(dfn_handle h, double *y, const double *dydx, double *x, double htry, double eps,
                  const double *yscal, double *hdid, double *hnext) {
+  DFN_API_BEGIN
   if (!h || !y || !dydx || !x || !yscal || !hdid || !hnext) return DFN_ERR_ARG;
   Engine &e = h->e;
   if (e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
@@ -1064,9 +1102,11 @@ API int dfn_rkqs(dfn_handle h, double *y, const double *dydx, double *x, double 
   DFN_HIP(hipStreamSynchronize(e.stream));
   DFN_HIP(hipMemcpy(y, e.Y.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats) {
+  DFN_API_BEGIN
   if (!h || nsteps < 0) return DFN_ERR_ARG;
   Engine &e = h->e;
   if (e.hp.kernel == DFN_KERNEL_HMATRIX && !e.active_ops()) return DFN_ERR_STATE;
@@ -1084,27 +1124,35 @@ API int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *s
     if (st == DFN_OK) st = s2;
   }
   return st;
+  DFN_API_END
 }
 
 API int dfn_set_output(dfn_handle h, const char *dir, int64_t stride_time, int64_t freq_writing_file) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   if (dir && h->e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
   return h->e.output_set(dir, stride_time, freq_writing_file);
+  DFN_API_END
 }
 
 API int dfn_flush_output(dfn_handle h) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   return h->e.output_write();
+  DFN_API_END
 }
 
 API int dfn_monitor(dfn_handle h, int32_t on) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   if (!on) { h->e.monitor = false; return DFN_OK; }
   if (h->e.form != DFN_FORM_REFERENCE) return DFN_ERR_STATE;
   return h->e.monitor_start();
+  DFN_API_END
 }
 
 API int dfn_monitor_take(dfn_handle h, int64_t cap, double *times, double *maxV, int64_t *count) {
+  DFN_API_BEGIN
   if (!h || !count || cap < 0) return DFN_ERR_ARG;
   Engine &e = h->e;
   const int nf = e.hp.nfault;
@@ -1117,9 +1165,11 @@ API int dfn_monitor_take(dfn_handle h, int64_t cap, double *times, double *maxV,
   e.mon_v.erase(e.mon_v.begin(), e.mon_v.begin() + n * nf);
   *count = n;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_moment(dfn_handle h, double *M0, double *slip) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   Engine &e = h->e;
   if (!e.monitor && e.slip.p == nullptr) return DFN_ERR_STATE;
@@ -1138,9 +1188,11 @@ API int dfn_moment(dfn_handle h, double *M0, double *slip) {
       M0[f] = m;
     }
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
+  DFN_API_BEGIN
   if (!h || !ms || reps < 1) return DFN_ERR_ARG;
   Engine &e = h->e;
   const double *dV = e.Y.as<double>();
@@ -1180,4 +1232,5 @@ API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
     ms[3] = ms[2];
   }
   return DFN_OK;
+  DFN_API_END
 }

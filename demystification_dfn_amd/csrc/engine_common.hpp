@@ -196,7 +196,9 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
 }
 
 // f(i) for i in [0, n) on up to 16 host threads (items handed out one at a time,
-// so each item should be a block- or panel-sized piece of work)
+// so each item should be a block- or panel-sized piece of work; f must not throw).
+// If a thread cannot be started, the threads that did start and the caller share
+// the work.
 template <class F>
 void parallel_for(int64_t n, F f) {
   const int nt = (int)std::min<int64_t>(n, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
@@ -205,7 +207,14 @@ void parallel_for(int64_t n, F f) {
     for (int64_t i; (i = next.fetch_add(1)) < n;) f(i);
   };
   std::vector<std::thread> pool;
-  for (int t = 1; t < nt; t++) pool.emplace_back(work);
+  pool.reserve(nt);
+  for (int t = 1; t < nt; t++) {
+    try {
+      pool.emplace_back(work);
+    } catch (...) {
+      break;
+    }
+  }
   work();
   for (auto &t : pool) t.join();
 }

@@ -697,8 +697,10 @@ struct dfn_ensemble_s {
 };
 
 #define API extern "C" __attribute__((visibility("default")))
+#include "api_guard.hpp"
 
 API int dfn_ensemble_create(const dfn_problem *problems, int32_t nnet, dfn_ensemble *out) {
+  DFN_API_BEGIN
   if (!out || !problems || nnet < 1) return DFN_ERR_ARG;
   *out = nullptr;
   int n = 0;
@@ -709,41 +711,53 @@ API int dfn_ensemble_create(const dfn_problem *problems, int32_t nnet, dfn_ensem
   if (st != DFN_OK) { delete h; return st; }
   *out = h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_destroy(dfn_ensemble h) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   delete h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_build_hmat(dfn_ensemble h) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   return h->e.build_hmat();
+  DFN_API_END
 }
 
 API int dfn_ensemble_set_precision(dfn_ensemble h, int32_t prec) {
+  DFN_API_BEGIN
   if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32)) return DFN_ERR_ARG;
   h->e.precision = prec;
   if (prec == DFN_PREC_FP32 && !h->e.ops32) return h->e.relayout();
   h->e.drop_graphs();
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_set_graph(dfn_ensemble h, int32_t use_graph) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   h->e.use_graph = use_graph != 0;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_solve(dfn_ensemble h, int64_t nsteps) {
+  DFN_API_BEGIN
   if (!h || nsteps < 0) return DFN_ERR_ARG;
   if (!h->e.active_ops()) return DFN_ERR_STATE;
   for (int64_t s = 0; s < nsteps; s++) DFN_TRY(h->e.step());
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_solve_until(dfn_ensemble h, double t_end, int64_t max_steps) {
+  DFN_API_BEGIN
   if (!h || !(t_end > 0.0) || max_steps < 0) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
   if (!e.active_ops()) return DFN_ERR_STATE;
@@ -758,9 +772,11 @@ API int dfn_ensemble_solve_until(dfn_ensemble h, double t_end, int64_t max_steps
   for (int i = 0; i < e.B; i++)
     if (e.time[i] < t_end) return DFN_ERR_STATE;  // max_steps ran out first
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_get_state(dfn_ensemble h, int32_t netw, double *Yh, double *Ph, double *Pdh, double *time) {
+  DFN_API_BEGIN
   if (!h || netw < 0 || netw >= h->e.B) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
   const size_t N = e.hp[netw].N, o = e.base[netw], T = e.Ntot;
@@ -772,9 +788,11 @@ API int dfn_ensemble_get_state(dfn_ensemble h, int32_t netw, double *Yh, double 
   if (Pdh) DFN_HIP(hipMemcpy(Pdh, e.Pdot.as<double>() + o, N * sizeof(double), hipMemcpyDeviceToHost));
   if (time) *time = e.time[netw];
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_get_stats(dfn_ensemble h, int32_t netw, dfn_run_stats *s) {
+  DFN_API_BEGIN
   if (!h || !s || netw < 0 || netw >= h->e.B) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
   *s = e.st[netw];
@@ -786,9 +804,11 @@ API int dfn_ensemble_get_stats(dfn_ensemble h, int32_t netw, dfn_run_stats *s) {
   DFN_HIP(hipMemcpy(V.data(), e.Y.as<double>() + e.base[netw], N * sizeof(double), hipMemcpyDeviceToHost));
   s->max_V = *std::max_element(V.begin(), V.end());
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_set_controller(dfn_ensemble h, int32_t mode, dfn_mlp m) {
+  DFN_API_BEGIN
   if (!h || mode < DFN_STEP_RKQS || mode > DFN_STEP_LEARNED) return DFN_ERR_ARG;
   if (mode == DFN_STEP_LEARNED && !m) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
@@ -797,17 +817,21 @@ API int dfn_ensemble_set_controller(dfn_ensemble h, int32_t mode, dfn_mlp m) {
   std::fill(e.fallback.begin(), e.fallback.end(), 0);
   std::fill(e.learned_last.begin(), e.learned_last.end(), 0);
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_set_recording(dfn_ensemble h, int32_t on) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
   e.recording = on != 0;
   std::fill(e.has_pend.begin(), e.has_pend.end(), 0);
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_take_records(dfn_ensemble h, float *X, float *T, int64_t cap, int64_t *count) {
+  DFN_API_BEGIN
   if (!h || !count || cap < 0) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
   const int64_t avail = (int64_t)e.rec_T.size();
@@ -819,9 +843,11 @@ API int dfn_ensemble_take_records(dfn_ensemble h, float *X, float *T, int64_t ca
   e.rec_T.erase(e.rec_T.begin(), e.rec_T.begin() + n);
   *count = n;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_ensemble_info(dfn_ensemble h, int64_t *info) {
+  DFN_API_BEGIN
   if (!h || !info) return DFN_ERR_ARG;
   EnsEngine &e = h->e;
   const dfn::DevOps *d = e.active_ops();
@@ -832,4 +858,5 @@ API int dfn_ensemble_info(dfn_ensemble h, int64_t *info) {
   info[4] = d ? (int64_t)d->alg_bytes : 0;
   info[5] = e.use_graph ? 1 : 0;
   return DFN_OK;
+  DFN_API_END
 }

@@ -471,8 +471,10 @@ struct dfn_h5_s {
 };
 
 #define API extern "C" __attribute__((visibility("default")))
+#include "api_guard.hpp"
 
 API int dfn_h5_open(const char *path, dfn_h5 *out) {
+  DFN_API_BEGIN
   if (!path || !out) return DFN_ERR_ARG;
   *out = nullptr;
   dfn_h5 h = new (std::nothrow) dfn_h5_s;
@@ -481,15 +483,19 @@ API int dfn_h5_open(const char *path, dfn_h5 *out) {
   if (st != DFN_OK) { delete h; return st; }
   *out = h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_h5_close(dfn_h5 h) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   delete h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_h5_info(dfn_h5 h, const char *path, int32_t *type_class, int32_t *rank, int64_t *dims) {
+  DFN_API_BEGIN
   if (!h || !path) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -499,9 +505,11 @@ API int dfn_h5_info(dfn_h5 h, const char *path, int32_t *type_class, int32_t *ra
   if (dims)
     for (size_t i = 0; i < ds.dims.size() && i < 8; i++) dims[i] = (int64_t)ds.dims[i];
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_h5_read_f64(dfn_h5 h, const char *path, double *buf, int64_t n) {
+  DFN_API_BEGIN
   if (!h || !path || !buf || n < 0) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -512,9 +520,11 @@ API int dfn_h5_read_f64(dfn_h5 h, const char *path, double *buf, int64_t n) {
   if (!h->f.raw(ds, r)) return DFN_ERR_UNSUPPORTED;
   for (int64_t i = 0; i < n; i++) buf[i] = h->f.number(ds, &r[(size_t)i * ds.size]);
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_h5_read_i64(dfn_h5 h, const char *path, int64_t *buf, int64_t n) {
+  DFN_API_BEGIN
   if (!h || !path || !buf || n < 0) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -525,9 +535,11 @@ API int dfn_h5_read_i64(dfn_h5 h, const char *path, int64_t *buf, int64_t n) {
   if (!h->f.raw(ds, r)) return DFN_ERR_UNSUPPORTED;
   for (int64_t i = 0; i < n; i++) buf[i] = h->f.integer(ds, &r[(size_t)i * ds.size]);
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_h5_read_string(dfn_h5 h, const char *path, char *buf, int32_t cap) {
+  DFN_API_BEGIN
   if (!h || !path || !buf || cap < 1) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -539,9 +551,12 @@ API int dfn_h5_read_string(dfn_h5 h, const char *path, char *buf, int32_t cap) {
   std::memcpy(buf, s.data(), n);
   buf[n] = 0;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_h5_write_f64(const char *path, int32_t n, const char *const *names, const int32_t *ranks,
                          const int64_t *dims, const double *const *data) {
+  DFN_API_BEGIN
   return h5_write_f64(path, n, names, ranks, dims, data);
+  DFN_API_END
 }

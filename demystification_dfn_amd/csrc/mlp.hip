@@ -299,8 +299,10 @@ struct dfn_mlp_s {
 dfn::Mlp *dfn::mlp_of(dfn_mlp h) { return h ? &h->m : nullptr; }
 
 #define API extern "C" __attribute__((visibility("default")))
+#include "api_guard.hpp"
 
 API int dfn_mlp_create(dfn_mlp *out) {
+  DFN_API_BEGIN
   if (!out) return DFN_ERR_ARG;
   *out = nullptr;
   int n = 0;
@@ -311,15 +313,19 @@ API int dfn_mlp_create(dfn_mlp *out) {
   if (st != DFN_OK) { delete h; return st; }
   *out = h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_mlp_destroy(dfn_mlp h) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   delete h;
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_mlp_set_params(dfn_mlp h, const float *params, const float *norm) {
+  DFN_API_BEGIN
   if (!h || !params) return DFN_ERR_ARG;
   if (norm) {  // the feature scales and the output scale must be positive
     for (int j = 0; j < DFN_MLP_INPUTS; j++)
@@ -327,9 +333,11 @@ API int dfn_mlp_set_params(dfn_mlp h, const float *params, const float *norm) {
     if (!(norm[2 * DFN_MLP_INPUTS + 1] > 0.0f)) return DFN_ERR_ARG;
   }
   return h->m.set_params(params, norm);
+  DFN_API_END
 }
 
 API int dfn_mlp_get_params(dfn_mlp h, float *params, float *norm) {
+  DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
   DFN_HIP(hipStreamSynchronize(h->m.stream));
   if (params) DFN_HIP(hipMemcpy(params, h->m.wm.p, sizeof(float) * DFN_MLP_PARAMS, hipMemcpyDeviceToHost));
@@ -339,9 +347,11 @@ API int dfn_mlp_get_params(dfn_mlp h, float *params, float *norm) {
     norm[2 * DFN_MLP_INPUTS + 1] = h->m.nm.sy;
   }
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_mlp_forward(dfn_mlp h, const float *X, int64_t n, float *Y) {
+  DFN_API_BEGIN
   if (!h || !X || !Y || n < 1 || n > INT32_MAX) return DFN_ERR_ARG;
   Mlp &m = h->m;
   DFN_TRY(m.ensure(n, n));
@@ -353,9 +363,11 @@ API int dfn_mlp_forward(dfn_mlp h, const float *X, int64_t n, float *Y) {
   DFN_HIP(hipStreamSynchronize(m.stream));
   for (int64_t s = 0; s < n; s++) Y[s] = dfn::mlp_output_scale(&m, y[s]);
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_mlp_loss_grad(dfn_mlp h, const float *X, const float *T, int64_t n, double *loss, float *grad) {
+  DFN_API_BEGIN
   if (!h || !X || !T || n < 1 || n > INT32_MAX) return DFN_ERR_ARG;
   Mlp &m = h->m;
   DFN_TRY(m.ensure(n, n));
@@ -367,10 +379,12 @@ API int dfn_mlp_loss_grad(dfn_mlp h, const float *X, const float *T, int64_t n, 
   if (grad) DFN_HIP(hipMemcpyAsync(grad, m.grad.p, sizeof(float) * DFN_MLP_PARAMS, hipMemcpyDeviceToHost, m.stream));
   DFN_HIP(hipStreamSynchronize(m.stream));
   return DFN_OK;
+  DFN_API_END
 }
 
 API int dfn_mlp_train(dfn_mlp h, const float *X, const float *T, int64_t n, int32_t epochs, int32_t batch, float lr,
                       uint64_t seed, double *final_loss) {
+  DFN_API_BEGIN
   if (!h || !X || !T || n < 1 || n > INT32_MAX || epochs < 0 || batch < 1 || !(lr > 0.0f)) return DFN_ERR_ARG;
   Mlp &m = h->m;
   const int b = (int)std::min<int64_t>(batch, n);
@@ -409,4 +423,5 @@ API int dfn_mlp_train(dfn_mlp h, const float *X, const float *T, int64_t n, int3
     *final_loss = tot / (double)cnt;
   }
   return DFN_OK;
+  DFN_API_END
 }
