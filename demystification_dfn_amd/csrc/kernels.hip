@@ -1086,29 +1086,37 @@ __device__ __forceinline__ void ll_sum_max(unsigned long long *rec, int wps, int
 // The crossing entries of one workgroup's span of rows, staged in LDS at the
 // start of a cooperative compute_P.  Entry j (in row order) contributes
 // (c1 v(ea) + c2 v(eb) + c3 v(ec)) / dv to its row (src/fluid_diffusion.f90:221-291);
-// term[j] holds that contribution for the current direction.
+// term[j] holds that contribution for the current direction, pa/pb/pc[j] the
+// direction at ea/eb/ec when that element lies outside the span.
 struct SpanEntries {
-  double *c1, *c2, *c3, *dv, *term;
+  double *c1, *c2, *c3, *dv, *term, *pa, *pb, *pc;
   int *ea, *eb, *ec;
   __device__ SpanEntries(double *base, int cap)
       : c1(base), c2(base + cap), c3(base + 2 * cap), dv(base + 3 * cap), term(base + 4 * cap),
-        ea(reinterpret_cast<int *>(base + 5 * cap)), eb(ea + cap), ec(ea + 2 * cap) {}
+        pa(base + 5 * cap), pb(base + 6 * cap), pc(base + 7 * cap),
+        ea(reinterpret_cast<int *>(base + 8 * cap)), eb(ea + cap), ec(ea + 2 * cap) {}
 };
 
 // Cooperative compute_P: every system (time step dt) is spread over `wps`
 // workgroups of 256 threads, each owning a span of kCgSpan rows, kCgEPT per
 // thread held in registers (x, r, p, A p and the stencil coefficients).  Per
 // iteration: the span's direction goes to LDS, with the halo values of the
-// neighbouring spans and the crossing operands outside the span received as
+// neighbouring spans and the crossing operands outside the span formed from
 // messages; the span's crossing terms are evaluated in parallel; A p row by row;
 // the p.Ap partials are exchanged as messages and summed in a fixed order, so
-// every workgroup of a system computes the same alpha; x and r are updated and
-// the r.r and max|r| partials exchanged the same way (same beta and stopping
-// decision everywhere); the new direction is sent for the elements other
-// workgroups read (the span ends and the crossing elements).  Systems never
-// wait for each other: one that converges early leaves the device to the
+// every workgroup of a system computes the same alpha; x and r are updated, the
+// new residual is sent for the elements other workgroups read (the span ends and
+// the crossing elements), and the r.r and max|r| partials are exchanged like
+// p.Ap (same beta and stopping decision everywhere).  A reader forms the new
+// direction at a remote element itself, fma(beta, p_old, r), from the residual
+// it received and the direction it kept from the last iteration: the owner
+// computes the same fma on the same doubles, so both hold the same direction,
+// and an iteration waits on two message exchanges instead of three.  Systems
+// never wait for each other: one that converges early leaves the device to the
 // others.  Message epochs are seq * 256 + a step number, so no word left by an
-// earlier launch can match.  Dynamic LDS: (kCgSpan + 2) doubles for p, then 5
+// earlier launch can match; a residual for the next step overwrites a word only
+// after the p.Ap exchange, which every workgroup of the system joins after its
+// last read of that word.  Dynamic LDS: (kCgSpan + 2) doubles for p, then 8
 // doubles and 3 ints per crossing entry of the span (cap entries; the launcher
 // checks that every span fits).
 __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int wps, int cap, unsigned seq,
@@ -1201,7 +1209,7 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       const double mv = coop_mv(A, X, k, xs[q], xe[q], P[k], vP, a1[q], a2[q], a3[q], d, dt, dq);
       r[q] = b - mv;
       p[q] = r[q];
-      if (shared & (1u << q)) ll_send(w0 + k, w1 + k, p[q], ep0 + 1);  // p_1 = r_0
+      if (shared & (1u << q)) ll_send(w0 + k, w1 + k, r[q], ep0 + 1);  // r_0, which is p_1
       part_rr += r[q] * r[q];
       part_em = fmax(part_em, fabs(r[q] / 1.0));
     }
@@ -1217,23 +1225,37 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   bool timed_out = __syncthreads_or(late);
   bool fail = false;
   int it = 0;
+  double beta = 0.0;      // of the last iteration
+  double halo_old = 0.0;  // thread 0: p at span0 - 1; the last thread: p at span1
+  // the direction at a remote element from its residual message (p_1 = r_0)
+  const auto remote_p = [&](double rv, double p_old) { return it == 0 ? rv : fma(beta, p_old, rv); };
   // every workgroup of the system receives the same partials and sums them in the
   // same order, so they all take the same decisions
   while (em / A.tol >= 1.0 && !timed_out) {
-    const unsigned ep = ep0 + (unsigned)it + 1;  // p_{it+1} and this iteration's partials
+    const unsigned ep = ep0 + (unsigned)it + 1;  // r_it for p_{it+1}, and this iteration's partials
 #pragma unroll
     for (int q = 0; q < kCgEPT; q++) {
       const int k = base + q * 256;
       if (k < n) lp[1 + k - span0] = p[q];
     }
-    if (threadIdx.x == 0 && span0 > 0) lp[0] = ll_recv(w0 + span0 - 1, w1 + span0 - 1, ep, late);
-    if (threadIdx.x == blockDim.x - 1 && span1 < n) lp[1 + span1 - span0] = ll_recv(w0 + span1, w1 + span1, ep, late);
+    if (threadIdx.x == 0 && span0 > 0) {
+      halo_old = remote_p(ll_recv(w0 + span0 - 1, w1 + span0 - 1, ep, late), halo_old);
+      lp[0] = halo_old;
+    }
+    if (threadIdx.x == blockDim.x - 1 && span1 < n) {
+      halo_old = remote_p(ll_recv(w0 + span1, w1 + span1, ep, late), halo_old);
+      lp[1 + span1 - span0] = halo_old;
+    }
     __syncthreads();  // p of the span and its halo are in LDS
     for (int j = threadIdx.x; j < nent; j += blockDim.x) {
       const int ea = E.ea[j], eb = E.eb[j], ec = E.ec[j];
-      const double va = (ea >= span0 && ea < span1) ? lp[1 + ea - span0] : ll_recv(w0 + ea, w1 + ea, ep, late);
-      const double vb = (eb >= span0 && eb < span1) ? lp[1 + eb - span0] : ll_recv(w0 + eb, w1 + eb, ep, late);
-      const double vc = (ec >= span0 && ec < span1) ? lp[1 + ec - span0] : ll_recv(w0 + ec, w1 + ec, ep, late);
+      double va, vb, vc;
+      if (ea >= span0 && ea < span1) va = lp[1 + ea - span0];
+      else va = E.pa[j] = remote_p(ll_recv(w0 + ea, w1 + ea, ep, late), E.pa[j]);
+      if (eb >= span0 && eb < span1) vb = lp[1 + eb - span0];
+      else vb = E.pb[j] = remote_p(ll_recv(w0 + eb, w1 + eb, ep, late), E.pb[j]);
+      if (ec >= span0 && ec < span1) vc = lp[1 + ec - span0];
+      else vc = E.pc[j] = remote_p(ll_recv(w0 + ec, w1 + ec, ep, late), E.pc[j]);
       const double num = E.c1[j] * va + E.c2[j] * vb + E.c3[j] * vc;
       E.term[j] = num / E.dv[j];
     }
@@ -1267,6 +1289,7 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
         r[q] = r[q] - alpha * ap[q];
         part_r1 += r[q] * r[q];
         part_m = fmax(part_m, fabs(r[q] / 1.0));
+        if (shared & (1u << q)) ll_send(w0 + k, w1 + k, r[q], ep + 1);
       }
     }
     block_sum_max(part_r1, part_m, red);
@@ -1276,13 +1299,9 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
     }
     double r1r1;
     ll_sum_max(rec, wps, 2, 4, ep, red, late, r1r1, em);
-    const double beta = r1r1 / rr;
+    beta = r1r1 / rr;
 #pragma unroll
-    for (int q = 0; q < kCgEPT; q++) {
-      const int k = base + q * 256;
-      p[q] = r[q] + beta * p[q];
-      if (k < n && (shared & (1u << q))) ll_send(w0 + k, w1 + k, p[q], ep + 1);
-    }
+    for (int q = 0; q < kCgEPT; q++) p[q] = fma(beta, p[q], r[q]);  // as remote_p
     rr = r1r1;
     it++;
     timed_out = __syncthreads_or(late);
@@ -1656,7 +1675,7 @@ void launch_stream_copy(const void *a, void *b, int64_t bytes, hipStream_t s) {
 }
 
 size_t cg_coop_lds_bytes(int cap) {
-  return sizeof(double) * (kCgSpan + 2) + (size_t)(cap > 1 ? cap : 1) * (5 * sizeof(double) + 3 * sizeof(int));
+  return sizeof(double) * (kCgSpan + 2) + (size_t)(cap > 1 ? cap : 1) * (8 * sizeof(double) + 3 * sizeof(int));
 }
 
 int cg_coop_capacity(int cap) {
