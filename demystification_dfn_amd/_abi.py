@@ -109,6 +109,8 @@ SIGNATURES = {
     "solve": [_H, C.c_int64, C.c_double, C.POINTER(DfnRunStats)],
     "set_formulation": [_H, C.c_int32],
     "tuning": [_H, _i32p, _dp],
+    "kernel_timing": [_H, C.c_int32],
+    "kernel_timing_read": [_H, _dp, _i64p],
     "monitor": [_H, C.c_int32],
     "monitor_take": [_H, C.c_int64, _dp, _dp, _i64p],
     "moment": [_H, _dp, _dp],

@@ -73,6 +73,14 @@ struct Engine {
   unsigned long long *h_fvmax = nullptr;  // pinned [nfault]
   std::vector<double> mon_t, mon_v;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // while kt_on, every stress evaluation records an event pair around its dominant
+  // kernel (the phase-2 panel launch, or the direct kernel); the pairs are summed
+  // once their launches are complete
+  bool kt_on = false;
+  std::vector<hipEvent_t> kt_ev;  // pairs: kt_ev[2 i], kt_ev[2 i + 1]
+  int kt_used = 0;
+  double kt_ms = 0.0;
+  int64_t kt_count = 0;
   // the step's compute_P batch can run on a second stream (default priority)
   // beside the k1 stress evaluation, which does not depend on it
   hipStream_t cg_stream = nullptr;
@@ -92,6 +100,7 @@ struct Engine {
     if (h_fvmax) (void)hipHostFree(h_fvmax);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t ev : kt_ev) (void)hipEventDestroy(ev);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (cg_stream) {
@@ -223,14 +232,43 @@ struct Engine {
   const DevOps *active_ops() const { return (precision == DFN_PREC_FP32 && ops32) ? ops32.get() : ops64.get(); }
 
   // stress rates sT (and sN) from the slip-rate vector d_V (device, original order)
+  // the event pair for the next timed launch (kt_on), or nullptr
+  hipEvent_t *kt_next() {
+    if (!kt_on) return nullptr;
+    if (kt_used >= 1024 && kt_collect() != DFN_OK) return nullptr;
+    while ((int)kt_ev.size() < 2 * (kt_used + 1)) {
+      hipEvent_t ev = nullptr;
+      if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+      kt_ev.push_back(ev);
+    }
+    return &kt_ev[2 * kt_used++];
+  }
+
+  // adds the durations of the recorded pairs (waits for the last one)
+  int kt_collect() {
+    if (kt_used == 0) return DFN_OK;
+    DFN_HIP(hipEventSynchronize(kt_ev[2 * kt_used - 1]));
+    for (int i = 0; i < kt_used; i++) {
+      float t = 0.0f;
+      DFN_HIP(hipEventElapsedTime(&t, kt_ev[2 * i], kt_ev[2 * i + 1]));
+      kt_ms += t;
+      kt_count++;
+    }
+    kt_used = 0;
+    return DFN_OK;
+  }
+
   int apply_kernel(const double *d_V, double *d_sT, double *d_sN) {
     const int N = hp.N;
+    hipEvent_t *kt = kt_next();
     if (hp.kernel == DFN_KERNEL_DIRECT) {
+      if (kt) DFN_HIP(hipEventRecord(kt[0], stream));
       if (hp.mode == DFN_MODE_III)
         launch_direct3(node.as<double>(), d_V, dpart.as<double>(), N, kernel_scale(), d_sT, stream);
       else
         launch_direct2(node.as<double>(), elem.as<double>(), nrm.as<double>(), d_V, dpart.as<double>(), N,
                        kernel_scale(), d_sT, d_sN, stream);
+      if (kt) DFN_HIP(hipEventRecord(kt[1], stream));
       return DFN_OK;
     }
     const DevOps *d = active_ops();
@@ -239,8 +277,10 @@ struct Engine {
     launch_gather(d_V, d->perm.as<int>(), w, N, stream);
     launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves);
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
+    if (kt) DFN_HIP(hipEventRecord(kt[0], stream));
     launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
                  d->ypart.as<double>(), stream, d->panel_rows);
+    if (kt) DFN_HIP(hipEventRecord(kt[1], stream));
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                     d->nops, kernel_scale(), d_sT, d->nops > 1 ? d_sN : d_sT, stream);
     return DFN_OK;
@@ -576,6 +616,7 @@ struct Engine {
     // cumtrapz integrates over the differences of the recorded times
     if (monitor) DFN_TRY(monitor_step(time - t_before));
     DFN_TRY(output_step());                         // save_data / write_data (:180-193)
+    if (kt_on) DFN_TRY(kt_collect());
     if (h_status->tensile) return DFN_ERR_TENSILE;  // check_errors :412-420 on the accepted state
     return DFN_OK;
   }
@@ -1187,6 +1228,29 @@ API int dfn_moment(dfn_handle h, double *M0, double *slip) {
       }
       M0[f] = m;
     }
+  return DFN_OK;
+  DFN_API_END
+}
+
+API int dfn_kernel_timing(dfn_handle h, int32_t on) {
+  DFN_API_BEGIN
+  if (!h) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  DFN_TRY(e.kt_collect());
+  e.kt_on = on != 0;
+  e.kt_ms = 0.0;
+  e.kt_count = 0;
+  return DFN_OK;
+  DFN_API_END
+}
+
+API int dfn_kernel_timing_read(dfn_handle h, double *ms_mean, int64_t *count) {
+  DFN_API_BEGIN
+  if (!h || !ms_mean || !count) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  DFN_TRY(e.kt_collect());
+  *count = e.kt_count;
+  *ms_mean = e.kt_count ? e.kt_ms / (double)e.kt_count : 0.0;
   return DFN_OK;
   DFN_API_END
 }

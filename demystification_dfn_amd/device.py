@@ -52,6 +52,17 @@ class DeviceSimulation(Simulation):
         return dict(vtx_waves=int(c[0]), vtx_order=int(c[1]), panel_rows=int(c[2]), panel_order=int(c[3]),
                     panel_split=int(c[4]), ms_default=float(ms[0]), ms_tuned=float(ms[1]))
 
+    def kernel_timing(self, on=True):
+        """Start (clearing the sums) or stop timing the dominant kernel of every
+        stress evaluation (dfn_kernel_timing)."""
+        _abi.check(self._fns["kernel_timing"](self._h, 1 if on else 0), "dfn_kernel_timing")
+
+    def kernel_timing_read(self):
+        """(mean ms per launch, launches timed) since the last start (dfn_kernel_timing_read)."""
+        ms, n = np.zeros(1), np.zeros(1, np.int64)
+        _abi.check(self._fns["kernel_timing_read"](self._h, _abi.dptr(ms), _abi.i64ptr(n)), "dfn_kernel_timing_read")
+        return float(ms[0]), int(n[0])
+
     def set_formulation(self, form):
         """dfn_set_formulation: _abi.FORM_REFERENCE or _abi.FORM_NEWTON_V."""
         _abi.check(self._fns["set_formulation"](self._h, int(form)), "dfn_set_formulation")

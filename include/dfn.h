@@ -256,6 +256,14 @@ int dfn_flush_output(dfn_handle h);
  * algorithmic bytes per application, ms[3] = algorithmic bytes of the dominant
  * kernel per launch.  ms must hold 4 doubles. */
 int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms);
+/* Per-launch timing of the dominant kernel of every stress evaluation (the
+ * phase-2 panel kernel of the H-matrix path, or the direct kernel) while the
+ * solver runs, with a HIP event pair around each launch on the handle's stream (no
+ * reference counterpart).  dfn_kernel_timing(h, 1) clears the sums and starts,
+ * 0 stops; dfn_kernel_timing_read gives the mean duration in ms and the number of
+ * launches timed since the last start. */
+int dfn_kernel_timing(dfn_handle h, int32_t on);
+int dfn_kernel_timing_read(dfn_handle h, double *ms_mean, int64_t *count);
 /* Measurement helper: a plain device-to-device copy kernel over `bytes` bytes,
  * run `reps` times on a private stream; *gbps = 2*bytes / mean time (read + write).
  * Used to scale hardware traffic counters and to state the achievable HBM rate. */

@@ -189,6 +189,25 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("mode", [2, 3])
+def test_kernel_timing_covers_every_stress_evaluation(mode):
+    """dfn_kernel_timing times one launch per friction-law evaluation of the solver
+    (the H-matrix panel kernel) and nothing once stopped."""
+    p = _hnet(mode, n_faults=20, seed=3)
+    d = dev(p)
+    d.initiate_hmat()
+    s0 = d.stats()
+    d.kernel_timing(True)
+    d.solve(6)
+    ms, n = d.kernel_timing_read()
+    s1 = d.stats()
+    assert n == s1["rhs_evaluations"] - s0["rhs_evaluations"] >= 36
+    assert 0.0 < ms < 100.0
+    d.kernel_timing(False)
+    d.solve(2)
+    assert d.kernel_timing_read() == (0.0, 0)
+
+
 @pytest.mark.parametrize("builder", ["device", "host"])
 @pytest.mark.parametrize("mode", [2, 3])
 def test_device_built_operator_matches_oracle_construction(mode, builder, monkeypatch):
