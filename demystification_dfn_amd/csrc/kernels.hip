@@ -170,7 +170,8 @@ __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restric
 }
 
 // z of each multi-chunk low-rank block = sum of its chunk partials, in chunk
-// order; one thread per (block, column).
+// order; one thread per (block, column).  The partials are loaded eight at a
+// time before they are added, so their latencies overlap.
 __global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ zr, int nblocks,
                                                       const double *__restrict__ zpart, double *__restrict__ w) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,8 +179,16 @@ __global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ z
   if (b >= nblocks) return;
   const ZRed r = zr[b];
   if (j >= r.k) return;
+  const double *src = zpart + r.zp_first + j;
   double s = 0.0;
-  for (int t = 0; t < r.ntask; t++) s += zpart[r.zp_first + t * r.k + j];
+  for (int t0 = 0; t0 < r.ntask; t0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = t0 + u < r.ntask ? src[(int64_t)(t0 + u) * r.k] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (t0 + u < r.ntask) s += v[u];
+  }
   w[r.zout + j] = s;
 }
 
@@ -321,8 +330,16 @@ __global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ f
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * nops) return;
   const int op = g / n, i = g - op * n;
+  const int q0 = fin_first[g], q1 = fin_first[g + 1];
   double s = 0.0;
-  for (int q = fin_first[g]; q < fin_first[g + 1]; q++) s += ypart[fin_idx[q]];
+  for (int q = q0; q < q1; q += 4) {  // four partials in flight, added in order
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = q + u < q1 ? ypart[fin_idx[q + u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (q + u < q1) s += v[u];
+  }
   double *y = op == 0 ? yT : yN;
   y[perm[i]] = scale * s;
 }
