@@ -479,7 +479,12 @@ struct Engine {
   // and yscal set; x is the step-start abscissa.  On success Y holds the new state.
   // stages_ready: CG slots 1..5 already hold compute_P(x + A_i*htry) (launched with the
   // step's t_try solve)
-  int rkqs_dev(double x, double htry, double eps, double *hdid, double *hnext, bool stages_ready = false) {
+  // rkqs_dev's answer when its first attempt ran before compute_P(t_try) had been
+  // read (t0_unread) and that solve failed: the attempt is dropped
+  static constexpr int kT0Failed = -1;
+
+  int rkqs_dev(double x, double htry, double eps, double *hdid, double *hnext, bool stages_ready = false,
+               bool t0_unread = false) {
     const int N = hp.N, n3 = 3 * N;
     const double SAFETY = 0.9, PGROW = -0.2, PSHRNK = -0.25, ERRCON = (double)1.89e-4f;
     double h = htry, errmax = 0.0;
@@ -505,7 +510,13 @@ struct Engine {
       }
       DFN_HIP(hipMemcpyAsync(h_status, status.p, sizeof(StatusWord), hipMemcpyDeviceToHost, stream));
       int fail = 0;
-      DFN_TRY(cg_collect(slot_base + 1, 5, &fail));
+      if (t0_unread) {  // compute_P(t_try) is read with the stage solves, in one copy
+        t0_unread = false;
+        DFN_TRY(cg_collect(slot_base, 6, &fail));
+        if (h_cgres[2 * slot_base]) return kT0Failed;
+      } else {
+        DFN_TRY(cg_collect(slot_base + 1, 5, &fail));
+      }
       if (fail) return DFN_ERR_CG;
       double em;
       std::memcpy(&em, &h_status->errmax_bits, sizeof(double));
@@ -566,7 +577,25 @@ struct Engine {
       DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
     }
     slot_base = 0;
-    DFN_TRY(cg_collect(0, 1, &fail));
+    double t_did = 0, t_next = 0;
+    // After a step that did not halve, stage 1 and the first attempt are issued
+    // before compute_P(t_try)'s result is read (one host synchronisation less per
+    // step); if that solve failed, the attempt's results are dropped and the step
+    // halves as below, as the reference does before its k1 evaluation.
+    const bool optimistic = batch && !spec && !halved_last;
+    bool attempted = false;
+    if (optimistic) {
+      StageArgs s1 = stage_args(Y.as<double>(), 0, ytA.as<double>(), t_try);
+      launch_stage(1, s1, stream);
+      st.rhs_evaluations++;
+      st.cg_solves -= 5;  // as below: the stages count their solves when they use them
+      const int rc = rkqs_dev(0.0, t_try, hp.tol, &t_did, &t_next, true, true);
+      if (rc == DFN_OK) attempted = true;
+      else if (rc == kT0Failed) fail = 1;
+      else return rc;
+    } else {
+      DFN_TRY(cg_collect(0, 1, &fail));
+    }
     while (fail) {  // :86-90: halve t_try until compute_P converges
       t_try = t_try / 2.0;
       st.cg_halvings++;
@@ -579,21 +608,23 @@ struct Engine {
       DFN_TRY(launch_try(stream));
       DFN_TRY(cg_collect(0, 1, &fail));
     }
-    if (batch) st.cg_solves -= spec ? 11 : 5;  // the stages' solves are counted when the stages use them
     halved_last = st.cg_halvings > halvings0;
-    // k1 = f(t_try, Y) and yscal (:101-102): the stress rates are those evaluated
-    // above; compute_P(t_try) of the RHS equals the call above unless the
-    // permeability evolves between calls
-    if (hp.coupling == 1) {
-      DFN_TRY(cg_launch(&t_try, 1, 0));
-      DFN_TRY(cg_collect(0, 1, &fail));
-      if (fail) return DFN_ERR_CG;
+    if (!attempted) {
+      if (batch) st.cg_solves -= spec ? 11 : 5;  // the stages' solves are counted when used
+      // k1 = f(t_try, Y) and yscal (:101-102): the stress rates are those evaluated
+      // above (again if a dropped attempt overwrote them); compute_P(t_try) of the
+      // RHS equals the call above unless the permeability evolves between calls
+      if (optimistic) DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
+      if (hp.coupling == 1) {
+        DFN_TRY(cg_launch(&t_try, 1, 0));
+        DFN_TRY(cg_collect(0, 1, &fail));
+        if (fail) return DFN_ERR_CG;
+      }
+      StageArgs s1 = stage_args(Y.as<double>(), 0, ytA.as<double>(), t_try);
+      launch_stage(1, s1, stream);
+      st.rhs_evaluations++;
+      DFN_TRY(rkqs_dev(0.0, t_try, hp.tol, &t_did, &t_next, batch));  // :111
     }
-    StageArgs s1 = stage_args(Y.as<double>(), 0, ytA.as<double>(), t_try);
-    launch_stage(1, s1, stream);
-    st.rhs_evaluations++;
-    double t_did = 0, t_next = 0;
-    DFN_TRY(rkqs_dev(0.0, t_try, hp.tol, &t_did, &t_next, batch));  // :111
     t_try = t_next;  // :117-120
     if (t_try > 24 * 60 * 60) t_try = 24 * 60 * 60;
     // commit compute_P(t_did) (:174): the stage-5 solve used dt = 0 + 1.0*h = t_did

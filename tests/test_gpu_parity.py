@@ -191,8 +191,8 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
 
 @pytest.mark.parametrize("mode", [2, 3])
 def test_kernel_timing_covers_every_stress_evaluation(mode):
-    """dfn_kernel_timing times one launch per friction-law evaluation of the solver
-    (the H-matrix panel kernel) and nothing once stopped."""
+    """dfn_kernel_timing times one launch per stress evaluation of the solver (the
+    H-matrix panel kernel) and nothing once stopped."""
     p = _hnet(mode, n_faults=20, seed=3)
     d = dev(p)
     d.initiate_hmat()
@@ -201,7 +201,12 @@ def test_kernel_timing_covers_every_stress_evaluation(mode):
     d.solve(6)
     ms, n = d.kernel_timing_read()
     s1 = d.stats()
-    assert n == s1["rhs_evaluations"] - s0["rhs_evaluations"] >= 36
+    rhs = s1["rhs_evaluations"] - s0["rhs_evaluations"]
+    # a step whose first attempt ran before its compute_P(t_try) failed evaluates
+    # the stress of its state once more
+    assert n >= rhs >= 36
+    if s1["cg_halvings"] == s0["cg_halvings"]:
+        assert n == rhs
     assert 0.0 < ms < 100.0
     d.kernel_timing(False)
     d.solve(2)
