@@ -358,6 +358,19 @@ __global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ f
 // a row in a fixed order (results do not depend on scheduling).  The weights
 // are formed from the slip as the tiles load their nodes.
 // ---------------------------------------------------------------------------
+// a / b for b > 0 (a squared distance between an element centre and a node)
+// without the IEEE divide sequence: the hardware reciprocal estimate refined by two
+// Newton-Raphson steps, y <- y + y (1 - b y), then one product; within about one
+// ulp of the rounded quotient
+__device__ __forceinline__ double quot_pos(double a, double b) {
+  double y = __builtin_amdgcn_rcp(b);
+  double e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-b, y, 1.0);
+  y = fma(y, e, y);
+  return a * y;
+}
+
 // Mode III weight of node l: -(s_l - s_{l-1}) with s_{-1} = s_N = 0
 __device__ __forceinline__ double dislocation_weight3(const double *__restrict__ slip, int l, int n) {
   if (l == 0) return -slip[0];
@@ -388,7 +401,7 @@ __global__ void __launch_bounds__(256) direct3_tile_kernel(const double *__restr
   for (int q = 0; q < cnt; q++) {
     const double dx = ex - sx[q], dy = ey - sy[q];
     const double r2 = dx * dx + dy * dy;
-    s += sc[q] * ((t1 * dx + t2 * dy) / r2);
+    s += sc[q] * quot_pos(t1 * dx + t2 * dy, r2);
   }
   part[(int64_t)blockIdx.y * n + k] = s;
 }
