@@ -143,3 +143,27 @@ def test_pi_and_learned_controllers_track_the_reference_controller():
             e = err(key, i, p)
             print(f"[gpu] net {i} {key}: errors {e} (rkqs {e_ref})", flush=True)
             assert np.all(e <= 10 * e_ref + 1e-6), (key, i, e, e_ref)
+
+
+def test_bench_ensemble_sampled_networks_equal_independent_runs():
+    """The bench's ensemble workload at its size (64 seeded networks, 123k elements,
+    FP64 with graphs): after 20 ensemble steps, eight networks spread over the size
+    range are bit-identical to their own single-network runs."""
+    from demystification_dfn_amd.device import DeviceEnsemble, DeviceSimulation
+    probs = problems.network_ensemble(64, faults=(5, 36), box=2500.0, seed=2024)
+    print(f"[gpu] bench ensemble: {len(probs)} networks, {sum(p.N for p in probs)} elements", flush=True)
+    ens = DeviceEnsemble(probs)
+    ens.initiate_hmat()
+    ens.solve(20)
+    order = np.argsort([p.N for p in probs])
+    for i in order[::9]:
+        p = probs[i]
+        d = DeviceSimulation(p)
+        d.initiate_hmat()
+        sd = d.solve(20)
+        Yd, Pd, _, td = d.get_state()
+        Ye, Pe, _, te = ens.get_state(int(i))
+        assert ens.stats(int(i))["accepted_steps"] == sd["accepted_steps"] == 20
+        assert te == td
+        np.testing.assert_array_equal(Ye, Yd)
+        np.testing.assert_array_equal(Pe, Pd)
