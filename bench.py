@@ -26,7 +26,7 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 # HBM bytes per launch of the dominant kernel, from two rocprofv3 PMC runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a 1 GiB copy kernel;
 # summary and commands in profiles/pmc_traffic_network80k_fp64.md.
-MEASURED_TRAFFIC = {("network800_80k", 2024): 2.6264e9}
+MEASURED_TRAFFIC = {("network800_80k", 2024): 2.4947e9}
 YEAR = 365.25 * 86400.0
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
