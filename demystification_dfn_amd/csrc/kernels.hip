@@ -195,9 +195,10 @@ __global__ void __launch_bounds__(256) zreduce_kernel(const ZRed *__restrict__ z
 typedef double dv2 __attribute__((ext_vector_type(2)));
 
 // Phase 2: one workgroup per panel task.  w entries of the panel's columns are
-// gathered into LDS once; each wave then streams two whole rows at a time
-// (row-major panel, rows padded to a multiple of 2 doubles / 4 floats so every
-// lane issues one 16-byte load per row: 1 KiB per wave instruction).
+// gathered into LDS once; each wave then streams R whole rows at a time
+// (row-major panel, rows padded with zero entries to a multiple of 128 bytes, so
+// every row starts on a cache line and every lane issues one 16-byte load per row:
+// 1 KiB per wave instruction).
 template <int R>
 __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict__ tasks, const double *__restrict__ pdata,
                                                         const int *__restrict__ widx, const double *__restrict__ w,
