@@ -142,10 +142,11 @@ struct PermArgs {
 };
 
 // launchers (kernels.hip)
-void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s);
-// waves: tasks (waves) per workgroup, 2, 4 or 8
+// xf: FP32 copy of x_h for FP32 operators (n + 4 floats), or nullptr
+void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s, float *xf = nullptr);
+// waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (FP32 operators)
 void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
-                int waves = 4);
+                int waves = 4, const float *xf = nullptr);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
 // rows: panel rows per wave step, 2 or 4
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,

@@ -274,8 +274,9 @@ struct Engine {
     const DevOps *d = active_ops();
     if (!d) return DFN_ERR_STATE;
     double *w = d->w.as<double>();
-    launch_gather(d_V, d->perm.as<int>(), w, N, stream);
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves);
+    launch_gather(d_V, d->perm.as<int>(), w, N, stream, d->xf.as<float>());
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves,
+               d->xf.as<float>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     if (kt) DFN_HIP(hipEventRecord(kt[0], stream));
     launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
@@ -1108,9 +1109,9 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
   double *dT = e.scratch3b.as<double>(), *dN = dT + N;
   DFN_HIP(hipMemcpyAsync(dV, xo.data(), N * sizeof(double), hipMemcpyHostToDevice, e.stream));
   double *w = d->w.as<double>();
-  dfn::launch_gather(dV, d->perm.as<int>(), w, N, e.stream);
+  dfn::launch_gather(dV, d->perm.as<int>(), w, N, e.stream, d->xf.as<float>());
   dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), e.stream,
-                  d->vtx_waves);
+                  d->vtx_waves, d->xf.as<float>());
   dfn::launch_zreduce(d->zred.as<dfn::ZRed>(), d->nzr, d->zpart.as<double>(), w, e.stream);
   dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(),
                     d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream, d->panel_rows);

@@ -76,6 +76,7 @@ struct DevOps {
   int n = 0, nops = 0;
   bool fp32 = false;
   DMem perm, vtasks, vdata, zred, zpart, ptasks, pdata, widx, cscale, w, ypart, fin_first, fin_idx;
+  DMem xf;  // FP32 operators: FP32 copy of x_h read by phase 1 (n + 4 floats, the tail zero)
   int nvt = 0, nzr = 0, npt = 0;
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
@@ -128,7 +129,8 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   auto measure = [&](int waves, int rows, double &ms) -> int {
     for (int rep = 0; rep < 10; rep++) {
       if (rep == 2) DFN_HIP(hipEventRecord(e0, nullptr));
-      launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.fp32, w, d.zpart.as<double>(), nullptr, waves);
+      launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.fp32, w, d.zpart.as<double>(), nullptr, waves,
+                 d.xf.as<float>());
       launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, nullptr);
       launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.fp32, d.widx.as<int>(), d.cscale.as<float>(), w,
                    d.ypart.as<double>(), nullptr, rows);
@@ -435,6 +437,10 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
   DFN_TRY(d.widx.upload(widx));
   DFN_TRY(d.cscale.upload(cscale));
   DFN_TRY(d.w.alloc(sizeof(double) * wsize));
+  if (d.fp32) {
+    DFN_TRY(d.xf.alloc(sizeof(float) * ((size_t)n + 4)));
+    DFN_HIP(hipMemset(d.xf.p, 0, sizeof(float) * ((size_t)n + 4)));
+  }
   DFN_TRY(d.ypart.alloc(sizeof(double) * std::max(yp, 1)));
   DFN_TRY(d.fin_first.upload(fin_first));
   DFN_TRY(d.fin_idx.upload(fin_idx));

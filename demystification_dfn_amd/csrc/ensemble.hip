@@ -431,8 +431,9 @@ struct EnsEngine {
   void apply(const double *dV) {
     const DevOps *d = active_ops();
     double *w = d->w.as<double>();
-    launch_gather(dV, d->perm.as<int>(), w, Ntot, stream);
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves);
+    launch_gather(dV, d->perm.as<int>(), w, Ntot, stream, d->xf.as<float>());
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves,
+               d->xf.as<float>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
                  d->ypart.as<double>(), stream, d->panel_rows);

@@ -89,10 +89,17 @@ __device__ __forceinline__ float wave_sum_full(float sum) {
 // ---------------------------------------------------------------------------
 // H-matvec
 // ---------------------------------------------------------------------------
+// x_h = x in H order; with FP32 operators also its FP32 copy, which phase 1 reads
+// (the FP32 body rounds each x_h entry to FP32 before use, so the copy holds the
+// same values and is read with one 16-byte load per lane instead of four 8-byte ones)
 __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restrict__ x, const int *__restrict__ perm,
-                                                          double *__restrict__ xh, int n) {
+                                                          double *__restrict__ xh, float *__restrict__ xf, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) xh[i] = x[perm[i]];
+  if (i < n) {
+    const double v = x[perm[i]];
+    xh[i] = v;
+    if (xf) xf[i] = (float)v;
+  }
 }
 
 // Phase 1: one wave per task.  The body is specialised on the rank K (a
@@ -104,7 +111,8 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 // K loads of one step read one contiguous run.
 template <int K, typename TV>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
-                                         double *__restrict__ zpart, double *__restrict__ wz, int lane) {
+                                         const float *__restrict__ xf, double *__restrict__ zpart,
+                                         double *__restrict__ wz, int lane) {
   constexpr int kVec = 16 / (int)sizeof(TV);
   typedef TV tvec __attribute__((ext_vector_type(kVec)));
   // FP64 storage accumulates in FP64; FP32 storage in FP32 (TV)
@@ -121,8 +129,16 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
     if (i >= task.len) break;
     const TV *Vs = V + (int64_t)s0 * K + lane * kVec;  // the strips before hold kStrip rows each
     TV xi[kVec];
+    if constexpr (sizeof(TV) == 4) {
+      // four consecutive entries of the FP32 copy (4-byte aligned; the copy is
+      // padded past n); past the chunk's rows they meet the zero rows of V
+      float4 x4;
+      __builtin_memcpy(&x4, xf + task.xcol + task.row0 + i, sizeof(x4));
+      xi[0] = x4.x; xi[1] = x4.y; xi[2] = x4.z; xi[3] = x4.w;
+    } else {
 #pragma unroll
-    for (int u = 0; u < kVec; u++) xi[u] = i + u < task.len ? (TV)x[i + u] : (TV)0;  // padding rows hold zeros
+      for (int u = 0; u < kVec; u++) xi[u] = i + u < task.len ? (TV)x[i + u] : (TV)0;  // padding rows hold zeros
+    }
     tvec v[K];
 #pragma unroll
     for (int j = 0; j < K; j++) v[j] = *reinterpret_cast<const tvec *>(Vs + j * hp);
@@ -148,7 +164,7 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
 template <typename TV>
 __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
                                                           const TV *__restrict__ vdata, double *__restrict__ w,
-                                                          double *__restrict__ zpart) {
+                                                          const float *__restrict__ xf, double *__restrict__ zpart) {
   // one task per wave, blockDim.x / 64 waves per workgroup; the task index is
   // uniform across the wave: scalar loads of the task and a scalar branch on its rank
   const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
@@ -157,7 +173,7 @@ __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restric
   const VTask task = tasks[t];
   switch (task.k) {
 #define DFN_VTX_CASE(K) \
-  case K: vtx_body<K, TV>(task, vdata, w, zpart, w, lane); break;
+  case K: vtx_body<K, TV>(task, vdata, w, xf, zpart, w, lane); break;
     DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
     DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
     DFN_VTX_CASE(13) DFN_VTX_CASE(14) DFN_VTX_CASE(15) DFN_VTX_CASE(16) DFN_VTX_CASE(17) DFN_VTX_CASE(18)
@@ -1555,20 +1571,20 @@ __global__ void __launch_bounds__(256) perm_x_kernel(const double *__restrict__ 
 // ---------------------------------------------------------------------------
 static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
-void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s) {
-  hipLaunchKernelGGL(gather_perm_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, x, perm, xh, n);
+void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s, float *xf) {
+  hipLaunchKernelGGL(gather_perm_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, x, perm, xh, xf, n);
 }
 
 void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
-                int waves) {
+                int waves, const float *xf) {
   if (nt <= 0) return;
   if (waves != 2 && waves != 8) waves = 4;
   if (fp32)
     hipLaunchKernelGGL(lowrank_vtx_kernel<float>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
-                       (const float *)vdata, w, zpart);
+                       (const float *)vdata, w, xf, zpart);
   else
     hipLaunchKernelGGL(lowrank_vtx_kernel<double>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
-                       (const double *)vdata, w, zpart);
+                       (const double *)vdata, w, xf, zpart);
 }
 
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s) {
