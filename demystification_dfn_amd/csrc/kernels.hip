@@ -849,15 +849,22 @@ __device__ __forceinline__ bool cg_view(CgArgs &A, CrossRows &X, const CgSys &S)
 }
 
 // M v for system s: rows (A1 v_{k-1} + A2 v_k + A3 v_{k+1}) / diag_k plus the crossing terms.
-__device__ void cg_matvec(const CgArgs &A, const CgSys &S, double dt, const double *v, double *out) {
+// out = (D^-1 M) v; with dot != nullptr and no crossing terms also this thread's
+// share of v.out, summed in the order of the caller's own loop over k
+__device__ void cg_matvec(const CgArgs &A, const CgSys &S, double dt, const double *v, double *out,
+                          double *dot = nullptr) {
   const int n = A.n;
+  double part = 0.0;
   for (int k = threadIdx.x; k < n; k += blockDim.x) {
     double r;
     if (k == 0) r = S.A2[0] * v[0] + S.A3[0] * v[1];
     else if (k == n - 1) r = S.A1[k] * v[k - 1] + S.A2[k] * v[k];
     else r = S.A1[k] * v[k - 1] + S.A2[k] * v[k] + S.A3[k] * v[k + 1];
-    out[k] = r / S.diag[k];
+    const double o = r / S.diag[k];
+    out[k] = o;
+    part += v[k] * o;
   }
+  if (dot && A.ncross == 0) *dot = part;
   if (A.ncross > 0) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -964,9 +971,13 @@ __global__ void __launch_bounds__(1024) cg_kernel(CgArgs A) {
   int it = 0, ier = 0;
   while (errmax >= 1.0) {
     it++;
-    cg_matvec(A, S, dt, S.p, S.Ap);
+    // p.Ap in the same pass as A p when no crossing term changes A p afterwards
     double part = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) part += S.p[k] * S.Ap[k];
+    cg_matvec(A, S, dt, S.p, S.Ap, &part);
+    if (A.ncross > 0) {
+      part = 0.0;
+      for (int k = threadIdx.x; k < n; k += blockDim.x) part += S.p[k] * S.Ap[k];
+    }
     const double pAp = block_sum(part, red);
     const double alpha = rr / pAp;
     double p1 = 0.0, em = 0.0;
