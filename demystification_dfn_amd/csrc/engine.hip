@@ -562,21 +562,16 @@ struct Engine {
       }
       return cg_launch_on(dts, spec ? 12 : 6, 0, cs);
     };
-    // With the one-workgroup-per-system CG (six workgroups) the first batch runs
-    // on the side stream beside the k1 stress evaluation, which needs only Y; the
-    // cooperative CG spans the device and slows a concurrent matvec, so it stays
-    // on the main stream.
-    if (N <= kCgLdsMax && cg_path <= 1) {
-      DFN_HIP(hipEventRecord(ev_fork, stream));
-      DFN_HIP(hipStreamWaitEvent(cg_stream, ev_fork, 0));
-      DFN_TRY(launch_try(cg_stream));
-      DFN_HIP(hipEventRecord(ev_join, cg_stream));
-      DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
-      DFN_HIP(hipStreamWaitEvent(stream, ev_join, 0));
-    } else {
-      DFN_TRY(launch_try(stream));
-      DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
-    }
+    // The first batch of pressure solves runs on the side stream beside the k1
+    // stress evaluation, which needs only Y (the cooperative CG too: its launch
+    // keeps its workgroups co-resident, and the latency-bound solve then hides
+    // behind the bandwidth-bound matvec: 4.31 -> 4.21 ms per step at 80k).
+    DFN_HIP(hipEventRecord(ev_fork, stream));
+    DFN_HIP(hipStreamWaitEvent(cg_stream, ev_fork, 0));
+    DFN_TRY(launch_try(cg_stream));
+    DFN_HIP(hipEventRecord(ev_join, cg_stream));
+    DFN_TRY(apply_kernel(Y.as<double>(), sT.as<double>(), sN.as<double>()));
+    DFN_HIP(hipStreamWaitEvent(stream, ev_join, 0));
     slot_base = 0;
     double t_did = 0, t_next = 0;
     // After a step that did not halve, stage 1 and the first attempt are issued
