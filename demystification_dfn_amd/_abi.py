@@ -102,6 +102,8 @@ SIGNATURES = {
     "get_state": [_H, _dp, _dp, _dp, C.POINTER(C.c_double)],
     "get_stats": [_H, C.POINTER(DfnRunStats)],
     "kernel": [_H, _dp, _dp, _dp],
+    # device pointers slip, stressT, stressN and a hipStream_t (NULL: default stream)
+    "kernel_noalloc": [_H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "hgemv": [_H, C.c_int32, C.c_double, C.c_double, _dp, _dp],
     "friction_law": [_H, C.c_double, _dp, _dp],
     "compute_P": [_H, C.c_double, _dp, _dp, _i32p],

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -19,6 +20,34 @@
 #include "engine_common.hpp"
 
 namespace dfn {
+
+// Scratch of dfn_kernel_noalloc: the per-tile partial sums of the dense direct
+// kernels, direct_chunks(N) * N doubles per operator (two operators in Mode II).
+// 16 MiB in every GPU the library runs on, held until the process ends and
+// shared by every call on that GPU.  Host code never uses the array's name as an
+// address: each device's copy is reached through the pointer hipGetSymbolAddress
+// returned with that device current.
+constexpr size_t kNoAllocDoubles = (size_t)1 << 21;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kNoAllocMaxDevices];
+
+// This GPU's copy of `scratch`, resolved once per device with that device current.
+static int noalloc_scratch(double **out) {
+  int dev = 0;
+  DFN_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return DFN_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    DFN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return DFN_ERR_HIP;
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return DFN_OK;
+}
 
 struct Engine {
   HostProblem hp;
@@ -146,8 +175,11 @@ struct Engine {
     DFN_TRY(src_tb.upload(hp.src_tb));
     DFN_TRY(src_te.upload(hp.src_te));
     DFN_TRY(cross.upload(hp.cross));
-    if (hp.kernel == DFN_KERNEL_DIRECT)
+    if (hp.kernel == DFN_KERNEL_DIRECT) {
       DFN_TRY(dpart.alloc(sizeof(double) * (size_t)direct_chunks(N) * N * (hp.mode == DFN_MODE_II ? 2 : 1)));
+      double *unused = nullptr;  // dfn_kernel_noalloc's scratch on this device
+      DFN_TRY(noalloc_scratch(&unused));
+    }
     // permeability_x (initialisation.f90:252-266): perm_x[k] = hmean(perm[k-1], perm[k]), k = 1..N-1
     std::vector<double> px(N + 1, 0.0);
     for (int kk = 1; kk < N; kk++)
@@ -1084,6 +1116,35 @@ API int dfn_kernel(dfn_handle h, const double *slip, double *sTh, double *sNh) {
   if (e.hp.mode == DFN_MODE_II)
     DFN_HIP(hipMemcpyAsync(sNh, e.sN.p, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
   DFN_HIP(hipStreamSynchronize(e.stream));
+  return DFN_OK;
+  DFN_API_END
+}
+
+API int dfn_kernel_noalloc(dfn_handle h, const double *slip, double *sT, double *sN, void *stream) {
+  DFN_API_BEGIN
+  if (!h || !slip || !sT) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  if (e.hp.kernel != DFN_KERNEL_DIRECT) return DFN_ERR_UNSUPPORTED;
+  const int N = e.hp.N;
+  const bool mode2 = e.hp.mode == DFN_MODE_II;
+  if (mode2 && !sN) return DFN_ERR_ARG;
+  const size_t need = (size_t)dfn::direct_chunks(N) * (size_t)N * (mode2 ? 2 : 1);
+  if (need > dfn::kNoAllocDoubles) {
+    std::fprintf(stderr,
+                 "dfn_kernel_noalloc: N = %d needs %zu scratch doubles, more than the %zu it holds per GPU "
+                 "(use dfn_kernel)\n",
+                 N, need, dfn::kNoAllocDoubles);
+    return DFN_ERR_ARG;
+  }
+  double *part = nullptr;
+  DFN_TRY(dfn::noalloc_scratch(&part));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (mode2)
+    dfn::launch_direct2(e.node.as<double>(), e.elem.as<double>(), e.nrm.as<double>(), slip, part, N,
+                        e.kernel_scale(), sT, sN, s);
+  else
+    dfn::launch_direct3(e.node.as<double>(), slip, part, N, e.kernel_scale(), sT, s);
+  DFN_HIP(hipGetLastError());
   return DFN_OK;
   DFN_API_END
 }

@@ -139,6 +139,19 @@ int dfn_get_stats(dfn_handle h, dfn_run_stats *stats);
  * stressN may be NULL in Mode III (the reference leaves it untouched there). */
 int dfn_kernel(dfn_handle h, const double *slip, double *stressT, double *stressN);
 
+/* dfn_kernel without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops); no counterpart in
+ * the reference.  Dense direct kernel handles only (DFN_KERNEL_DIRECT; an H-matrix
+ * handle gives DFN_ERR_UNSUPPORTED).  slip, stressT and stressN (Mode II; may be
+ * NULL in Mode III) are DEVICE pointers of N doubles; the kernels are queued on
+ * `stream` (a hipStream_t; NULL = the default stream) and the call returns without
+ * waiting for them.  The per-tile partial sums live in a fixed 16 MiB device array
+ * per GPU (ceil((N+1)/64) * N doubles per operator must fit in 2^21 doubles:
+ * N <= 11 583 in Mode III, N <= 8 191 in Mode II; larger N gives DFN_ERR_ARG and a
+ * message on stderr).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads, other handles) race. */
+int dfn_kernel_noalloc(dfn_handle h, const double *slip, double *stressT, double *stressN, void *stream);
+
 /* y <- alpha*H*x + beta*y for operator op in H order (Hgemv_right,
  * src/hmatrix/calculation_hmat.f90:3-165), x and y of length N. */
 int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const double *x, double *y);
