@@ -586,12 +586,23 @@ static int compression_low_rank(int m, int n, int inrank, double *vA, double *vB
 /* compute_full_ACA, src/hmatrix/ACA_algorithm.f90:665-826, on block rows
  * r0..r0+m-1, cols c0..c0+n-1 (0-based H order).  A (m x max_rank) and
  * B (n x max_rank) must be zeroed by the caller.  Returns low_rank. */
+static int full_aca_on(double *R, int m, int n, double eps, int max_rank, double *A, double *B);
+
 static int full_aca(const hgeom *g, eval_fn ev, int r0, int m, int c0, int n, double eps,
                     int max_rank, double *A, double *B) {
   size_t mn = (size_t)m * n;
   double *R = (double *)malloc(sizeof(double) * mn);
   for (int i = 0; i < m; i++) /* :90-95 */
     for (int j = 0; j < n; j++) R[i + (size_t)j * m] = ev(g, r0 + i, c0 + j);
+  int low_rank = full_aca_on(R, m, n, eps, max_rank, A, B);
+  free(R);
+  return low_rank;
+}
+
+/* The full-pivot iteration of compute_full_ACA (:96-166) on the assembled block R
+ * (m x n, column-major, overwritten by the residual). */
+static int full_aca_on(double *R, int m, int n, double eps, int max_rank, double *A, double *B) {
+  size_t mn = (size_t)m * n;
   double nA = 0.0; /* ZLANGE('F') */
   for (size_t t = 0; t < mn; t++) nA = nA + R[t] * R[t];
   nA = sqrt(nA);
@@ -623,7 +634,6 @@ static int full_aca(const hgeom *g, eval_fn ev, int r0, int m, int c0, int n, do
     if (ind_rec + 1 > max_rank) { low_rank = ind_rec; break; } /* memory check :153-159 */
   }
   if (low_rank == 0) low_rank = ind_rec; /* :162-166 */
-  free(R);
   return low_rank;
 }
 
@@ -1574,6 +1584,27 @@ API int dfno_hmat_tree(dfn_handle hh, int32_t op, int32_t *mat_corr, int32_t *nb
       bltree[3 * b] = H->bl_type[b]; bltree[3 * b + 1] = H->bl_row[b]; bltree[3 * b + 2] = H->bl_col[b];
       for (int j = 0; j < 6; j++) blinfo[6 * b + j] = H->info[6 * b + j];
     }
+  return DFN_OK;
+}
+
+/* compression_low_rank (H_matrix_recompression.f90:145-301) on caller factors:
+ * A (m x k) and B (n x k), column-major, are overwritten by the recompressed
+ * factors; *rank = the rank kept. */
+API int dfno_debug_compress(int32_t m, int32_t n, int32_t k, double *A, double *B, double eps, int32_t *rank) {
+  if (m < 1 || n < 1 || k < 1 || !A || !B || !rank) return DFN_ERR_ARG;
+  *rank = compression_low_rank(m, n, k, A, B, eps);
+  return DFN_OK;
+}
+
+/* The full-pivot ACA of compute_full_ACA (ACA_algorithm.f90:665-826) on a caller
+ * block M (m x n, column-major, overwritten by the residual); A (m x max_rank)
+ * and B (n x max_rank) receive the factors; *rank = low_rank. */
+API int dfno_debug_full_aca(int32_t m, int32_t n, double *M, double eps, int32_t max_rank, double *A, double *B,
+                            int32_t *rank) {
+  if (m < 1 || n < 1 || max_rank < 1 || !M || !A || !B || !rank) return DFN_ERR_ARG;
+  memset(A, 0, sizeof(double) * (size_t)m * max_rank);
+  memset(B, 0, sizeof(double) * (size_t)n * max_rank);
+  *rank = full_aca_on(M, m, n, eps, max_rank, A, B);
   return DFN_OK;
 }
 

@@ -248,3 +248,32 @@ class RefSimulation:
         assert nbc.value <= cap_cl and nbb.value <= cap_bl
         return dict(mat_corr=mc, ind_clusters=ind[:2 * nbc.value], bltree=blt[:3 * nbb.value].reshape(-1, 3),
                     info=inf[:6 * nbb.value].reshape(-1, 6))
+
+
+def compress(A, B, eps):
+    """compression_low_rank on factors A (m x k), B (n x k): (A', B', rank)."""
+    load()
+    fn = _lib.dfno_debug_compress
+    fn.argtypes = [C.c_int32, C.c_int32, C.c_int32, _dp, _dp, C.c_double, _i32p]
+    m, k = A.shape
+    n = B.shape[0]
+    a = np.asfortranarray(A, np.float64).copy(order="F")
+    b = np.asfortranarray(B, np.float64).copy(order="F")
+    r = C.c_int32()
+    assert fn(m, n, k, a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), float(eps), C.byref(r)) == 0
+    return a[:, :r.value], b[:, :r.value], r.value
+
+
+def full_aca(M, eps, max_rank):
+    """The full-pivot ACA of compute_full_ACA on the block M: (A, B, rank)."""
+    load()
+    fn = _lib.dfno_debug_full_aca
+    fn.argtypes = [C.c_int32, C.c_int32, _dp, C.c_double, C.c_int32, _dp, _dp, _i32p]
+    m, n = M.shape
+    R = np.asfortranarray(M, np.float64).copy(order="F")
+    A = np.zeros((m, max_rank), order="F")
+    B = np.zeros((n, max_rank), order="F")
+    r = C.c_int32()
+    assert fn(m, n, R.ctypes.data_as(_dp), float(eps), int(max_rank), A.ctypes.data_as(_dp), B.ctypes.data_as(_dp),
+              C.byref(r)) == 0
+    return A[:, :r.value], B[:, :r.value], r.value

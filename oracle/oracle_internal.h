@@ -4,7 +4,25 @@
  * cpu_baseline leg of bench.py as the checker.  Nothing in the product links,
  * loads or calls it.
  *
- * Parity status per function (what pins it) is listed in oracle/README.md.
+ * What pins each restated function (tests/ file in brackets):
+ *   directIII / directII, eval_kernel_ModeIII / _ModeII_T / _ModeII_N, both
+ *     friction laws, compute_P (crossings, both quirks, permeability_coupling = 1),
+ *     rkqs / rkck, the solver loop, cluster and block cluster trees: bit-equal to
+ *     the reference's own Fortran built by Makefile.ref, on committed fixtures and
+ *     live [test_oracle_golden.py];
+ *   geometry and loading projections (initialisation.f90:139-206): Mohr's-circle
+ *     closed forms on faults of every strike, all five stress rates
+ *     [test_oracle_setup_pins.py];
+ *   find_fault_crossing (initialisation.f90:279-344): hand-built junctions and
+ *     near misses at the single-precision threshold, brute force over node pairs,
+ *     the generator's crossing count [test_oracle_setup_pins.py];
+ *   full-pivot ACA and recompression rank rules: numpy SVD of factors with a known
+ *     spectrum and of every built block [test_oracle_setup_pins.py]; the operator
+ *     against the dense kernel to 20 eps_ACA [test_oracle_closed_forms.py];
+ *   Hgemv_right: dense reconstruction of the block lists [test_oracle_closed_forms.py].
+ * Parity unpinned: the loop order of the reference BLAS ZGEMV inside Hgemv_right
+ * and the LAPACK QR/SVD inside the recompression (neither library is in the
+ * image; only their products are pinned, as above).
  */
 #ifndef DFN_ORACLE_INTERNAL_H
 #define DFN_ORACLE_INTERNAL_H

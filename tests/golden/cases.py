@@ -33,6 +33,27 @@ def crossing_network():
                             sources=[(2, 3, 1e-5, 0.0, 1e5)])
 
 
+def coupling_network():
+    """The crossing network with permeability_coupling = 1 (src/fluid_diffusion.f90:68-70,
+    src/initialisation.f90:252-266): each compute_P sets the permeability from the
+    effective normal stress, k = (k* - kmin) exp(-|sigmaN + P| / Snk) + kmin, with
+    Snk = 20 MPa and kmin a tenth of the initial permeability on the real elements."""
+    p = crossing_network()
+    p.name = "golden_coupling_network"
+    p.permeability_coupling = 1
+    real = np.ones(p.N, bool)
+    real[p.fake_indices()] = False
+    p.arrays["kmin"] = np.where(real, 0.1 * p.arrays["permeability"], 0.0)
+    p.arrays["Snk"] = np.where(real, 20e6, 1.0)
+    return p
+
+
+def mode3_network():
+    """The 20-fault seeded network of tree_network in Mode III (kernel entries of
+    eval_kernel_ModeIII on faults of every strike)."""
+    return problems.network_small(n_faults=20, seed=3, mode=3, name="network_small_20_seed3_mode3")
+
+
 def tree_network():
     """Seeded 20-fault network for the cluster / block cluster trees (leaf 16, eta 2)."""
     q = problems.network_small(n_faults=20, seed=3)
@@ -112,3 +133,53 @@ def connected_network_spans(static_kernel=0):
     return problems.fault_network(n_faults=120, box=2000.0, lmin=100.0, lmax=800.0, alpha=0.5, seed=3, eta=2.0,
                                   eps_aca=1e-6, nleaf=32, tol=1e-8, snap=True, static_kernel=static_kernel,
                                   name="connected_network_120")
+
+
+def _strike_segment(theta, x0, y0, length, n):
+    t = np.linspace(0.0, 1.0, n + 1)
+    return np.stack([x0 + length * np.cos(theta) * t, y0 + length * np.sin(theta) * t], 1)
+
+
+def _faults_problem(segs, mode, name, seed=0, **kw):
+    rng = np.random.default_rng(seed)
+    faults = []
+    for nd in segs:
+        n = nd.shape[0] - 1
+        one = np.ones(n)
+        faults.append(dict(node=nd, a=0.01 * one, b=0.012 * one, Dc=1e-3 * one, f0=0.6 * one, V0=1e-6 * one,
+                           sigmaN=-30e6 * (1.0 + 0.1 * rng.random(n)), V=1e-12 * one, theta=np.log(1e6) * one,
+                           P=0 * one, permeability=1e-15 * one, Snk=one, Lk=one, Tk=one,
+                           shear_loading_dot=1e-2 * rng.standard_normal(n),
+                           normal_loading_dot=1e-2 * rng.standard_normal(n)))
+    nels, node, arr = problems.concatenate_faults(faults)
+    return problems.Problem(name, mode, 0, nels, node, arr, **kw)
+
+
+STRIKE_FAN_RATES = dict(Sigma11_dot=0.31, Sigma12_dot=-0.17, Sigma22_dot=0.05, Sigma31_dot=0.23, Sigma32_dot=-0.41)
+
+
+def strike_fan(mode):
+    """Seven straight faults of strikes 0 to 3.1 rad (fake elements between them at
+    other strikes) under all five uniform stress rates, per-element loading rates
+    and normal stresses: the loading projections of src/initialisation.f90:159-200."""
+    thetas = [0.0, 0.3, 1.1, np.pi / 2, 2.0, 2.9, 3.1]
+    segs = [_strike_segment(th, 100.0 * i, 37.0 * i, 200.0 + 13.0 * i, 8 + i) for i, th in enumerate(thetas)]
+    return _faults_problem(segs, mode, f"strike_fan_mode{mode}", **STRIKE_FAN_RATES)
+
+
+def junction_network():
+    """Five Mode II faults of 10 m elements with a T junction (fault 2 starts on
+    node 6 of fault 1), an X crossing (node 2 of fault 5 on node 5 of fault 2) and
+    two near misses of node 11 and node 3 of fault 1 at 0.099999999 m and
+    0.099999997 m, either side of the reference's threshold 0.01f * min ds
+    (src/initialisation.f90:299): three crossings.  A 10 MPa pulse on element 5,
+    beside the junction."""
+    f1 = _strike_segment(0.0, 0.0, 0.0, 100.0, 10)
+    f2 = _strike_segment(np.pi / 2, 50.0, 0.0, 80.0, 8)
+    f3 = np.stack([np.full(7, 100.0), -0.099999999 - 10.0 * np.arange(7)], 1)
+    f4 = np.stack([np.full(6, 20.0), -0.099999997 - 10.0 * np.arange(6)], 1)
+    f5 = _strike_segment(0.0, 40.0, 40.0, 20.0, 2)
+    p = _faults_problem([f1, f2, f3, f4, f5], 2, "junction_network", permeability_coupling=0)
+    p.arrays["P"][4] = 10e6
+    p.arrays["permeability"][:] = np.where(p.arrays["permeability"] > 0, 1e-13, 0.0)
+    return p

@@ -3,7 +3,9 @@
 Run in a container where /root/reference exists and oracle/_ref/libdfn_ref.so has
 been built (make -C oracle -f Makefile.ref).  Produces tests/golden/*.npz: inputs
 and the reference's outputs for the kernel, the friction law, compute_P, rkqs,
-20-30 solver steps and the cluster / block cluster trees.
+20-30 solver steps, the cluster / block cluster trees, the kernel-entry functions
+of the H-matrix (eval_kernel_hmatrix.f90) and the permeability-coupled pressure
+solve (permeability_coupling = 1).
 """
 import os
 import sys
@@ -46,6 +48,30 @@ def run_case(prob, xs, dts, nsteps, seed):
     return out
 
 
+def kernel_entries(prob, seed, nsample):
+    """Entries of eval_kernel_ModeIII (Mode III) or eval_kernel_ModeII_T / _N
+    (Mode II) at the diagonal, both neighbour diagonals, 64 seeded entries in the
+    row and in the column of every fake element and nsample seeded (receiver,
+    source) pairs; the whole matrix when it has at most 2000 entries."""
+    r = O.RefSimulation(prob)
+    N = prob.N
+    if N * N <= 2000:
+        k, l = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+        k, l = k.ravel(), l.ravel()
+    else:
+        rng = np.random.default_rng(seed)
+        fake = prob.fake_indices()
+        ar = np.arange(N)
+        fk = np.repeat(fake, 64)
+        k = np.concatenate([ar, ar[:-1], ar[1:], fk, rng.integers(0, N, fk.size), rng.integers(0, N, nsample)])
+        l = np.concatenate([ar, ar[1:], ar[:-1], rng.integers(0, N, fk.size), fk, rng.integers(0, N, nsample)])
+    assert N < 32768
+    out = {"k": k.astype(np.int16), "l": l.astype(np.int16)}
+    for op in ((0,) if prob.fracture_mode == 3 else (0, 1)):
+        out[f"op{op}"] = r.eval_matrix(op)[k, l]
+    return out
+
+
 def main():
     assert O.RefSimulation.available(), "build oracle/_ref first (make -C oracle -f Makefile.ref)"
     p1 = cases.single_fault_small()
@@ -69,6 +95,19 @@ def main():
         out = cases.problem_arrays(p3)
         out.update(run_case(p3, [1.0, 1000.0, 20000.0], [10.0, 5000.0, 86400.0], 20, 2))
         np.savez_compressed(os.path.join(HERE, "network_rate_strengthening.npz"), **out)
+    ent = {}
+    for name, prob, seed in (("single_fault_300", p1, 3), ("crossing_network", p2, 4), ("network20", q, 5),
+                             ("network20_mode3", cases.mode3_network(), 6)):
+        for key, v in kernel_entries(prob, seed, 1500).items():
+            ent[f"{name}__{key}"] = v
+    if os.path.exists(cfg):
+        for key, v in kernel_entries(cases.reference_network(), 7, 1500).items():
+            ent[f"network_rate_strengthening__{key}"] = v
+    np.savez_compressed(os.path.join(HERE, "kernel_entries.npz"), **ent)
+    # permeability_coupling = 1: compute_P at three steps and 30 solver steps
+    pc = cases.coupling_network()
+    np.savez_compressed(os.path.join(HERE, "coupling_network.npz"),
+                        **run_case(pc, [1.0, 1000.0, 20000.0], [10.0, 5000.0, 86400.0], 30, 8))
     print("written")
 
 

@@ -431,3 +431,76 @@ def test_connected_network_spans_trajectory():
     for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
         assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)
     assert np.max(np.abs(Pd - Po)) <= 1e-9 * np.abs(Po).max() + 10 * p.tol_solver
+
+
+# ---------------------------------------------------------------------------
+# The device library's own setup (host code of dfn_create) against the oracle,
+# whose geometry, loading projections and crossings the CPU tests pin
+# (test_oracle_setup_pins.py).
+@pytest.mark.parametrize("mode", [2, 3])
+def test_setup_loading_projections_match_the_oracle(mode):
+    """Faults of every strike under all five stress rates: the uploaded state
+    (Mode III adds the projected stress rates to sigma_N) and the friction law,
+    which reads the projected loading rates, equal the oracle's."""
+    p = cases.strike_fan(mode)
+    d, o = dev(p), OracleSimulation(p)
+    Yd, Yo = d.get_state()[0], o.get_state()[0]
+    np.testing.assert_allclose(Yd, Yo, rtol=1e-15, atol=0)
+    for x in (1.0, 1000.0, 20000.0):
+        yd, yo = d.friction_law(x, Yo), o.friction_law(x, Yo)
+        assert close_elementwise(yd, yo, 1e-9, 1e-300 + np.abs(yo).max() * 1e-12), x
+
+
+@pytest.mark.parametrize("cg_path", CG_PATHS)
+def test_junction_crossings_pressure_and_trajectory(cg_path, monkeypatch):
+    """A T junction, an X crossing and a near miss on each side of the crossing
+    threshold: compute_P and 10 solver steps, which carry the crossing terms of
+    the pressure stencil, against the oracle."""
+    monkeypatch.setenv("DFN_CG_PATH", cg_path)
+    p = cases.junction_network()
+    d, o = dev(p), OracleSimulation(p)
+    for dt in (10.0, 5000.0, 86400.0):
+        a, ad, ea = d.compute_P(dt)
+        b, bd, eb = o.compute_P(dt)
+        assert ea == eb, dt
+        if ea == 0:
+            assert np.max(np.abs(a - b)) <= 1e-9 * np.abs(b).max() + 10 * p.tol_solver, dt
+    sd, so = d.solve(10), o.solve(10)
+    assert sd["accepted_steps"] == so["accepted_steps"] and sd["rejected_attempts"] == so["rejected_attempts"]
+    Yd, Yo = d.get_state()[0], o.get_state()[0]
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)
+
+
+@pytest.mark.parametrize("cg_path", CG_PATHS)
+def test_permeability_coupling_pressure_and_trajectory(cg_path, monkeypatch):
+    """permeability_coupling = 1 (src/fluid_diffusion.f90:68-70): every compute_P
+    first sets the permeability from the effective normal stress of the step-start
+    state; the pressures, the permeability the steps leave and 30 solver steps
+    against the oracle, which tests/golden/coupling_network.npz pins to the
+    reference build."""
+    monkeypatch.setenv("DFN_CG_PATH", cg_path)
+    p = cases.coupling_network()
+    d, o = dev(p), OracleSimulation(p)
+    for dt in (10.0, 5000.0, 86400.0):
+        a, ad, ea = d.compute_P(dt)
+        b, bd, eb = o.compute_P(dt)
+        assert ea == eb, dt
+        if ea == 0:
+            assert np.max(np.abs(a - b)) <= 1e-9 * np.abs(b).max() + 10 * p.tol_solver, dt
+    sd, so = d.solve(30), o.solve(30)
+    assert sd["accepted_steps"] == so["accepted_steps"] == 30
+    assert sd["rejected_attempts"] == so["rejected_attempts"]
+    assert sd["cg_halvings"] == so["cg_halvings"]
+    Yd, Pd, _, td = d.get_state()
+    Yo, Po, _, to = o.get_state()
+    assert abs(td - to) <= 1e-8 * to
+    N = p.N
+    real = np.ones(N, bool)
+    real[p.fake_indices()] = False
+    for sl in (slice(0, N), slice(N, 2 * N), slice(2 * N, 3 * N)):
+        assert close_elementwise(Yd[sl][real], Yo[sl][real], 1e-8, 1e-300)
+    assert np.max(np.abs(Pd - Po)) <= 1e-8 * np.abs(Po).max() + 10 * p.tol_solver

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import cases
-from oracle.oracle import OracleSimulation, RefSimulation, geometry, hmat_tree
+from oracle.oracle import OracleSimulation, RefSimulation, eval_matrix, geometry, hmat_tree
 
 
 def _load(golden_dir, name):
@@ -16,7 +16,8 @@ def _load(golden_dir, name):
 
 @pytest.mark.parametrize("fixture,make", [("single_fault_300.npz", cases.single_fault_small),
                                           ("crossing_network.npz", cases.crossing_network),
-                                          ("network_rate_strengthening.npz", cases.reference_network)])
+                                          ("network_rate_strengthening.npz", cases.reference_network),
+                                          ("coupling_network.npz", cases.coupling_network)])
 def test_oracle_matches_reference_fixtures(golden_dir, fixture, make):
     g = _load(golden_dir, fixture)
     prob = make()
@@ -75,3 +76,32 @@ def test_oracle_matches_live_reference_network():
     r.solve(25)
     for a, b in zip(o.get_state(), r.get_state()):
         np.testing.assert_array_equal(a, b)
+
+
+ENTRY_CASES = [("single_fault_300", cases.single_fault_small), ("crossing_network", cases.crossing_network),
+               ("network20", cases.tree_network), ("network20_mode3", cases.mode3_network),
+               ("network_rate_strengthening", cases.reference_network)]
+
+
+@pytest.mark.parametrize("name,make", ENTRY_CASES)
+def test_oracle_kernel_entries_match_reference_fixture(golden_dir, name, make):
+    """The H-matrix entry functions eval_kernel_ModeIII / _ModeII_T / _ModeII_N
+    (src/hmatrix/eval_kernel_hmatrix.f90:7-199) against the reference build's, bit
+    for bit, at the diagonal, the neighbour diagonals, the fake elements' rows and
+    columns and seeded pairs (faults of every strike, fake elements between them)."""
+    z = np.load(os.path.join(golden_dir, "kernel_entries.npz"))
+    prob = make()
+    o = OracleSimulation(prob)
+    k, l = z[f"{name}__k"].astype(np.int64), z[f"{name}__l"].astype(np.int64)
+    for op in ((0,) if prob.fracture_mode == 3 else (0, 1)):
+        np.testing.assert_array_equal(eval_matrix(o, op)[k, l], z[f"{name}__op{op}"])
+
+
+@pytest.mark.skipif(not RefSimulation.available(), reason="reference build oracle/_ref not present")
+@pytest.mark.parametrize("make", [cases.tree_network, cases.mode3_network])
+def test_oracle_kernel_matrices_match_live_reference(make):
+    """Whole kernel matrices of a 20-fault network against the reference build."""
+    prob = make()
+    o, r = OracleSimulation(prob), RefSimulation(prob)
+    for op in ((0,) if prob.fracture_mode == 3 else (0, 1)):
+        np.testing.assert_array_equal(eval_matrix(o, op), r.eval_matrix(op))
