@@ -17,6 +17,7 @@ DFN_ERR_TENSILE = 5
 DFN_ERR_STATE = 6
 DFN_ERR_NOMEM = 7
 DFN_ERR_UNSUPPORTED = 8
+DFN_ERR_CG_TIMEOUT = 9
 
 ERROR_NAMES = {
     DFN_OK: "ok",
@@ -28,6 +29,7 @@ ERROR_NAMES = {
     DFN_ERR_STATE: "call out of order",
     DFN_ERR_NOMEM: "allocation failed",
     DFN_ERR_UNSUPPORTED: "unsupported option",
+    DFN_ERR_CG_TIMEOUT: "compute_P workgroups lost contact",
 }
 
 MODE_II, MODE_III = 2, 3

@@ -118,6 +118,8 @@ struct CgArgs {
   const int64_t *cross;
   const CgNet *nets;   // null: one network described by the fields above
   const int *active;   // multi-network launches: systems of networks with active[net] == 0 return at once
+  int late_part;       // test hook of the cooperative kernel: workgroup late_part - 1 of every system
+                       // reports that a message never arrived (0: off)
 };
 
 // Crossing rows of the fluid-diffusion stencil: entries (crossing, slot) of each
@@ -201,12 +203,13 @@ void launch_restart_stage2(const double *y, const double *k1, double h, int n3, 
 void launch_cg(const CgArgs &a, int nsys, hipStream_t s);
 bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s);
 // cooperative multi-workgroup CG; cap = the most crossing entries of any span of
-// kCgSpan rows; seq in [1, 2^24) differs from the seq of every earlier launch on
-// the same records and direction buffers since they were last cleared; records
-// holds kCgRecordWords words per workgroup of each system.  Returns false if a
-// span holds more than kCgCoopMaxEntries or one system's workgroups cannot be
-// co-resident.
-bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, unsigned seq,
+// kCgSpan rows; max_blocks = cg_coop_capacity(cap) on the current device; seq in
+// [1, 2^24) differs from the seq of every earlier launch on the same records and
+// direction buffers since they were last cleared; records holds kCgRecordWords
+// words per workgroup of each system.  Returns false if a span holds more than
+// kCgCoopMaxEntries or one system's workgroups cannot be co-resident.  A
+// system's result[0] is 2 when a workgroup waited in vain for a message.
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, int max_blocks, unsigned seq,
                     unsigned long long *records, hipStream_t s);
 // workgroups of cg_coop_kernel that can be resident together (0: none / unsupported)
 int cg_coop_capacity(int cap);

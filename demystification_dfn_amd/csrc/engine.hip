@@ -72,6 +72,9 @@ struct Engine {
   // environment variable DFN_CG_PATH ("lds", "coop", "plain") at dfn_create, so
   // that every kernel can be checked on small problems.
   int cg_path = 0;
+  // test hook (DFN_CG_LATE_PART, read at dfn_create): workgroup n - 1 of every
+  // cooperative compute_P system reports a message that never arrived; 0 off
+  int cg_late_part = 0;
   DMem status;
   StatusWord *h_status = nullptr;  // pinned
   int *h_cgres = nullptr;          // pinned [kSlots*2]
@@ -146,6 +149,7 @@ struct Engine {
       const std::string v(cp);
       cg_path = v == "lds" ? 1 : v == "coop" ? 2 : v == "plain" ? 3 : 0;
     }
+    if (const char *lp = std::getenv("DFN_CG_LATE_PART")) cg_late_part = std::max(0, std::atoi(lp));
     int dev = 0;
     DFN_HIP(hipGetDevice(&dev));
     DFN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -369,6 +373,7 @@ struct Engine {
     A.src_te = src_te.as<double>();
     A.ncross = hp.ncross();
     A.cross = cross.as<int64_t>();
+    A.late_part = cg_late_part;
     bool launched = false;
     if (N <= kCgLdsMax && cg_path <= 1) {
       CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
@@ -383,7 +388,9 @@ struct Engine {
         DFN_HIP(hipMemsetAsync(cg_work.p, 0, sizeof(double) * (size_t)kSlots * 12 * N, cs));
         cg_seq = 1;
       }
-      launched = launch_cg_coop(A, X, nsys, span_entries, cg_seq, cg_records.as<unsigned long long>(), cs);
+      if (coop_capacity < 0) coop_capacity = cg_coop_capacity(span_entries);
+      launched = launch_cg_coop(A, X, nsys, span_entries, coop_capacity, cg_seq, cg_records.as<unsigned long long>(),
+                                cs);
       if (!launched) coop_ok = false;  // grid too large to be co-resident: one workgroup per system
     }
     if (!launched) launch_cg(A, nsys, cs);
@@ -396,6 +403,8 @@ struct Engine {
     return DFN_OK;
   }
 
+  // reads the results of CG slots slot0..; a solve whose workgroups lost contact
+  // (result 2) is an error of the device, not a reason to halve the step
   int cg_collect(int slot0, int nsys, int *any_fail) {
     DFN_HIP(hipMemcpyAsync(h_cgres + 2 * slot0, cg_res.as<int>() + 2 * slot0, sizeof(int) * 2 * nsys,
                            hipMemcpyDeviceToHost, stream));
@@ -403,6 +412,10 @@ struct Engine {
     *any_fail = 0;
     for (int s = slot0; s < slot0 + nsys; s++) {
       st.cg_iterations += h_cgres[2 * s + 1];
+      if (h_cgres[2 * s] == 2) {
+        std::fprintf(stderr, "dfn: compute_P workgroups lost contact (a message wait gave up)\n");
+        return DFN_ERR_CG_TIMEOUT;
+      }
       if (h_cgres[2 * s]) *any_fail = 1;
     }
     return DFN_OK;

@@ -44,6 +44,10 @@ struct Dataset {
   bool filtered = false;
 };
 
+// Thrown by the reader's byte accessors for a read outside the file; the entry
+// points turn it into DFN_ERR_UNSUPPORTED (a truncated or corrupt file).
+struct H5Malformed {};
+
 struct H5 {
   std::vector<uint8_t> d;
   uint64_t root_oh = 0;
@@ -51,9 +55,14 @@ struct H5 {
 
   bool in(uint64_t off, uint64_t n) const { return off <= d.size() && n <= d.size() - off; }
   uint64_t u(uint64_t off, int n) const {  // little-endian unsigned of n bytes
+    if (!in(off, (uint64_t)n)) throw H5Malformed{};
     uint64_t v = 0;
     for (int i = n - 1; i >= 0; i--) v = (v << 8) | d[off + i];
     return v;
+  }
+  uint8_t b(uint64_t off) const {  // one byte
+    if (off >= d.size()) throw H5Malformed{};
+    return d[off];
   }
 
   int open(const char *path) {
@@ -77,7 +86,7 @@ struct H5 {
 
   // messages of a version-1 object header (following continuation blocks)
   bool messages(uint64_t oh, std::vector<std::pair<int, uint64_t>> &out, std::vector<uint16_t> &sizes) const {
-    if (!in(oh, 16) || d[oh] != 1) return false;
+    if (!in(oh, 16) || b(oh) != 1) return false;
     const int nmsg = (int)u(oh + 2, 2);
     std::vector<std::pair<uint64_t, uint64_t>> blocks{{oh + 16, u(oh + 8, 4)}};
     int seen = 0;
@@ -101,10 +110,10 @@ struct H5 {
 
   // object header address of `name` in the group whose symbol table is (btree, heap)
   uint64_t group_find(uint64_t btree, uint64_t heap, const std::string &name, int depth = 0) const {
-    if (depth > 32 || !in(btree, 24) || std::memcmp(&d[btree], "TREE", 4) != 0 || d[btree + 4] != 0) return kUndef;
+    if (depth > 32 || !in(btree, 24) || std::memcmp(&d[btree], "TREE", 4) != 0 || b(btree + 4) != 0) return kUndef;
     if (!in(heap, 32) || std::memcmp(&d[heap], "HEAP", 4) != 0) return kUndef;
     const uint64_t hdata = u(heap + 24, 8), hsize = u(heap + 8, 8);
-    const int level = d[btree + 5], used = (int)u(btree + 6, 2);
+    const int level = b(btree + 5), used = (int)u(btree + 6, 2);
     for (int e = 0; e < used; e++) {
       const uint64_t child = u(btree + 24 + 8 + (uint64_t)e * 16, 8);
       if (level > 0) {
@@ -157,53 +166,71 @@ struct H5 {
     if (!messages(oh, ms, sz)) return DFN_ERR_UNSUPPORTED;
     bool have_space = false, have_type = false;
     for (size_t i = 0; i < ms.size(); i++) {
-      const uint64_t p = ms[i].second;
+      const uint64_t p = ms[i].second, end = p + sz[i];  // every field read lies in [p, end)
+      const auto fits = [&](uint64_t q, uint64_t n) { return q >= p && q <= end && n <= end - q; };
       switch (ms[i].first) {
         case 0x01: {  // dataspace
-          const int ver = d[p], rank = d[p + 1], flags = d[p + 2];
-          uint64_t q = ver == 1 ? p + 8 : p + 4;
+          if (!fits(p, 4)) return DFN_ERR_UNSUPPORTED;
+          const int ver = b(p), rank = b(p + 1);
+          const uint64_t q = ver == 1 ? p + 8 : p + 4;
+          if (rank > 32 || !fits(q, 8ull * rank)) return DFN_ERR_UNSUPPORTED;
           ds.dims.clear();
           for (int r = 0; r < rank; r++) ds.dims.push_back(u(q + 8ull * r, 8));
-          (void)flags;
           have_space = true;
           break;
         }
         case 0x03: {  // datatype
-          const int cls = d[p] & 0x0f;
+          if (!fits(p, 8)) return DFN_ERR_UNSUPPORTED;
+          const int cls = b(p) & 0x0f;
           const uint32_t bits = (uint32_t)u(p + 1, 3);
-          ds.size = (int)u(p + 4, 4);
+          const uint64_t size = u(p + 4, 4);
           if (cls == 0) { ds.cls = 0; ds.big_endian = bits & 1; ds.is_signed = (bits >> 3) & 1; }
           else if (cls == 1) { ds.cls = 1; ds.big_endian = bits & 1; }
           else if (cls == 3) { ds.cls = 2; }
           else if (cls == 9 && (bits & 0x0f) == 1) { ds.cls = 2; ds.vlen = true; }
           else return DFN_ERR_UNSUPPORTED;
+          // numbers of 1 to 8 bytes (floats of 4 or 8), fixed strings of 1 byte to 1 MiB,
+          // variable-length strings as 16-byte heap references
+          if (ds.cls == 0 && (size < 1 || size > 8)) return DFN_ERR_UNSUPPORTED;
+          if (ds.cls == 1 && size != 4 && size != 8) return DFN_ERR_UNSUPPORTED;
+          if (ds.cls == 2 && !ds.vlen && (size < 1 || size > (1u << 20))) return DFN_ERR_UNSUPPORTED;
+          if (ds.vlen && size != 16) return DFN_ERR_UNSUPPORTED;
+          ds.size = (int)size;
           have_type = true;
           break;
         }
         case 0x08: {  // data layout
-          const int ver = d[p];
+          if (!fits(p, 4)) return DFN_ERR_UNSUPPORTED;
+          const int ver = b(p);
           if (ver == 3) {
-            ds.layout = d[p + 1];
+            ds.layout = b(p + 1);
             if (ds.layout == 0) {
               const uint64_t n = u(p + 2, 2);
-              if (!in(p + 4, n)) return DFN_ERR_UNSUPPORTED;
+              if (!fits(p + 4, n)) return DFN_ERR_UNSUPPORTED;
               ds.compact.assign(d.begin() + (long)(p + 4), d.begin() + (long)(p + 4 + n));
             } else if (ds.layout == 1) {
+              if (!fits(p + 2, 16)) return DFN_ERR_UNSUPPORTED;
               ds.addr = u(p + 2, 8);
               ds.nbytes = u(p + 10, 8);
             } else {
               return DFN_ERR_UNSUPPORTED;  // chunked
             }
           } else if (ver == 1 || ver == 2) {
-            const int rank = d[p + 1];
-            ds.layout = d[p + 2];
+            const int rank = b(p + 1);
+            ds.layout = b(p + 2);
             uint64_t q = p + 8;
-            if (ds.layout == 1) { ds.addr = u(q, 8); q += 8; }
-            else if (ds.layout != 0) return DFN_ERR_UNSUPPORTED;
+            if (ds.layout == 1) {
+              if (!fits(q, 8)) return DFN_ERR_UNSUPPORTED;
+              ds.addr = u(q, 8);
+              q += 8;
+            } else if (ds.layout != 0) {
+              return DFN_ERR_UNSUPPORTED;
+            }
             q += 4ull * rank;
             if (ds.layout == 0) {
+              if (!fits(q, 4)) return DFN_ERR_UNSUPPORTED;
               const uint64_t n = u(q, 4);
-              if (!in(q + 4, n)) return DFN_ERR_UNSUPPORTED;
+              if (!fits(q + 4, n)) return DFN_ERR_UNSUPPORTED;
               ds.compact.assign(d.begin() + (long)(q + 4), d.begin() + (long)(q + 4 + n));
             }
           } else {
@@ -220,27 +247,42 @@ struct H5 {
     }
     if (!have_space || !have_type || ds.layout < 0) return DFN_ERR_UNSUPPORTED;
     if (ds.filtered) return DFN_ERR_UNSUPPORTED;
+    uint64_t n = 0;
+    if (!count(ds, n) || n == 0) return DFN_ERR_UNSUPPORTED;  // overflowing or empty extents
+    uint64_t bytes = 0;
+    if (__builtin_mul_overflow(n, (uint64_t)ds.size, &bytes)) return DFN_ERR_UNSUPPORTED;
     return DFN_OK;
   }
 
+  // element count (false if the product of the extents overflows)
+  static bool count(const Dataset &ds, uint64_t &n) {
+    n = 1;
+    for (uint64_t x : ds.dims)
+      if (__builtin_mul_overflow(n, x, &n)) return false;
+    return true;
+  }
   static uint64_t count(const Dataset &ds) {
-    uint64_t n = 1;
-    for (uint64_t x : ds.dims) n *= x;
-    return n;
+    uint64_t n = 0;
+    return count(ds, n) ? n : 0;
   }
 
-  // raw bytes of the dataset (count * size); unwritten contiguous data reads as zeros
+  // raw bytes of the dataset (count * size); unwritten contiguous data reads as zeros.
+  // The bytes are checked to lie in the file before anything is allocated.
   bool raw(const Dataset &ds, std::vector<uint8_t> &out) const {
-    const uint64_t n = count(ds) * (uint64_t)ds.size;
-    out.assign(n, 0);
+    uint64_t c = 0, n = 0;
+    if (!count(ds, c) || __builtin_mul_overflow(c, (uint64_t)ds.size, &n) || n == 0) return false;
     if (ds.layout == 0) {
       if (ds.compact.size() < n) return false;
-      std::memcpy(out.data(), ds.compact.data(), n);
+      out.assign(ds.compact.begin(), ds.compact.begin() + (long)n);
       return true;
     }
-    if (ds.addr == kUndef) return true;
+    if (ds.addr == kUndef) {
+      if (n > d.size() + (1ull << 20)) return false;  // unwritten data larger than any real input
+      out.assign(n, 0);
+      return true;
+    }
     if (!in(ds.addr, n)) return false;
-    std::memcpy(out.data(), &d[ds.addr], n);
+    out.assign(d.begin() + (long)ds.addr, d.begin() + (long)(ds.addr + n));
     return true;
   }
 
@@ -268,7 +310,7 @@ struct H5 {
   // the first string of a string dataset
   bool string(const Dataset &ds, std::string &s) const {
     std::vector<uint8_t> r;
-    if (!raw(ds, r) || r.empty()) return false;
+    if (!raw(ds, r) || r.empty() || (ds.vlen && r.size() < 16)) return false;
     if (!ds.vlen) {
       s.assign(reinterpret_cast<const char *>(r.data()), strnlen(reinterpret_cast<const char *>(r.data()), ds.size));
       return true;
@@ -473,8 +515,17 @@ struct dfn_h5_s {
 #define API extern "C" __attribute__((visibility("default")))
 #include "api_guard.hpp"
 
+// reads outside a truncated or corrupt file end the call with DFN_ERR_UNSUPPORTED
+#define H5_READ_BEGIN try {
+#define H5_READ_END                       \
+  }                                       \
+  catch (const H5Malformed &) {           \
+    return DFN_ERR_UNSUPPORTED;           \
+  }
+
 API int dfn_h5_open(const char *path, dfn_h5 *out) {
   DFN_API_BEGIN
+  H5_READ_BEGIN
   if (!path || !out) return DFN_ERR_ARG;
   *out = nullptr;
   dfn_h5 h = new (std::nothrow) dfn_h5_s;
@@ -483,6 +534,7 @@ API int dfn_h5_open(const char *path, dfn_h5 *out) {
   if (st != DFN_OK) { delete h; return st; }
   *out = h;
   return DFN_OK;
+  H5_READ_END
   DFN_API_END
 }
 
@@ -496,6 +548,7 @@ API int dfn_h5_close(dfn_h5 h) {
 
 API int dfn_h5_info(dfn_h5 h, const char *path, int32_t *type_class, int32_t *rank, int64_t *dims) {
   DFN_API_BEGIN
+  H5_READ_BEGIN
   if (!h || !path) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -505,11 +558,13 @@ API int dfn_h5_info(dfn_h5 h, const char *path, int32_t *type_class, int32_t *ra
   if (dims)
     for (size_t i = 0; i < ds.dims.size() && i < 8; i++) dims[i] = (int64_t)ds.dims[i];
   return DFN_OK;
+  H5_READ_END
   DFN_API_END
 }
 
 API int dfn_h5_read_f64(dfn_h5 h, const char *path, double *buf, int64_t n) {
   DFN_API_BEGIN
+  H5_READ_BEGIN
   if (!h || !path || !buf || n < 0) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -520,11 +575,13 @@ API int dfn_h5_read_f64(dfn_h5 h, const char *path, double *buf, int64_t n) {
   if (!h->f.raw(ds, r)) return DFN_ERR_UNSUPPORTED;
   for (int64_t i = 0; i < n; i++) buf[i] = h->f.number(ds, &r[(size_t)i * ds.size]);
   return DFN_OK;
+  H5_READ_END
   DFN_API_END
 }
 
 API int dfn_h5_read_i64(dfn_h5 h, const char *path, int64_t *buf, int64_t n) {
   DFN_API_BEGIN
+  H5_READ_BEGIN
   if (!h || !path || !buf || n < 0) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -535,11 +592,13 @@ API int dfn_h5_read_i64(dfn_h5 h, const char *path, int64_t *buf, int64_t n) {
   if (!h->f.raw(ds, r)) return DFN_ERR_UNSUPPORTED;
   for (int64_t i = 0; i < n; i++) buf[i] = h->f.integer(ds, &r[(size_t)i * ds.size]);
   return DFN_OK;
+  H5_READ_END
   DFN_API_END
 }
 
 API int dfn_h5_read_string(dfn_h5 h, const char *path, char *buf, int32_t cap) {
   DFN_API_BEGIN
+  H5_READ_BEGIN
   if (!h || !path || !buf || cap < 1) return DFN_ERR_ARG;
   Dataset ds;
   const int st = h->f.dataset(path, ds);
@@ -551,6 +610,7 @@ API int dfn_h5_read_string(dfn_h5 h, const char *path, char *buf, int32_t cap) {
   std::memcpy(buf, s.data(), n);
   buf[n] = 0;
   return DFN_OK;
+  H5_READ_END
   DFN_API_END
 }
 

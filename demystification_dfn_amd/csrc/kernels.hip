@@ -1365,6 +1365,8 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
     timed_out = __syncthreads_or(late);
     if (it > kCgMaxIter) { fail = true; break; }
   }
+  // test hook: the host makes one workgroup of every system report a lost message
+  if (A.late_part == part + 1) timed_out = true;
   int bad = 0;
   if (!fail) {
     const bool small = dt < (double)1e-9f;
@@ -1378,16 +1380,18 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
       }
     }
   }
-  // the overflow flags go to the system's first workgroup, which reports
+  // every workgroup's flag goes to the system's first workgroup, which reports:
+  // 2 if any workgroup waited in vain for a message (its results are not to be
+  // trusted), else 1 if any x overflowed or the solve failed
   const double b2 = block_max(bad ? 1.0 : 0.0, red);
-  if (threadIdx.x == 0) ll_send(myrec + 0, myrec + 1, b2, ep0 + 255);
+  const double flag = timed_out ? 2.0 : b2;
+  if (threadIdx.x == 0) ll_send(myrec + 0, myrec + 1, flag, ep0 + 255);
   if (part == 0) {
-    double nbad, anybad;
-    ll_sum_max(rec, wps, 0, 0, ep0 + 255, red, late, nbad, anybad);
-    timed_out = __syncthreads_or(late) || timed_out;
-    // a message that never arrived reports as a failed solve (2)
+    double nflag, maxflag;
+    ll_sum_max(rec, wps, 0, 0, ep0 + 255, red, late, nflag, maxflag);
+    timed_out = __syncthreads_or(late) || maxflag >= 2.0;
     if (threadIdx.x == 0) {
-      S.result[0] = timed_out ? 2 : ((fail || anybad > 0.0) ? 1 : 0);
+      S.result[0] = timed_out ? 2 : ((fail || maxflag > 0.0) ? 1 : 0);
       S.result[1] = it;
     }
   }
@@ -1748,23 +1752,10 @@ int cg_coop_capacity(int cap) {
   return per_cu * cus;
 }
 
-bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, unsigned seq,
+bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, int max_blocks, unsigned seq,
                     unsigned long long *records, hipStream_t s) {
   if (cap > kCgCoopMaxEntries) return false;
   const size_t sh = cg_coop_lds_bytes(cap);
-  static size_t cached_sh = 0;
-  static int max_blocks = -1;
-  if (max_blocks < 0 || sh != cached_sh) {
-    int per_cu = 0, dev = 0, cus = 0, coop = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cg_coop_kernel, 256, sh) != hipSuccess)
-      max_blocks = 0;
-    else
-      max_blocks = per_cu * cus;
-    cached_sh = sh;
-  }
   int wps = (a.n + kCgSpan - 1) / kCgSpan;
   if (wps > max_blocks) return false;
   // as many systems per cooperative launch as can be resident together

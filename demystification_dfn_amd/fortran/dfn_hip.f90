@@ -18,6 +18,7 @@ private
 
 integer(c_int), parameter, public :: DFN_OK = 0, DFN_MODE_II = 2, DFN_MODE_III = 3
 integer(c_int), parameter, public :: DFN_KERNEL_DIRECT = 0, DFN_KERNEL_HMATRIX = 1
+integer(c_int), parameter, public :: DFN_ERR_CG_TIMEOUT = 9
 
 type, bind(C), public :: dfn_problem
   integer(c_int32_t) :: fracture_mode, static_kernel, nb_fault, nb_element
@@ -197,7 +198,9 @@ subroutine ode_RateStateAgingHIP(x, nv, Y, Y_dot)
   end if
 end subroutine
 
-! compute_P (src/fluid_diffusion.f90:24): ier = 1 on CG failure, outputs then untouched
+! compute_P (src/fluid_diffusion.f90:24): ier = 1 on CG failure, outputs then untouched.
+! A solve whose device workgroups lost contact is a fault of the device, not a
+! reason to halve the step: it stops.
 subroutine compute_P_hip(dt, P_temp, P_dot_temp, ier)
   real(kind=8) :: dt
   real(kind=8), dimension(n_el) :: P_temp, P_dot_temp
@@ -205,6 +208,10 @@ subroutine compute_P_hip(dt, P_temp, P_dot_temp, ier)
   integer(c_int32_t) :: e
   integer(c_int) :: s
   s = dfn_compute_P(dfn_hip_handle, dt, P_temp, P_dot_temp, e)
+  if (s == DFN_ERR_CG_TIMEOUT) then
+    print *, 'dfn_compute_P: device workgroups lost contact, status ', s
+    stop
+  end if
   ier = int(e)
   if (s /= DFN_OK) ier = 1
 end subroutine

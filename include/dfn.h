@@ -36,7 +36,10 @@ enum {
   DFN_ERR_TENSILE = 5,    /* sigmaN > 0 somewhere (solver.f90:412-420) */
   DFN_ERR_STATE = 6,      /* call out of order (e.g. H-matrix kernel before dfn_build_hmat) */
   DFN_ERR_NOMEM = 7,      /* host or device allocation failed */
-  DFN_ERR_UNSUPPORTED = 8 /* option the library does not implement */
+  DFN_ERR_UNSUPPORTED = 8, /* option the library does not implement */
+  DFN_ERR_CG_TIMEOUT = 9   /* the workgroups of a multi-workgroup compute_P lost contact (a bounded
+                              message wait gave up): a device fault, not non-convergence; the
+                              handle's state is left at the start of the failed step */
 };
 
 enum { DFN_MODE_II = 2, DFN_MODE_III = 3 };               /* hyperparameter/fracture_mode */

@@ -84,3 +84,28 @@ def test_null_pointers_and_bad_options_are_argument_errors():
     sd, so = d.solve(3), o.solve(3)
     assert sd["accepted_steps"] == so["accepted_steps"] == 3
     np.testing.assert_allclose(d.get_state()[0], o.get_state()[0], rtol=1e-8, atol=1e-300)
+
+
+@pytest.mark.parametrize("late_part", [1, 2, 3])
+def test_cooperative_compute_P_reports_any_workgroup_that_lost_contact(late_part, monkeypatch):
+    """The multi-workgroup compute_P of a 4855-element network spans three
+    workgroups per system.  A host switch (DFN_CG_LATE_PART) makes one of them
+    report a message wait that gave up, while every kernel still runs to its end:
+    whichever workgroup it is, the solve reports it (DFN_ERR_CG_TIMEOUT), instead
+    of a converged pressure or a step halving; the same network without the
+    switch solves."""
+    from demystification_dfn_amd.device import DeviceSimulation
+    monkeypatch.setenv("DFN_CG_PATH", "coop")
+    p = cases.connected_network_spans()
+    print(f"[gpu] errors coop timeout part {late_part} N={p.N}", flush=True)
+    monkeypatch.setenv("DFN_CG_LATE_PART", str(late_part))
+    d = DeviceSimulation(p)
+    assert _status(d.compute_P, 5000.0) == _abi.DFN_ERR_CG_TIMEOUT
+    assert _status(d.solve, 1) == _abi.DFN_ERR_CG_TIMEOUT
+    assert d.stats()["accepted_steps"] == 0 and d.stats()["cg_halvings"] == 0
+    d.close()
+    monkeypatch.setenv("DFN_CG_LATE_PART", "0")
+    d = DeviceSimulation(p)
+    Pt, _, ier = d.compute_P(5000.0)
+    assert ier == 0 and np.isfinite(Pt).all()
+    assert d.solve(1)["accepted_steps"] == 1
