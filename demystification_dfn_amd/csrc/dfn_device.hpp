@@ -22,7 +22,7 @@ struct VTask {
   int32_t xcol;   // H index of the block's first column
   int32_t zp;     // output offset in zpart (blocks split in several chunks)
   int32_t wout;   // >= 0: single-chunk block, z written straight to w[wout + j]
-  int32_t pad;
+  int32_t net;    // network of the block (ensembles; 0 for one network)
 };
 
 // z of one low-rank block: w[zout + j] = sum_t zpart[zp_first + t*k + j]
@@ -40,6 +40,8 @@ struct PTask {
   int32_t L;
   int32_t yp;
   int32_t nd;
+  int32_t net;  // network of the panel's rows (ensembles; 0 for one network)
+  int32_t pad;
 };
 
 struct RhsParams {
@@ -120,6 +122,9 @@ struct CgArgs {
   const int *active;   // multi-network launches: systems of networks with active[net] == 0 return at once
   int late_part;       // test hook of the cooperative kernel: workgroup late_part - 1 of every system
                        // reports that a message never arrived (0: off)
+  // multi-network cooperative launches (may be null): per workgroup {system, part,
+  // the system's workgroup count, the system's first workgroup in the launch}
+  const int4 *wg_map;
 };
 
 // Crossing rows of the fluid-diffusion stencil: entries (crossing, slot) of each
@@ -146,13 +151,14 @@ struct PermArgs {
 // launchers (kernels.hip)
 // xf: FP32 copy of x_h for FP32 operators (n + 4 floats), or nullptr
 void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s, float *xf = nullptr);
-// waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (FP32 operators)
+// waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (FP32 operators);
+// active (ensembles, may be null): tasks of networks with active[net] == 0 return at once
 void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
-                int waves = 4, const float *xf = nullptr);
+                int waves = 4, const float *xf = nullptr, const int *active = nullptr);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
-// rows: panel rows per wave step, 2 or 4
+// rows: panel rows per wave step, 2 or 4; active as for launch_vtx
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
-                  const double *w, double *ypart, hipStream_t s, int rows = 2);
+                  const double *w, double *ypart, hipStream_t s, int rows = 2, const int *active = nullptr);
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
                      double scale, double *yT, double *yN, hipStream_t s);
 // dense direct kernels: node columns per tile, number of tiles along the nodes,
@@ -211,6 +217,10 @@ bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s
 // system's result[0] is 2 when a workgroup waited in vain for a message.
 bool launch_cg_coop(const CgArgs &a, const CrossRows &xr, int nsys, int cap, int max_blocks, unsigned seq,
                     unsigned long long *records, hipStream_t s);
+// Multi-network cooperative launch of nwg workgroups described by a.wg_map (all
+// co-resident; nwg <= cg_coop_capacity(cap)); false if the launch fails.
+bool launch_cg_coop_map(const CgArgs &a, const CrossRows &xr, int nwg, int cap, unsigned seq,
+                        unsigned long long *records, hipStream_t s);
 // workgroups of cg_coop_kernel that can be resident together (0: none / unsupported)
 int cg_coop_capacity(int cap);
 void launch_perm_update(const PermArgs &a, hipStream_t s);

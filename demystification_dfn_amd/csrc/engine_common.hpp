@@ -231,8 +231,11 @@ inline double pow2_scale(const double *v, size_t n) {
   return std::ldexp(1.0, e);
 }
 
+// hnet (ensembles, may be null): the network of every H position; each phase-1 and
+// phase-2 task records the network its block or panel belongs to, so that the
+// kernels can skip the tasks of networks that sit out an evaluation.
 template <class T>
-int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
+int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::vector<int> *hnet = nullptr) {
   const bool scaled = sizeof(T) == 4;  // FP32 storage keeps each block divided by its power-of-two scale
   const int n = ops[0]->n;
   for (int o = 1; o < nops; o++)
@@ -298,6 +301,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
         t.zp = single ? -1 : zp;
         t.scale = vs;
         t.wout = single ? wz : -1;
+        t.net = hnet ? (*hnet)[H.l_c0[b]] : 0;
         vt.push_back(t);
         if (!single) zp += k;
         r.ntask++;
@@ -398,6 +402,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d) {
         t.L = Lp;
         t.yp = yp;
         t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, (int64_t)ndense - c0));
+        t.net = hnet ? (*hnet)[r0] : 0;
         for (int j = 0; j < Lp; j++) {
           widx.push_back(j < L ? (int)cols[cbase + c0 + j].wi : 0);
           if (scaled) cscale.push_back(j < L ? (float)cols[cbase + c0 + j].scale : 1.0f);

@@ -110,7 +110,22 @@ struct EnsEngine {
   // state and stage buffers
   DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot;
   // per-network parameters read by the captured launches
-  DMem d_h, d_ttry, d_tol, d_eta, d_active, status, cg_work, cg_res, sys_all, sys_stage;
+  DMem d_h, d_ttry, d_tol, d_eta, d_active, status, cg_work, cg_res;
+  // compute_P systems, (network, slot) with the slots of a network consecutive:
+  // networks of at most kCgLdsMax elements (one workgroup per system, vectors in
+  // registers) and larger ones (the cooperative kernel, each system over
+  // ceil(N / kCgSpan) workgroups, or the plain one-workgroup kernel), each list for
+  // all six slots and for the five stage slots of a retried attempt
+  DMem sys_small_all, sys_small_stage, sys_large_all, sys_large_stage;
+  int n_small = 0, n_large = 0, max_small = 0, max_large = 0;
+  // cooperative launches of the large networks' systems: groups of whole systems
+  // whose workgroups can all be resident; per group, its map entries
+  bool coop = false;
+  int coop_cap = 0;  // most crossing entries in one span of any large network
+  struct CoopGroup { int first, nwg; };
+  std::vector<CoopGroup> coop_all, coop_stage;
+  DMem map_all, map_stage, coop_records;
+  unsigned cg_seq = 0;
   // pinned host mirrors
   double *h_h = nullptr, *h_ttry = nullptr;
   int *h_active = nullptr, *h_cgres = nullptr;
@@ -267,23 +282,7 @@ struct EnsEngine {
     for (int i = 0; i < B; i++) { tol[i] = hp[i].tol; eta[i] = hp[i].mu / (2.0 * hp[i].cs); }
     DFN_TRY(d_tol.upload(tol));
     DFN_TRY(d_eta.upload(eta));
-    // (network, slot) systems: all six slots, and slots 1..5 for retried attempts
-    std::vector<CgSys> all((size_t)B * kEnsSlots), stage((size_t)B * (kEnsSlots - 1));
-    for (int i = 0; i < B; i++)
-      for (int sl = 0; sl < kEnsSlots; sl++) {
-        CgSys c{};
-        c.dt = 0.0;
-        c.diag = cg_vec(sl, 0) + base[i]; c.A1 = cg_vec(sl, 1) + base[i]; c.A2 = cg_vec(sl, 2) + base[i];
-        c.A3 = cg_vec(sl, 3) + base[i]; c.b = cg_vec(sl, 4) + base[i]; c.x = cg_vec(sl, 5) + base[i];
-        c.r = cg_vec(sl, 6) + base[i]; c.p = cg_vec(sl, 7) + base[i]; c.Ap = cg_vec(sl, 8) + base[i];
-        c.Pt = cg_vec(sl, 10) + base[i]; c.Pd = cg_vec(sl, 11) + base[i];
-        c.result = cg_res.as<int>() + 2 * ((size_t)i * kEnsSlots + sl);
-        c.net = i;
-        all[(size_t)i * kEnsSlots + sl] = c;
-        if (sl > 0) stage[(size_t)i * (kEnsSlots - 1) + sl - 1] = c;
-      }
-    DFN_TRY(sys_all.upload(all));
-    DFN_TRY(sys_stage.upload(stage));
+    DFN_TRY(setup_cg_systems(xstart));
     DFN_HIP(hipHostMalloc((void **)&h_h, sizeof(double) * B, hipHostMallocDefault));
     DFN_HIP(hipHostMalloc((void **)&h_ttry, sizeof(double) * B, hipHostMallocDefault));
     DFN_HIP(hipHostMalloc((void **)&h_active, sizeof(int) * B, hipHostMallocDefault));
@@ -308,6 +307,78 @@ struct EnsEngine {
     has_pend.assign(B, 0);
     pend_skip.assign(B, 0);
     pend_feat.assign((size_t)DFN_MLP_INPUTS * B, 0.0f);
+    return DFN_OK;
+  }
+
+  // compute_P system lists and the cooperative launch groups (see the members)
+  int setup_cg_systems(const std::vector<int> &xstart) {
+    DFN_HIP(hipMemset(cg_work.p, 0, sizeof(double) * (size_t)kEnsSlots * 12 * Ntot));  // message epochs 0
+    // DFN_CG_PATH=plain (read at creation, as the single-network engine does): the
+    // plain one-workgroup kernel for every network
+    const char *cp = std::getenv("DFN_CG_PATH");
+    const bool plain = cp && std::string(cp) == "plain";
+    std::vector<CgSys> sa, ss, la, ls;
+    std::vector<int> large;
+    for (int i = 0; i < B; i++) {
+      const bool small = !plain && hp[i].N <= kCgLdsMax;
+      if (small) { n_small++; max_small = std::max(max_small, hp[i].N); }
+      else { n_large++; max_large = std::max(max_large, hp[i].N); large.push_back(i); }
+      for (int sl = 0; sl < kEnsSlots; sl++) {
+        CgSys c{};
+        c.dt = 0.0;
+        c.diag = cg_vec(sl, 0) + base[i]; c.A1 = cg_vec(sl, 1) + base[i]; c.A2 = cg_vec(sl, 2) + base[i];
+        c.A3 = cg_vec(sl, 3) + base[i]; c.b = cg_vec(sl, 4) + base[i]; c.x = cg_vec(sl, 5) + base[i];
+        c.r = cg_vec(sl, 6) + base[i]; c.p = cg_vec(sl, 7) + base[i]; c.Ap = cg_vec(sl, 8) + base[i];
+        c.Pt = cg_vec(sl, 10) + base[i]; c.Pd = cg_vec(sl, 11) + base[i];
+        c.result = cg_res.as<int>() + 2 * ((size_t)i * kEnsSlots + sl);
+        c.net = i;
+        (small ? sa : la).push_back(c);
+        if (sl > 0) (small ? ss : ls).push_back(c);
+      }
+    }
+    DFN_TRY(sys_small_all.upload(sa));
+    DFN_TRY(sys_small_stage.upload(ss));
+    DFN_TRY(sys_large_all.upload(la));
+    DFN_TRY(sys_large_stage.upload(ls));
+    if (n_large == 0) return DFN_OK;
+    // the cooperative kernel for the large networks when every span's crossing
+    // entries fit its LDS and one network's workgroups can be resident together
+    coop_cap = 0;
+    for (int i : large)
+      for (int s0 = 0; s0 < hp[i].N; s0 += kCgSpan)
+        coop_cap = std::max(coop_cap, xstart[base[i] + std::min(s0 + kCgSpan, hp[i].N)] - xstart[base[i] + s0]);
+    const int capacity = plain ? 0 : cg_coop_capacity(coop_cap);
+    int max_wps = 0;
+    for (int i : large) max_wps = std::max(max_wps, (hp[i].N + kCgSpan - 1) / kCgSpan);
+    coop = capacity > 0 && max_wps <= capacity;
+    if (!coop) return DFN_OK;
+    // whole systems packed in list order into launches of at most `capacity` workgroups
+    auto plan = [&](int nslot, std::vector<CoopGroup> &groups, DMem &map) -> int {
+      std::vector<int4> m;
+      groups.clear();
+      int sysi = 0, in_group = 0, group_first = 0;
+      for (int i : large)
+        for (int sl = 0; sl < nslot; sl++, sysi++) {
+          const int wps = (hp[i].N + kCgSpan - 1) / kCgSpan;
+          if (in_group + wps > capacity) {
+            groups.push_back({group_first, in_group});
+            group_first = (int)m.size();
+            in_group = 0;
+          }
+          for (int q = 0; q < wps; q++) m.push_back(make_int4(sysi, q, wps, in_group));
+          in_group += wps;
+        }
+      groups.push_back({group_first, in_group});
+      return map.upload(m);
+    };
+    DFN_TRY(plan(kEnsSlots, coop_all, map_all));
+    DFN_TRY(plan(kEnsSlots - 1, coop_stage, map_stage));
+    // message records: kCgRecordWords per workgroup of the largest launch
+    int most = 0;
+    for (const CoopGroup &g : coop_all) most = std::max(most, g.nwg);
+    DFN_TRY(coop_records.alloc(sizeof(unsigned long long) * kCgRecordWords * (size_t)most));
+    DFN_HIP(hipMemset(coop_records.p, 0, sizeof(unsigned long long) * kCgRecordWords * (size_t)most));
+    use_graph = false;  // cooperative launches are issued one by one, not captured
     return DFN_OK;
   }
 
@@ -417,11 +488,14 @@ struct EnsEngine {
     ops32.reset();
     if (!built) return DFN_OK;
     const HostOperator *list[2] = {&cat[0], &cat[1]};
+    // the network of each H position: network b owns [base_b, base_b + N_b)
+    std::vector<int> hnet(Ntot);
+    for (int i = 0; i < B; i++) std::fill(hnet.begin() + base[i], hnet.begin() + base[i] + hp[i].N, i);
     ops64.reset(new DevOps);
-    DFN_TRY(layout_ops<double>(list, nops, *ops64));
+    DFN_TRY(layout_ops<double>(list, nops, *ops64, &hnet));
     if (precision == DFN_PREC_FP32) {
       ops32.reset(new DevOps);
-      DFN_TRY(layout_ops<float>(list, nops, *ops32));
+      DFN_TRY(layout_ops<float>(list, nops, *ops32, &hnet));
     }
     return DFN_OK;
   }
@@ -432,11 +506,13 @@ struct EnsEngine {
     const DevOps *d = active_ops();
     double *w = d->w.as<double>();
     launch_gather(dV, d->perm.as<int>(), w, Ntot, stream, d->xf.as<float>());
+    // the tasks of networks that sit out this attempt (accepted, or waiting for
+    // others) return at once: a retry streams only the retrying networks' operators
     launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves,
-               d->xf.as<float>());
+               d->xf.as<float>(), d_active.as<int>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
-                 d->ypart.as<double>(), stream, d->panel_rows);
+                 d->ypart.as<double>(), stream, d->panel_rows, d_active.as<int>());
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), Ntot,
                     d->nops, kscale, sT.as<double>(), d->nops > 1 ? sN.as<double>() : sT.as<double>(), stream);
   }
@@ -486,17 +562,39 @@ struct EnsEngine {
     return A;
   }
 
-  void cg_launch(bool stages_only, hipStream_t cs) {
-    CgSys *sys = stages_only ? sys_stage.as<CgSys>() : sys_all.as<CgSys>();
-    const int nslot = stages_only ? kEnsSlots - 1 : kEnsSlots;
-    launch_ens_cg_dt(sys, B, nslot, d_ttry.as<double>(), d_h.as<double>(), stages_only ? 1 : 0, cs);
-    const CgArgs A = cg_args(sys);
-    if (max_n <= kCgLdsMax) {
-      CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
-      launch_cg_lds_multi(A, X, B * nslot, max_n, cs);
-    } else {
-      launch_cg(A, B * nslot, cs);
+  int cg_launch(bool stages_only, hipStream_t cs) {
+    const int nslot = stages_only ? kEnsSlots - 1 : kEnsSlots, first = stages_only ? 1 : 0;
+    CrossRows X{xrow_start.as<int>(), xrow_ent.as<int>()};
+    if (n_small > 0) {
+      CgSys *sys = stages_only ? sys_small_stage.as<CgSys>() : sys_small_all.as<CgSys>();
+      launch_ens_cg_dt(sys, n_small, nslot, d_ttry.as<double>(), d_h.as<double>(), first, cs);
+      CgArgs A = cg_args(sys);
+      A.n = max_small;
+      launch_cg_lds_multi(A, X, n_small * nslot, max_small, cs);
     }
+    if (n_large > 0) {
+      CgSys *sys = stages_only ? sys_large_stage.as<CgSys>() : sys_large_all.as<CgSys>();
+      launch_ens_cg_dt(sys, n_large, nslot, d_ttry.as<double>(), d_h.as<double>(), first, cs);
+      CgArgs A = cg_args(sys);
+      A.n = max_large;
+      if (!coop) {
+        launch_cg(A, n_large * nslot, cs);
+        return DFN_OK;
+      }
+      const std::vector<CoopGroup> &groups = stages_only ? coop_stage : coop_all;
+      const int4 *map = (stages_only ? map_stage : map_all).as<int4>();
+      for (const CoopGroup &g : groups) {
+        if (++cg_seq >= kCgMaxSeq) {  // epochs would repeat: clear every message word first
+          DFN_HIP(hipMemsetAsync(coop_records.p, 0, coop_records.bytes, cs));
+          DFN_HIP(hipMemsetAsync(cg_work.p, 0, sizeof(double) * (size_t)kEnsSlots * 12 * Ntot, cs));
+          cg_seq = 1;
+        }
+        A.wg_map = map + g.first;
+        if (!launch_cg_coop_map(A, X, g.nwg, coop_cap, cg_seq, coop_records.as<unsigned long long>(), cs))
+          return DFN_ERR_HIP;
+      }
+    }
+    return DFN_OK;
   }
 
   // stages 2..6 of rkck (ode_bs.f90:199-224) for the active networks
@@ -516,7 +614,7 @@ struct EnsEngine {
   void seq_try() {
     (void)hipEventRecord(ev_fork, stream);
     (void)hipStreamWaitEvent(cg_stream, ev_fork, 0);
-    cg_launch(false, cg_stream);
+    seq_rc = cg_launch(false, cg_stream);
     (void)hipEventRecord(ev_join, cg_stream);
     apply(Y.as<double>());
     (void)hipStreamWaitEvent(stream, ev_join, 0);
@@ -526,10 +624,11 @@ struct EnsEngine {
 
   // retried attempt with reduced steps: compute_P of slots 1..5, restart, stages 2..6
   void seq_retry() {
-    cg_launch(true, stream);
+    seq_rc = cg_launch(true, stream);
     launch_ens_restart_stage2(stage_args(Y.as<double>(), 0, ytA.as<double>()), stream);
     attempt();
   }
+  int seq_rc = DFN_OK;  // status of the last sequence's compute_P launches
 
   int capture(void (EnsEngine::*fn)(), hipGraphExec_t *exec) {
     hipGraph_t g = nullptr;
@@ -554,6 +653,7 @@ struct EnsEngine {
     } else {
       if (retry) seq_retry();
       else seq_try();
+      DFN_TRY(seq_rc);
       DFN_HIP(hipGetLastError());
     }
     return DFN_OK;
@@ -610,6 +710,8 @@ struct EnsEngine {
         const int *r0 = h_cgres + 2 * ((size_t)i * kEnsSlots);
         st[i].cg_iterations += r0[1];
         st[i].cg_solves += 1;
+        for (int sl = 0; sl < kEnsSlots; sl++)
+          if (r0[2 * sl] == 2) return DFN_ERR_CG_TIMEOUT;  // a cooperative solve lost a message
         if (r0[0]) {  // compute_P(t_try) failed: halve and redo this network
           t_try[i] = t_try[i] / 2.0;
           clipped[i] = 0;
@@ -637,6 +739,7 @@ struct EnsEngine {
         int fail = 0;
         for (int sl = 1; sl < kEnsSlots; sl++) {
           st[i].cg_iterations += r[2 * sl + 1];
+          if (r[2 * sl] == 2) return DFN_ERR_CG_TIMEOUT;
           if (r[2 * sl]) fail = 1;
         }
         st[i].cg_solves += 5;
@@ -743,7 +846,7 @@ API int dfn_ensemble_set_precision(dfn_ensemble h, int32_t prec) {
 API int dfn_ensemble_set_graph(dfn_ensemble h, int32_t use_graph) {
   DFN_API_BEGIN
   if (!h) return DFN_ERR_ARG;
-  h->e.use_graph = use_graph != 0;
+  h->e.use_graph = use_graph != 0 && !h->e.coop;  // cooperative launches are not captured
   return DFN_OK;
   DFN_API_END
 }

@@ -164,13 +164,15 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
 template <typename TV>
 __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
                                                           const TV *__restrict__ vdata, double *__restrict__ w,
-                                                          const float *__restrict__ xf, double *__restrict__ zpart) {
+                                                          const float *__restrict__ xf, double *__restrict__ zpart,
+                                                          const int *__restrict__ active) {
   // one task per wave, blockDim.x / 64 waves per workgroup; the task index is
   // uniform across the wave: scalar loads of the task and a scalar branch on its rank
   const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (t >= ntasks) return;
   const VTask task = tasks[t];
+  if (active && !active[task.net]) return;  // the block's network sits out this evaluation
   switch (task.k) {
 #define DFN_VTX_CASE(K) \
   case K: vtx_body<K, TV>(task, vdata, w, xf, zpart, w, lane); break;
@@ -218,10 +220,11 @@ typedef double dv2 __attribute__((ext_vector_type(2)));
 template <int R>
 __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict__ tasks, const double *__restrict__ pdata,
                                                         const int *__restrict__ widx, const double *__restrict__ w,
-                                                        double *__restrict__ ypart) {
+                                                        double *__restrict__ ypart, const int *__restrict__ active) {
   __shared__ double2 wl2[kPanelMaxCols / 2];
   double *wl = reinterpret_cast<double *>(wl2);
   const PTask task = tasks[blockIdx.x];
+  if (active && !active[task.net]) return;  // the panel's network sits out this evaluation
   for (int j = threadIdx.x; j < task.L; j += 256) wl[j] = w[widx[task.widx + j]];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -301,10 +304,12 @@ __device__ __forceinline__ void acc4(fv4 v, float4 wv, int j, int nd, float &s, 
 template <int R>
 __global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict__ tasks, const float *__restrict__ pdata,
                                                         const int *__restrict__ widx, const float *__restrict__ cscale,
-                                                        const double *__restrict__ w, double *__restrict__ ypart) {
+                                                        const double *__restrict__ w, double *__restrict__ ypart,
+                                                        const int *__restrict__ active) {
   __shared__ float4 wl4[kPanelMaxCols / 4];
   float *wl = reinterpret_cast<float *>(wl4);
   const PTask task = tasks[blockIdx.x];
+  if (active && !active[task.net]) return;  // the panel's network sits out this evaluation
   for (int j = threadIdx.x; j < task.L; j += 256)
     wl[j] = (float)(w[widx[task.widx + j]] * (double)cscale[task.widx + j]);
   __syncthreads();
@@ -779,7 +784,7 @@ __global__ void __launch_bounds__(256) ens_cg_dt_kernel(CgSys *sys, int nnet, in
                                                         const double *h, int first_slot) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nnet * nslot) return;
-  const int b = q / nslot, slot = first_slot + q % nslot;
+  const int b = sys[q].net, slot = first_slot + q % nslot;  // nslot consecutive systems per network
   const double A[6] = {0.0, CK::A2, CK::A3, CK::A4, CK::A5, CK::A6};
   sys[q].dt = slot == 0 ? t_try[b] : 0.0 + A[slot] * h[b];
 }
@@ -1181,8 +1186,15 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
                                                       unsigned long long *records) {
   extern __shared__ double coop_lds[];
   __shared__ double red[16];
-  const int sys = blockIdx.x / wps, part = blockIdx.x - sys * wps;
+  int sys, part, first;  // this workgroup's system, its span index, the system's first workgroup
+  if (A.wg_map) {        // systems of several networks, of their own sizes
+    const int4 m = A.wg_map[blockIdx.x];
+    sys = m.x; part = m.y; wps = m.z; first = m.w;
+  } else {
+    sys = blockIdx.x / wps; part = blockIdx.x - sys * wps; first = sys * wps;
+  }
   const CgSys S = A.sys[sys];
+  if (!cg_view(A, X, S)) return;  // every workgroup of a network that sits out returns
   const int n = A.n;
   const double dt = S.dt, c = A.cfac;
   const double *ds = A.ds, *kx = A.perm_x, *kxd = A.perm_x_diag, *P = A.P;
@@ -1194,7 +1206,7 @@ __global__ void __launch_bounds__(256) cg_coop_kernel(CgArgs A, CrossRows X, int
   // direction messages, one per element: the two words of element e are w0[e], w1[e]
   unsigned long long *w0 = reinterpret_cast<unsigned long long *>(S.p);
   unsigned long long *w1 = reinterpret_cast<unsigned long long *>(S.Ap);
-  unsigned long long *rec = records + (size_t)kCgRec * sys * wps;
+  unsigned long long *rec = records + (size_t)kCgRec * first;
   unsigned long long *myrec = rec + (size_t)kCgRec * part;
   const unsigned ep0 = seq << 8;
   double *lp = coop_lds;  // lp[1 + k - span0] = p at element k; lp[0] and lp[1 + span1 - span0] are the halo
@@ -1591,15 +1603,15 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
 }
 
 void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
-                int waves, const float *xf) {
+                int waves, const float *xf, const int *active) {
   if (nt <= 0) return;
   if (waves != 2 && waves != 8) waves = 4;
   if (fp32)
     hipLaunchKernelGGL(lowrank_vtx_kernel<float>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
-                       (const float *)vdata, w, xf, zpart);
+                       (const float *)vdata, w, xf, zpart, active);
   else
     hipLaunchKernelGGL(lowrank_vtx_kernel<double>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
-                       (const double *)vdata, w, xf, zpart);
+                       (const double *)vdata, w, xf, zpart, active);
 }
 
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s) {
@@ -1608,16 +1620,20 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
 }
 
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
-                  const double *w, double *ypart, hipStream_t s, int rows) {
+                  const double *w, double *ypart, hipStream_t s, int rows, const int *active) {
   if (nt <= 0) return;
   if (fp32 && rows == 4)
-    hipLaunchKernelGGL(panel_kernel_f32<4>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart);
+    hipLaunchKernelGGL(panel_kernel_f32<4>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart,
+                       active);
   else if (fp32)
-    hipLaunchKernelGGL(panel_kernel_f32<2>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart);
+    hipLaunchKernelGGL(panel_kernel_f32<2>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart,
+                       active);
   else if (rows == 4)
-    hipLaunchKernelGGL(panel_kernel_f64<4>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
+    hipLaunchKernelGGL(panel_kernel_f64<4>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart,
+                       active);
   else
-    hipLaunchKernelGGL(panel_kernel_f64<2>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart);
+    hipLaunchKernelGGL(panel_kernel_f64<2>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart,
+                       active);
 }
 
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
@@ -1792,6 +1808,17 @@ bool launch_cg_lds(const CgArgs &a, const CrossRows &xr, int nsys, hipStream_t s
 }
 
 // every system's network has at most max_n elements; the kernel is picked by max_n
+bool launch_cg_coop_map(const CgArgs &a, const CrossRows &xr, int nwg, int cap, unsigned seq,
+                        unsigned long long *records, hipStream_t s) {
+  if (cap > kCgCoopMaxEntries || !a.wg_map || nwg < 1) return false;
+  const size_t sh = cg_coop_lds_bytes(cap);
+  CgArgs A = a;
+  CrossRows X = xr;
+  int wps = 0;  // taken from the map
+  void *args[] = {&A, &X, &wps, &cap, &seq, &records};
+  return hipLaunchCooperativeKernel((const void *)cg_coop_kernel, dim3(nwg), dim3(256), args, sh, s) == hipSuccess;
+}
+
 void launch_cg_lds_multi(const CgArgs &a, const CrossRows &xr, int nsys, int max_n, hipStream_t s) {
   (void)cg_lds_dispatch(a, xr, nsys, max_n, s);
 }

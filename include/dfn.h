@@ -332,7 +332,9 @@ int dfn_ensemble_build_hmat(dfn_ensemble e);
 /* DFN_PREC_FP64 / DFN_PREC_FP32 operator storage, as dfn_set_precision. */
 int dfn_ensemble_set_precision(dfn_ensemble e, int32_t precision);
 /* 1 (default): the per-attempt launch sequences are captured as HIP graphs and
- * replayed; 0: the same launches are issued one by one. */
+ * replayed; 0: the same launches are issued one by one.  Ensembles with networks
+ * above 4096 elements solve their pressure systems with cooperative launches,
+ * which are always issued one by one (the request is ignored there). */
 int dfn_ensemble_set_graph(dfn_ensemble e, int32_t use_graph);
 /* Every network advances nsteps accepted steps. */
 int dfn_ensemble_solve(dfn_ensemble e, int64_t nsteps);

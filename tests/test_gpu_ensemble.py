@@ -72,15 +72,61 @@ def test_ensemble_fp32_operators_equal_independent_fp32_runs():
     _compare(_nets(2, n=4, seed=9), 5, prec=_abi.PREC_FP32)
 
 
-def test_ensemble_with_a_large_network_uses_the_plain_compute_P(monkeypatch):
-    """A network above the one-workgroup compute_P size: the ensemble solves every
-    system with the plain one-workgroup kernel, as a single run does under
-    DFN_CG_PATH=plain."""
+@pytest.mark.parametrize("cg_path", ["auto", "plain"])
+def test_ensemble_with_a_large_network(cg_path, monkeypatch):
+    """A network above the one-workgroup compute_P size beside two small ones: the
+    ensemble solves the large network's systems with the cooperative kernel (by
+    default; each system over its own span of workgroups, the launches mapping
+    workgroups to systems) or the plain one-workgroup kernel (DFN_CG_PATH=plain),
+    the small ones with the one-workgroup LDS kernel, as single runs do."""
+    monkeypatch.setenv("DFN_CG_PATH", cg_path)
     big = problems.fault_network(n_faults=120, box=2000.0, lmin=100.0, lmax=800.0, alpha=0.5, seed=3, eta=2.0,
                                  eps_aca=1e-6, nleaf=32, tol=1e-8)
     assert big.N > 4096
     probs = [big] + _nets(2, n=2, seed=13)
-    _compare(probs, 3, env={"DFN_CG_PATH": "plain"}, monkeypatch=monkeypatch)
+    _compare(probs, 3)
+
+
+@pytest.mark.timeout(900)
+def test_full_size_ensemble_networks_equal_independent_runs():
+    """The 64-network ensemble at its stated size (50-400 faults per network in
+    7 km boxes, ~1.4M elements): after 10 FP64 steps and 10 more with FP32
+    operators, the smallest, the median and the largest network are bit-identical
+    to their own runs of the same 20 steps."""
+    import time
+    from demystification_dfn_amd.device import DeviceEnsemble, DeviceSimulation
+    probs = problems.network_ensemble(64, faults=(50, 400), box=7000.0, seed=2024)
+    Ns = np.array([p.N for p in probs])
+    picks = [int(np.argmin(Ns)), int(np.argsort(Ns)[len(Ns) // 2]), int(np.argmax(Ns))]
+    print(f"[gpu] full-size ensemble: {len(probs)} networks, {Ns.sum()} elements, checking {picks} "
+          f"(N={Ns[picks].tolist()})", flush=True)
+    t0 = time.perf_counter()
+    ens = DeviceEnsemble(probs)
+    ens.initiate_hmat()
+    print(f"[gpu] ensemble built in {time.perf_counter() - t0:.1f} s", flush=True)
+    ens.solve(10)
+    ens.set_precision(_abi.PREC_FP32)
+    print(f"[gpu] 10 FP64 steps done, FP32 layout at {time.perf_counter() - t0:.1f} s", flush=True)
+    ens.solve(10)
+    print(f"[gpu] 10 FP32 steps done at {time.perf_counter() - t0:.1f} s", flush=True)
+    for i in picks:
+        d = DeviceSimulation(probs[i])
+        d.initiate_hmat()
+        d.solve(10)
+        d.set_precision(_abi.PREC_FP32)
+        sd = d.solve(10)
+        Yd, Pd, Pdd, td = d.get_state()
+        Ye, Pe, Pde, te = ens.get_state(i)
+        se = ens.stats(i)
+        print(f"[gpu] network {i} N={probs[i].N}: accepted {se['accepted_steps']} rejected "
+              f"{se['rejected_attempts']} t={te:.6e}", flush=True)
+        assert se["accepted_steps"] == sd["accepted_steps"] == 20
+        assert se["rejected_attempts"] == sd["rejected_attempts"]
+        assert te == td
+        np.testing.assert_array_equal(Ye, Yd)
+        np.testing.assert_array_equal(Pe, Pd)
+        np.testing.assert_array_equal(Pde, Pdd)
+        d.close()
 
 
 def test_pi_and_learned_controllers_track_the_reference_controller():
