@@ -248,7 +248,7 @@ struct H5 {
     if (!have_space || !have_type || ds.layout < 0) return DFN_ERR_UNSUPPORTED;
     if (ds.filtered) return DFN_ERR_UNSUPPORTED;
     uint64_t n = 0;
-    if (!count(ds, n) || n == 0) return DFN_ERR_UNSUPPORTED;  // overflowing or empty extents
+    if (!count(ds, n)) return DFN_ERR_UNSUPPORTED;  // overflowing extents
     uint64_t bytes = 0;
     if (__builtin_mul_overflow(n, (uint64_t)ds.size, &bytes)) return DFN_ERR_UNSUPPORTED;
     return DFN_OK;
@@ -270,7 +270,9 @@ struct H5 {
   // The bytes are checked to lie in the file before anything is allocated.
   bool raw(const Dataset &ds, std::vector<uint8_t> &out) const {
     uint64_t c = 0, n = 0;
-    if (!count(ds, c) || __builtin_mul_overflow(c, (uint64_t)ds.size, &n) || n == 0) return false;
+    if (!count(ds, c) || __builtin_mul_overflow(c, (uint64_t)ds.size, &n)) return false;
+    out.clear();
+    if (n == 0) return true;  // an empty dataset
     if (ds.layout == 0) {
       if (ds.compact.size() < n) return false;
       out.assign(ds.compact.begin(), ds.compact.begin() + (long)n);
