@@ -149,6 +149,8 @@ SIGNATURES = {
     "ensemble_get_state": [_H, C.c_int32, _dp, _dp, _dp, C.POINTER(C.c_double)],
     "ensemble_get_stats": [_H, C.c_int32, C.POINTER(DfnRunStats)],
     "ensemble_info": [_H, _i64p],
+    "ensemble_kernel_timing": [_H, C.c_int32],
+    "ensemble_kernel_timing_read": [_H, _dp, _i64p, _dp],
     "ensemble_set_controller": [_H, C.c_int32, _H],
     "ensemble_set_recording": [_H, C.c_int32],
     "ensemble_take_records": [_H, _fp, _fp, C.c_int64, _i64p],

@@ -197,9 +197,9 @@ void launch_save_output(const double *Y, const double *P, const double *sT, cons
                         int64_t ld, int64_t col, hipStream_t s);
 void launch_monitor(const double *Vold, const double *Vnew, double h, const int *fault_of, int n, double *slip,
                     unsigned long long *fvmax, int nfault, hipStream_t s);
-// per-network max |V| over real elements (bits of non-negative doubles, cleared first)
-void launch_ens_vmax(const double *V, const int *net, const uint8_t *fake, int n, unsigned long long *vmax, int nnet,
-                     hipStream_t s);
+// per network, over its real elements: max |V|, mean log10 |V|, fraction with
+// |V| > 1e-3 m/s (out[3 b + 0..2]), reduced in a fixed order
+void launch_ens_vstats(const double *V, const uint8_t *fake, const CgNet *nets, int nnet, double *out, hipStream_t s);
 // dt of every (network, slot) system of an ensemble compute_P launch: slot 0 and,
 // when with_stages, slots 1..5 (x + A_i h with x = 0), from t_try (slot 0) and h
 void launch_ens_cg_dt(CgSys *sys, int nnet, int nslot, const double *t_try, const double *h, int first_slot,

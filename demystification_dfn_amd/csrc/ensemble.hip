@@ -137,6 +137,14 @@ struct EnsEngine {
   std::vector<dfn_run_stats> st;
   hipGraphExec_t g_try = nullptr, g_retry = nullptr;
   const DevOps *graph_ops = nullptr;
+  // live timing of the phase-2 panel kernel (dfn_ensemble_kernel_timing): an event
+  // pair around each panel launch of an evaluation in which every network takes
+  // part (launches issued one by one while it is on)
+  bool kt_on = false, kt_graph = true, all_active = false;
+  std::vector<hipEvent_t> kt_ev;
+  int kt_used = 0;
+  double kt_ms = 0.0;
+  int64_t kt_count = 0;
 
   // step-size control: 0 = rkqs's proposal (the reference), 1 = PI controller,
   // 2 = the proposal network with the PI controller as fallback
@@ -144,12 +152,12 @@ struct EnsEngine {
   Mlp *mlp = nullptr;
   bool recording = false;
   std::vector<float> rec_X, rec_T;  // (features, log10 accepted-step ratio) pairs
-  std::vector<double> prev_hdid, prev_err, prev_vmax, pend_href;
+  std::vector<double> prev_err, prev_err2, prev_lmax, pend_href;
   std::vector<int> fallback, learned_last, has_pend, pend_skip;
   std::vector<float> pend_feat;
-  DMem vmax_bits, feat_d, out_d;
+  DMem vstats, feat_d, out_d;
   float *h_feat = nullptr, *h_out = nullptr;
-  unsigned long long *h_vmax = nullptr;
+  double *h_vstats = nullptr;  // per network: max |V|, mean log10 |V|, fraction above 1e-3 m/s
   static constexpr int kFallbackSteps = 3;
 
   double *kk(int i) const { return kbuf.as<double>() + (size_t)i * 3 * Ntot; }
@@ -157,6 +165,7 @@ struct EnsEngine {
 
   ~EnsEngine() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t ev : kt_ev) (void)hipEventDestroy(ev);
     drop_graphs();
     if (h_h) (void)hipHostFree(h_h);
     if (h_ttry) (void)hipHostFree(h_ttry);
@@ -166,7 +175,7 @@ struct EnsEngine {
     if (h_nets) (void)hipHostFree(h_nets);
     if (h_feat) (void)hipHostFree(h_feat);
     if (h_out) (void)hipHostFree(h_out);
-    if (h_vmax) (void)hipHostFree(h_vmax);
+    if (h_vstats) (void)hipHostFree(h_vstats);
     ops64.reset();
     ops32.reset();
     if (cg_stream) (void)hipStreamSynchronize(cg_stream);
@@ -292,15 +301,15 @@ struct EnsEngine {
     time.assign(B, 0.0);
     t_try.assign(B, 100.0);
     st.assign(B, dfn_run_stats{});
-    DFN_TRY(vmax_bits.alloc(sizeof(unsigned long long) * B));
+    DFN_TRY(vstats.alloc(sizeof(double) * 3 * B));
     DFN_TRY(feat_d.alloc(sizeof(float) * DFN_MLP_INPUTS * B));
     DFN_TRY(out_d.alloc(sizeof(float) * B));
     DFN_HIP(hipHostMalloc((void **)&h_feat, sizeof(float) * DFN_MLP_INPUTS * B, hipHostMallocDefault));
     DFN_HIP(hipHostMalloc((void **)&h_out, sizeof(float) * B, hipHostMallocDefault));
-    DFN_HIP(hipHostMalloc((void **)&h_vmax, sizeof(unsigned long long) * B, hipHostMallocDefault));
-    prev_hdid.assign(B, 0.0);
-    prev_err.assign(B, 0.0);
-    prev_vmax.assign(B, 0.0);
+    DFN_HIP(hipHostMalloc((void **)&h_vstats, sizeof(double) * 3 * B, hipHostMallocDefault));
+    prev_err.assign(B, -1.0);   // < 0: no earlier accepted step
+    prev_err2.assign(B, -1.0);
+    prev_lmax.assign(B, 0.0);
     pend_href.assign(B, 0.0);
     fallback.assign(B, 0);
     learned_last.assign(B, 0);
@@ -391,17 +400,25 @@ struct EnsEngine {
     return h * fac;
   }
 
-  // the proposal network's features of network i after a step
-  void features(int i, double hdid, double href, double err, int nrej, double vmax, float *f) const {
-    const double tiny = 1e-30;
-    f[0] = (float)std::log10(std::fmax(hdid, tiny));
-    f[1] = (float)std::log10(std::fmax(href, tiny));
-    f[2] = (float)std::log10(std::fmax(err, 1e-10));
-    f[3] = (float)nrej;
-    f[4] = (float)std::log10(std::fmax(vmax, tiny));
-    f[5] = prev_hdid[i] > 0.0 ? (float)std::log10(hdid / prev_hdid[i]) : 0.0f;
-    f[6] = prev_hdid[i] > 0.0 ? (float)std::log10(std::fmax(prev_err[i], 1e-10)) : f[2];
-    f[7] = prev_vmax[i] > 0.0 ? (float)std::log10(std::fmax(vmax, tiny) / prev_vmax[i]) : 0.0f;
+  // the proposal network's features of network i after an accepted step of size
+  // hdid with scaled error err: max and mean log10 |V|, the rate of change of max
+  // log10 |V| over the step, the scaled errors of the last three accepted steps
+  // (log10; the oldest repeat the newest at the start), log10 of the step and the
+  // fraction of elements slipping faster than 1e-3 m/s
+  void features(int i, double hdid, double err, const double *vs, float *f) const {
+    const double lmax = std::log10(std::fmax(vs[0], 1e-30));
+    const bool first = prev_err[i] < 0.0;
+    const double e0 = std::log10(std::fmax(err, 1e-10));
+    const double e1 = first ? e0 : std::log10(std::fmax(prev_err[i], 1e-10));
+    const double e2 = prev_err2[i] < 0.0 ? e1 : std::log10(std::fmax(prev_err2[i], 1e-10));
+    f[0] = (float)lmax;
+    f[1] = (float)vs[1];
+    f[2] = first ? 0.0f : (float)((lmax - prev_lmax[i]) / std::fmax(hdid, 1e-30));
+    f[3] = (float)e0;
+    f[4] = (float)e1;
+    f[5] = (float)e2;
+    f[6] = (float)std::log10(std::fmax(hdid, 1e-30));
+    f[7] = (float)vs[2];
   }
 
   // next trial steps from this step's outcome (t_try before the one-day cap)
@@ -413,17 +430,14 @@ struct EnsEngine {
         if (stepping[i]) t_try[i] = href[i];
       return DFN_OK;
     }
-    launch_ens_vmax(Y.as<double>(), net.as<int>(), fake.as<uint8_t>(), Ntot, vmax_bits.as<unsigned long long>(), B,
-                    stream);
-    DFN_HIP(hipMemcpyAsync(h_vmax, vmax_bits.p, sizeof(unsigned long long) * B, hipMemcpyDeviceToHost, stream));
+    launch_ens_vstats(Y.as<double>(), fake.as<uint8_t>(), cgnets.as<CgNet>(), B, vstats.as<double>(), stream);
+    DFN_HIP(hipMemcpyAsync(h_vstats, vstats.p, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, stream));
     DFN_HIP(hipStreamSynchronize(stream));
     std::vector<int> use_mlp(B, 0);
     for (int i = 0; i < B; i++) {
       if (!stepping[i]) continue;
-      double vmax;
-      std::memcpy(&vmax, &h_vmax[i], sizeof(double));
       float *f = h_feat + (size_t)i * DFN_MLP_INPUTS;
-      features(i, hdid[i], href[i], err[i], nrej[i], vmax, f);
+      features(i, hdid[i], err[i], h_vstats + 3 * (size_t)i, f);
       if (recording) {
         if (has_pend[i] && !pend_skip[i] && !guarded[i] && !clipped[i]) {
           rec_X.insert(rec_X.end(), pend_feat.begin() + (size_t)i * DFN_MLP_INPUTS,
@@ -442,9 +456,9 @@ struct EnsEngine {
       else use_mlp[i] = 1;
       learned_last[i] = use_mlp[i];
       if (controller == 2 && !use_mlp[i] && fallback[i] > 0) fallback[i]--;
-      prev_hdid[i] = hdid[i];
+      prev_err2[i] = prev_err[i];
       prev_err[i] = err[i];
-      prev_vmax[i] = std::fmax(vmax, 1e-30);
+      prev_lmax[i] = f[0];
     }
     if (controller == 2) {
       if (!mlp) return DFN_ERR_STATE;
@@ -502,8 +516,33 @@ struct EnsEngine {
 
   const DevOps *active_ops() const { return (precision == DFN_PREC_FP32 && ops32) ? ops32.get() : ops64.get(); }
 
+  hipEvent_t *kt_next() {
+    if (!kt_on || !all_active) return nullptr;
+    if (kt_used >= 1024 && kt_collect() != DFN_OK) return nullptr;
+    while ((int)kt_ev.size() < 2 * (kt_used + 1)) {
+      hipEvent_t ev = nullptr;
+      if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+      kt_ev.push_back(ev);
+    }
+    return &kt_ev[2 * kt_used++];
+  }
+
+  int kt_collect() {
+    if (kt_used == 0) return DFN_OK;
+    DFN_HIP(hipEventSynchronize(kt_ev[2 * kt_used - 1]));
+    for (int i = 0; i < kt_used; i++) {
+      float t = 0.0f;
+      DFN_HIP(hipEventElapsedTime(&t, kt_ev[2 * i], kt_ev[2 * i + 1]));
+      kt_ms += t;
+      kt_count++;
+    }
+    kt_used = 0;
+    return DFN_OK;
+  }
+
   void apply(const double *dV) {
     const DevOps *d = active_ops();
+    hipEvent_t *kt = kt_next();
     double *w = d->w.as<double>();
     launch_gather(dV, d->perm.as<int>(), w, Ntot, stream, d->xf.as<float>());
     // the tasks of networks that sit out this attempt (accepted, or waiting for
@@ -511,8 +550,10 @@ struct EnsEngine {
     launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves,
                d->xf.as<float>(), d_active.as<int>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
+    if (kt) (void)hipEventRecord(kt[0], stream);
     launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
                  d->ypart.as<double>(), stream, d->panel_rows, d_active.as<int>());
+    if (kt) (void)hipEventRecord(kt[1], stream);
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), Ntot,
                     d->nops, kscale, sT.as<double>(), d->nops > 1 ? sN.as<double>() : sT.as<double>(), stream);
   }
@@ -696,13 +737,16 @@ struct EnsEngine {
     // first attempt (with compute_P(t_try) halving, :81-92)
     std::vector<int> act = todo;
     for (;;) {
+      all_active = true;
       for (int i = 0; i < B; i++) {
         h_ttry[i] = t_try[i];
         h_h[i] = t_try[i];
         h_active[i] = act[i];
+        if (!act[i]) all_active = false;
       }
       DFN_TRY(upload_params());
       DFN_TRY(run(false));
+      all_active = false;
       DFN_TRY(readback());
       bool again = false;
       for (int i = 0; i < B; i++) {
@@ -946,6 +990,38 @@ API int dfn_ensemble_take_records(dfn_ensemble h, float *X, float *T, int64_t ca
   e.rec_X.erase(e.rec_X.begin(), e.rec_X.begin() + n * DFN_MLP_INPUTS);
   e.rec_T.erase(e.rec_T.begin(), e.rec_T.begin() + n);
   *count = n;
+  return DFN_OK;
+  DFN_API_END
+}
+
+API int dfn_ensemble_kernel_timing(dfn_ensemble h, int32_t on) {
+  DFN_API_BEGIN
+  if (!h) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  if (on) {
+    if (!e.kt_on) e.kt_graph = e.use_graph;
+    e.use_graph = false;  // events are recorded around launches issued one by one
+    e.kt_on = true;
+    e.kt_used = 0;
+    e.kt_ms = 0.0;
+    e.kt_count = 0;
+  } else if (e.kt_on) {
+    DFN_TRY(e.kt_collect());
+    e.kt_on = false;
+    e.use_graph = e.kt_graph && !e.coop;
+  }
+  return DFN_OK;
+  DFN_API_END
+}
+
+API int dfn_ensemble_kernel_timing_read(dfn_ensemble h, double *ms_mean, int64_t *count, double *bytes) {
+  DFN_API_BEGIN
+  if (!h) return DFN_ERR_ARG;
+  EnsEngine &e = h->e;
+  DFN_TRY(e.kt_collect());
+  if (ms_mean) *ms_mean = e.kt_count ? e.kt_ms / (double)e.kt_count : 0.0;
+  if (count) *count = e.kt_count;
+  if (bytes) *bytes = e.active_ops() ? e.active_ops()->panel_bytes : 0.0;
   return DFN_OK;
   DFN_API_END
 }

@@ -728,25 +728,6 @@ __global__ void __launch_bounds__(256) ens_restart_stage2_kernel(EnsStageArgs E)
   }
 }
 
-// max |V| over each network's real elements, as the bits of a non-negative double
-__global__ void __launch_bounds__(256) ens_vmax_kernel(const double *__restrict__ V, const int *__restrict__ net,
-                                                       const uint8_t *__restrict__ fake, int n,
-                                                       unsigned long long *__restrict__ vmax) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nb = k < n ? net[k] : -1;
-  const double v = (k < n && !fake[k]) ? fabs(V[k]) : 0.0;
-  const int first = __shfl(nb, 0, 64);
-  if (__all(nb == first)) {
-    const double m = wave_max(v);
-    if ((threadIdx.x & 63) == 0 && first >= 0) atomicMax(&vmax[first], (unsigned long long)__double_as_longlong(m));
-  } else if (nb >= 0) {
-    atomicMax(&vmax[nb], (unsigned long long)__double_as_longlong(v));
-  }
-}
-
-// After an accepted step of length h: slip += (V_old + V_new) h / 2 per element
-// (the trapezoid rule of a cumulative time integral) and max |V_new| per fault
-// (bits of non-negative doubles; fake elements, fault -1, are skipped).
 __global__ void __launch_bounds__(256) monitor_kernel(const double *__restrict__ Vold, const double *__restrict__ Vnew,
                                                       double h, const int *__restrict__ fault_of, int n,
                                                       double *__restrict__ slip, unsigned long long *__restrict__ fvmax) {
@@ -819,6 +800,34 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   double s = 0.0;
   for (int q = 0; q < nw; q++) s = fmax(s, red[q]);
   return s;
+}
+
+// Per network (one workgroup each, in a fixed order): over its real elements, max
+// |V|, the mean of log10 |V| and the fraction with |V| > 1e-3 m/s; out[3 b + 0..2].
+__global__ void __launch_bounds__(256) ens_vstats_kernel(const double *__restrict__ V,
+                                                         const uint8_t *__restrict__ fake,
+                                                         const CgNet *__restrict__ nets, double *__restrict__ out) {
+  __shared__ double red[16];
+  const CgNet c = nets[blockIdx.x];
+  double mx = 0.0, sl = 0.0, fast = 0.0, nreal = 0.0;
+  for (int k = threadIdx.x; k < c.n; k += blockDim.x) {
+    const int e = c.base + k;
+    if (fake[e]) continue;
+    const double v = fabs(V[e]);
+    mx = fmax(mx, v);
+    sl += log10(fmax(v, 1e-30));
+    fast += v > 1e-3 ? 1.0 : 0.0;
+    nreal += 1.0;
+  }
+  mx = block_max(mx, red);
+  sl = block_sum(sl, red);
+  fast = block_sum(fast, red);
+  nreal = block_sum(nreal, red);
+  if (threadIdx.x == 0) {
+    out[3 * blockIdx.x] = mx;
+    out[3 * blockIdx.x + 1] = sl / fmax(nreal, 1.0);
+    out[3 * blockIdx.x + 2] = fast / fmax(nreal, 1.0);
+  }
 }
 
 __device__ __forceinline__ double hmean(double pa, double pb) { return 2.0 * pa * pb / (pa + pb); }
@@ -1689,10 +1698,8 @@ void launch_ens_restart_stage2(const EnsStageArgs &a, hipStream_t s) {
   hipLaunchKernelGGL(ens_restart_stage2_kernel, dim3(nblk(a.a.n, 256)), dim3(256), 0, s, a);
 }
 
-void launch_ens_vmax(const double *V, const int *net, const uint8_t *fake, int n, unsigned long long *vmax, int nnet,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(fill_u64_kernel, dim3(nblk(nnet, 256)), dim3(256), 0, s, vmax, nnet);
-  hipLaunchKernelGGL(ens_vmax_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, V, net, fake, n, vmax);
+void launch_ens_vstats(const double *V, const uint8_t *fake, const CgNet *nets, int nnet, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(ens_vstats_kernel, dim3(nnet), dim3(256), 0, s, V, fake, nets, out);
 }
 
 // One save of the solver output (save_data, src/solver.f90:211-222): column col

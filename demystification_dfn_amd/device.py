@@ -189,6 +189,16 @@ class DeviceEnsemble:
         return dict(zip(("networks", "elements", "max_elements", "S_entries", "alg_bytes", "graph"),
                         (int(x) for x in v)))
 
+    def kernel_timing(self, on=True):
+        """Live per-launch timing of the phase-2 panel kernel (dfn_ensemble_kernel_timing)."""
+        _abi.check(self._fns["ensemble_kernel_timing"](self._h, 1 if on else 0), "dfn_ensemble_kernel_timing")
+
+    def kernel_timing_read(self):
+        ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+        _abi.check(self._fns["ensemble_kernel_timing_read"](self._h, C.byref(ms), C.byref(n), C.byref(b)),
+                   "dfn_ensemble_kernel_timing_read")
+        return ms.value, n.value, b.value
+
 
 class DeviceMlp:
     """The step-size proposal network on the device (dfn_mlp_*): 8 -> 64 -> 64 -> 1,

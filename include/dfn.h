@@ -360,11 +360,12 @@ int dfn_ensemble_get_stats(dfn_ensemble e, int32_t net, dfn_run_stats *stats);
 #define DFN_STEP_LEARNED 2
 int dfn_ensemble_set_controller(dfn_ensemble e, int32_t mode, dfn_mlp m);
 /* Records (features, target) pairs for the proposal network after every step:
- * features of network i after step n (log10 accepted step, log10 rkqs proposal,
- * log10 scaled error, rejected attempts, log10 max |V|, log10 step growth,
- * log10 previous scaled error, log10 max |V| growth) and target log10(accepted
- * step n+1 / rkqs proposal after step n); steps set by the one-day cap or the
- * injection guard are left out. */
+ * features of network i after accepted step n: max over its real elements of
+ * log10 |V|, their mean log10 |V|, the rate of change of max log10 |V| over the
+ * step (per second), log10 of the scaled error (errmax) of steps n, n-1 and n-2,
+ * log10 of the step, and the fraction of elements with |V| > 1e-3 m/s; target
+ * log10(accepted step n+1 / rkqs proposal after step n).  Steps set by the
+ * one-day cap, the injection guard or a final time are left out. */
 int dfn_ensemble_set_recording(dfn_ensemble e, int32_t on);
 /* Moves up to cap recorded pairs (X: 8 floats each, T: 1) out of the ensemble;
  * with X or T NULL, *count = pairs available. */
@@ -372,6 +373,14 @@ int dfn_ensemble_take_records(dfn_ensemble e, float *X, float *T, int64_t cap, i
 /* info[0..5] = networks, total elements, largest network, matrix entries streamed
  * per stress evaluation, algorithmic bytes per stress evaluation, graphs on. */
 int dfn_ensemble_info(dfn_ensemble e, int64_t *info);
+/* Per-launch timing of the phase-2 panel kernel while the ensemble steps (no
+ * reference counterpart): with timing on, every stress evaluation in which all
+ * networks take part records a HIP event pair around its panel launch on the
+ * ensemble's stream (the launches are then issued one by one).  1 clears the
+ * sums and starts, 0 stops; _read gives the mean duration in ms, the launches
+ * timed and the kernel's algorithmic bytes per such launch. */
+int dfn_ensemble_kernel_timing(dfn_ensemble e, int32_t on);
+int dfn_ensemble_kernel_timing_read(dfn_ensemble e, double *ms_mean, int64_t *count, double *bytes);
 
 #ifdef __cplusplus
 }

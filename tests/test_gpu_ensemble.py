@@ -213,3 +213,45 @@ def test_bench_ensemble_sampled_networks_equal_independent_runs():
         assert te == td
         np.testing.assert_array_equal(Ye, Yd)
         np.testing.assert_array_equal(Pe, Pd)
+
+
+def test_trained_proposal_network_against_rkqs():
+    """The proposal network trained on the GPU on 1e6 recorded pairs
+    (demystification_dfn_amd/data/step_network.npz, written by
+    train_step_network) on held-out networks: rkqs (src/ode_solver/ode_bs.f90:
+    156-170) and the learned controller step the same 40 seeded networks
+    (tolerances 1e-6 to 1e-10) from t = 0 to a common final time.  Reported: the
+    rejected-attempt fraction, the stress evaluations (6 per first attempt, 5 per
+    retried one) and the aggregate simulated time per wall hour of each.  The
+    learned controller may not cost more: at most 1 % more stress evaluations and
+    no more rejected attempts than rkqs."""
+    import time
+    from demystification_dfn_amd import train_step_network as tsn
+    from demystification_dfn_amd.device import DeviceEnsemble, DeviceMlp
+    params, norm, meta = tsn.load()
+    assert meta["recording"]["pairs"] >= 1_000_000
+    assert meta["training"]["loss_final"] < 0.5 * meta["training"]["loss_initial"]
+    mlp = DeviceMlp()
+    mlp.set_params(params, norm)
+    probs = tsn.recording_ensemble(40, seed=99)  # seeds 11.. were recorded for training
+    t_end = 2.0e6
+    out = {}
+    for key, mode in (("rkqs", _abi.STEP_RKQS), ("learned", _abi.STEP_LEARNED)):
+        ens = DeviceEnsemble(probs)
+        ens.initiate_hmat()
+        ens.set_controller(mode, mlp if mode == _abi.STEP_LEARNED else None)
+        w0 = time.perf_counter()
+        ens.solve_until(t_end, max_steps=200000)
+        wall = time.perf_counter() - w0
+        st = [ens.stats(i) for i in range(len(probs))]
+        acc = sum(s["accepted_steps"] for s in st)
+        rej = sum(s["rejected_attempts"] for s in st)
+        out[key] = dict(accepted=acc, rejected=rej, rejected_fraction=rej / (acc + rej), evaluations=6 * acc + 5 * rej,
+                        network_yr_per_hr=len(probs) * t_end / 3.15576e7 / (wall / 3600.0))
+        print(f"[gpu] {key}: {out[key]}", flush=True)
+        ens.close()
+    r, l = out["rkqs"], out["learned"]
+    print(f"[gpu] learned / rkqs: evaluations {l['evaluations'] / r['evaluations']:.4f}, rejected "
+          f"{l['rejected']} vs {r['rejected']}, rate {l['network_yr_per_hr'] / r['network_yr_per_hr']:.3f}", flush=True)
+    assert l["rejected"] <= r["rejected"]
+    assert l["evaluations"] <= 1.01 * r["evaluations"]
