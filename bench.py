@@ -1,15 +1,23 @@
 """Benchmark of the per-RK-stage hot path on one MI355X.
 
-A step is one accepted step of the solver loop (src/solver.f90:59-198): CG
-pressure solves, the stress-transfer H-matvec and the rate-state RHS at every
-Cash-Karp stage, the embedded error norm and the step controller, all on the
-device.  The default workload is the seeded 800-fault Mode II network (~80k
-elements, H-matrix eta 2 / ACA 1e-6 / leaf 64, FP64).
+A step is one accepted step of the solver loop (src/solver.f90:59-198) of every
+network: CG pressure solves, the stress-transfer H-matvec and the rate-state RHS
+at every Cash-Karp stage, the embedded error norm and the step controller, all
+on the device.
 
-Prints ONE JSON line: simulated years per wall-clock hour (whole job), the
-roofline of the dominant kernel (the H-matvec panel kernel, HIP events on the
-library's stream) and a CPU baseline (the oracle, timed on this host's cores on
-a bounded sample continuing from the GPU's final state).
+The default workload is the headline one: an ensemble of 64 seeded fault
+networks of 50-400 faults each (~1.37M elements, H-matrix eta 2 / ACA 1e-6 /
+leaf 64) advanced together, with FP32 operator storage (FP64 state, RHS, CG and
+error norm) and the learned step-size proposals (data/step_network.npz), against
+the plain FP64 path (one dfn_handle per network, FP64 operators, rkqs) over the
+same simulated interval of every network.
+
+Prints ONE JSON line: simulated network-years per wall-clock hour of the whole
+ensemble, the ratio to the plain FP64 path, the roofline of the dominant kernel
+(the FP32 phase-2 panel kernel, HIP events on the ensemble's stream), a CPU
+baseline (the oracle on this host's cores, from the state at the start of the
+timed window), and the 80k-element single network (FP64 and FP32 operators) and
+its hydraulically connected variant as sub-objects.
 """
 import argparse
 import json
@@ -61,14 +69,37 @@ def make_workload(name, seed):
     return p, desc
 
 
-def cpu_baseline_reference(sim, p, seconds):
+def cpu_info():
+    """The host cores this job may use: OMP_NUM_THREADS when the host sets it
+    (the cores allotted per GPU), else the cores of this process's affinity mask;
+    with the CPU model and the logical CPU count for the record."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(omp) if omp.isdigit() and int(omp) > 0 else affinity
+    return dict(model=model, logical_cpus=os.cpu_count(), affinity_cpus=affinity, omp_num_threads=omp or None,
+                threads=threads)
+
+
+def cpu_baseline_reference(state, p, seconds):
     """The reference's own Fortran modules (oracle/_ref, built with the reference's
-    -O1 serial flags) continuing from the device's state, timed for ~`seconds`."""
+    -O1 serial flags) from `state` = (Y, P, time, t_try), timed for ~`seconds`."""
     from oracle.oracle import RefSimulation
     r = RefSimulation(p)
-    Y, P, _, t = sim.get_state()
+    Y, P, t, t_try = state
     r.set_state(Y, P, t)
-    r.set_t_try(sim.stats()["t_try"])
+    r.set_t_try(t_try)
     steps = 0
     w0 = time.perf_counter()
     while time.perf_counter() - w0 < seconds:
@@ -76,35 +107,46 @@ def cpu_baseline_reference(sim, p, seconds):
     wall = time.perf_counter() - w0
     sim_dt = r.get_state()[3] - t
     return dict(value=(sim_dt / YEAR) / (wall / 3600.0), unit="simulated yr / wall hr", cores=1, kind="reference",
+                steps=steps, wall_s=wall, sim_s=sim_dt, ms_per_step=1e3 * wall / max(steps, 1),
+                mean_dt_s=sim_dt / max(steps, 1),
                 sample=f"{steps} accepted steps of the reference Fortran (directIII, friction law, compute_P, rkqs; "
-                       f"-O1, one thread) continuing from the GPU's final state ({wall:.1f} s wall, {sim_dt:.3e} s "
-                       f"simulated)")
+                       f"-O1, one thread) from the GPU's state at the start of its timed window ({wall:.1f} s wall, "
+                       f"{sim_dt:.3e} s simulated)")
 
 
-def cpu_baseline(sim, p, seconds, threads):
-    """The oracle (plain-C restatement) continuing from the device's state on the
-    same operator block lists, timed for ~`seconds` of wall time."""
+def cpu_baseline(sim, state, p, seconds, threads, t_stop=None):
+    """The oracle (plain-C restatement) from `state` = (Y, P, time, t_try) on the
+    same operator block lists as the device handle `sim`, timed for ~`seconds` of
+    wall time (or until t_stop)."""
     from oracle.oracle import OracleSimulation, RefSimulation
     if p.static_kernel == 0 and RefSimulation.available():
-        return cpu_baseline_reference(sim, p, seconds)
+        return cpu_baseline_reference(state, p, seconds)
     o = OracleSimulation(p, threads=threads)
     if p.static_kernel == 1:
         for op in ((0, 1) if p.fracture_mode == 2 else (0,)):
             o.hmat_import(op, sim.hmat_export(op))
-    Y, P, _, t = sim.get_state()
+    Y, P, t, t_try = state
     o.set_state(Y, P, t)
-    o.set_t_try(sim.stats()["t_try"])
+    o.set_t_try(t_try)
     t_start = o.stats()["time"]
     steps = 0
     w0 = time.perf_counter()
-    while time.perf_counter() - w0 < seconds:
+    while time.perf_counter() - w0 < seconds and (t_stop is None or o.stats()["time"] < t_stop):
         o.solve(1)
         steps += 1
     wall = time.perf_counter() - w0
     sim_dt = o.stats()["time"] - t_start
+    o.close()
     return dict(value=(sim_dt / YEAR) / (wall / 3600.0), unit="simulated yr / wall hr", cores=threads,
-                kind="port", sample=f"{steps} accepted steps continuing from the GPU's final state "
-                                   f"({wall:.1f} s wall, {sim_dt:.3e} s simulated, OpenMP matvec)")
+                kind="port", steps=steps, wall_s=wall, sim_s=sim_dt, ms_per_step=1e3 * wall / max(steps, 1),
+                mean_dt_s=sim_dt / max(steps, 1),
+                sample=f"{steps} accepted steps from the GPU's state at the start of its timed window "
+                       f"({wall:.1f} s wall, {sim_dt:.3e} s simulated, OpenMP matvec on {threads} threads)")
+
+
+def device_state(sim):
+    Y, P, _, t = sim.get_state()
+    return (Y, P, t, sim.stats()["t_try"])
 
 
 def mixed_precision(sim, p, args, kb64):
@@ -272,43 +314,13 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="network80k")
-    ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mixed-steps", type=int, default=300,
-                    help="steps of the FP32-operator (mixed precision) run on the same workload; 0 skips it")
-    ap.add_argument("--mixed-vectors", type=int, default=1000,
-                    help="random log-uniform slip-rate vectors for the FP32-vs-FP64 stress-rate error")
-    ap.add_argument("--networks", type=int, default=64, help="networks of the ensemble workload")
-    ap.add_argument("--ensemble-faults", default="5,36",
-                    help="fault-count range per network of the ensemble workload (50,400 with --ensemble-box 7000 "
-                         "is the 1.5M-element ensemble)")
-    ap.add_argument("--ensemble-box", type=float, default=2500.0, help="box size (m) of the ensemble's networks")
-    ap.add_argument("--ensemble-single-sample", type=int, default=16,
-                    help="ensemble networks also run one handle at a time, for the batching comparison")
-    ap.add_argument("--no-graph", action="store_true",
-                    help="ensemble workload: issue the launches one by one instead of replaying HIP graphs")
-    ap.add_argument("--learned-steps", type=int, default=300,
-                    help="ensemble workload: steps of the recording run for the step-size network; 0 skips it")
-    ap.add_argument("--learned-epochs", type=int, default=30)
-    ap.add_argument("--copy-probe-gb", type=float, default=2.0,
-                    help="size of the copy kernel whose rate is reported as the achievable HBM bandwidth")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: every workload here runs on a single GPU", file=sys.stderr)
-        sys.exit(2)
-
-    from demystification_dfn_amd.device import DeviceSimulation, bench_stream, device_info
-    log("[bench]", device_info())
-    if args.workload == "ensemble":
-        return run_ensemble(args)
-    p, desc = make_workload(args.workload, args.seed)
+def run_network(workload, args, mixed=True, cpu_seconds=None):
+    """One network on one dfn_handle: warmup, `steps` timed solver steps, the
+    dominant kernel timed live over further steps, the isolated matvec, FP32
+    operators (config 3) and the CPU baseline from the state at the start of the
+    timed window.  Returns the result dict."""
+    from demystification_dfn_amd.device import DeviceSimulation
+    p, desc = make_workload(workload, args.seed)
     log(f"[bench] workload {desc['workload']} N={p.N}")
     sim = DeviceSimulation(p)
     t0 = time.perf_counter()
@@ -325,7 +337,7 @@ def main():
     log(f"[bench] H-matrix setup {setup_s:.1f} s {hm}")
     sim.solve(args.warmup)
     s0 = sim.stats()
-    log(f"[bench] warmup done t={s0['time']:.3e}")
+    state0 = device_state(sim)  # the CPU baseline starts where the timed window starts
     w0 = time.perf_counter()
     sim.solve(args.steps)
     wall = time.perf_counter() - w0
@@ -347,7 +359,6 @@ def main():
         "direct3_tile_kernel + reduce (dense dislocation kernel evaluated on the fly, 11 flops per "
         "element-node pair)" if p.fracture_mode == 3 else
         "direct2_tile_kernel + reduces (dense dislocation kernel evaluated on the fly, 40 flops per pair)")
-    copy_gbs = bench_stream(int(args.copy_probe_gb * (1 << 30)), 20) if args.copy_probe_gb > 0 else None
     ms_dom = ms_live if n_live > 0 else kb["ms_dominant"]
     timing = dict(launches_timed=n_live, ms_per_launch_isolated=kb["ms_dominant"],
                   timing=f"mean over the launches of {n_extra} solver steps after the timed ones, HIP events "
@@ -371,24 +382,249 @@ def main():
         data="synthetic (seeded generator)", config=desc,
         roofline=roof,
         hmatvec=matvec,
-        copy_kernel_GB_s=copy_gbs,
         run=dict(accepted=s1["accepted_steps"] - s0["accepted_steps"],
                  rejected=s1["rejected_attempts"] - s0["rejected_attempts"],
-                 cg_iterations=s1["cg_iterations"] - s0["cg_iterations"], sim_time_s=s1["time"],
-                 max_V=s1["max_V"], setup_s=setup_s),
+                 cg_iterations=s1["cg_iterations"] - s0["cg_iterations"], window_start_s=s0["time"],
+                 sim_time_s=s1["time"], mean_accepted_dt_s=sim_dt / args.steps, max_V=s1["max_V"],
+                 setup_s=setup_s),
     )
-    if args.mixed_steps > 0 and p.static_kernel == 1:
+    if mixed and args.mixed_steps > 0 and p.static_kernel == 1:
         out["mixed"] = mixed_precision(sim, p, args, kb)
-    if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        log(f"[bench] cpu baseline on {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(sim, p, args.cpu_seconds, threads)
-        if out["cpu_baseline"]["kind"] == "port" and threads > 1:
-            # the reference as built runs on one thread
-            log("[bench] cpu baseline on 1 thread")
-            out["cpu_baseline_1thread"] = cpu_baseline(sim, p, args.cpu_seconds / 2, 1)
+    cpu_seconds = args.cpu_seconds if cpu_seconds is None else cpu_seconds
+    if not args.no_cpu and cpu_seconds > 0:
+        ci = cpu_info()
+        log(f"[bench] cpu baseline on {ci['threads']} threads")
+        cb = cpu_baseline(sim, state0, p, cpu_seconds, ci["threads"])
+        cb.update(cpu=ci, gpu_ms_per_step=out["ms_per_step"], gpu_mean_dt_s=sim_dt / args.steps,
+                  gpu_vs_cpu_sim_rate=value / cb["value"] if cb["value"] > 0 else None,
+                  gpu_vs_cpu_per_step=cb["ms_per_step"] / out["ms_per_step"])
+        out["cpu_baseline"] = cb
+        if cb["kind"] == "port" and ci["threads"] > 1:
+            log("[bench] cpu baseline on 1 thread")  # the reference as built runs on one thread
+            out["cpu_baseline_1thread"] = cpu_baseline(sim, state0, p, cpu_seconds / 2, 1)
     else:
         out["cpu_baseline"] = None
+    sim.close()
+    return out
+
+
+def run_headline(args):
+    """Configs 4 and 5: the 64-network ensemble with FP32 operators and the
+    learned step-size proposals, against the plain FP64 path (one dfn_handle per
+    network, FP64 operators, rkqs) over the same simulated interval of every
+    network; then, continuing, the ensemble with rkqs in FP32 and in FP64."""
+    from demystification_dfn_amd import _abi, problems
+    from demystification_dfn_amd import train_step_network as tsn
+    from demystification_dfn_amd.device import DeviceEnsemble, DeviceMlp, DeviceSimulation
+    lo, hi = (int(v) for v in args.ensemble_faults.split(","))
+    probs = problems.network_ensemble(args.networks, faults=(lo, hi), box=args.ensemble_box, seed=args.seed)
+    B = len(probs)
+    Ns = np.array([p.N for p in probs])
+    desc = dict(workload=f"ensemble{B}_{Ns.sum() // 1000}k_fp32_learned", networks=B,
+                faults_per_network=f"{lo}-{hi}", box_m=args.ensemble_box,
+                length_law="p(L)~L^-0.1 on [50 m, 2 km]", element_m=10, fracture_mode="II",
+                kernel="hmatrix", eta=2.0, eps_aca=1e-6, leaf=64, elements=int(Ns.sum()), min_elements=int(Ns.min()),
+                max_elements=int(Ns.max()), seed=args.seed,
+                friction="per network: a~U[0.008,0.012] per fault, b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa",
+                pulse="10 MPa on one seeded element per network (fluid diffusion on)",
+                precision="FP32 operator blocks (power-of-two block scales, FP32 accumulation, compensated "
+                          "near-field sums); FP64 state, RHS, CG and error norm",
+                controller="learned proposals (data/step_network.npz) with the PI controller as fallback; the "
+                           "Cash-Karp error test accepts or rejects every step")
+    log(f"[bench] headline: ensemble of {B} networks, {Ns.sum()} elements (max {Ns.max()})")
+    params, norm, meta = tsn.load()
+    mlp = DeviceMlp()
+    mlp.set_params(params, norm)
+    ens = DeviceEnsemble(probs)
+    t0 = time.perf_counter()
+    ens.initiate_hmat()
+    setup_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.set_precision(_abi.PREC_FP32)
+    layout32_s = time.perf_counter() - t0
+    log(f"[bench] ensemble setup {setup_s:.1f} s, FP32 layout {layout32_s:.1f} s {ens.info()}")
+    ens.set_controller(_abi.STEP_LEARNED, mlp)
+
+    def window(steps):
+        ta = np.array([ens.stats(i)["time"] for i in range(B)])
+        acc0 = sum(ens.stats(i)["accepted_steps"] for i in range(B))
+        rej0 = sum(ens.stats(i)["rejected_attempts"] for i in range(B))
+        w0 = time.perf_counter()
+        ens.solve(steps)
+        wall = time.perf_counter() - w0
+        tb = np.array([ens.stats(i)["time"] for i in range(B)])
+        acc = sum(ens.stats(i)["accepted_steps"] for i in range(B)) - acc0
+        rej = sum(ens.stats(i)["rejected_attempts"] for i in range(B)) - rej0
+        sim = float((tb - ta).sum())
+        return ta, tb, wall, dict(value=(sim / YEAR) / (wall / 3600.0), unit="simulated network-yr / wall hr",
+                                  steps=steps, ms_per_step=1e3 * wall / steps, mean_accepted_dt_s=sim / max(acc, 1),
+                                  accepted=acc, rejected=rej, rejected_fraction=rej / max(acc + rej, 1),
+                                  window_start_s_median=float(np.median(ta)))
+
+    ens.solve(args.warmup)
+    tA, tB, wall, head = window(args.steps)
+    value = head["value"]
+    log(f"[bench] mixed+learned: {args.steps} steps in {wall:.2f} s ({head['ms_per_step']:.2f} ms/step)")
+    # the dominant kernel (FP32 phase-2 panels), HIP events on the ensemble's stream
+    # over further steps in which every network takes part
+    n_extra = min(args.steps, 10)
+    ens.kernel_timing(True)
+    ens.solve(n_extra)
+    ms_dom, n_dom, bytes_dom = ens.kernel_timing_read()
+    ens.kernel_timing(False)
+    achieved = bytes_dom / (ms_dom * 1e-3) / 1e9 if n_dom else None
+    log(f"[bench] panel_kernel_f32: {ms_dom:.3f} ms mean over {n_dom} launches")
+    roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=achieved / HBM_PEAK_GBS if achieved else None,
+                traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed)),
+                kernel="panel_kernel_f32 (H-matvec phase 2, FP32 blocks, the whole ensemble's panels)",
+                bytes_per_launch=bytes_dom, ms_per_launch=ms_dom, launches_timed=n_dom,
+                timing=f"mean over the panel launches of {n_extra} ensemble steps after the timed ones in which every "
+                       f"network took part, HIP events on the ensemble's stream")
+    info32 = ens.info()
+    # the same ensemble, continuing: rkqs with FP32 operators, then with FP64 operators
+    ens.set_controller(_abi.STEP_RKQS)
+    _, _, _, rkqs32 = window(args.steps)
+    ens.set_precision(_abi.PREC_FP64)
+    info64 = ens.info()
+    _, _, _, rkqs64 = window(args.steps)
+    ens.close()
+    log(f"[bench] continuing: FP32 rkqs {rkqs32['ms_per_step']:.2f} ms/step, FP64 rkqs {rkqs64['ms_per_step']:.2f}")
+    # the plain FP64 path: every network on its own dfn_handle, from t = 0 to the start
+    # of the ensemble's timed window (untimed), then timed over the same interval
+    order = np.argsort(Ns)
+    sample = [int(order[B // 4]), int(order[B // 2])] if not args.no_cpu and args.cpu_seconds > 0 else []
+    plain_wall, plain_steps, plain_rej, states, per_net = 0.0, 0, 0, {}, {}
+    for i, p in enumerate(probs):
+        d = DeviceSimulation(p)
+        d.initiate_hmat()
+        d.solve(10 ** 9, t_final=float(tA[i]))
+        s0 = d.stats()
+        if i in sample:
+            states[i] = device_state(d)
+        w0 = time.perf_counter()
+        d.solve(10 ** 9, t_final=float(tB[i]))
+        w = time.perf_counter() - w0
+        s1 = d.stats()
+        plain_wall += w
+        plain_steps += s1["accepted_steps"] - s0["accepted_steps"]
+        plain_rej += s1["rejected_attempts"] - s0["rejected_attempts"]
+        per_net[i] = (w, s1["accepted_steps"] - s0["accepted_steps"])
+        if i in sample:
+            states[i] = (states[i], d)  # keep the handle for the CPU baseline's block lists
+        else:
+            d.close()
+        if i % 16 == 15:
+            log(f"[bench] plain FP64 path: {i + 1}/{B} networks")
+    sim_total = float((tB - tA).sum())
+    plain_value = (sim_total / YEAR) / (plain_wall / 3600.0)
+    ratio = value / plain_value
+    log(f"[bench] plain FP64 path {plain_value:.1f} network-yr/hr; mixed+learned ensemble {ratio:.2f}x")
+    out = dict(
+        metric=METRIC, value=value, unit="simulated network-yr / wall hr", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
+        dtype="f32 operators / f64 state", data="synthetic (seeded generator)", config=desc,
+        roofline=roof,
+        vs_plain_fp64=dict(plain_value=plain_value, unit="simulated network-yr / wall hr", ratio=ratio,
+                           target_ratio=20.0, meets_target=bool(ratio >= 20.0), plain_wall_s=plain_wall,
+                           plain_accepted=plain_steps, plain_rejected=plain_rej,
+                           note="the same 64 networks, one dfn_handle each (FP64 operators, rkqs), each stepped "
+                                "from t=0 to the start of the ensemble's timed window (untimed) and then timed to "
+                                "the window's end: the same simulated interval as the ensemble's, per network"),
+        window=head,
+        ensemble=dict(setup_s=setup_s, fp32_layout_s=layout32_s, S_entries=info64["S_entries"],
+                      alg_bytes_per_stress_eval_fp64=info64["alg_bytes"],
+                      alg_bytes_per_stress_eval_fp32=info32["alg_bytes"],
+                      continuing_fp32_rkqs=rkqs32, continuing_fp64_rkqs=rkqs64,
+                      note="continuing windows follow the timed one in simulated time; rates over different "
+                           "windows compare per step (ms_per_step) rather than per simulated year"),
+        proposal_network=dict(pairs=meta["recording"]["pairs"], loss_initial=meta["training"]["loss_initial"],
+                              loss_final=meta["training"]["loss_final"], features=meta["features"]),
+    )
+    # CPU baseline: the oracle on two sampled networks (25th and 50th percentile of
+    # size) from the plain path's state at the start of the timed window
+    if sample:
+        ci = cpu_info()
+        runs, gpu_w, gpu_sim = [], 0.0, 0.0
+        for i in sample:
+            st, d = states[i]
+            log(f"[bench] cpu baseline: network {i} (N={probs[i].N}) on {ci['threads']} threads")
+            r = cpu_baseline(d, st, probs[i], args.cpu_seconds / len(sample), ci["threads"], t_stop=float(tB[i]))
+            d.close()
+            r["network"], r["N"] = i, int(probs[i].N)
+            # the plain GPU path over the same network's window, at the same start
+            w, n = per_net[i]
+            r["gpu_plain_ms_per_step"] = 1e3 * w / max(n, 1)
+            runs.append(r)
+            gpu_w += w
+            gpu_sim += float(tB[i] - tA[i])
+        cpu_sim = sum(r["sim_s"] for r in runs)
+        cpu_wall = sum(r["wall_s"] for r in runs)
+        cpu_rate = (cpu_sim / YEAR) / (cpu_wall / 3600.0)
+        gpu_rate = (gpu_sim / YEAR) / (gpu_w / 3600.0)
+        # over the whole ensemble, as the sampled networks' CPU / plain-GPU ratio
+        est = cpu_rate * plain_value / gpu_rate
+        out["cpu_baseline"] = dict(
+            value=est, unit="simulated network-yr / wall hr", cores=ci["threads"], kind="port",
+            sample=(f"the oracle (OpenMP matvec on {ci['threads']} threads) on networks {sample} (N = "
+                    f"{[int(probs[i].N) for i in sample]}, the 25th and 50th size percentiles) from the GPU's state "
+                    f"at the start of the timed window, {cpu_wall:.1f} s wall in all; value = their CPU rate scaled "
+                    f"to the 64 networks by the plain GPU path's rate on them against its rate on all 64"),
+            measured_on_sample=dict(value=cpu_rate, unit="simulated network-yr / wall hr", runs=runs),
+            plain_gpu_on_sample=gpu_rate, plain_gpu_vs_cpu=gpu_rate / cpu_rate if cpu_rate > 0 else None, cpu=ci)
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="headline",
+                    help="headline (the 64-network ensemble, FP32 operators + learned step vs the plain FP64 path), "
+                         "network80k, network80k_connected, single_fault, ensemble")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sub", action="store_true", help="headline: skip the 80k-network sub-runs")
+    ap.add_argument("--mixed-steps", type=int, default=20,
+                    help="steps of the FP32-operator (mixed precision) run on a single network; 0 skips it")
+    ap.add_argument("--mixed-vectors", type=int, default=1000,
+                    help="random log-uniform slip-rate vectors for the FP32-vs-FP64 stress-rate error")
+    ap.add_argument("--networks", type=int, default=64, help="networks of the ensemble workloads")
+    ap.add_argument("--ensemble-faults", default="50,400", help="fault-count range per network of the ensembles")
+    ap.add_argument("--ensemble-box", type=float, default=7000.0, help="box size (m) of the ensemble's networks")
+    ap.add_argument("--ensemble-single-sample", type=int, default=16,
+                    help="--workload ensemble: networks also run one handle at a time")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="--workload ensemble: issue the launches one by one instead of replaying HIP graphs")
+    ap.add_argument("--learned-steps", type=int, default=0,
+                    help="--workload ensemble: steps of a recording run that trains the step-size network; 0 skips")
+    ap.add_argument("--learned-epochs", type=int, default=30)
+    ap.add_argument("--copy-probe-gb", type=float, default=2.0,
+                    help="size of the copy kernel whose rate is reported as the achievable HBM bandwidth")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every workload here runs on a single GPU", file=sys.stderr)
+        sys.exit(2)
+
+    from demystification_dfn_amd.device import bench_stream, device_info
+    log("[bench]", device_info())
+    if args.workload == "ensemble":
+        return run_ensemble(args)
+    if args.workload != "headline":
+        out = run_network(args.workload, args)
+        out["copy_kernel_GB_s"] = bench_stream(int(args.copy_probe_gb * (1 << 30)), 20) if args.copy_probe_gb else None
+        print(json.dumps(out), flush=True)
+        return
+    out = run_headline(args)
+    if not args.no_sub:
+        # config 2 (FP64, with the FP32-operator run of config 3) and its connected variant
+        out["network80k"] = run_network("network80k", args, mixed=True, cpu_seconds=args.cpu_seconds / 2)
+        out["network80k_connected"] = run_network("network80k_connected", args, mixed=False, cpu_seconds=0)
+    out["copy_kernel_GB_s"] = bench_stream(int(args.copy_probe_gb * (1 << 30)), 20) if args.copy_probe_gb else None
     print(json.dumps(out), flush=True)
 
 
