@@ -150,23 +150,33 @@ def device_state(sim):
 
 
 def mixed_precision(sim, p, args, kb64):
-    """FP32 operator storage (per-block power-of-two scales, FP32 accumulation,
-    compensated near-field sums) with FP64 state, RHS, CG and error norm: stress-rate
-    error against the FP64 operator over random log-uniform slip rates in [1e-12, 1]
-    m/s, then the same number of solver steps continued from the FP64 run's state."""
+    """Config 3 on a single network: FP32 operator storage (per-block power-of-two
+    scales, FP32 accumulation, compensated near-field sums) and FP32 low-rank
+    factors beside FP64 near-field blocks, each against the FP64 operator over
+    random log-uniform slip rates in [1e-12, 1] m/s (norm-wise and on the rows above
+    1e-3 of the largest stress rate); then FP32 solver steps continued from the
+    FP64 run's state.  State, RHS, CG and error norm stay FP64 throughout."""
     from demystification_dfn_amd import _abi
     rng = np.random.default_rng(7)
-    errs, sig = [], []
+    errs = {_abi.PREC_FP32: ([], []), _abi.PREC_MIXED: ([], [])}
     for _ in range(args.mixed_vectors):
         V = 10.0 ** rng.uniform(-12.0, 0.0, p.N)
         sim.set_precision(_abi.PREC_FP64)
         a = sim.kernel(V)
-        sim.set_precision(_abi.PREC_FP32)
-        b = sim.kernel(V)
-        for y64, y32 in zip(a, b):
-            errs.append(np.linalg.norm(y32 - y64) / np.linalg.norm(y64))
-            big = np.abs(y64) > 1e-3 * np.abs(y64).max()
-            sig.append(np.max(np.abs(y32[big] - y64[big]) / np.abs(y64[big])))
+        for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
+            sim.set_precision(prec)
+            b = sim.kernel(V)
+            for y64, y in zip(a, b):
+                errs[prec][0].append(np.linalg.norm(y - y64) / np.linalg.norm(y64))
+                big = np.abs(y64) > 1e-3 * np.abs(y64).max()
+                errs[prec][1].append(np.max(np.abs(y[big] - y64[big]) / np.abs(y64[big])))
+
+    def stats(prec):
+        e, g = errs[prec]
+        return dict(vectors=args.mixed_vectors, norm_max=float(np.max(e)), norm_median=float(np.median(e)),
+                    significant_rows_max=float(np.max(g)), significant_rows_median=float(np.median(g)))
+    sim.set_precision(_abi.PREC_MIXED)
+    kbm = sim.bench_kernel(reps=20)
     sim.set_precision(_abi.PREC_FP32)
     tune32 = sim.tuning()
     kb = sim.bench_kernel(reps=20)
@@ -178,12 +188,15 @@ def mixed_precision(sim, p, args, kb64):
     sim.set_precision(_abi.PREC_FP64)
     sim_dt = s1["time"] - s0["time"]
     return dict(value=(sim_dt / YEAR) / (wall / 3600.0), unit="simulated yr / wall hr", steps=args.mixed_steps,
-                ms_per_step=1e3 * wall / args.mixed_steps,
+                ms_per_step=1e3 * wall / args.mixed_steps, mean_accepted_dt_s=sim_dt / args.mixed_steps,
                 hmatvec_ms=kb["ms_kernel"], hmatvec_GB_s=kb["alg_bytes"] / (kb["ms_kernel"] * 1e-3) / 1e9,
                 hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kb["ms_kernel"],
                 panel_GB_s=kb["dominant_bytes"] / (kb["ms_dominant"] * 1e-3) / 1e9,
-                stress_rate_rel_err=dict(vectors=args.mixed_vectors, norm_max=float(np.max(errs)),
-                                         norm_median=float(np.median(errs)), significant_rows_max=float(np.max(sig))),
+                stress_rate_rel_err=stats(_abi.PREC_FP32),
+                fp64_near_field=dict(stress_rate_rel_err=stats(_abi.PREC_MIXED), hmatvec_ms=kbm["ms_kernel"],
+                                     alg_bytes=kbm["alg_bytes"],
+                                     hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kbm["ms_kernel"],
+                                     note="FP32 low-rank factors beside FP64 near-field blocks (DFN_PREC_MIXED)"),
                 autotune=tune32, note="continues from the FP64 run's final state; operators FP32, state FP64")
 
 

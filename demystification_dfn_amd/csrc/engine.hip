@@ -55,7 +55,8 @@ struct Engine {
   bool have_op[2] = {false, false};
   int precision = DFN_PREC_FP64;
   hipStream_t stream = nullptr;
-  std::unique_ptr<DevOps> ops64, ops32;
+  std::unique_ptr<DevOps> ops64, ops32;  // FP64 layout; the reduced-precision layout of kind ops32_kind
+  int ops32_kind = -1;
   // problem data on the device
   DMem node, elem, nrm, ds, a, b, Dc, f0, V0, nld, sld, fake, perm_star, kmin, Snk, src_elem, src_Q, src_tb, src_te,
       cross, dpart;
@@ -258,14 +259,27 @@ struct Engine {
       if (!have_op[o]) return DFN_OK;  // incomplete until every operator exists
     ops64.reset(new DevOps);
     DFN_TRY(layout_ops<double>(list, nops, *ops64));
-    if (precision == DFN_PREC_FP32) {
-      ops32.reset(new DevOps);
-      DFN_TRY(layout_ops<float>(list, nops, *ops32));
-    }
+    if (precision != DFN_PREC_FP64) DFN_TRY(layout_reduced());
     return DFN_OK;
   }
 
-  const DevOps *active_ops() const { return (precision == DFN_PREC_FP32 && ops32) ? ops32.get() : ops64.get(); }
+  // the FP32 layout (DFN_PREC_FP32) or FP32 low-rank factors beside FP64 near-field
+  // blocks (DFN_PREC_MIXED), for the current precision
+  int layout_reduced() {
+    ops32.reset();
+    ops32_kind = -1;
+    if (!ops64) return DFN_OK;  // no operator yet
+    const int nops = hp.mode == DFN_MODE_II ? 2 : 1;
+    const HostOperator *list[2] = {&hop[0], &hop[1]};
+    ops32.reset(new DevOps);
+    DFN_TRY(layout_ops<float>(list, nops, *ops32, nullptr, precision == DFN_PREC_MIXED));
+    ops32_kind = precision;
+    return DFN_OK;
+  }
+
+  const DevOps *active_ops() const {
+    return (precision != DFN_PREC_FP64 && ops32 && ops32_kind == precision) ? ops32.get() : ops64.get();
+  }
 
   // stress rates sT (and sN) from the slip-rate vector d_V (device, original order)
   // the event pair for the next timed launch (kt_on), or nullptr
@@ -315,8 +329,7 @@ struct Engine {
                d->xf.as<float>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     if (kt) DFN_HIP(hipEventRecord(kt[0], stream));
-    launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
-                 d->ypart.as<double>(), stream, d->panel_rows);
+    launch_panels(*d, w, stream, d->panel_rows);
     if (kt) DFN_HIP(hipEventRecord(kt[1], stream));
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                     d->nops, kernel_scale(), d_sT, d->nops > 1 ? d_sN : d_sT, stream);
@@ -931,9 +944,9 @@ API int dfn_build_hmat(dfn_handle h) {
 
 API int dfn_set_precision(dfn_handle h, int32_t prec) {
   DFN_API_BEGIN
-  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32)) return DFN_ERR_ARG;
+  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32 && prec != DFN_PREC_MIXED)) return DFN_ERR_ARG;
   h->e.precision = prec;
-  if (prec == DFN_PREC_FP32 && !h->e.ops32) return h->e.relayout();
+  if (prec != DFN_PREC_FP64 && h->e.ops32_kind != prec) return h->e.layout_reduced();
   return DFN_OK;
   DFN_API_END
 }
@@ -1182,8 +1195,7 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
   dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), e.stream,
                   d->vtx_waves, d->xf.as<float>());
   dfn::launch_zreduce(d->zred.as<dfn::ZRed>(), d->nzr, d->zpart.as<double>(), w, e.stream);
-  dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(),
-                    d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream, d->panel_rows);
+  dfn::launch_panels(*d, w, e.stream, d->panel_rows);
   dfn::launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                   d->nops, 1.0, dT, d->nops > 1 ? dN : dT, e.stream);
   DFN_HIP(hipMemcpyAsync(yT.data(), op == 0 ? dT : dN, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
@@ -1379,8 +1391,7 @@ API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
     double *w = d->w.as<double>();
     DFN_HIP(hipEventRecord(e.ev0, e.stream));
     for (int r = 0; r < reps; r++)
-      dfn::launch_panel(d->ptasks.as<dfn::PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(),
-                        d->cscale.as<float>(), w, d->ypart.as<double>(), e.stream, d->panel_rows);
+      dfn::launch_panels(*d, w, e.stream, d->panel_rows);
     DFN_HIP(hipEventRecord(e.ev1, e.stream));
     DFN_HIP(hipEventSynchronize(e.ev1));
     DFN_HIP(hipEventElapsedTime(&t, e.ev0, e.ev1));

@@ -13,6 +13,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "dfn_device.hpp"
@@ -74,8 +75,14 @@ struct DMem {
 // T and N in Mode II, sharing the permutation and the input vector).
 struct DevOps {
   int n = 0, nops = 0;
-  bool fp32 = false;
+  bool fp32 = false;  // phase 1 reads FP32 V factors (and the FP32 copy of x_h)
   DMem perm, vtasks, vdata, zred, zpart, ptasks, pdata, widx, cscale, w, ypart, fin_first, fin_idx;
+  // phase-2 panel sets: set A (ptasks, pdata, widx, cscale) in FP32 when pa_fp32, else
+  // FP64; with FP64 near-field storage beside FP32 low-rank factors (DFN_PREC_MIXED),
+  // set A holds the near-field (dense) columns in FP64 and set B the U columns in FP32
+  bool pa_fp32 = false;
+  DMem ptasks_b, pdata_b, widx_b, cscale_b;
+  int npt_b = 0;
   DMem xf;  // FP32 operators: FP32 copy of x_h read by phase 1 (n + 4 floats, the tail zero)
   int nvt = 0, nzr = 0, npt = 0;
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
@@ -108,6 +115,16 @@ inline std::vector<PTask> panel_schedule(const std::vector<PTask> &pt, int split
   return out;
 }
 
+// Phase 2 of a stress evaluation: every panel set of the layout (rows: panel rows
+// per wave step; active as for launch_panel)
+inline void launch_panels(const DevOps &d, const double *w, hipStream_t s, int rows, const int *active = nullptr) {
+  launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.pa_fp32, d.widx.as<int>(), d.cscale.as<float>(), w,
+               d.ypart.as<double>(), s, rows, active);
+  if (d.npt_b > 0)
+    launch_panel(d.ptasks_b.as<PTask>(), d.npt_b, d.pdata_b.p, true, d.widx_b.as<int>(), d.cscale_b.as<float>(), w,
+                 d.ypart.as<double>(), s, rows, active);
+}
+
 // Autotuner of the stress-evaluation launch shapes, run once per operator layout on
 // the operator itself: phase-1 tasks (waves) per workgroup {2, 4, 8}; the phase-1
 // task order (as built; grouped by rank bucket with the largest ranks first, so a
@@ -120,7 +137,8 @@ inline std::vector<PTask> panel_schedule(const std::vector<PTask> &pt, int split
 // is 1 % faster.  DFN_AUTOTUNE=alt takes non-default shapes (8 waves, largest
 // phase-1 tasks first, 4 rows, 16-row panel tasks) without timing, so that a test
 // can check that the shapes agree.
-inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<PTask> &pt) {
+inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<PTask> &pt,
+                    const std::vector<PTask> &pt_b) {
   hipEvent_t e0, e1;
   DFN_HIP(hipEventCreate(&e0));
   DFN_HIP(hipEventCreate(&e1));
@@ -132,8 +150,7 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
       launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.fp32, w, d.zpart.as<double>(), nullptr, waves,
                  d.xf.as<float>());
       launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, nullptr);
-      launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.fp32, d.widx.as<int>(), d.cscale.as<float>(), w,
-                   d.ypart.as<double>(), nullptr, rows);
+      launch_panels(d, w, nullptr, rows);
     }
     DFN_HIP(hipEventRecord(e1, nullptr));
     DFN_HIP(hipEventSynchronize(e1));
@@ -154,7 +171,11 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   auto porder = [&](int order, int split) -> int {  // order 0: as built
     const std::vector<PTask> p = order ? panel_schedule(pt, split) : pt;
     d.npt = (int)p.size();
-    return d.ptasks.upload(p);
+    DFN_TRY(d.ptasks.upload(p));
+    if (pt_b.empty()) return DFN_OK;
+    const std::vector<PTask> q = order ? panel_schedule(pt_b, split) : pt_b;
+    d.npt_b = (int)q.size();
+    return d.ptasks_b.upload(q);
   };
   int waves = 4, order = 0, rows = 2, pord = 0, split = 0;
   const char *mode = std::getenv("DFN_AUTOTUNE");
@@ -234,9 +255,13 @@ inline double pow2_scale(const double *v, size_t n) {
 // hnet (ensembles, may be null): the network of every H position; each phase-1 and
 // phase-2 task records the network its block or panel belongs to, so that the
 // kernels can skip the tasks of networks that sit out an evaluation.
+// T: storage of the low-rank factors (and of the near-field blocks unless dense64,
+// which keeps them in FP64 in their own panel set).
 template <class T>
-int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::vector<int> *hnet = nullptr) {
+int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::vector<int> *hnet = nullptr,
+               bool dense64 = false) {
   const bool scaled = sizeof(T) == 4;  // FP32 storage keeps each block divided by its power-of-two scale
+  dense64 = dense64 && scaled;
   const int n = ops[0]->n;
   for (int o = 1; o < nops; o++)
     if (ops[o]->perm != ops[0]->perm) return DFN_ERR_UNSUPPORTED;  // T and N share the cluster tree
@@ -329,20 +354,18 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   std::vector<int> row_leaf(n);
   for (int l = 0; l < nleaf; l++)
     for (int i = 0; i < H0.leaf_m[l]; i++) row_leaf[H0.leaf_r0[l] + i] = l;
-  std::vector<PTask> pt;
-  std::vector<int> widx;
-  std::vector<float> cscale;
   std::vector<std::vector<double>> dscale(nops), uscale(nops);
-  int64_t reserve = 0;
+  int64_t S_dense = 0;
   for (int o = 0; o < nops; o++) {
     const HostOperator &H = *ops[o];
-    reserve += (int64_t)H.d_data.size() + (int64_t)H.l_u.size();
+    S_dense += (int64_t)H.d_data.size();
     dscale[o].assign(H.d_m.size(), 1.0);
     uscale[o].assign(H.l_m.size(), 1.0);
     if (scaled) {
-      parallel_for((int64_t)H.d_m.size(), [&](int64_t b) {
-        dscale[o][b] = pow2_scale(H.d_data.data() + H.d_off[b], (size_t)H.d_m[b] * H.d_n[b]);
-      });
+      if (!dense64)
+        parallel_for((int64_t)H.d_m.size(), [&](int64_t b) {
+          dscale[o][b] = pow2_scale(H.d_data.data() + H.d_off[b], (size_t)H.d_m[b] * H.d_n[b]);
+        });
       parallel_for((int64_t)H.l_m.size(), [&](int64_t b) {
         uscale[o][b] = pow2_scale(H.l_u.data() + H.l_uoff[b], (size_t)H.l_m[b] * H.l_k[b]);
       });
@@ -352,12 +375,50 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   // panels are planned here and written by parallel_for below
   struct Col { int64_t wi; const double *src; double scale; };
   struct PFill { int64_t poff, col0; int L, Lp, m; };
+  struct PanelSet {
+    int esize;    // bytes per stored entry
+    bool scales;  // FP32 storage: per-column block scales
+    std::vector<PTask> pt;
+    std::vector<int> widx;
+    std::vector<float> cscale;
+    std::vector<PFill> pfill;
+    int64_t psize = 0, entries = 0;
+  };
+  PanelSet setA{dense64 ? 8 : (int)sizeof(T), scaled && !dense64}, setB{(int)sizeof(T), scaled};
   std::vector<Col> cols;
-  std::vector<PFill> pfill;
-  int64_t psize = 0;
-  const int64_t panel_entries = reserve;  // dense + U entries, every one stored once
   std::vector<std::vector<int>> fin((size_t)n * nops);
   int yp = 0;
+  // tasks of the columns cols[c0, c0 + ncols) of one panel (rows r0.., m of them) into
+  // set S; the first nd of them are near-field columns (compensated in FP32 panels)
+  auto emit = [&](PanelSet &S, int o, int r0, int m, int net, int64_t c0, int64_t ncols, int64_t nd) {
+    // rows padded with zero entries (on column 0 of w) to a multiple of 128 bytes, so
+    // every row starts on a 128-byte line: a line shared by two rows is fetched
+    // twice by the streaming loads (9 % more bytes fetched and 5-10 % more time
+    // with 16-byte rows); a zero entry adds an exact zero to its lane's sum
+    const int pad = 128 / S.esize;
+    S.entries += ncols * m;
+    for (int64_t q0 = 0; q0 < ncols; q0 += kPanelMaxCols) {
+      const int L = (int)std::min<int64_t>(kPanelMaxCols, ncols - q0);
+      const int Lp = (L + pad - 1) / pad * pad;
+      PTask t{};
+      t.poff = S.psize;
+      t.widx = (int64_t)S.widx.size();
+      t.m = m;
+      t.L = Lp;
+      t.yp = yp;
+      t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, nd - q0));
+      t.net = net;
+      for (int j = 0; j < Lp; j++) {
+        S.widx.push_back(j < L ? (int)cols[c0 + q0 + j].wi : 0);
+        if (S.scales) S.cscale.push_back(j < L ? (float)cols[c0 + q0 + j].scale : 1.0f);
+      }
+      S.pfill.push_back({S.psize, c0 + q0, L, Lp, m});
+      S.psize += (int64_t)m * Lp;
+      for (int r = 0; r < m; r++) fin[(size_t)o * n + r0 + r].push_back(yp + r);
+      yp += m;
+      S.pt.push_back(t);
+    }
+  };
   for (int o = 0; o < nops; o++) {
     const HostOperator &H = *ops[o];
     // covering lists: (kind, block) per leaf; dense first then low-rank, block order
@@ -372,75 +433,76 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
       const int r0 = H0.leaf_r0[l], m = H0.leaf_m[l];
       // this panel's columns, near-field (dense) columns first
       const int64_t cbase = (int64_t)cols.size();
-      int ndense = 0;
+      int64_t ndense = 0;
       for (auto [kind, b] : cover[l]) {
         if (kind == 0) {
-          const double sc = scaled ? dscale[o][b] : 1.0;
           const double *F = H.d_data.data() + H.d_off[b] + (r0 - H.d_r0[b]);
-          for (int j = 0; j < H.d_n[b]; j++) cols.push_back({(int64_t)H.d_c0[b] + j, F + (int64_t)j * H.d_m[b], sc});
+          for (int j = 0; j < H.d_n[b]; j++)
+            cols.push_back({(int64_t)H.d_c0[b] + j, F + (int64_t)j * H.d_m[b], dscale[o][b]});
           ndense += H.d_n[b];
         } else {
-          const double sc = scaled ? uscale[o][b] : 1.0;
           const double *U = H.l_u.data() + H.l_uoff[b] + (r0 - H.l_r0[b]);
           for (int j = 0; j < H.l_k[b]; j++)
-            cols.push_back({zbase[o] + zoff[o][b] + j, U + (int64_t)j * H.l_m[b], sc});
+            cols.push_back({zbase[o] + zoff[o][b] + j, U + (int64_t)j * H.l_m[b], uscale[o][b]});
         }
       }
-      // rows padded with zero entries (on column 0 of w) to a multiple of 128 bytes, so
-      // every row starts on a 128-byte line: a line shared by two rows is fetched
-      // twice by the streaming loads (9 % more bytes fetched and 5-10 % more time
-      // with 16-byte rows); a zero entry adds an exact zero to its lane's sum
-      const int pad = 128 / (int)sizeof(T);
       const int64_t ncols = (int64_t)cols.size() - cbase;
-      for (int64_t c0 = 0; c0 < ncols; c0 += kPanelMaxCols) {
-        const int L = (int)std::min<int64_t>(kPanelMaxCols, ncols - c0);
-        const int Lp = (L + pad - 1) / pad * pad;
-        PTask t{};
-        t.poff = psize;
-        t.widx = (int64_t)widx.size();
-        t.m = m;
-        t.L = Lp;
-        t.yp = yp;
-        t.nd = (int)std::max<int64_t>(0, std::min<int64_t>(L, (int64_t)ndense - c0));
-        t.net = hnet ? (*hnet)[r0] : 0;
-        for (int j = 0; j < Lp; j++) {
-          widx.push_back(j < L ? (int)cols[cbase + c0 + j].wi : 0);
-          if (scaled) cscale.push_back(j < L ? (float)cols[cbase + c0 + j].scale : 1.0f);
-        }
-        pfill.push_back({psize, cbase + c0, L, Lp, m});
-        psize += (int64_t)m * Lp;
-        for (int r = 0; r < m; r++) fin[(size_t)o * n + r0 + r].push_back(yp + r);
-        yp += m;
-        pt.push_back(t);
+      const int net = hnet ? (*hnet)[r0] : 0;
+      if (dense64) {  // near-field columns in FP64 (set A), then the U columns in FP32 (set B)
+        emit(setA, o, r0, m, net, cbase, ndense, ndense);
+        emit(setB, o, r0, m, net, cbase + ndense, ncols - ndense, 0);
+      } else {
+        emit(setA, o, r0, m, net, cbase, ncols, ndense);
       }
     }
   }
-  // panel rows, zero entries padding each row to 16 bytes
-  std::unique_ptr<T[]> pdata(new T[std::max<int64_t>(psize, 1)]);
-  parallel_for((int64_t)pfill.size(), [&](int64_t q) {
-    const PFill &f = pfill[q];
-    const Col *c = cols.data() + f.col0;
-    for (int r = 0; r < f.m; r++) {
-      T *row = pdata.get() + f.poff + (int64_t)r * f.Lp;
-      for (int j = 0; j < f.Lp; j++) row[j] = j < f.L ? (T)(c[j].src[r] / c[j].scale) : (T)0;
-    }
-  });
+  // panel rows, zero entries padding each row to its 128-byte multiple
+  auto fill = [&](const PanelSet &S, auto *pdata) {
+    typedef std::remove_pointer_t<decltype(pdata)> E;
+    parallel_for((int64_t)S.pfill.size(), [&](int64_t q) {
+      const PFill &f = S.pfill[q];
+      const Col *c = cols.data() + f.col0;
+      for (int r = 0; r < f.m; r++) {
+        E *row = pdata + f.poff + (int64_t)r * f.Lp;
+        for (int j = 0; j < f.Lp; j++) row[j] = j < f.L ? (E)(c[j].src[r] / c[j].scale) : (E)0;
+      }
+    });
+  };
+  if (dense64) {
+    std::unique_ptr<double[]> pa(new double[std::max<int64_t>(setA.psize, 1)]);
+    std::unique_ptr<T[]> pb(new T[std::max<int64_t>(setB.psize, 1)]);
+    fill(setA, pa.get());
+    fill(setB, pb.get());
+    DFN_TRY(d.pdata.upload(pa.get(), (size_t)setA.psize));
+    DFN_TRY(d.pdata_b.upload(pb.get(), (size_t)setB.psize));
+  } else {
+    std::unique_ptr<T[]> pa(new T[std::max<int64_t>(setA.psize, 1)]);
+    fill(setA, pa.get());
+    DFN_TRY(d.pdata.upload(pa.get(), (size_t)setA.psize));
+  }
+  const std::vector<PTask> &pt = setA.pt;
   std::vector<int> fin_first((size_t)n * nops + 1, 0), fin_idx;
   for (size_t g = 0; g < fin.size(); g++) {
     fin_first[g] = (int)fin_idx.size();
     fin_idx.insert(fin_idx.end(), fin[g].begin(), fin[g].end());
   }
   fin_first[fin.size()] = (int)fin_idx.size();
-  if (widx.size() > (size_t)INT32_MAX || zp < 0 || yp < 0) return DFN_ERR_UNSUPPORTED;
+  if (setA.widx.size() + setB.widx.size() > (size_t)INT32_MAX || zp < 0 || yp < 0) return DFN_ERR_UNSUPPORTED;
   DFN_TRY(d.perm.upload(H0.perm));
   DFN_TRY(d.vtasks.upload(vt));
   DFN_TRY(d.vdata.upload(vdata.get(), (size_t)vsize));
   DFN_TRY(d.zred.upload(zr));
   DFN_TRY(d.zpart.alloc(sizeof(double) * std::max(zp, 1)));
   DFN_TRY(d.ptasks.upload(pt));
-  DFN_TRY(d.pdata.upload(pdata.get(), (size_t)psize));
-  DFN_TRY(d.widx.upload(widx));
-  DFN_TRY(d.cscale.upload(cscale));
+  DFN_TRY(d.widx.upload(setA.widx));
+  DFN_TRY(d.cscale.upload(setA.cscale));
+  d.pa_fp32 = scaled && !dense64;
+  d.npt_b = (int)setB.pt.size();
+  if (dense64) {
+    DFN_TRY(d.ptasks_b.upload(setB.pt));
+    DFN_TRY(d.widx_b.upload(setB.widx));
+    DFN_TRY(d.cscale_b.upload(setB.cscale));
+  }
   DFN_TRY(d.w.alloc(sizeof(double) * wsize));
   if (d.fp32) {
     DFN_TRY(d.xf.alloc(sizeof(float) * ((size_t)n + 4)));
@@ -454,10 +516,12 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   d.npt = (int)pt.size();
   d.S_total = S_total;
   // bytes: factors + x read + y written, per operator (x shared in Mode II)
-  d.alg_bytes = (double)sizeof(T) * (double)S_total + 8.0 * n + 8.0 * n * nops;
-  d.panel_bytes = (double)sizeof(T) * (double)panel_entries + 4.0 * (double)cscale.size() + 8.0 * (double)yp;
+  const double dense_extra = dense64 ? (8.0 - (double)sizeof(T)) * (double)S_dense : 0.0;
+  d.alg_bytes = (double)sizeof(T) * (double)S_total + dense_extra + 8.0 * n + 8.0 * n * nops;
+  d.panel_bytes = (double)setA.esize * (double)setA.entries + (double)setB.esize * (double)setB.entries +
+                  4.0 * (double)(setA.cscale.size() + setB.cscale.size()) + 8.0 * (double)yp;
   const char *at = std::getenv("DFN_AUTOTUNE");
-  if (!(at && std::string(at) == "0")) DFN_TRY(autotune(d, vt, pt));
+  if (!(at && std::string(at) == "0")) DFN_TRY(autotune(d, vt, pt, setB.pt));
   return DFN_OK;
 }
 

@@ -101,7 +101,8 @@ struct EnsEngine {
   hipStream_t stream = nullptr;
   hipStream_t cg_stream = nullptr;  // the first attempt's pressure solves run beside the k1 stress evaluation
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  std::unique_ptr<DevOps> ops64, ops32;
+  std::unique_ptr<DevOps> ops64, ops32;  // FP64 layout; the reduced-precision layout of kind ops32_kind
+  int ops32_kind = -1;
   HostOperator cat[2];  // concatenated operators (kept for the FP32 layout)
 
   // concatenated problem data
@@ -507,14 +508,28 @@ struct EnsEngine {
     for (int i = 0; i < B; i++) std::fill(hnet.begin() + base[i], hnet.begin() + base[i] + hp[i].N, i);
     ops64.reset(new DevOps);
     DFN_TRY(layout_ops<double>(list, nops, *ops64, &hnet));
-    if (precision == DFN_PREC_FP32) {
-      ops32.reset(new DevOps);
-      DFN_TRY(layout_ops<float>(list, nops, *ops32, &hnet));
-    }
+    if (precision != DFN_PREC_FP64) DFN_TRY(layout_reduced());
     return DFN_OK;
   }
 
-  const DevOps *active_ops() const { return (precision == DFN_PREC_FP32 && ops32) ? ops32.get() : ops64.get(); }
+  // the reduced-precision layout of the current precision (see Engine::layout_reduced)
+  int layout_reduced() {
+    drop_graphs();
+    ops32.reset();
+    ops32_kind = -1;
+    if (!built) return DFN_OK;
+    const HostOperator *list[2] = {&cat[0], &cat[1]};
+    std::vector<int> hnet(Ntot);
+    for (int i = 0; i < B; i++) std::fill(hnet.begin() + base[i], hnet.begin() + base[i] + hp[i].N, i);
+    ops32.reset(new DevOps);
+    DFN_TRY(layout_ops<float>(list, nops, *ops32, &hnet, precision == DFN_PREC_MIXED));
+    ops32_kind = precision;
+    return DFN_OK;
+  }
+
+  const DevOps *active_ops() const {
+    return (precision != DFN_PREC_FP64 && ops32 && ops32_kind == precision) ? ops32.get() : ops64.get();
+  }
 
   hipEvent_t *kt_next() {
     if (!kt_on || !all_active) return nullptr;
@@ -551,8 +566,7 @@ struct EnsEngine {
                d->xf.as<float>(), d_active.as<int>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     if (kt) (void)hipEventRecord(kt[0], stream);
-    launch_panel(d->ptasks.as<PTask>(), d->npt, d->pdata.p, d->fp32, d->widx.as<int>(), d->cscale.as<float>(), w,
-                 d->ypart.as<double>(), stream, d->panel_rows, d_active.as<int>());
+    launch_panels(*d, w, stream, d->panel_rows, d_active.as<int>());
     if (kt) (void)hipEventRecord(kt[1], stream);
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), Ntot,
                     d->nops, kscale, sT.as<double>(), d->nops > 1 ? sN.as<double>() : sT.as<double>(), stream);
@@ -879,9 +893,9 @@ API int dfn_ensemble_build_hmat(dfn_ensemble h) {
 
 API int dfn_ensemble_set_precision(dfn_ensemble h, int32_t prec) {
   DFN_API_BEGIN
-  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32)) return DFN_ERR_ARG;
+  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32 && prec != DFN_PREC_MIXED)) return DFN_ERR_ARG;
   h->e.precision = prec;
-  if (prec == DFN_PREC_FP32 && !h->e.ops32) return h->e.relayout();
+  if (prec != DFN_PREC_FP64 && h->e.ops32_kind != prec) return h->e.layout_reduced();
   h->e.drop_graphs();
   return DFN_OK;
   DFN_API_END

@@ -45,7 +45,10 @@ enum {
 enum { DFN_MODE_II = 2, DFN_MODE_III = 3 };               /* hyperparameter/fracture_mode */
 enum { DFN_KERNEL_DIRECT = 0, DFN_KERNEL_HMATRIX = 1 };   /* hyperparameter/static_kernel */
 enum { DFN_OP_T = 0, DFN_OP_N = 1 };                      /* shear (Mode II/III) and normal (Mode II) operator */
-enum { DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1 };            /* storage of the operator blocks */
+/* storage of the operator blocks: all FP64; all FP32 (power-of-two block scales,
+ * FP32 accumulation, compensated near-field sums); FP32 low-rank factors beside
+ * FP64 near-field (dense) blocks */
+enum { DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1, DFN_PREC_MIXED = 2 };
 
 /* Problem description: the content of problems/<name>/config
  * (src/configuration.f90:70-379) as flat arrays.  Per-element arrays have

@@ -80,3 +80,31 @@ def test_full_size_fp32_operator_error(pair):
         d.set_precision(_abi.PREC_FP64)
     print(f"[gpu] full-size FP32 operators: worst norm-wise error {worst:.2e}", flush=True)
     assert worst <= 1e-6
+
+
+def test_full_size_mixed_operator_error(pair):
+    """The 80k network with FP32 low-rank factors and FP64 near-field blocks: the
+    stress-rate error over 20 log-uniform slip-rate vectors, norm-wise and on the
+    rows above 1e-3 of the largest stress rate (bar 1e-5), beside all-FP32 storage."""
+    p, d, _ = pair
+    rng = np.random.default_rng(13)
+    worst = {_abi.PREC_FP32: [0.0, 0.0], _abi.PREC_MIXED: [0.0, 0.0]}
+    try:
+        for _ in range(20):
+            V = 10.0 ** rng.uniform(-12.0, 0.0, p.N)
+            d.set_precision(_abi.PREC_FP64)
+            a = d.kernel(V)
+            for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
+                d.set_precision(prec)
+                b = d.kernel(V)
+                for y, y64 in zip(b, a):
+                    big = np.abs(y64) > 1e-3 * np.abs(y64).max()
+                    w = worst[prec]
+                    w[0] = max(w[0], _rel(y, y64))
+                    w[1] = max(w[1], float(np.max(np.abs(y[big] - y64[big]) / np.abs(y64[big]))))
+    finally:
+        d.set_precision(_abi.PREC_FP64)
+    print(f"[gpu] full-size norm-wise / significant-row errors: FP32 {worst[_abi.PREC_FP32]}, "
+          f"mixed {worst[_abi.PREC_MIXED]}", flush=True)
+    assert worst[_abi.PREC_MIXED][0] <= 1e-6
+    assert worst[_abi.PREC_MIXED][1] <= 1e-5

@@ -160,7 +160,7 @@ def test_hmatvec_on_identical_block_lists(mode):
         assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
 
 
-@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_FP32])
+@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_FP32, _abi.PREC_MIXED])
 def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
     """The autotuner only picks launch shapes that keep every sum in the same order:
     the stress rates of the tuned layout, of the default shapes and of every
@@ -296,12 +296,46 @@ def test_fp32_operator_storage_error(mode):
                 assert err > 0  # really a different storage
 
 
-def test_fp32_operator_trajectory_tracks_fp64():
+@pytest.mark.parametrize("mode", [2, 3])
+def test_mixed_operator_storage_error(mode):
+    """FP32 low-rank factors beside FP64 near-field blocks (DFN_PREC_MIXED) vs the
+    FP64 operator on log-uniform slip rates in [1e-12, 1] m/s: norm-wise within
+    1e-6, and on every row above 1e-3 of the largest stress rate within 1e-5
+    relative -- the rows where all-FP32 storage loses the most, because a
+    near-field entry stored in FP32 keeps its rounding times the row's
+    cancellation ratio."""
+    p = _hnet(mode, n_faults=20, seed=9)
+    d = dev(p)
+    d.initiate_hmat()
+    rng = np.random.default_rng(3)
+    worst = {_abi.PREC_FP32: 0.0, _abi.PREC_MIXED: 0.0}
+    for _ in range(20):
+        V = 10.0 ** rng.uniform(-12, 0, p.N)
+        d.set_precision(_abi.PREC_FP64)
+        a = d.kernel(V)
+        for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
+            d.set_precision(prec)
+            b = d.kernel(V)
+            for y64, y in zip(a, b):
+                if np.linalg.norm(y64) == 0:
+                    continue
+                if prec == _abi.PREC_MIXED:
+                    assert 0 < rel(y, y64) <= 1e-6
+                big = np.abs(y64) > 1e-3 * np.abs(y64).max()
+                worst[prec] = max(worst[prec], float(np.max(np.abs(y[big] - y64[big]) / np.abs(y64[big]))))
+    print(f"[gpu] significant-row max error: FP32 {worst[_abi.PREC_FP32]:.2e}, mixed {worst[_abi.PREC_MIXED]:.2e}",
+          flush=True)
+    assert worst[_abi.PREC_MIXED] <= 1e-5
+    assert worst[_abi.PREC_MIXED] < worst[_abi.PREC_FP32]
+
+
+@pytest.mark.parametrize("prec", [_abi.PREC_FP32, _abi.PREC_MIXED])
+def test_reduced_operator_trajectory_tracks_fp64(prec):
     p = _hnet(2)
     a, b = dev(p), dev(p)
     a.initiate_hmat()
     b.initiate_hmat()
-    b.set_precision(_abi.PREC_FP32)
+    b.set_precision(prec)
     a.solve(15)
     b.solve(15)
     Ya, Yb = a.get_state()[0], b.get_state()[0]
