@@ -127,6 +127,17 @@ struct EnsEngine {
   std::vector<CoopGroup> coop_all, coop_stage;
   DMem map_all, map_stage, coop_records;
   unsigned cg_seq = 0;
+  // Launches issued one by one with cooperative pressure solves: the groups hold the
+  // systems slot by slot (every network's slot 0 first), an event follows each
+  // group and the small networks' launch, and the stage kernel that reads slot s
+  // waits only for the group that completes slot s, so the later solves run beside
+  // the stress evaluations of the earlier stages.
+  std::vector<int> last_group_all, last_group_stage;  // per list slot: the group completing it
+  std::vector<hipEvent_t> ev_group;
+  hipEvent_t ev_small = nullptr;
+  const std::vector<int> *cur_last = nullptr;  // the last cg_launch's list
+  int cur_first = 0;                           // its first slot
+  bool per_slot() const { return coop && !use_graph; }
   // pinned host mirrors
   double *h_h = nullptr, *h_ttry = nullptr;
   int *h_active = nullptr, *h_cgres = nullptr;
@@ -167,6 +178,9 @@ struct EnsEngine {
   ~EnsEngine() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (hipEvent_t ev : kt_ev) (void)hipEventDestroy(ev);
+    if (cg_stream) (void)hipStreamSynchronize(cg_stream);
+    for (hipEvent_t ev : ev_group) (void)hipEventDestroy(ev);
+    if (ev_small) (void)hipEventDestroy(ev_small);
     drop_graphs();
     if (h_h) (void)hipHostFree(h_h);
     if (h_ttry) (void)hipHostFree(h_ttry);
@@ -362,13 +376,17 @@ struct EnsEngine {
     for (int i : large) max_wps = std::max(max_wps, (hp[i].N + kCgSpan - 1) / kCgSpan);
     coop = capacity > 0 && max_wps <= capacity;
     if (!coop) return DFN_OK;
-    // whole systems packed in list order into launches of at most `capacity` workgroups
-    auto plan = [&](int nslot, std::vector<CoopGroup> &groups, DMem &map) -> int {
+    // whole systems packed slot by slot (every large network's first slot, then its
+    // second, ...) into launches of at most `capacity` workgroups; the list itself
+    // keeps each network's slots together (system index li * nslot + sl)
+    auto plan = [&](int nslot, std::vector<CoopGroup> &groups, DMem &map, std::vector<int> &last) -> int {
       std::vector<int4> m;
       groups.clear();
-      int sysi = 0, in_group = 0, group_first = 0;
-      for (int i : large)
-        for (int sl = 0; sl < nslot; sl++, sysi++) {
+      last.assign(nslot, 0);
+      int in_group = 0, group_first = 0;
+      for (int sl = 0; sl < nslot; sl++) {
+        for (size_t li = 0; li < large.size(); li++) {
+          const int i = large[li], sysi = (int)li * nslot + sl;
           const int wps = (hp[i].N + kCgSpan - 1) / kCgSpan;
           if (in_group + wps > capacity) {
             groups.push_back({group_first, in_group});
@@ -378,11 +396,16 @@ struct EnsEngine {
           for (int q = 0; q < wps; q++) m.push_back(make_int4(sysi, q, wps, in_group));
           in_group += wps;
         }
+        last[sl] = (int)groups.size();  // the group being filled holds this slot's last system
+      }
       groups.push_back({group_first, in_group});
       return map.upload(m);
     };
-    DFN_TRY(plan(kEnsSlots, coop_all, map_all));
-    DFN_TRY(plan(kEnsSlots - 1, coop_stage, map_stage));
+    DFN_TRY(plan(kEnsSlots, coop_all, map_all, last_group_all));
+    DFN_TRY(plan(kEnsSlots - 1, coop_stage, map_stage, last_group_stage));
+    ev_group.assign(std::max(coop_all.size(), coop_stage.size()), nullptr);
+    for (hipEvent_t &ev : ev_group) DFN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    DFN_HIP(hipEventCreateWithFlags(&ev_small, hipEventDisableTiming));
     // message records: kCgRecordWords per workgroup of the largest launch
     int most = 0;
     for (const CoopGroup &g : coop_all) most = std::max(most, g.nwg);
@@ -626,6 +649,7 @@ struct EnsEngine {
       CgArgs A = cg_args(sys);
       A.n = max_small;
       launch_cg_lds_multi(A, X, n_small * nslot, max_small, cs);
+      if (per_slot()) DFN_HIP(hipEventRecord(ev_small, cs));
     }
     if (n_large > 0) {
       CgSys *sys = stages_only ? sys_large_stage.as<CgSys>() : sys_large_all.as<CgSys>();
@@ -638,7 +662,10 @@ struct EnsEngine {
       }
       const std::vector<CoopGroup> &groups = stages_only ? coop_stage : coop_all;
       const int4 *map = (stages_only ? map_stage : map_all).as<int4>();
-      for (const CoopGroup &g : groups) {
+      cur_last = stages_only ? &last_group_stage : &last_group_all;
+      cur_first = first;
+      for (size_t gi = 0; gi < groups.size(); gi++) {
+        const CoopGroup &g = groups[gi];
         if (++cg_seq >= kCgMaxSeq) {  // epochs would repeat: clear every message word first
           DFN_HIP(hipMemsetAsync(coop_records.p, 0, coop_records.bytes, cs));
           DFN_HIP(hipMemsetAsync(cg_work.p, 0, sizeof(double) * (size_t)kEnsSlots * 12 * Ntot, cs));
@@ -647,17 +674,26 @@ struct EnsEngine {
         A.wg_map = map + g.first;
         if (!launch_cg_coop_map(A, X, g.nwg, coop_cap, cg_seq, coop_records.as<unsigned long long>(), cs))
           return DFN_ERR_HIP;
+        if (per_slot()) DFN_HIP(hipEventRecord(ev_group[gi], cs));
       }
     }
     return DFN_OK;
   }
 
-  // stages 2..6 of rkck (ode_bs.f90:199-224) for the active networks
-  void attempt() {
+  // the stage kernel about to read pressure slot `slot` waits for its solves
+  void wait_slot(int slot) {
+    if (n_small > 0) (void)hipStreamWaitEvent(stream, ev_small, 0);
+    if (n_large > 0 && cur_last) (void)hipStreamWaitEvent(stream, ev_group[(*cur_last)[slot - cur_first]], 0);
+  }
+
+  // stages 2..6 of rkck (ode_bs.f90:199-224) for the active networks; with per-slot
+  // waits each stage kernel first waits for its own pressure slot
+  void attempt(bool waits = false) {
     launch_ens_clear_status(status.as<StatusWord>(), d_active.as<int>(), B, stream);
     double *in = ytA.as<double>(), *out = ytB.as<double>();
     for (int s = 2; s <= 6; s++) {
       apply(in);
+      if (waits) wait_slot(s - 1);
       const EnsStageArgs E = stage_args(in, s - 1, s == 6 ? Ynew.as<double>() : out);
       launch_ens_stage(s, E, stream);
       std::swap(in, out);
@@ -667,21 +703,29 @@ struct EnsEngine {
   // first attempt of a step: compute_P of all six slots, k1 and yscal, stages 2..6
   // (the pressure solves on the side stream, joined before stage 1, which reads them)
   void seq_try() {
+    const bool ps = per_slot();
     (void)hipEventRecord(ev_fork, stream);
     (void)hipStreamWaitEvent(cg_stream, ev_fork, 0);
     seq_rc = cg_launch(false, cg_stream);
-    (void)hipEventRecord(ev_join, cg_stream);
+    if (!ps) (void)hipEventRecord(ev_join, cg_stream);
     apply(Y.as<double>());
-    (void)hipStreamWaitEvent(stream, ev_join, 0);
+    if (ps) wait_slot(0);
+    else (void)hipStreamWaitEvent(stream, ev_join, 0);
     launch_ens_stage(1, stage_args(Y.as<double>(), 0, ytA.as<double>()), stream);
-    attempt();
+    attempt(ps);
   }
 
   // retried attempt with reduced steps: compute_P of slots 1..5, restart, stages 2..6
+  // (with per-slot waits the solves run on the side stream beside the stages)
   void seq_retry() {
-    seq_rc = cg_launch(true, stream);
+    const bool ps = per_slot();
+    if (ps) {
+      (void)hipEventRecord(ev_fork, stream);
+      (void)hipStreamWaitEvent(cg_stream, ev_fork, 0);
+    }
+    seq_rc = cg_launch(true, ps ? cg_stream : stream);
     launch_ens_restart_stage2(stage_args(Y.as<double>(), 0, ytA.as<double>()), stream);
-    attempt();
+    attempt(ps);
   }
   int seq_rc = DFN_OK;  // status of the last sequence's compute_P launches
 
