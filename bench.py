@@ -38,7 +38,7 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 MEASURED_TRAFFIC = {("network800_80k", 2024): 2.4947e9, ("ensemble64_1374k_fp32_learned", 2024): 1.5468e10}
 YEAR = 365.25 * 86400.0
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 peak (vector and matrix cores alike)
 
 
 def log(*a):
@@ -160,10 +160,16 @@ def mixed_precision(sim, p, args, kb64):
     from demystification_dfn_amd import _abi
     rng = np.random.default_rng(7)
     errs = {_abi.PREC_FP32: ([], []), _abi.PREC_MIXED: ([], [])}
-    for _ in range(args.mixed_vectors):
-        V = 10.0 ** rng.uniform(-12.0, 0.0, p.N)
-        sim.set_precision(_abi.PREC_FP64)
-        a = sim.kernel(V)
+    # the FP64 stress rates of every vector in one multi-RHS application
+    X = 10.0 ** rng.uniform(-12.0, 0.0, (args.mixed_vectors, p.N))
+    w0 = time.perf_counter()
+    Y64 = sim.kernel_multi(X)
+    multi_wall = time.perf_counter() - w0
+    multi = multi_rhs_bench(sim, X, kb64)
+    multi["wall_s_with_host_copies"] = multi_wall
+    for r in range(args.mixed_vectors):
+        V = X[r]
+        a = tuple(y[r] for y in Y64 if y is not None)
         for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
             sim.set_precision(prec)
             b = sim.kernel(V)
@@ -198,7 +204,23 @@ def mixed_precision(sim, p, args, kb64):
                                      alg_bytes=kbm["alg_bytes"],
                                      hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kbm["ms_kernel"],
                                      note="FP32 low-rank factors beside FP64 near-field blocks (DFN_PREC_MIXED)"),
-                autotune=tune32, note="continues from the FP64 run's final state; operators FP32, state FP64")
+                autotune=tune32, fp64_multi_rhs=multi,
+                note="continues from the FP64 run's final state; operators FP32, state FP64")
+
+
+def multi_rhs_bench(sim, X, kb64):
+    """The FP64 H-matrix applied to the rows of X in one multi-RHS call
+    (dfn_kernel_multi, phase 2 on the matrix cores), timed on the device with the
+    vectors resident, against as many one-vector applications."""
+    b = sim.bench_kernel_multi(X[:1024], reps=3)
+    n = min(len(X), 1024)
+    tflops = b["panel_flops"] / (b["ms_panel"] * 1e-3) / 1e12
+    return dict(vectors=n, ms=b["ms_apply"], ms_per_vector=b["ms_apply"] / n,
+                one_vector_ms=kb64["ms_kernel"], speedup_vs_one_vector_calls=kb64["ms_kernel"] * n / b["ms_apply"],
+                panel_ms=b["ms_panel"], panel_TFLOP_s=tflops, panel_frac_fp64_peak=tflops / FP64_PEAK_TFLOPS,
+                alg_GB_s=b["alg_bytes"] / (b["ms_apply"] * 1e-3) / 1e9,
+                note="phase-2 flops = 2 x stored entries x vectors rounded up to 64; peak = FP64 matrix peak "
+                     "(equal to the FP64 vector peak on MI355X)")
 
 
 def learned_step(args, probs):

@@ -106,6 +106,8 @@ SIGNATURES = {
     "kernel": [_H, _dp, _dp, _dp],
     # device pointers slip, stressT, stressN and a hipStream_t (NULL: default stream)
     "kernel_noalloc": [_H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "kernel_multi": [_H, C.c_int32, _dp, _dp, _dp],
+    "bench_kernel_multi": [_H, C.c_int32, _dp, C.c_int32, _dp],
     "hgemv": [_H, C.c_int32, C.c_double, C.c_double, _dp, _dp],
     "friction_law": [_H, C.c_double, _dp, _dp],
     "compute_P": [_H, C.c_double, _dp, _dp, _i32p],

@@ -159,6 +159,22 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
 // rows: panel rows per wave step, 2 or 4; active as for launch_vtx
 void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
                   const double *w, double *ypart, hipStream_t s, int rows = 2, const int *active = nullptr);
+// Multi-RHS stress evaluation (FP64 layout): X holds nrhs vectors of n (vector
+// after vector, original order), y{T,N} likewise; R is nrhs rounded up to a multiple
+// of 64; W ((w size) x R), zpart ((zpart size) x R), ypart ((ypart size) x R) and Yh
+// (nops n x R) are work arrays, row-major with row stride R; ev (optional) gets
+// events recorded before and after the phase-2 kernel.
+struct MrhsLayout {
+  const VTask *vt;
+  const double *vdata;
+  const ZRed *zr;
+  const PTask *pt;
+  const double *pdata;
+  const int *widx, *fin_first, *fin_idx, *perm;
+  int nvt, nzr, npt, n, nops;
+};
+void launch_mrhs(const MrhsLayout &d, const double *X, int nrhs, int R, double *W, double *zpart, double *ypart,
+                 double *Yh, double scale, double *yT, double *yN, hipStream_t s, hipEvent_t *ev);
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,
                      double scale, double *yT, double *yN, hipStream_t s);
 // dense direct kernels: node columns per tile, number of tiles along the nodes,

@@ -64,6 +64,10 @@ struct Engine {
   // state and stage buffers
   DMem Y, Ynew, yscal, sT, sN, kbuf, ytA, ytB, P, Pdot, scratch3, scratch3b;
   DMem cg_work, cg_sys, cg_res, cg_records, xrow_start, xrow_ent;
+  // multi-RHS stress evaluation (dfn_kernel_multi): vector batches of up to mr_R
+  // (a multiple of 64), input and output vectors and the FP64 layout's work arrays
+  DMem mr_X, mr_Y, mr_W, mr_zp, mr_yp, mr_Yh;
+  int mr_R = 0;
   bool coop_ok = true;
   int span_entries = 0;  // most crossing entries in one span of kCgSpan rows (cooperative CG)
   unsigned cg_seq = 0;   // sequence number of the last cooperative CG launch
@@ -333,6 +337,43 @@ struct Engine {
     if (kt) DFN_HIP(hipEventRecord(kt[1], stream));
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                     d->nops, kernel_scale(), d_sT, d->nops > 1 ? d_sN : d_sT, stream);
+    return DFN_OK;
+  }
+
+  // work arrays for multi-RHS batches of R vectors (R a multiple of 64)
+  int mrhs_reserve(int R) {
+    if (R <= mr_R) return DFN_OK;
+    const DevOps &d = *ops64;
+    const size_t n = hp.N, r = R;
+    DFN_TRY(mr_X.alloc(sizeof(double) * n * r));
+    DFN_TRY(mr_Y.alloc(sizeof(double) * n * r * d.nops));
+    DFN_TRY(mr_W.alloc(d.w.bytes * r));
+    DFN_TRY(mr_zp.alloc(d.zpart.bytes * r));
+    DFN_TRY(mr_yp.alloc(d.ypart.bytes * r));
+    DFN_TRY(mr_Yh.alloc(sizeof(double) * n * r * d.nops));
+    mr_R = R;
+    return DFN_OK;
+  }
+
+  // largest multi-RHS batch: at most 1024 vectors and about 48 GiB of work arrays
+  int mrhs_batch(int nrhs) const {
+    const DevOps &d = *ops64;
+    const double per = (double)d.w.bytes + d.zpart.bytes + d.ypart.bytes + 8.0 * hp.N * (1 + 2 * d.nops);
+    int cap = (int)std::min(1024.0, std::floor(48.0 * (1 << 30) / per / 64) * 64);
+    cap = std::max(cap, 64);
+    return std::min(cap, (nrhs + 63) / 64 * 64);
+  }
+
+  // stress rates of nb vectors (device, vector after vector, original order) with the
+  // FP64 layout into mr_Y (T vectors, then N vectors); ev: events around phase 2
+  int apply_multi(const double *dX, int nb, hipEvent_t *ev = nullptr) {
+    const DevOps &d = *ops64;
+    const int R = (nb + 63) / 64 * 64;
+    DFN_TRY(mrhs_reserve(R));
+    double *Y = mr_Y.as<double>();
+    launch_mrhs(d, dX, nb, R, mr_W.as<double>(), mr_zp.as<double>(), mr_yp.as<double>(), mr_Yh.as<double>(),
+                kernel_scale(), Y, Y + (size_t)hp.N * nb, stream, ev);
+    DFN_HIP(hipGetLastError());
     return DFN_OK;
   }
 
@@ -1142,6 +1183,73 @@ API int dfn_kernel(dfn_handle h, const double *slip, double *sTh, double *sNh) {
   if (e.hp.mode == DFN_MODE_II)
     DFN_HIP(hipMemcpyAsync(sNh, e.sN.p, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
   DFN_HIP(hipStreamSynchronize(e.stream));
+  return DFN_OK;
+  DFN_API_END
+}
+
+API int dfn_kernel_multi(dfn_handle h, int32_t nrhs, const double *slip, double *sTh, double *sNh) {
+  DFN_API_BEGIN
+  if (!h || nrhs < 0 || (nrhs > 0 && (!slip || !sTh))) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  const bool mode2 = e.hp.mode == DFN_MODE_II;
+  if (mode2 && nrhs > 0 && !sNh) return DFN_ERR_ARG;
+  if (e.hp.kernel != DFN_KERNEL_HMATRIX || !e.ops64) return DFN_ERR_STATE;
+  const size_t N = e.hp.N;
+  const int B = e.mrhs_batch(nrhs);
+  for (int r0 = 0; r0 < nrhs; r0 += B) {
+    const int nb = std::min(B, nrhs - r0);
+    DFN_TRY(e.mrhs_reserve((nb + 63) / 64 * 64));
+    DFN_HIP(hipMemcpyAsync(e.mr_X.p, slip + r0 * N, nb * N * sizeof(double), hipMemcpyHostToDevice, e.stream));
+    DFN_TRY(e.apply_multi(e.mr_X.as<double>(), nb));
+    const double *Y = e.mr_Y.as<double>();
+    DFN_HIP(hipMemcpyAsync(sTh + r0 * N, Y, nb * N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    if (mode2)
+      DFN_HIP(hipMemcpyAsync(sNh + r0 * N, Y + nb * N, nb * N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
+    DFN_HIP(hipStreamSynchronize(e.stream));
+  }
+  return DFN_OK;
+  DFN_API_END
+}
+
+API int dfn_bench_kernel_multi(dfn_handle h, int32_t nrhs, const double *slip, int32_t reps, double *out) {
+  DFN_API_BEGIN
+  if (!h || !slip || !out || nrhs < 1 || reps < 1) return DFN_ERR_ARG;
+  Engine &e = h->e;
+  if (e.hp.kernel != DFN_KERNEL_HMATRIX || !e.ops64) return DFN_ERR_STATE;
+  if (e.mrhs_batch(nrhs) < nrhs) return DFN_ERR_UNSUPPORTED;
+  const size_t N = e.hp.N;
+  DFN_TRY(e.mrhs_reserve((nrhs + 63) / 64 * 64));
+  DFN_HIP(hipMemcpyAsync(e.mr_X.p, slip, nrhs * N * sizeof(double), hipMemcpyHostToDevice, e.stream));
+  DFN_TRY(e.apply_multi(e.mr_X.as<double>(), nrhs));  // warm
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } evs;
+  hipEvent_t *ev = evs.e;
+  DFN_HIP(hipEventCreate(&ev[0]));
+  DFN_HIP(hipEventCreate(&ev[1]));
+  float t = 0, tp = 0, tsum = 0;
+  DFN_HIP(hipStreamSynchronize(e.stream));
+  DFN_HIP(hipEventRecord(e.ev0, e.stream));
+  for (int r = 0; r < reps; r++) DFN_TRY(e.apply_multi(e.mr_X.as<double>(), nrhs));
+  DFN_HIP(hipEventRecord(e.ev1, e.stream));
+  DFN_HIP(hipEventSynchronize(e.ev1));
+  DFN_HIP(hipEventElapsedTime(&t, e.ev0, e.ev1));
+  for (int r = 0; r < reps; r++) {  // the phase-2 kernel alone, launch by launch
+    DFN_TRY(e.apply_multi(e.mr_X.as<double>(), nrhs, ev));
+    DFN_HIP(hipEventSynchronize(ev[1]));
+    DFN_HIP(hipEventElapsedTime(&tp, ev[0], ev[1]));
+    tsum += tp;
+  }
+  const dfn::DevOps &d = *e.ops64;
+  const int R = (nrhs + 63) / 64 * 64;
+  out[0] = (double)t / reps;
+  out[1] = (double)tsum / reps;
+  out[2] = 2.0 * (double)d.S_total * R;                           // phase-2 flops of one launch
+  out[3] = 8.0 * (double)d.S_total + 16.0 * (double)N * nrhs * d.nops;  // matrix once + vectors in/out
   return DFN_OK;
   DFN_API_END
 }

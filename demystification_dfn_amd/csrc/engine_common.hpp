@@ -115,6 +115,18 @@ inline std::vector<PTask> panel_schedule(const std::vector<PTask> &pt, int split
   return out;
 }
 
+// Multi-RHS stress evaluation with an FP64 layout (d.fp32 false, one panel set)
+inline void launch_mrhs(const DevOps &d, const double *X, int nrhs, int R, double *W, double *zpart, double *ypart,
+                        double *Yh, double scale, double *yT, double *yN, hipStream_t s, hipEvent_t *ev = nullptr) {
+  MrhsLayout m;
+  m.vt = d.vtasks.as<VTask>(); m.nvt = d.nvt; m.vdata = static_cast<const double *>(d.vdata.p);
+  m.zr = d.zred.as<ZRed>(); m.nzr = d.nzr;
+  m.pt = d.ptasks.as<PTask>(); m.npt = d.npt; m.pdata = static_cast<const double *>(d.pdata.p);
+  m.widx = d.widx.as<int>(); m.fin_first = d.fin_first.as<int>(); m.fin_idx = d.fin_idx.as<int>();
+  m.perm = d.perm.as<int>(); m.n = d.n; m.nops = d.nops;
+  launch_mrhs(m, X, nrhs, R, W, zpart, ypart, Yh, scale, yT, yN, s, ev);
+}
+
 // Phase 2 of a stress evaluation: every panel set of the layout (rows: panel rows
 // per wave step; active as for launch_panel)
 inline void launch_panels(const DevOps &d, const double *w, hipStream_t s, int rows, const int *active = nullptr) {

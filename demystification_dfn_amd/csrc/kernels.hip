@@ -367,6 +367,220 @@ __global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ f
 }
 
 // ---------------------------------------------------------------------------
+// Multi-RHS stress transfer: the H-matrix applied to R slip-rate vectors at once,
+// both phases as dense products on the matrix cores (v_mfma_f64_16x16x4f64) over
+// tiles of 16 vectors.  Every vector set is row-major with a stride R that is a
+// multiple of 64: the gathered input and the z values W[q R + r], the phase-1
+// partials zpart[q R + r], the panel row partials ypart[q R + r].
+// Operand layout of the FP64 16x16x4 product: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator element e of lane l is C[l / 16 + 4 e][l % 16].
+// ---------------------------------------------------------------------------
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+// W[i R + r] = X[r n + perm[i]] (r < nrhs; 0 for the padding vectors), in 32 x 32 tiles
+__global__ void __launch_bounds__(256) mrhs_gather_kernel(const double *__restrict__ X, const int *__restrict__ perm,
+                                                          int n, int nrhs, int R, double *__restrict__ W) {
+  __shared__ double t[32][33];
+  const int i0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int q = ty; q < 32; q += 8) {
+    const int i = i0 + tx, r = r0 + q;
+    t[q][tx] = (i < n && r < nrhs) ? X[(int64_t)r * n + perm[i]] : 0.0;
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int i = i0 + q, r = r0 + tx;
+    if (i < n) W[(int64_t)i * R + r] = t[tx][q];
+  }
+}
+
+// Phase 1: z_j[r] = scale * sum_i V[i][j] X[xcol + row0 + i][r] for j < k, one wave
+// per (task, group of 64 vectors): products of the task's V^T (k x len, in K
+// tiles of 16) with its X rows (len x 64, four tiles of 16), 4 rows per product.
+// The loads of 16 rows are issued together before their products.
+__global__ void __launch_bounds__(256) mrhs_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
+                                                       const double *__restrict__ vdata, double *__restrict__ W,
+                                                       double *__restrict__ zpart, int R, int groups) {
+  const int wv = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63, am = lane & 15, ak = lane >> 4;
+  const int t = wv / groups, g = wv - t * groups;
+  if (t >= ntasks) return;
+  const VTask task = tasks[t];
+  const int K = task.k, len = task.len;
+  const bool two = K > 16;
+  const double *V = vdata + task.voff;
+  const double *X = W + (int64_t)(task.xcol + task.row0) * R + g * 64 + am;
+  d4v acc[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = d4v{0.0, 0.0, 0.0, 0.0};
+  for (int i0 = 0; i0 < len; i0 += 16) {
+    double a[4][2], x[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + 4 * u + ak;
+      const bool in = i < len;
+      // V of a task is stored strip by strip (128 rows each for FP64), column after column
+      const int s0 = (i / 128) * 128, hp = (min(128, len - s0) + 1) & ~1;
+      const double *Vi = V + (int64_t)s0 * K + (i - s0);
+      a[u][0] = (in && am < K) ? Vi[(int64_t)am * hp] : 0.0;
+      a[u][1] = (in && two && 16 + am < K) ? Vi[(int64_t)(16 + am) * hp] : 0.0;
+#pragma unroll
+      for (int tile = 0; tile < 4; tile++) x[u][tile] = in ? X[(int64_t)i * R + tile * 16] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int tile = 0; tile < 4; tile++) {
+        acc[0][tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][0], x[u][tile], acc[0][tile], 0, 0, 0);
+        if (two) acc[1][tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][1], x[u][tile], acc[1][tile], 0, 0, 0);
+      }
+  }
+#pragma unroll
+  for (int kt = 0; kt < 2; kt++)
+#pragma unroll
+    for (int tile = 0; tile < 4; tile++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int j = kt * 16 + ak + 4 * e;
+        if (j < K) {
+          double *dst = task.wout >= 0 ? W + (int64_t)(task.wout + j) * R : zpart + (int64_t)(task.zp + j) * R;
+          dst[g * 64 + tile * 16 + am] = acc[kt][tile][e] * task.scale;
+        }
+      }
+}
+
+// z of each multi-chunk block: the sum of its chunk partials, in chunk order
+__global__ void __launch_bounds__(256) mrhs_zreduce_kernel(const ZRed *__restrict__ zr, int nblocks,
+                                                           const double *__restrict__ zpart, double *__restrict__ W,
+                                                           int R) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = 32 * (int64_t)R;
+  const int b = (int)(g / per), j = (int)((g % per) / R), r = (int)(g % R);
+  if (b >= nblocks) return;
+  const ZRed z = zr[b];
+  if (j >= z.k) return;
+  double s = 0.0;
+  for (int t = 0; t < z.ntask; t++) s += zpart[(int64_t)(z.zp_first + t * z.k + j) * R + r];
+  W[(int64_t)(z.zout + j) * R + r] = s;
+}
+
+// Phase 2: ypart[yp + row][r] = sum_c P[row][c] W[widx[c]][r]; one workgroup per
+// (panel task, group of 64 vectors), wave w on vectors 16 w..16 w + 15 of the group.
+// The panel is staged through LDS 64 rows x 32 columns at a time (double-buffered,
+// one barrier per chunk) and the loads of the next chunk (its panel entries, its
+// gathered vector entries) are in flight while the current chunk's products run;
+// the column indices run one chunk further ahead.  Row tiles of 16 past the
+// panel's last row are skipped.
+__global__ void __launch_bounds__(256) mrhs_panel_kernel(const PTask *__restrict__ tasks,
+                                                         const double *__restrict__ pdata,
+                                                         const int *__restrict__ widx, const double *__restrict__ W,
+                                                         double *__restrict__ ypart, int R, int groups) {
+  __shared__ double As[2][64][33];
+  const int t = blockIdx.x / groups, g = blockIdx.x - t * groups;
+  const PTask task = tasks[t];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, am = lane & 15, ak = lane >> 4;
+  const int rhs = g * 64 + wave * 16 + am;
+  const double *P = pdata + task.poff;
+  const int *wi = widx + task.widx;
+  const int m = task.m, L = task.L;
+  // (a panel with no columns still writes its zero rows)
+  const int nch = max(1, (L + 31) >> 5), nrb = (m + 63) >> 6, nsteps = nch * nrb;
+  // step q: row block q / nch, columns 32 (q % nch) ..
+  double pa[8], pb[8], cb[8];
+  int pw[8];
+  auto load_a = [&](int q) {
+    const int row0 = (q / nch) * 64, c0 = (q % nch) * 32;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int r = (threadIdx.x >> 5) + 8 * u, c = threadIdx.x & 31;
+      pa[u] = (row0 + r < m && c0 + c < L) ? P[(int64_t)(row0 + r) * L + c0 + c] : 0.0;
+    }
+  };
+  auto load_w = [&](int q) {
+    const int c0 = (q % nch) * 32;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int c = c0 + 4 * u + ak;
+      pw[u] = c < L ? wi[c] : -1;
+    }
+  };
+  auto load_b = [&]() {
+#pragma unroll
+    for (int u = 0; u < 8; u++) pb[u] = pw[u] >= 0 ? W[(int64_t)pw[u] * R + rhs] : 0.0;
+  };
+  load_a(0);
+  load_w(0);
+  load_b();
+  if (nsteps > 1) load_w(1);
+  d4v acc[4];
+  for (int q = 0; q < nsteps; q++) {
+    const int buf = q & 1, rb = q / nch, ch = q - rb * nch;
+#pragma unroll
+    for (int u = 0; u < 8; u++) As[buf][(threadIdx.x >> 5) + 8 * u][threadIdx.x & 31] = pa[u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) cb[u] = pb[u];
+    __syncthreads();
+    if (q + 1 < nsteps) {
+      load_a(q + 1);
+      load_b();
+      if (q + 2 < nsteps) load_w(q + 2);
+    }
+    if (ch == 0)
+#pragma unroll
+      for (int rt = 0; rt < 4; rt++) acc[rt] = d4v{0.0, 0.0, 0.0, 0.0};
+    const int row0 = rb * 64;
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+      if (row0 + rt * 16 < m)
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[buf][rt * 16 + am][4 * u + ak], cb[u], acc[rt], 0, 0, 0);
+    if (ch == nch - 1)
+#pragma unroll
+      for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int row = row0 + rt * 16 + ak + 4 * e;
+          if (row < m) ypart[(int64_t)(task.yp + row) * R + rhs] = acc[rt][e];
+        }
+  }
+}
+
+// Yh[(o n + i) R + r] = sum of the partials of H row i of operator o, in order
+__global__ void __launch_bounds__(256) mrhs_finalize_kernel(const int *__restrict__ fin_first,
+                                                            const int *__restrict__ fin_idx,
+                                                            const double *__restrict__ ypart, int n, int nops, int R,
+                                                            double *__restrict__ Yh) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)n * nops * R) return;
+  const int64_t row = g / R;
+  const int r = (int)(g - row * R);
+  double s = 0.0;
+  for (int q = fin_first[row]; q < fin_first[row + 1]; q++) s += ypart[(int64_t)fin_idx[q] * R + r];
+  Yh[g] = s;
+}
+
+// y[r n + perm[i]] = scale * Yh[i R + r] (r < nrhs), in 32 x 32 tiles
+__global__ void __launch_bounds__(256) mrhs_scatter_kernel(const double *__restrict__ Yh, const int *__restrict__ perm,
+                                                           int n, int nrhs, int R, double scale,
+                                                           double *__restrict__ y) {
+  __shared__ double t[32][33];
+  const int i0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int q = ty; q < 32; q += 8) {
+    const int i = i0 + q, r = r0 + tx;
+    t[q][tx] = i < n ? Yh[(int64_t)i * R + r] : 0.0;
+  }
+  __syncthreads();
+  for (int q = ty; q < 32; q += 8) {
+    const int i = i0 + tx, r = r0 + q;
+    if (i < n && r < nrhs) y[(int64_t)r * n + perm[i]] = scale * t[tx][q];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Direct kernels (src/kernel_calculation/directModeIII.f:7-74, directModeII.f90:7-107)
 // ---------------------------------------------------------------------------
 // Dense direct kernels (src/kernel_calculation/directModeIII.f:7-74,
@@ -1649,6 +1863,28 @@ void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, 
                      double scale, double *yT, double *yN, hipStream_t s) {
   hipLaunchKernelGGL(finalize_kernel, dim3(nblk((int64_t)n * nops, 256)), dim3(256), 0, s, fin_first, fin_idx, perm,
                      ypart, n, nops, scale, yT, yN);
+}
+
+void launch_mrhs(const MrhsLayout &d, const double *X, int nrhs, int R, double *W, double *zpart, double *ypart,
+                 double *Yh, double scale, double *yT, double *yN, hipStream_t s, hipEvent_t *ev) {
+  const int n = d.n, groups = R / 64;
+  hipLaunchKernelGGL(mrhs_gather_kernel, dim3(nblk(n, 32), R / 32), dim3(256), 0, s, X, d.perm, n, nrhs, R, W);
+  if (d.nvt > 0)
+    hipLaunchKernelGGL(mrhs_vtx_kernel, dim3(nblk((int64_t)d.nvt * groups, 4)), dim3(256), 0, s, d.vt, d.nvt,
+                       d.vdata, W, zpart, R, groups);
+  if (d.nzr > 0)
+    hipLaunchKernelGGL(mrhs_zreduce_kernel, dim3(nblk((int64_t)d.nzr * 32 * R, 256)), dim3(256), 0, s, d.zr, d.nzr,
+                       zpart, W, R);
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if (d.npt > 0)
+    hipLaunchKernelGGL(mrhs_panel_kernel, dim3(d.npt * groups), dim3(256), 0, s, d.pt, d.pdata, d.widx, W, ypart, R,
+                       groups);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL(mrhs_finalize_kernel, dim3(nblk((int64_t)n * d.nops * R, 256)), dim3(256), 0, s, d.fin_first,
+                     d.fin_idx, ypart, n, d.nops, R, Yh);
+  for (int o = 0; o < d.nops; o++)
+    hipLaunchKernelGGL(mrhs_scatter_kernel, dim3(nblk(n, 32), R / 32), dim3(256), 0, s, Yh + (int64_t)o * n * R,
+                       d.perm, n, nrhs, R, scale, o == 0 ? yT : yN);
 }
 
 int direct_chunks(int n) { return (n + kDirectCols) / kDirectCols; }

@@ -99,6 +99,28 @@ class DeviceSimulation(Simulation):
         _abi.check(self._fns["moment"](self._h, _abi.dptr(m), _abi.dptr(s)), "dfn_moment")
         return m, s
 
+    def kernel_multi(self, X):
+        """Stress rates of the rows of X (nrhs x N slip-rate vectors) with the FP64
+        H-matrix in one multi-RHS application (dfn_kernel_multi): (sT, sN) of the
+        same shape (sN None in Mode III)."""
+        X = np.ascontiguousarray(X, np.float64)
+        if X.ndim != 2 or X.shape[1] != self.N:
+            raise ValueError(f"kernel_multi: X must be (nrhs, {self.N})")
+        sT = np.zeros_like(X)
+        sN = np.zeros_like(X) if self.mode == _abi.MODE_II else None
+        _abi.check(self._fns["kernel_multi"](self._h, X.shape[0], _abi.dptr(X), _abi.dptr(sT),
+                                             _abi.dptr(sN) if sN is not None else None), "dfn_kernel_multi")
+        return sT, sN
+
+    def bench_kernel_multi(self, X, reps=5):
+        """dfn_bench_kernel_multi on the rows of X."""
+        X = np.ascontiguousarray(X, np.float64)
+        out = np.zeros(4)
+        _abi.check(self._fns["bench_kernel_multi"](self._h, X.shape[0], _abi.dptr(X), int(reps), _abi.dptr(out)),
+                   "dfn_bench_kernel_multi")
+        return dict(ms_apply=float(out[0]), ms_panel=float(out[1]), panel_flops=float(out[2]),
+                    alg_bytes=float(out[3]))
+
     def bench_kernel(self, reps=20):
         ms = np.zeros(4)
         _abi.check(self._fns["bench_kernel"](self._h, int(reps), _abi.dptr(ms)), "dfn_bench_kernel")

@@ -145,6 +145,16 @@ int dfn_get_stats(dfn_handle h, dfn_run_stats *stats);
  * stressN may be NULL in Mode III (the reference leaves it untouched there). */
 int dfn_kernel(dfn_handle h, const double *slip, double *stressT, double *stressN);
 
+/* dfn_kernel over nrhs slip-rate vectors at once (H-matrix handles, FP64 operators
+ * whatever dfn_set_precision chose): slip holds nrhs vectors of N doubles one after
+ * another, stressT (and stressN in Mode II) likewise.  The reference applies its
+ * H-matrix to one vector per call (Hgemv_right, src/hmatrix/calculation_hmat.f90:
+ * 122-159, ZGEMV per block); here both phases are dense products on the matrix
+ * cores over tiles of 16 vectors, in batches of up to 1024 vectors.  Each output
+ * agrees with dfn_kernel's for the same vector to rounding (the sums run in
+ * another order).  A dense direct handle gives DFN_ERR_STATE. */
+int dfn_kernel_multi(dfn_handle h, int32_t nrhs, const double *slip, double *stressT, double *stressN);
+
 /* dfn_kernel without any allocation, for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops); no counterpart in
  * the reference.  Dense direct kernel handles only (DFN_KERNEL_DIRECT; an H-matrix
@@ -282,6 +292,13 @@ int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms);
  * 0 stops; dfn_kernel_timing_read gives the mean duration in ms and the number of
  * launches timed since the last start. */
 int dfn_kernel_timing(dfn_handle h, int32_t on);
+/* Times dfn_kernel_multi's device work on nrhs (<= one batch) host vectors already
+ * copied to the device: out[0] = ms per whole multi-RHS application, out[1] = ms
+ * per phase-2 (panel product) launch, out[2] = its flops (2 x matrix entries x
+ * vectors rounded up to 64), out[3] = algorithmic HBM bytes of one application
+ * (every stored entry once, the vectors read and written once).  No reference
+ * counterpart. */
+int dfn_bench_kernel_multi(dfn_handle h, int32_t nrhs, const double *slip, int32_t reps, double *out);
 int dfn_kernel_timing_read(dfn_handle h, double *ms_mean, int64_t *count);
 /* Measurement helper: a plain device-to-device copy kernel over `bytes` bytes,
  * run `reps` times on a private stream; *gbps = 2*bytes / mean time (read + write).
