@@ -167,12 +167,15 @@ def mixed_precision(sim, p, args, kb64):
     multi_wall = time.perf_counter() - w0
     multi = multi_rhs_bench(sim, X, kb64)
     multi["wall_s_with_host_copies"] = multi_wall
-    for r in range(args.mixed_vectors):
-        V = X[r]
-        a = tuple(y[r] for y in Y64 if y is not None)
-        for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
-            sim.set_precision(prec)
-            b = sim.kernel(V)
+    log(f"[bench] FP64 multi-RHS over {multi['vectors']} vectors: {multi['ms']:.1f} ms, phase 2 at "
+        f"{multi['panel_TFLOP_s']:.1f} TFLOP/s")
+    # one precision switch each (a switch rebuilds the reduced-precision layout)
+    for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
+        sim.set_precision(prec)
+        log(f"[bench] stress-rate error over {args.mixed_vectors} vectors, precision {prec}")
+        for r in range(args.mixed_vectors):
+            a = tuple(y[r] for y in Y64 if y is not None)
+            b = sim.kernel(X[r])
             for y64, y in zip(a, b):
                 errs[prec][0].append(np.linalg.norm(y - y64) / np.linalg.norm(y64))
                 big = np.abs(y64) > 1e-3 * np.abs(y64).max()
@@ -219,7 +222,7 @@ def multi_rhs_bench(sim, X, kb64):
                 one_vector_ms=kb64["ms_kernel"], speedup_vs_one_vector_calls=kb64["ms_kernel"] * n / b["ms_apply"],
                 panel_ms=b["ms_panel"], panel_TFLOP_s=tflops, panel_frac_fp64_peak=tflops / FP64_PEAK_TFLOPS,
                 alg_GB_s=b["alg_bytes"] / (b["ms_apply"] * 1e-3) / 1e9,
-                note="phase-2 flops = 2 x stored entries x vectors rounded up to 64; peak = FP64 matrix peak "
+                note="phase-2 flops = 2 x panel entries (dense and U) x vectors rounded up to 64; peak = FP64 matrix peak "
                      "(equal to the FP64 vector peak on MI355X)")
 
 

@@ -49,15 +49,17 @@ def build(verbose=True):
 
 
 def build_fortran():
-    """The ISO_C_BINDING layer (fortran/dfn_hip.f90) and the Fortran host driver
-    that calls the library through it (fortran/drive_solver.f90)."""
+    """The ISO_C_BINDING layer (fortran/dfn_hip.f90) and the Fortran host drivers
+    that call the library through it (fortran/drive_solver.f90: one problem;
+    fortran/drive_ensemble.f90: an ensemble with the proposal network)."""
     bld = os.path.join(FORT, "build")
     os.makedirs(bld, exist_ok=True)
     mod_o = os.path.join(bld, "dfn_hip.o")
     subprocess.run([FLANG, "-O2", "-J", bld, "-c", os.path.join(FORT, "dfn_hip.f90"), "-o", mod_o], check=True)
-    subprocess.run([FLANG, "-O2", "-J", bld, "-I", bld, os.path.join(FORT, "drive_solver.f90"), mod_o,
-                    "-L" + CSRC, "-ldfn", "-Wl,-rpath,$ORIGIN/../../csrc", "-o", os.path.join(bld, "drive_solver")],
-                   check=True)
+    for prog in ("drive_solver", "drive_ensemble"):
+        subprocess.run([FLANG, "-O2", "-J", bld, "-I", bld, os.path.join(FORT, prog + ".f90"), mod_o,
+                        "-L" + CSRC, "-ldfn", "-Wl,-rpath,$ORIGIN/../../csrc", "-o", os.path.join(bld, prog)],
+                       check=True)
 
 
 if __name__ == "__main__":

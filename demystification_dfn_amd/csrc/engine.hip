@@ -257,6 +257,7 @@ struct Engine {
   int relayout() {
     ops64.reset();
     ops32.reset();
+    mr_R = 0;
     const int nops = hp.mode == DFN_MODE_II ? 2 : 1;
     const HostOperator *list[2] = {&hop[0], &hop[1]};
     for (int o = 0; o < nops; o++)
@@ -1248,7 +1249,7 @@ API int dfn_bench_kernel_multi(dfn_handle h, int32_t nrhs, const double *slip, i
   const int R = (nrhs + 63) / 64 * 64;
   out[0] = (double)t / reps;
   out[1] = (double)tsum / reps;
-  out[2] = 2.0 * (double)d.S_total * R;                           // phase-2 flops of one launch
+  out[2] = 2.0 * (double)d.S_panel * R;                           // phase-2 flops of one launch
   out[3] = 8.0 * (double)d.S_total + 16.0 * (double)N * nrhs * d.nops;  // matrix once + vectors in/out
   return DFN_OK;
   DFN_API_END

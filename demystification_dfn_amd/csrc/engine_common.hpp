@@ -88,6 +88,7 @@ struct DevOps {
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
   int64_t S_total = 0;   // matrix entries streamed per application
+  int64_t S_panel = 0;   // of which in the phase-2 panels (dense and U entries)
   // launch shapes picked by autotune (none changes a sum): phase-1 tasks per
   // workgroup, phase-1 task order (0 as built, 1 grouped by rank, 2 largest tasks
   // first), panel rows per wave step, panel task order (0 as built, 1 largest
@@ -295,7 +296,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   std::vector<VTask> vt;
   std::vector<ZRed> zr;
   int zp = 0;
-  int64_t S_total = 0;
+  int64_t S_total = 0, S_panel = 0;
   // V of each phase-1 task (a chunk of at most kVChunkRows rows of one block) is
   // stored strip by strip: a strip is the 64 x kVec rows one wave instruction covers,
   // and holds the strip's rows of column 0, then of column 1, ... (the last strip's
@@ -346,6 +347,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
       if (!single) zr.push_back(r);
     }
     S_total += (int64_t)H.l_v.size() + (int64_t)H.l_u.size() + (int64_t)H.d_data.size();
+    S_panel += (int64_t)H.l_u.size() + (int64_t)H.d_data.size();
   }
   std::unique_ptr<T[]> vdata(new T[std::max<int64_t>(vsize, 1)]);
   parallel_for((int64_t)vfill.size(), [&](int64_t q) {
@@ -527,6 +529,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   d.nzr = (int)zr.size();
   d.npt = (int)pt.size();
   d.S_total = S_total;
+  d.S_panel = S_panel;
   // bytes: factors + x read + y written, per operator (x shared in Mode II)
   const double dense_extra = dense64 ? (8.0 - (double)sizeof(T)) * (double)S_dense : 0.0;
   d.alg_bytes = (double)sizeof(T) * (double)S_total + dense_extra + 8.0 * n + 8.0 * n * nops;

@@ -1,8 +1,11 @@
 ! dfn_hip -- ISO_C_BINDING layer between a Fortran host (the DemystiFicatioN
 ! solver) and the gfx950 library libdfn.so (include/dfn.h).
 !
-! 1. bind(C) mirrors of dfn_problem / dfn_run_stats and interfaces to every
-!    extern "C" entry point.
+! 1. bind(C) mirrors of dfn_problem / dfn_run_stats and interfaces to the
+!    extern "C" entry points a Fortran host calls: one problem (dfn_*), the
+!    multi-RHS stress evaluation, ensembles of networks (dfn_ensemble_*) and the
+!    step-size proposal network (dfn_mlp_*); dfn_hip_problem fills a dfn_problem
+!    from the reference's own arrays.
 ! 2. Procedures with the reference's own interfaces, so that the reference's
 !    pointers and calls can be bound to the device without changing their callers:
 !      hip_kernel             -> kernel          (src/variables.f90:17-26)
@@ -19,6 +22,9 @@ private
 integer(c_int), parameter, public :: DFN_OK = 0, DFN_MODE_II = 2, DFN_MODE_III = 3
 integer(c_int), parameter, public :: DFN_KERNEL_DIRECT = 0, DFN_KERNEL_HMATRIX = 1
 integer(c_int), parameter, public :: DFN_ERR_CG_TIMEOUT = 9
+integer(c_int), parameter, public :: DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1, DFN_PREC_MIXED = 2
+integer(c_int), parameter, public :: DFN_STEP_RKQS = 0, DFN_STEP_PI = 1, DFN_STEP_LEARNED = 2
+integer(c_int), parameter, public :: DFN_MLP_INPUTS = 8, DFN_MLP_PARAMS = 4801
 
 type, bind(C), public :: dfn_problem
   integer(c_int32_t) :: fracture_mode, static_kernel, nb_fault, nb_element
@@ -43,7 +49,15 @@ end type
 
 public :: dfn_create, dfn_destroy, dfn_build_hmat, dfn_set_state, dfn_get_state, dfn_kernel, dfn_friction_law
 public :: dfn_compute_P, dfn_rkqs, dfn_solve, dfn_get_stats, dfn_set_output, dfn_flush_output
-public :: dfn_hip_setup, dfn_hip_close, hip_kernel, ode_RateStateAgingHIP, compute_P_hip, rkqs_hip, dfn_hip_handle
+public :: dfn_set_precision, dfn_kernel_multi
+public :: dfn_ensemble_create, dfn_ensemble_destroy, dfn_ensemble_build_hmat, dfn_ensemble_set_precision
+public :: dfn_ensemble_set_graph, dfn_ensemble_solve, dfn_ensemble_solve_until, dfn_ensemble_get_state
+public :: dfn_ensemble_get_stats, dfn_ensemble_set_controller, dfn_ensemble_set_recording
+public :: dfn_ensemble_take_records, dfn_ensemble_info, dfn_ensemble_kernel_timing, dfn_ensemble_kernel_timing_read
+public :: dfn_mlp_create, dfn_mlp_destroy, dfn_mlp_set_params, dfn_mlp_get_params, dfn_mlp_forward
+public :: dfn_mlp_loss_grad, dfn_mlp_train
+public :: dfn_hip_problem, dfn_hip_setup, dfn_hip_close, hip_kernel, ode_RateStateAgingHIP, compute_P_hip, rkqs_hip
+public :: dfn_hip_handle
 
 interface
   integer(c_int) function dfn_create(p, h) bind(C, name='dfn_create')
@@ -122,6 +136,151 @@ interface
     import :: c_ptr, c_int
     type(c_ptr), value :: h
   end function
+  integer(c_int) function dfn_set_precision(h, precision) bind(C, name='dfn_set_precision')
+    import :: c_ptr, c_int, c_int32_t
+    type(c_ptr), value :: h
+    integer(c_int32_t), value :: precision
+  end function
+  ! nrhs slip-rate vectors of N values one after another (slip(N, nrhs) in Fortran order)
+  integer(c_int) function dfn_kernel_multi(h, nrhs, slip, stressT, stressN) bind(C, name='dfn_kernel_multi')
+    import :: c_ptr, c_int, c_int32_t, c_double
+    type(c_ptr), value :: h
+    integer(c_int32_t), value :: nrhs
+    real(c_double), intent(in) :: slip(*)
+    real(c_double), intent(out) :: stressT(*), stressN(*)
+  end function
+
+  ! -- ensembles of independent networks (one shared device layout) ---------
+  integer(c_int) function dfn_ensemble_create(problems, nnet, e) bind(C, name='dfn_ensemble_create')
+    import :: dfn_problem, c_ptr, c_int, c_int32_t
+    type(dfn_problem), intent(in) :: problems(*)
+    integer(c_int32_t), value :: nnet
+    type(c_ptr), intent(out) :: e
+  end function
+  integer(c_int) function dfn_ensemble_destroy(e) bind(C, name='dfn_ensemble_destroy')
+    import :: c_ptr, c_int
+    type(c_ptr), value :: e
+  end function
+  integer(c_int) function dfn_ensemble_build_hmat(e) bind(C, name='dfn_ensemble_build_hmat')
+    import :: c_ptr, c_int
+    type(c_ptr), value :: e
+  end function
+  integer(c_int) function dfn_ensemble_set_precision(e, precision) bind(C, name='dfn_ensemble_set_precision')
+    import :: c_ptr, c_int, c_int32_t
+    type(c_ptr), value :: e
+    integer(c_int32_t), value :: precision
+  end function
+  integer(c_int) function dfn_ensemble_set_graph(e, use_graph) bind(C, name='dfn_ensemble_set_graph')
+    import :: c_ptr, c_int, c_int32_t
+    type(c_ptr), value :: e
+    integer(c_int32_t), value :: use_graph
+  end function
+  integer(c_int) function dfn_ensemble_solve(e, nsteps) bind(C, name='dfn_ensemble_solve')
+    import :: c_ptr, c_int, c_int64_t
+    type(c_ptr), value :: e
+    integer(c_int64_t), value :: nsteps
+  end function
+  integer(c_int) function dfn_ensemble_solve_until(e, t_end, max_steps) bind(C, name='dfn_ensemble_solve_until')
+    import :: c_ptr, c_int, c_double, c_int64_t
+    type(c_ptr), value :: e
+    real(c_double), value :: t_end
+    integer(c_int64_t), value :: max_steps
+  end function
+  ! net is 0-based; Y = [V | theta | sigmaN] (3N), P and P_dot (N)
+  integer(c_int) function dfn_ensemble_get_state(e, net, Y, P, P_dot, time) bind(C, name='dfn_ensemble_get_state')
+    import :: c_ptr, c_int, c_int32_t, c_double
+    type(c_ptr), value :: e
+    integer(c_int32_t), value :: net
+    real(c_double), intent(out) :: Y(*), P(*), P_dot(*), time
+  end function
+  integer(c_int) function dfn_ensemble_get_stats(e, net, s) bind(C, name='dfn_ensemble_get_stats')
+    import :: c_ptr, c_int, c_int32_t, dfn_run_stats
+    type(c_ptr), value :: e
+    integer(c_int32_t), value :: net
+    type(dfn_run_stats), intent(out) :: s
+  end function
+  ! mode DFN_STEP_*; m: a dfn_mlp handle for DFN_STEP_LEARNED (c_null_ptr otherwise)
+  integer(c_int) function dfn_ensemble_set_controller(e, mode, m) bind(C, name='dfn_ensemble_set_controller')
+    import :: c_ptr, c_int, c_int32_t
+    type(c_ptr), value :: e
+    integer(c_int32_t), value :: mode
+    type(c_ptr), value :: m
+  end function
+  integer(c_int) function dfn_ensemble_set_recording(e, on) bind(C, name='dfn_ensemble_set_recording')
+    import :: c_ptr, c_int, c_int32_t
+    type(c_ptr), value :: e
+    integer(c_int32_t), value :: on
+  end function
+  ! X(DFN_MLP_INPUTS, cap) and T(cap); c_null_ptr for X or T only counts the pairs
+  integer(c_int) function dfn_ensemble_take_records(e, X, T, cap, count) bind(C, name='dfn_ensemble_take_records')
+    import :: c_ptr, c_int, c_int64_t
+    type(c_ptr), value :: e, X, T
+    integer(c_int64_t), value :: cap
+    integer(c_int64_t), intent(out) :: count
+  end function
+  integer(c_int) function dfn_ensemble_info(e, info) bind(C, name='dfn_ensemble_info')
+    import :: c_ptr, c_int, c_int64_t
+    type(c_ptr), value :: e
+    integer(c_int64_t), intent(out) :: info(6)
+  end function
+  integer(c_int) function dfn_ensemble_kernel_timing(e, on) bind(C, name='dfn_ensemble_kernel_timing')
+    import :: c_ptr, c_int, c_int32_t
+    type(c_ptr), value :: e
+    integer(c_int32_t), value :: on
+  end function
+  integer(c_int) function dfn_ensemble_kernel_timing_read(e, ms_mean, count, bytes) &
+      bind(C, name='dfn_ensemble_kernel_timing_read')
+    import :: c_ptr, c_int, c_double, c_int64_t
+    type(c_ptr), value :: e
+    real(c_double), intent(out) :: ms_mean, bytes
+    integer(c_int64_t), intent(out) :: count
+  end function
+
+  ! -- step-size proposal network ------------------------------------------
+  integer(c_int) function dfn_mlp_create(m) bind(C, name='dfn_mlp_create')
+    import :: c_ptr, c_int
+    type(c_ptr), intent(out) :: m
+  end function
+  integer(c_int) function dfn_mlp_destroy(m) bind(C, name='dfn_mlp_destroy')
+    import :: c_ptr, c_int
+    type(c_ptr), value :: m
+  end function
+  ! params(DFN_MLP_PARAMS), norm(2 DFN_MLP_INPUTS + 2) (c_null_ptr keeps the normalisation)
+  integer(c_int) function dfn_mlp_set_params(m, params, norm) bind(C, name='dfn_mlp_set_params')
+    import :: c_ptr, c_int, c_float
+    type(c_ptr), value :: m
+    real(c_float), intent(in) :: params(*)
+    type(c_ptr), value :: norm
+  end function
+  integer(c_int) function dfn_mlp_get_params(m, params, norm) bind(C, name='dfn_mlp_get_params')
+    import :: c_ptr, c_int, c_float
+    type(c_ptr), value :: m
+    real(c_float), intent(out) :: params(*), norm(*)
+  end function
+  integer(c_int) function dfn_mlp_forward(m, X, n, Y) bind(C, name='dfn_mlp_forward')
+    import :: c_ptr, c_int, c_float, c_int64_t
+    type(c_ptr), value :: m
+    real(c_float), intent(in) :: X(*)
+    integer(c_int64_t), value :: n
+    real(c_float), intent(out) :: Y(*)
+  end function
+  integer(c_int) function dfn_mlp_loss_grad(m, X, T, n, loss, grad) bind(C, name='dfn_mlp_loss_grad')
+    import :: c_ptr, c_int, c_float, c_double, c_int64_t
+    type(c_ptr), value :: m
+    real(c_float), intent(in) :: X(*), T(*)
+    integer(c_int64_t), value :: n
+    real(c_double), intent(out) :: loss
+    real(c_float), intent(out) :: grad(*)
+  end function
+  integer(c_int) function dfn_mlp_train(m, X, T, n, epochs, batch, lr, seed, final_loss) bind(C, name='dfn_mlp_train')
+    import :: c_ptr, c_int, c_float, c_double, c_int64_t, c_int32_t
+    type(c_ptr), value :: m
+    real(c_float), intent(in) :: X(*), T(*)
+    integer(c_int64_t), value :: n, seed
+    integer(c_int32_t), value :: epochs, batch
+    real(c_float), value :: lr
+    real(c_double), intent(out) :: final_loss
+  end function
 end interface
 
 type(c_ptr), save :: dfn_hip_handle = c_null_ptr
@@ -129,14 +288,16 @@ integer, save :: n_el = 0
 
 contains
 
-! Creates the device problem from the reference's own arrays (the content of
-! src/variables.f90 after config()/ini_value()), and builds the H-matrix when
-! static_kernel is 'hmatrix'.  Every array argument stays owned by the caller.
-subroutine dfn_hip_setup(mode, kernel, nb_fault, nb_element, fnel, node, a, b, Dc, f0, V0, nld, sld, V, theta, &
-                         sigmaN, P, permeability, kmin, kmax, Snk, Lk, Tk, mu, cs, cp, sdot, porosity, fluid_comp, &
-                         rock_comp, dyn_viscosity, tol_solver, coupling, nb_source, src_fault, src_index, src_Q, &
-                         src_tbeg, src_tend, eta, eps_aca, nleaf, ier)
-  integer, intent(in) :: mode, kernel, nb_fault, nb_element, coupling, nb_source, nleaf
+! Fills a dfn_problem with the addresses of the reference's own arrays (the
+! content of src/variables.f90 after config()/ini_value()) and its scalars.  The
+! arrays must stay allocated until the problem has been handed to dfn_create or
+! dfn_ensemble_create.
+subroutine dfn_hip_problem(prob, mode, kernel, nb_fault, nb_element, fnel, node, a, b, Dc, f0, V0, nld, sld, V, &
+                           theta, sigmaN, P, permeability, kmin, kmax, Snk, Lk, Tk, mu, cs, cp, sdot, porosity, &
+                           fluid_comp, rock_comp, dyn_viscosity, tol_solver, coupling, nb_source, src_fault, &
+                           src_index, src_Q, src_tbeg, src_tend, eta, eps_aca, nleaf, max_rank)
+  type(dfn_problem), intent(out) :: prob
+  integer, intent(in) :: mode, kernel, nb_fault, nb_element, coupling, nb_source, nleaf, max_rank
   integer(c_int64_t), intent(in), target :: fnel(nb_fault), src_fault(*), src_index(*)
   real(c_double), intent(in), target :: node(2, nb_element+1)
   real(c_double), intent(in), target, dimension(nb_element) :: a, b, Dc, f0, V0, nld, sld, V, theta, sigmaN, P
@@ -144,8 +305,6 @@ subroutine dfn_hip_setup(mode, kernel, nb_fault, nb_element, fnel, node, a, b, D
   real(c_double), intent(in), target :: src_Q(*), src_tbeg(*), src_tend(*)
   real(c_double), intent(in) :: mu, cs, cp, sdot(5), porosity, fluid_comp, rock_comp, dyn_viscosity, tol_solver
   real(c_double), intent(in) :: eta, eps_aca
-  integer, intent(out) :: ier
-  type(dfn_problem) :: prob
   prob%fracture_mode = mode; prob%static_kernel = kernel; prob%nb_fault = nb_fault; prob%nb_element = nb_element
   prob%fault_nb_element = c_loc(fnel); prob%node = c_loc(node)
   prob%a = c_loc(a); prob%b = c_loc(b); prob%Dc = c_loc(Dc); prob%f0 = c_loc(f0); prob%V0 = c_loc(V0)
@@ -161,7 +320,30 @@ subroutine dfn_hip_setup(mode, kernel, nb_fault, nb_element, fnel, node, a, b, D
   prob%permeability_coupling = coupling; prob%nb_source = nb_source
   prob%source_fault = c_loc(src_fault); prob%source_index = c_loc(src_index)
   prob%source_Q = c_loc(src_Q); prob%source_t_beg = c_loc(src_tbeg); prob%source_t_end = c_loc(src_tend)
-  prob%eta = eta; prob%eps_aca = eps_aca; prob%nleaf = nleaf; prob%max_rank = 20
+  prob%eta = eta; prob%eps_aca = eps_aca; prob%nleaf = nleaf; prob%max_rank = max_rank
+end subroutine
+
+! Creates the device problem from the reference's own arrays, and builds the
+! H-matrix when static_kernel is 'hmatrix'.  Every array argument stays owned by
+! the caller.
+subroutine dfn_hip_setup(mode, kernel, nb_fault, nb_element, fnel, node, a, b, Dc, f0, V0, nld, sld, V, theta, &
+                         sigmaN, P, permeability, kmin, kmax, Snk, Lk, Tk, mu, cs, cp, sdot, porosity, fluid_comp, &
+                         rock_comp, dyn_viscosity, tol_solver, coupling, nb_source, src_fault, src_index, src_Q, &
+                         src_tbeg, src_tend, eta, eps_aca, nleaf, ier)
+  integer, intent(in) :: mode, kernel, nb_fault, nb_element, coupling, nb_source, nleaf
+  integer(c_int64_t), intent(in), target :: fnel(nb_fault), src_fault(*), src_index(*)
+  real(c_double), intent(in), target :: node(2, nb_element+1)
+  real(c_double), intent(in), target, dimension(nb_element) :: a, b, Dc, f0, V0, nld, sld, V, theta, sigmaN, P
+  real(c_double), intent(in), target, dimension(nb_element) :: permeability, kmin, kmax, Snk, Lk, Tk
+  real(c_double), intent(in), target :: src_Q(*), src_tbeg(*), src_tend(*)
+  real(c_double), intent(in) :: mu, cs, cp, sdot(5), porosity, fluid_comp, rock_comp, dyn_viscosity, tol_solver
+  real(c_double), intent(in) :: eta, eps_aca
+  integer, intent(out) :: ier
+  type(dfn_problem) :: prob
+  call dfn_hip_problem(prob, mode, kernel, nb_fault, nb_element, fnel, node, a, b, Dc, f0, V0, nld, sld, V, theta, &
+                       sigmaN, P, permeability, kmin, kmax, Snk, Lk, Tk, mu, cs, cp, sdot, porosity, fluid_comp, &
+                       rock_comp, dyn_viscosity, tol_solver, coupling, nb_source, src_fault, src_index, src_Q, &
+                       src_tbeg, src_tend, eta, eps_aca, nleaf, 20)
   call dfn_hip_close()
   ier = dfn_create(prob, dfn_hip_handle)
   if (ier /= DFN_OK) return

@@ -294,8 +294,8 @@ int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms);
 int dfn_kernel_timing(dfn_handle h, int32_t on);
 /* Times dfn_kernel_multi's device work on nrhs (<= one batch) host vectors already
  * copied to the device: out[0] = ms per whole multi-RHS application, out[1] = ms
- * per phase-2 (panel product) launch, out[2] = its flops (2 x matrix entries x
- * vectors rounded up to 64), out[3] = algorithmic HBM bytes of one application
+ * per phase-2 (panel product) launch, out[2] = its flops (2 x panel entries, the
+ * dense and U ones, x vectors rounded up to 64), out[3] = algorithmic HBM bytes of one application
  * (every stored entry once, the vectors read and written once).  No reference
  * counterpart. */
 int dfn_bench_kernel_multi(dfn_handle h, int32_t nrhs, const double *slip, int32_t reps, double *out);

@@ -64,3 +64,72 @@ def test_fortran_host_output_files(tmp_path):
         a, b = read_output(str(fdir / nm)), read_output(str(pdir / nm))
         for k in OUTPUT_FIELDS:
             np.testing.assert_array_equal(a[k], b[k], err_msg=f"{nm}/{k}")
+
+
+ENS_DRIVER = os.path.join(ROOT, "demystification_dfn_amd", "fortran", "build", "drive_ensemble")
+
+
+def _ens_nets():
+    from demystification_dfn_amd import _abi, problems
+    return [problems.network_small(n_faults=nf, seed=100 + i, mode=2, eta=2.0, eps_aca=1e-6, nleaf=16,
+                                   static_kernel=_abi.KERNEL_HMATRIX, tol=tol)
+            for i, (nf, tol) in enumerate(((9, 1e-6), (14, 1e-9), (11, 1e-7)))]
+
+
+@pytest.mark.parametrize("learned", [False, True])
+def test_fortran_host_ensemble(tmp_path, learned):
+    """An ensemble driven from Fortran through the dfn_ensemble_* / dfn_mlp_*
+    interfaces of dfn_hip (fortran/drive_ensemble.f90) equals the same calls made
+    from Python, bit for bit: every network's state and step counts, the recorded
+    (features, target) pairs and, with the learned controller (FP32 operators and
+    the trained proposal network), the network's outputs on those pairs and one
+    training epoch on them."""
+    from demystification_dfn_amd import _abi
+    from demystification_dfn_amd.device import DeviceEnsemble, DeviceMlp
+    from demystification_dfn_amd.train_step_network import load
+    assert os.path.exists(ENS_DRIVER), "Fortran ensemble driver not built (__graft_entry__.build())"
+    probs = _ens_nets()
+    files = []
+    for i, p in enumerate(probs):
+        files.append(str(tmp_path / f"net{i}.bin"))
+        fortran_io.write_problem(p, files[-1], 0)
+    prec = _abi.PREC_FP32 if learned else _abi.PREC_FP64
+    ctrl = _abi.STEP_LEARNED if learned else _abi.STEP_RKQS
+    wfile = "-"
+    if learned:
+        params, norm, _ = load()
+        wfile = str(tmp_path / "weights.bin")
+        fortran_io.write_weights(wfile, params, norm)
+    nsteps = 40
+    fout = str(tmp_path / "ens.bin")
+    print(f"[gpu] fortran ensemble N={[p.N for p in probs]} learned={learned}", flush=True)
+    subprocess.run([ENS_DRIVER, fout, str(nsteps), str(prec), str(ctrl), wfile] + files, check=True, timeout=300)
+    got = fortran_io.read_ensemble_result(fout, [p.N for p in probs], _abi.MLP_PARAMS)
+
+    ens = DeviceEnsemble(probs)
+    ens.initiate_hmat()
+    ens.set_precision(prec)
+    mlp = None
+    if learned:
+        mlp = DeviceMlp()
+        mlp.set_params(params, norm)
+    ens.set_controller(ctrl, mlp)
+    ens.set_recording(True)
+    ens.solve(nsteps)
+    for i, g in enumerate(got["nets"]):
+        Y, P, _, t = ens.get_state(i)
+        s = ens.stats(i)
+        assert g["time"] == t and g["accepted"] == s["accepted_steps"] == nsteps
+        assert g["rejected"] == s["rejected_attempts"]
+        np.testing.assert_array_equal(g["Y"], Y)
+        np.testing.assert_array_equal(g["P"], P)
+    X, T = ens.take_records()
+    assert len(T) > 0
+    np.testing.assert_array_equal(got["X"], X)
+    np.testing.assert_array_equal(got["T"], T)
+    if learned:
+        np.testing.assert_array_equal(got["Y"], mlp.forward(X))
+        loss = mlp.train(X, T, epochs=1, batch=64, lr=1e-3, seed=5)
+        assert got["loss"] == loss
+        np.testing.assert_array_equal(got["params"], mlp.get_params()[0])
+    ens.close()
