@@ -1,7 +1,8 @@
-"""Device kernels of the step-size proposal network against the FP64 numpy
-restatement (oracle/mlp_oracle.py, itself pinned by finite differences)
-evaluated at the device's BF16-rounded weights.  Tolerances cover FP32
-accumulation: 1e-5 relative on outputs and the loss, 1e-4 on the gradient."""
+"""Device kernels of the step-size proposal network (every product on the FP32
+matrix cores, the gradient summed over workgroups in a fixed order) against the
+FP64 numpy restatement (oracle/mlp_oracle.py, itself pinned by finite
+differences) evaluated at the device's BF16-rounded weights.  Tolerances cover
+FP32 accumulation: 1e-5 relative on outputs and the loss, 1e-4 on the gradient."""
 import numpy as np
 import pytest
 
