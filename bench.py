@@ -7,16 +7,18 @@ on the device.
 
 The default workload is the headline one: an ensemble of 64 seeded fault
 networks of 50-400 faults each (~1.37M elements, H-matrix eta 2 / ACA 1e-6 /
-leaf 64) advanced together, with FP32 operator storage (FP64 state, RHS, CG and
-error norm) and the learned step-size proposals (data/step_network.npz), against
+leaf 64) advanced together, with mixed-precision operators (FP64 near-field
+blocks, FP32 low-rank factors read into FP64 products and sums; FP64 state, RHS,
+CG and error norm) and the learned step-size proposals (data/step_network.npz), against
 the plain FP64 path (one dfn_handle per network, FP64 operators, rkqs) over the
 same simulated interval of every network.
 
 Prints ONE JSON line: simulated network-years per wall-clock hour of the whole
 ensemble, the ratio to the plain FP64 path, the roofline of the dominant kernel
-(the FP32 phase-2 panel kernel, HIP events on the ensemble's stream), a CPU
+(the phase-2 panel kernels, HIP events on the ensemble's stream), a CPU
 baseline (the oracle on this host's cores, from the state at the start of the
-timed window), and the 80k-element single network (FP64 and FP32 operators) and
+timed window), and the 80k-element single network (FP64 operators; FP32 and
+mixed operators with their stress-rate errors over 1000 vectors) and
 its hydraulically connected variant as sub-objects.
 """
 import argparse
@@ -34,8 +36,8 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 # HBM bytes per launch of the dominant kernel, from two rocprofv3 PMC runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a 1 GiB copy kernel
 # (FETCH x 2.000, WRITE x 1.000); summaries and commands in
-# profiles/pmc_traffic_network80k_fp64.md and profiles/pmc_traffic_ensemble64_fp32.md.
-MEASURED_TRAFFIC = {("network800_80k", 2024): 2.4947e9, ("ensemble64_1374k_fp32_learned", 2024): 1.5468e10}
+# profiles/pmc_traffic_network80k_fp64.md and profiles/pmc_traffic_ensemble64_mixed.md.
+MEASURED_TRAFFIC = {("network800_80k", 2024): 2.4947e9}
 YEAR = 365.25 * 86400.0
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 peak (vector and matrix cores alike)
@@ -152,14 +154,15 @@ def device_state(sim):
 
 def mixed_precision(sim, p, args, kb64):
     """Config 3 on a single network: FP32 operator storage (per-block power-of-two
-    scales, FP32 accumulation, compensated near-field sums) and FP32 low-rank
-    factors beside FP64 near-field blocks, each against the FP64 operator over
+    scales, FP32 accumulation, compensated near-field sums), the same storage read
+    into FP64 products and sums, and FP64 near-field blocks beside FP32 low-rank
+    factors read into FP64 sums (the mixed mode), each against the FP64 operator over
     random log-uniform slip rates in [1e-12, 1] m/s (norm-wise and on the rows above
     1e-3 of the largest stress rate); then FP32 solver steps continued from the
     FP64 run's state.  State, RHS, CG and error norm stay FP64 throughout."""
     from demystification_dfn_amd import _abi
     rng = np.random.default_rng(7)
-    errs = {_abi.PREC_FP32: ([], []), _abi.PREC_MIXED: ([], [])}
+    errs = {_abi.PREC_FP32: ([], []), _abi.PREC_MIXED: ([], []), _abi.PREC_FP32_STORE: ([], [])}
     # the FP64 stress rates of every vector in one multi-RHS application
     X = 10.0 ** rng.uniform(-12.0, 0.0, (args.mixed_vectors, p.N))
     w0 = time.perf_counter()
@@ -170,7 +173,7 @@ def mixed_precision(sim, p, args, kb64):
     log(f"[bench] FP64 multi-RHS over {multi['vectors']} vectors: {multi['ms']:.1f} ms, phase 2 at "
         f"{multi['panel_TFLOP_s']:.1f} TFLOP/s")
     # one precision switch each (a switch rebuilds the reduced-precision layout)
-    for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
+    for prec in (_abi.PREC_FP32, _abi.PREC_FP32_STORE, _abi.PREC_MIXED):
         sim.set_precision(prec)
         log(f"[bench] stress-rate error over {args.mixed_vectors} vectors, precision {prec}")
         for r in range(args.mixed_vectors):
@@ -206,7 +209,11 @@ def mixed_precision(sim, p, args, kb64):
                 fp64_near_field=dict(stress_rate_rel_err=stats(_abi.PREC_MIXED), hmatvec_ms=kbm["ms_kernel"],
                                      alg_bytes=kbm["alg_bytes"],
                                      hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kbm["ms_kernel"],
-                                     note="FP32 low-rank factors beside FP64 near-field blocks (DFN_PREC_MIXED)"),
+                                     note="FP64 near-field blocks beside FP32 low-rank factors read into FP64 "
+                                          "products and sums (DFN_PREC_MIXED)"),
+                fp32_storage_fp64_sums=dict(stress_rate_rel_err=stats(_abi.PREC_FP32_STORE),
+                                            note="the FP32 storage read into FP64 products and sums "
+                                                 "(DFN_PREC_FP32_STORE): the error of the storage alone"),
                 autotune=tune32, fp64_multi_rhs=multi,
                 note="continues from the FP64 run's final state; operators FP32, state FP64")
 
@@ -448,10 +455,12 @@ def run_network(workload, args, mixed=True, cpu_seconds=None):
 
 
 def run_headline(args):
-    """Configs 4 and 5: the 64-network ensemble with FP32 operators and the
+    """Configs 4 and 5: the 64-network ensemble with mixed-precision operators
+    (FP64 near-field blocks, FP32 low-rank factors read into FP64 sums) and the
     learned step-size proposals, against the plain FP64 path (one dfn_handle per
     network, FP64 operators, rkqs) over the same simulated interval of every
-    network; then, continuing, the ensemble with rkqs in FP32 and in FP64."""
+    network; then, continuing, the ensemble with rkqs in mixed precision and in
+    FP64."""
     from demystification_dfn_amd import _abi, problems
     from demystification_dfn_amd import train_step_network as tsn
     from demystification_dfn_amd.device import DeviceEnsemble, DeviceMlp, DeviceSimulation
@@ -459,15 +468,15 @@ def run_headline(args):
     probs = problems.network_ensemble(args.networks, faults=(lo, hi), box=args.ensemble_box, seed=args.seed)
     B = len(probs)
     Ns = np.array([p.N for p in probs])
-    desc = dict(workload=f"ensemble{B}_{Ns.sum() // 1000}k_fp32_learned", networks=B,
+    desc = dict(workload=f"ensemble{B}_{Ns.sum() // 1000}k_mixed_learned", networks=B,
                 faults_per_network=f"{lo}-{hi}", box_m=args.ensemble_box,
                 length_law="p(L)~L^-0.1 on [50 m, 2 km]", element_m=10, fracture_mode="II",
                 kernel="hmatrix", eta=2.0, eps_aca=1e-6, leaf=64, elements=int(Ns.sum()), min_elements=int(Ns.min()),
                 max_elements=int(Ns.max()), seed=args.seed,
                 friction="per network: a~U[0.008,0.012] per fault, b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa",
                 pulse="10 MPa on one seeded element per network (fluid diffusion on)",
-                precision="FP32 operator blocks (power-of-two block scales, FP32 accumulation, compensated "
-                          "near-field sums); FP64 state, RHS, CG and error norm",
+                precision="DFN_PREC_MIXED: FP64 near-field blocks; FP32 low-rank factors (power-of-two block "
+                          "scales) read into FP64 products and sums; FP64 state, RHS, CG and error norm",
                 controller="learned proposals (data/step_network.npz) with the PI controller as fallback; the "
                            "Cash-Karp error test accepts or rejects every step")
     log(f"[bench] headline: ensemble of {B} networks, {Ns.sum()} elements (max {Ns.max()})")
@@ -479,9 +488,9 @@ def run_headline(args):
     ens.initiate_hmat()
     setup_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ens.set_precision(_abi.PREC_FP32)
+    ens.set_precision(_abi.PREC_MIXED)
     layout32_s = time.perf_counter() - t0
-    log(f"[bench] ensemble setup {setup_s:.1f} s, FP32 layout {layout32_s:.1f} s {ens.info()}")
+    log(f"[bench] ensemble setup {setup_s:.1f} s, mixed layout {layout32_s:.1f} s {ens.info()}")
     ens.set_controller(_abi.STEP_LEARNED, mlp)
 
     def window(steps):
@@ -504,7 +513,7 @@ def run_headline(args):
     tA, tB, wall, head = window(args.steps)
     value = head["value"]
     log(f"[bench] mixed+learned: {args.steps} steps in {wall:.2f} s ({head['ms_per_step']:.2f} ms/step)")
-    # the dominant kernel (FP32 phase-2 panels), HIP events on the ensemble's stream
+    # the dominant kernel (the phase-2 panels), HIP events on the ensemble's stream
     # over further steps in which every network takes part
     n_extra = min(args.steps, 10)
     ens.kernel_timing(True)
@@ -512,23 +521,24 @@ def run_headline(args):
     ms_dom, n_dom, bytes_dom = ens.kernel_timing_read()
     ens.kernel_timing(False)
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9 if n_dom else None
-    log(f"[bench] panel_kernel_f32: {ms_dom:.3f} ms mean over {n_dom} launches")
+    log(f"[bench] phase-2 panels: {ms_dom:.3f} ms mean over {n_dom} launches")
     roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=achieved / HBM_PEAK_GBS if achieved else None,
                 traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed)),
-                kernel="panel_kernel_f32 (H-matvec phase 2, FP32 blocks, the whole ensemble's panels)",
+                kernel="phase-2 panels (panel_kernel_f64 on the FP64 near-field set, then panel_kernel_f32w on "
+                       "the FP32 low-rank set; the whole ensemble's panels)",
                 bytes_per_launch=bytes_dom, ms_per_launch=ms_dom, launches_timed=n_dom,
                 timing=f"mean over the panel launches of {n_extra} ensemble steps after the timed ones in which every "
                        f"network took part, HIP events on the ensemble's stream")
     info32 = ens.info()
-    # the same ensemble, continuing: rkqs with FP32 operators, then with FP64 operators
+    # the same ensemble, continuing: rkqs with mixed-precision operators, then with FP64 operators
     ens.set_controller(_abi.STEP_RKQS)
     _, _, _, rkqs32 = window(args.steps)
     ens.set_precision(_abi.PREC_FP64)
     info64 = ens.info()
     _, _, _, rkqs64 = window(args.steps)
     ens.close()
-    log(f"[bench] continuing: FP32 rkqs {rkqs32['ms_per_step']:.2f} ms/step, FP64 rkqs {rkqs64['ms_per_step']:.2f}")
+    log(f"[bench] continuing: mixed rkqs {rkqs32['ms_per_step']:.2f} ms/step, FP64 rkqs {rkqs64['ms_per_step']:.2f}")
     # the plain FP64 path: every network on its own dfn_handle, from t = 0 to the start
     # of the ensemble's timed window (untimed), then timed over the same interval
     mlp.close()
@@ -537,7 +547,8 @@ def run_headline(args):
     if args.no_plain:  # profiling runs: the ensemble's kernels only
         return dict(metric=METRIC, value=value, unit="simulated network-yr / wall hr", n_gpus=1, steps=args.steps,
                     warmup=args.warmup, ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak",
-                    vs_baseline=None, dtype="f32 operators / f64 state", data="synthetic (seeded generator)",
+                    vs_baseline=None, dtype="f64 near-field / f32 low-rank storage, f64 arithmetic and state",
+                    data="synthetic (seeded generator)",
                     config=desc, roofline=roof, window=head, cpu_baseline=None,
                     note="--no-plain: the plain FP64 path and the CPU baseline were not run")
     plain_wall, plain_steps, plain_rej, states, per_net = 0.0, 0, 0, {}, {}
@@ -569,7 +580,8 @@ def run_headline(args):
     out = dict(
         metric=METRIC, value=value, unit="simulated network-yr / wall hr", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f32 operators / f64 state", data="synthetic (seeded generator)", config=desc,
+        dtype="f64 near-field / f32 low-rank storage, f64 arithmetic and state", data="synthetic (seeded generator)",
+        config=desc,
         roofline=roof,
         vs_plain_fp64=dict(plain_value=plain_value, unit="simulated network-yr / wall hr", ratio=ratio,
                            target_ratio=20.0, meets_target=bool(ratio >= 20.0), plain_wall_s=plain_wall,
@@ -578,10 +590,10 @@ def run_headline(args):
                                 "from t=0 to the start of the ensemble's timed window (untimed) and then timed to "
                                 "the window's end: the same simulated interval as the ensemble's, per network"),
         window=head,
-        ensemble=dict(setup_s=setup_s, fp32_layout_s=layout32_s, S_entries=info64["S_entries"],
+        ensemble=dict(setup_s=setup_s, mixed_layout_s=layout32_s, S_entries=info64["S_entries"],
                       alg_bytes_per_stress_eval_fp64=info64["alg_bytes"],
-                      alg_bytes_per_stress_eval_fp32=info32["alg_bytes"],
-                      continuing_fp32_rkqs=rkqs32, continuing_fp64_rkqs=rkqs64,
+                      alg_bytes_per_stress_eval_mixed=info32["alg_bytes"],
+                      continuing_mixed_rkqs=rkqs32, continuing_fp64_rkqs=rkqs64,
                       note="continuing windows follow the timed one in simulated time; rates over different "
                            "windows compare per step (ms_per_step) rather than per simulated year"),
         proposal_network=dict(pairs=meta["recording"]["pairs"], loss_initial=meta["training"]["loss_initial"],
@@ -629,7 +641,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="headline",
-                    help="headline (the 64-network ensemble, FP32 operators + learned step vs the plain FP64 path), "
+                    help="headline (the 64-network ensemble, mixed operators + learned step vs the plain FP64 path), "
                          "network80k, network80k_connected, single_fault, ensemble")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)

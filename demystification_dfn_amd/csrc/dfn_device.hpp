@@ -151,13 +151,17 @@ struct PermArgs {
 // launchers (kernels.hip)
 // xf: FP32 copy of x_h for FP32 operators (n + 4 floats), or nullptr
 void launch_gather(const double *x, const int *perm, double *xh, int n, hipStream_t s, float *xf = nullptr);
-// waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (FP32 operators);
+// storage and arithmetic of an operator's factors and panels (mode of launch_vtx and
+// launch_panel): FP64; FP32 with FP32 products and sums (compensated on near-field
+// columns); FP32 storage widened into FP64 products and sums
+enum { kStoreF64 = 0, kStoreF32 = 1, kStoreF32Acc64 = 2 };
+// waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (kStoreF32);
 // active (ensembles, may be null): tasks of networks with active[net] == 0 return at once
-void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
+void launch_vtx(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
                 int waves = 4, const float *xf = nullptr, const int *active = nullptr);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
 // rows: panel rows per wave step, 2 or 4; active as for launch_vtx
-void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
+void launch_panel(const PTask *t, int nt, const void *pdata, int mode, const int *widx, const float *cscale,
                   const double *w, double *ypart, hipStream_t s, int rows = 2, const int *active = nullptr);
 // Multi-RHS stress evaluation (FP64 layout): X holds nrhs vectors of n (vector
 // after vector, original order), y{T,N} likewise; R is nrhs rounded up to a multiple

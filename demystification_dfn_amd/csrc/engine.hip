@@ -268,8 +268,9 @@ struct Engine {
     return DFN_OK;
   }
 
-  // the FP32 layout (DFN_PREC_FP32) or FP32 low-rank factors beside FP64 near-field
-  // blocks (DFN_PREC_MIXED), for the current precision
+  // the FP32 layout (DFN_PREC_FP32), FP32 low-rank factors read into FP64 products
+  // and sums beside FP64 near-field blocks (DFN_PREC_MIXED) or the FP32 layout read
+  // into FP64 products and sums (DFN_PREC_FP32_STORE), for the current precision
   int layout_reduced() {
     ops32.reset();
     ops32_kind = -1;
@@ -277,7 +278,8 @@ struct Engine {
     const int nops = hp.mode == DFN_MODE_II ? 2 : 1;
     const HostOperator *list[2] = {&hop[0], &hop[1]};
     ops32.reset(new DevOps);
-    DFN_TRY(layout_ops<float>(list, nops, *ops32, nullptr, precision == DFN_PREC_MIXED));
+    DFN_TRY(layout_ops<float>(list, nops, *ops32, nullptr, precision == DFN_PREC_MIXED,
+                                   precision == DFN_PREC_FP32_STORE || precision == DFN_PREC_MIXED));
     ops32_kind = precision;
     return DFN_OK;
   }
@@ -330,7 +332,7 @@ struct Engine {
     if (!d) return DFN_ERR_STATE;
     double *w = d->w.as<double>();
     launch_gather(d_V, d->perm.as<int>(), w, N, stream, d->xf.as<float>());
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves,
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->vmode(), w, d->zpart.as<double>(), stream, d->vtx_waves,
                d->xf.as<float>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     if (kt) DFN_HIP(hipEventRecord(kt[0], stream));
@@ -986,7 +988,7 @@ API int dfn_build_hmat(dfn_handle h) {
 
 API int dfn_set_precision(dfn_handle h, int32_t prec) {
   DFN_API_BEGIN
-  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32 && prec != DFN_PREC_MIXED)) return DFN_ERR_ARG;
+  if (!h || (prec < DFN_PREC_FP64 || prec > DFN_PREC_FP32_STORE)) return DFN_ERR_ARG;
   h->e.precision = prec;
   if (prec != DFN_PREC_FP64 && h->e.ops32_kind != prec) return h->e.layout_reduced();
   return DFN_OK;
@@ -1301,7 +1303,7 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
   DFN_HIP(hipMemcpyAsync(dV, xo.data(), N * sizeof(double), hipMemcpyHostToDevice, e.stream));
   double *w = d->w.as<double>();
   dfn::launch_gather(dV, d->perm.as<int>(), w, N, e.stream, d->xf.as<float>());
-  dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), e.stream,
+  dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->vmode(), w, d->zpart.as<double>(), e.stream,
                   d->vtx_waves, d->xf.as<float>());
   dfn::launch_zreduce(d->zred.as<dfn::ZRed>(), d->nzr, d->zpart.as<double>(), w, e.stream);
   dfn::launch_panels(*d, w, e.stream, d->panel_rows);

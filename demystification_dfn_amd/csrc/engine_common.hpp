@@ -84,6 +84,12 @@ struct DevOps {
   DMem ptasks_b, pdata_b, widx_b, cscale_b;
   int npt_b = 0;
   DMem xf;  // FP32 operators: FP32 copy of x_h read by phase 1 (n + 4 floats, the tail zero)
+  // FP32 storage read into FP64 products and sums (DFN_PREC_FP32_STORE, and the
+  // low-rank factors of DFN_PREC_MIXED)
+  bool acc64 = false;
+  int vmode() const { return fp32 ? (acc64 ? kStoreF32Acc64 : kStoreF32) : kStoreF64; }
+  int pmode_a() const { return pa_fp32 ? (acc64 ? kStoreF32Acc64 : kStoreF32) : kStoreF64; }
+  int pmode_b() const { return acc64 ? kStoreF32Acc64 : kStoreF32; }
   int nvt = 0, nzr = 0, npt = 0;
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
@@ -131,10 +137,10 @@ inline void launch_mrhs(const DevOps &d, const double *X, int nrhs, int R, doubl
 // Phase 2 of a stress evaluation: every panel set of the layout (rows: panel rows
 // per wave step; active as for launch_panel)
 inline void launch_panels(const DevOps &d, const double *w, hipStream_t s, int rows, const int *active = nullptr) {
-  launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.pa_fp32, d.widx.as<int>(), d.cscale.as<float>(), w,
+  launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.pmode_a(), d.widx.as<int>(), d.cscale.as<float>(), w,
                d.ypart.as<double>(), s, rows, active);
   if (d.npt_b > 0)
-    launch_panel(d.ptasks_b.as<PTask>(), d.npt_b, d.pdata_b.p, true, d.widx_b.as<int>(), d.cscale_b.as<float>(), w,
+    launch_panel(d.ptasks_b.as<PTask>(), d.npt_b, d.pdata_b.p, d.pmode_b(), d.widx_b.as<int>(), d.cscale_b.as<float>(), w,
                  d.ypart.as<double>(), s, rows, active);
 }
 
@@ -160,7 +166,7 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   auto measure = [&](int waves, int rows, double &ms) -> int {
     for (int rep = 0; rep < 10; rep++) {
       if (rep == 2) DFN_HIP(hipEventRecord(e0, nullptr));
-      launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.fp32, w, d.zpart.as<double>(), nullptr, waves,
+      launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.vmode(), w, d.zpart.as<double>(), nullptr, waves,
                  d.xf.as<float>());
       launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, nullptr);
       launch_panels(d, w, nullptr, rows);
@@ -272,9 +278,10 @@ inline double pow2_scale(const double *v, size_t n) {
 // which keeps them in FP64 in their own panel set).
 template <class T>
 int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::vector<int> *hnet = nullptr,
-               bool dense64 = false) {
+               bool dense64 = false, bool acc64 = false) {
   const bool scaled = sizeof(T) == 4;  // FP32 storage keeps each block divided by its power-of-two scale
   dense64 = dense64 && scaled;
+  d.acc64 = acc64 && scaled;
   const int n = ops[0]->n;
   for (int o = 1; o < nops; o++)
     if (ops[o]->perm != ops[0]->perm) return DFN_ERR_UNSUPPORTED;  // T and N share the cluster tree

@@ -545,7 +545,8 @@ struct EnsEngine {
     std::vector<int> hnet(Ntot);
     for (int i = 0; i < B; i++) std::fill(hnet.begin() + base[i], hnet.begin() + base[i] + hp[i].N, i);
     ops32.reset(new DevOps);
-    DFN_TRY(layout_ops<float>(list, nops, *ops32, &hnet, precision == DFN_PREC_MIXED));
+    DFN_TRY(layout_ops<float>(list, nops, *ops32, &hnet, precision == DFN_PREC_MIXED,
+                                   precision == DFN_PREC_FP32_STORE || precision == DFN_PREC_MIXED));
     ops32_kind = precision;
     return DFN_OK;
   }
@@ -585,7 +586,7 @@ struct EnsEngine {
     launch_gather(dV, d->perm.as<int>(), w, Ntot, stream, d->xf.as<float>());
     // the tasks of networks that sit out this attempt (accepted, or waiting for
     // others) return at once: a retry streams only the retrying networks' operators
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->fp32, w, d->zpart.as<double>(), stream, d->vtx_waves,
+    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->vmode(), w, d->zpart.as<double>(), stream, d->vtx_waves,
                d->xf.as<float>(), d_active.as<int>());
     launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
     if (kt) (void)hipEventRecord(kt[0], stream);
@@ -937,7 +938,7 @@ API int dfn_ensemble_build_hmat(dfn_ensemble h) {
 
 API int dfn_ensemble_set_precision(dfn_ensemble h, int32_t prec) {
   DFN_API_BEGIN
-  if (!h || (prec != DFN_PREC_FP64 && prec != DFN_PREC_FP32 && prec != DFN_PREC_MIXED)) return DFN_ERR_ARG;
+  if (!h || (prec < DFN_PREC_FP64 || prec > DFN_PREC_FP32_STORE)) return DFN_ERR_ARG;
   h->e.precision = prec;
   if (prec != DFN_PREC_FP64 && h->e.ops32_kind != prec) return h->e.layout_reduced();
   h->e.drop_graphs();

@@ -109,16 +109,18 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 // moves.  The task's V is stored strip by strip (64 vectors of rows, column after
 // column; the last strip's height padded with zero rows to a whole vector), so the
 // K loads of one step read one contiguous run.
-template <int K, typename TV>
+template <int K, typename TV, bool ACC64>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
                                          const float *__restrict__ xf, double *__restrict__ zpart,
                                          double *__restrict__ wz, int lane) {
   constexpr int kVec = 16 / (int)sizeof(TV);
   typedef TV tvec __attribute__((ext_vector_type(kVec)));
-  // FP64 storage accumulates in FP64; FP32 storage in FP32 (TV)
-  TV acc[K];
+  // FP64 storage accumulates in FP64; FP32 storage in FP32 (TV), or with ACC64 in
+  // FP64 from the FP64 input (the stored entries widened exactly)
+  typedef typename std::conditional<ACC64, double, TV>::type TA;
+  TA acc[K];
 #pragma unroll
-  for (int j = 0; j < K; j++) acc[j] = (TV)0;
+  for (int j = 0; j < K; j++) acc[j] = (TA)0;
   constexpr int kStrip = 64 * kVec;
   const TV *V = vdata + task.voff;
   const double *x = w + task.xcol + task.row0;
@@ -128,8 +130,8 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
     const int hp = (h + kVec - 1) / kVec * kVec;  // the strip's column stride
     if (i >= task.len) break;
     const TV *Vs = V + (int64_t)s0 * K + lane * kVec;  // the strips before hold kStrip rows each
-    TV xi[kVec];
-    if constexpr (sizeof(TV) == 4) {
+    TA xi[kVec];
+    if constexpr (sizeof(TV) == 4 && !ACC64) {
       // four consecutive entries of the FP32 copy (4-byte aligned; the copy is
       // padded past n); past the chunk's rows they meet the zero rows of V
       float4 x4;
@@ -137,7 +139,7 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
       xi[0] = x4.x; xi[1] = x4.y; xi[2] = x4.z; xi[3] = x4.w;
     } else {
 #pragma unroll
-      for (int u = 0; u < kVec; u++) xi[u] = i + u < task.len ? (TV)x[i + u] : (TV)0;  // padding rows hold zeros
+      for (int u = 0; u < kVec; u++) xi[u] = i + u < task.len ? (TA)x[i + u] : (TA)0;  // padding rows hold zeros
     }
     tvec v[K];
 #pragma unroll
@@ -145,13 +147,13 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
 #pragma unroll
     for (int j = 0; j < K; j++)
 #pragma unroll
-      for (int u = 0; u < kVec; u++) acc[j] += v[j][u] * xi[u];
+      for (int u = 0; u < kVec; u++) acc[j] += (TA)v[j][u] * xi[u];
   }
   // all 64 lanes are active here: DPP wave totals, lane j keeps column j's
-  TV mine = (TV)0;
+  TA mine = (TA)0;
 #pragma unroll
   for (int j = 0; j < K; j++) {
-    const TV t = wave_sum_full(acc[j]);
+    const TA t = wave_sum_full(acc[j]);
     if (lane == j) mine = t;
   }
   if (lane < K) {
@@ -161,7 +163,7 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
   }
 }
 
-template <typename TV>
+template <typename TV, bool ACC64>
 __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
                                                           const TV *__restrict__ vdata, double *__restrict__ w,
                                                           const float *__restrict__ xf, double *__restrict__ zpart,
@@ -175,7 +177,7 @@ __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restric
   if (active && !active[task.net]) return;  // the block's network sits out this evaluation
   switch (task.k) {
 #define DFN_VTX_CASE(K) \
-  case K: vtx_body<K, TV>(task, vdata, w, xf, zpart, w, lane); break;
+  case K: vtx_body<K, TV, ACC64>(task, vdata, w, xf, zpart, w, lane); break;
     DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
     DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
     DFN_VTX_CASE(13) DFN_VTX_CASE(14) DFN_VTX_CASE(15) DFN_VTX_CASE(16) DFN_VTX_CASE(17) DFN_VTX_CASE(18)
@@ -339,6 +341,50 @@ __global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict_
     for (int q = 0; q < R; q++) {
       double t = ((double)s[q] - (double)c[q]) + (double)a[q];
       t = wave_sum(t);
+      if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
+    }
+  }
+}
+
+// Phase 2 over FP32 storage with FP64 arithmetic: the gathered entries stay FP64
+// (in 16 KB of LDS), every stored entry is widened exactly and multiplied and
+// summed in FP64, so the only rounding left beside FP64 is that of the storage.
+template <int R>
+__global__ void __launch_bounds__(256) panel_kernel_f32w(const PTask *__restrict__ tasks,
+                                                         const float *__restrict__ pdata, const int *__restrict__ widx,
+                                                         const float *__restrict__ cscale, const double *__restrict__ w,
+                                                         double *__restrict__ ypart, const int *__restrict__ active) {
+  __shared__ double2 wl2[kPanelMaxCols / 2];
+  double *wl = reinterpret_cast<double *>(wl2);
+  const PTask task = tasks[blockIdx.x];
+  if (active && !active[task.net]) return;  // the panel's network sits out this evaluation
+  for (int j = threadIdx.x; j < task.L; j += 256) wl[j] = w[widx[task.widx + j]] * (double)cscale[task.widx + j];
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int L4 = task.L >> 2;
+  const fv4 *P = reinterpret_cast<const fv4 *>(pdata + task.poff);
+  for (int r = wave * R; r < task.m; r += 4 * R) {
+    const fv4 *row[R];
+    double a[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      row[q] = P + (int64_t)(r + q < task.m ? r + q : r) * L4;
+      a[q] = 0.0;
+    }
+#pragma unroll 2
+    for (int j = lane; j < L4; j += 64) {
+      const double2 w01 = wl2[2 * j], w23 = wl2[2 * j + 1];
+      fv4 v[R];
+#pragma unroll
+      for (int q = 0; q < R; q++) v[q] = __builtin_nontemporal_load(row[q] + j);
+#pragma unroll
+      for (int q = 0; q < R; q++)
+        a[q] = fma((double)v[q].w, w23.y, fma((double)v[q].z, w23.x, fma((double)v[q].y, w01.y,
+                   fma((double)v[q].x, w01.x, a[q]))));
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const double t = wave_sum(a[q]);
       if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
     }
   }
@@ -1825,15 +1871,18 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
   hipLaunchKernelGGL(gather_perm_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, x, perm, xh, xf, n);
 }
 
-void launch_vtx(const VTask *t, int nt, const void *vdata, bool fp32, double *w, double *zpart, hipStream_t s,
+void launch_vtx(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
                 int waves, const float *xf, const int *active) {
   if (nt <= 0) return;
   if (waves != 2 && waves != 8) waves = 4;
-  if (fp32)
-    hipLaunchKernelGGL(lowrank_vtx_kernel<float>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+  if (mode == kStoreF32)
+    hipLaunchKernelGGL((lowrank_vtx_kernel<float, false>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+                       (const float *)vdata, w, xf, zpart, active);
+  else if (mode == kStoreF32Acc64)
+    hipLaunchKernelGGL((lowrank_vtx_kernel<float, true>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
                        (const float *)vdata, w, xf, zpart, active);
   else
-    hipLaunchKernelGGL(lowrank_vtx_kernel<double>, dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+    hipLaunchKernelGGL((lowrank_vtx_kernel<double, true>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
                        (const double *)vdata, w, xf, zpart, active);
 }
 
@@ -1842,10 +1891,17 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
   hipLaunchKernelGGL(zreduce_kernel, dim3(nblk((int64_t)nb * 32, 256)), dim3(256), 0, s, zr, nb, zpart, w);
 }
 
-void launch_panel(const PTask *t, int nt, const void *pdata, bool fp32, const int *widx, const float *cscale,
+void launch_panel(const PTask *t, int nt, const void *pdata, int mode, const int *widx, const float *cscale,
                   const double *w, double *ypart, hipStream_t s, int rows, const int *active) {
   if (nt <= 0) return;
-  if (fp32 && rows == 4)
+  const bool fp32 = mode == kStoreF32;
+  if (mode == kStoreF32Acc64 && rows == 4)
+    hipLaunchKernelGGL(panel_kernel_f32w<4>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w,
+                       ypart, active);
+  else if (mode == kStoreF32Acc64)
+    hipLaunchKernelGGL(panel_kernel_f32w<2>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w,
+                       ypart, active);
+  else if (fp32 && rows == 4)
     hipLaunchKernelGGL(panel_kernel_f32<4>, dim3(nt), dim3(256), 0, s, t, (const float *)pdata, widx, cscale, w, ypart,
                        active);
   else if (fp32)

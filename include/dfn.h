@@ -48,7 +48,7 @@ enum { DFN_OP_T = 0, DFN_OP_N = 1 };                      /* shear (Mode II/III)
 /* storage of the operator blocks: all FP64; all FP32 (power-of-two block scales,
  * FP32 accumulation, compensated near-field sums); FP32 low-rank factors beside
  * FP64 near-field (dense) blocks */
-enum { DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1, DFN_PREC_MIXED = 2 };
+enum { DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1, DFN_PREC_MIXED = 2, DFN_PREC_FP32_STORE = 3 };
 
 /* Problem description: the content of problems/<name>/config
  * (src/configuration.f90:70-379) as flat arrays.  Per-element arrays have
@@ -108,7 +108,14 @@ int dfn_destroy(dfn_handle h);
  * problem's mode: Mode III builds the shear operator, Mode II the normal then the
  * shear operator (src/initialisation.f90:210-248).  No-op for the direct kernel. */
 int dfn_build_hmat(dfn_handle h);
-/* Operator storage precision for subsequent kernel calls (DFN_PREC_FP64 default). */
+/* Operator storage precision for subsequent kernel calls (DFN_PREC_FP64 default):
+ * DFN_PREC_FP32: factors and near-field blocks in FP32 (per-block power-of-two
+ *   scales), FP32 products and sums, compensated on near-field columns;
+ * DFN_PREC_MIXED: FP64 near-field blocks beside FP32 low-rank factors (stored as
+ *   for DFN_PREC_FP32) read into FP64 products and sums;
+ * DFN_PREC_FP32_STORE: the FP32 storage of DFN_PREC_FP32 (same bytes) read into
+ *   FP64 products and sums, so that only the storage rounds.
+ * No reference counterpart (the reference is FP64 throughout). */
 int dfn_set_precision(dfn_handle h, int32_t precision);
 /* stats[0..9] = {nb_dense, nb_lowrank, S_dense, S_lowrank_U, S_lowrank_V, max_rank,
  *                nbclus, nbblclus, n, sum_rank} of operator op. */
@@ -349,7 +356,7 @@ int dfn_ensemble_destroy(dfn_ensemble e);
 /* Builds every network's operators (as dfn_build_hmat) and the block-diagonal
  * device layout of the whole ensemble. */
 int dfn_ensemble_build_hmat(dfn_ensemble e);
-/* DFN_PREC_FP64 / DFN_PREC_FP32 operator storage, as dfn_set_precision. */
+/* Operator storage precision (DFN_PREC_*), as dfn_set_precision. */
 int dfn_ensemble_set_precision(dfn_ensemble e, int32_t precision);
 /* 1 (default): the per-attempt launch sequences are captured as HIP graphs and
  * replayed; 0: the same launches are issued one by one.  Ensembles with networks

@@ -160,7 +160,7 @@ def test_hmatvec_on_identical_block_lists(mode):
         assert rel(d.Hgemv_right(x, op=op), o.Hgemv_right(x, op=op)) <= 1e-12
 
 
-@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_FP32, _abi.PREC_MIXED])
+@pytest.mark.parametrize("prec", [_abi.PREC_FP64, _abi.PREC_FP32, _abi.PREC_MIXED, _abi.PREC_FP32_STORE])
 def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
     """The autotuner only picks launch shapes that keep every sum in the same order:
     the stress rates of the tuned layout, of the default shapes and of every
@@ -275,10 +275,12 @@ def test_hmatrix_solver_trajectory_mode2():
     assert close_elementwise(Yd[N:2 * N][real], Yo[N:2 * N][real], 1e-8, 1e-300)
 
 
+@pytest.mark.parametrize("prec", [_abi.PREC_FP32, _abi.PREC_FP32_STORE])
 @pytest.mark.parametrize("mode", [2, 3])
-def test_fp32_operator_storage_error(mode):
-    """Precision form: FP32 factors with per-block scaling vs the FP64 operator, on
-    log-uniform slip rates in [1e-12, 1] m/s; the bar is 1e-5 relative."""
+def test_fp32_operator_storage_error(mode, prec):
+    """Precision form: FP32 factors with per-block scaling (FP32 sums, or FP64 sums
+    over the same storage) vs the FP64 operator, on log-uniform slip rates in
+    [1e-12, 1] m/s; the bar is 1e-5 relative."""
     p = _hnet(mode, n_faults=20, seed=9)
     d = dev(p)
     d.initiate_hmat()
@@ -287,7 +289,7 @@ def test_fp32_operator_storage_error(mode):
         V = 10.0 ** rng.uniform(-12, 0, p.N)
         d.set_precision(_abi.PREC_FP64)
         a = d.kernel(V)
-        d.set_precision(_abi.PREC_FP32)
+        d.set_precision(prec)
         b = d.kernel(V)
         for y64, y32 in zip(a, b):
             if np.linalg.norm(y64) > 0:
@@ -298,22 +300,22 @@ def test_fp32_operator_storage_error(mode):
 
 @pytest.mark.parametrize("mode", [2, 3])
 def test_mixed_operator_storage_error(mode):
-    """FP32 low-rank factors beside FP64 near-field blocks (DFN_PREC_MIXED) vs the
-    FP64 operator on log-uniform slip rates in [1e-12, 1] m/s: norm-wise within
-    1e-6, and on every row above 1e-3 of the largest stress rate within 1e-5
-    relative -- the rows where all-FP32 storage loses the most, because a
-    near-field entry stored in FP32 keeps its rounding times the row's
-    cancellation ratio."""
+    """FP64 near-field blocks beside FP32 low-rank factors read into FP64 products
+    and sums (DFN_PREC_MIXED) vs the FP64 operator on log-uniform slip rates in
+    [1e-12, 1] m/s: norm-wise within 1e-6, and on every row above 1e-3 of the
+    largest stress rate within 1e-5 relative -- the rows where all-FP32 storage
+    loses the most, because an entry stored in FP32 keeps its rounding times the
+    row's cancellation ratio."""
     p = _hnet(mode, n_faults=20, seed=9)
     d = dev(p)
     d.initiate_hmat()
     rng = np.random.default_rng(3)
-    worst = {_abi.PREC_FP32: 0.0, _abi.PREC_MIXED: 0.0}
+    worst = {_abi.PREC_FP32: 0.0, _abi.PREC_FP32_STORE: 0.0, _abi.PREC_MIXED: 0.0}
     for _ in range(20):
         V = 10.0 ** rng.uniform(-12, 0, p.N)
         d.set_precision(_abi.PREC_FP64)
         a = d.kernel(V)
-        for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
+        for prec in (_abi.PREC_FP32, _abi.PREC_FP32_STORE, _abi.PREC_MIXED):
             d.set_precision(prec)
             b = d.kernel(V)
             for y64, y in zip(a, b):
@@ -323,13 +325,13 @@ def test_mixed_operator_storage_error(mode):
                     assert 0 < rel(y, y64) <= 1e-6
                 big = np.abs(y64) > 1e-3 * np.abs(y64).max()
                 worst[prec] = max(worst[prec], float(np.max(np.abs(y[big] - y64[big]) / np.abs(y64[big]))))
-    print(f"[gpu] significant-row max error: FP32 {worst[_abi.PREC_FP32]:.2e}, mixed {worst[_abi.PREC_MIXED]:.2e}",
-          flush=True)
+    print(f"[gpu] significant-row max error: FP32 {worst[_abi.PREC_FP32]:.2e}, FP32 storage with FP64 sums "
+          f"{worst[_abi.PREC_FP32_STORE]:.2e}, mixed {worst[_abi.PREC_MIXED]:.2e}", flush=True)
     assert worst[_abi.PREC_MIXED] <= 1e-5
     assert worst[_abi.PREC_MIXED] < worst[_abi.PREC_FP32]
 
 
-@pytest.mark.parametrize("prec", [_abi.PREC_FP32, _abi.PREC_MIXED])
+@pytest.mark.parametrize("prec", [_abi.PREC_FP32, _abi.PREC_MIXED, _abi.PREC_FP32_STORE])
 def test_reduced_operator_trajectory_tracks_fp64(prec):
     p = _hnet(2)
     a, b = dev(p), dev(p)
