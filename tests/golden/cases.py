@@ -6,6 +6,12 @@ import numpy as np
 from demystification_dfn_amd import problems
 
 
+def config1_single_fault():
+    """Config 1: BP6-QD-A-like fault, 4000 elements over 40 km, Mode III, dense
+    kernel, tol 1e-8, injection at the centre for 100 days."""
+    return problems.single_fault(n=4000, length=40e3, tol=1e-8, name="config1_single_fault_4000")
+
+
 def single_fault_small():
     """BP6-QD-A-like fault, 300 elements over 3 km, dense Mode III, tol 1e-8."""
     return problems.single_fault(n=300, length=3000.0, tol=1e-8, name="golden_single_fault_300")
