@@ -37,7 +37,8 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a 1 GiB copy kernel
 # (FETCH x 2.000, WRITE x 1.000); summaries and commands in
 # profiles/pmc_traffic_network80k_fp64.md and profiles/pmc_traffic_ensemble64_mixed.md.
-MEASURED_TRAFFIC = {("network800_80k", 2024): 2.4947e9}
+# Keyed by (workload, seed, phase 1 fused with the near-field panels).
+MEASURED_TRAFFIC = {("network800_80k", 2024, False): 2.4947e9}
 YEAR = 365.25 * 86400.0
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 peak (vector and matrix cores alike)
@@ -413,7 +414,7 @@ def run_network(workload, args, mixed=True, cpu_seconds=None):
     whole = kb["alg_bytes"] / (kb["ms_kernel"] * 1e-3) / 1e9
     if p.static_kernel == 1:
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed)), kernel=dominant,
+                    traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed, False)), kernel=dominant,
                     bytes_per_launch=kb["dominant_bytes"], ms_per_launch=ms_dom, **timing)
         matvec = dict(ms=kb["ms_kernel"], alg_bytes=kb["alg_bytes"], GB_s=whole, frac_hbm=whole / HBM_PEAK_GBS, **hm)
     else:
@@ -513,23 +514,27 @@ def run_headline(args):
     tA, tB, wall, head = window(args.steps)
     value = head["value"]
     log(f"[bench] mixed+learned: {args.steps} steps in {wall:.2f} s ({head['ms_per_step']:.2f} ms/step)")
-    # the dominant kernel (the phase-2 panels), HIP events on the ensemble's stream
-    # over further steps in which every network takes part
+    # the dominant kernel, HIP events on the ensemble's stream over further steps in
+    # which every network takes part
     n_extra = min(args.steps, 10)
     ens.kernel_timing(True)
     ens.solve(n_extra)
     ms_dom, n_dom, bytes_dom = ens.kernel_timing_read()
     ens.kernel_timing(False)
     achieved = bytes_dom / (ms_dom * 1e-3) / 1e9 if n_dom else None
-    log(f"[bench] phase-2 panels: {ms_dom:.3f} ms mean over {n_dom} launches")
+    fused = bool(ens.info()["fused_near"])
+    dom_name = ("vtx_near_kernel: phase 1 over the FP32 low-rank V factors (FP64 sums) and the FP64 near-field "
+                "panels in one launch; the whole ensemble's tasks" if fused else
+                "phase-2 panels (panel_kernel_f64 on the FP64 near-field set, then panel_kernel_f32w on the FP32 "
+                "low-rank set; the whole ensemble's panels)")
+    log(f"[bench] dominant kernel ({'phase 1 + near-field' if fused else 'phase-2 panels'}): {ms_dom:.3f} ms mean "
+        f"over {n_dom} launches")
     roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=achieved / HBM_PEAK_GBS if achieved else None,
-                traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed)),
-                kernel="phase-2 panels (panel_kernel_f64 on the FP64 near-field set, then panel_kernel_f32w on "
-                       "the FP32 low-rank set; the whole ensemble's panels)",
-                bytes_per_launch=bytes_dom, ms_per_launch=ms_dom, launches_timed=n_dom,
-                timing=f"mean over the panel launches of {n_extra} ensemble steps after the timed ones in which every "
-                       f"network took part, HIP events on the ensemble's stream")
+                traffic=MEASURED_TRAFFIC.get((desc["workload"], args.seed, fused)),
+                kernel=dom_name, bytes_per_launch=bytes_dom, ms_per_launch=ms_dom, launches_timed=n_dom,
+                timing=f"mean over the dominant launches of {n_extra} ensemble steps after the timed ones in which "
+                       f"every network took part, HIP events on the ensemble's stream")
     info32 = ens.info()
     # the same ensemble, continuing: rkqs with mixed-precision operators, then with FP64 operators
     ens.set_controller(_abi.STEP_RKQS)

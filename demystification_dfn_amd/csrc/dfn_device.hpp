@@ -9,6 +9,7 @@ namespace dfn {
 constexpr int kMaxRank = 32;          // largest low-rank block rank the phase-1 kernel keeps in registers
 constexpr int kPanelMaxCols = 2048;   // columns per phase-2 panel task (w staged in 16 KB of LDS)
 constexpr int kVChunkRows = 512;      // rows of V per phase-1 wave task
+constexpr int kXPad = 4;              // zero entries after x_h in w: phase 1 reads whole 16-byte vectors
 constexpr int kCgMaxIter = 200;       // compute_P gives up after 200 iterations (fluid_diffusion.f90:120)
 
 // Phase 1: out[j] = scale * sum_{i<len} V[row0+i, j] * w[xcol+row0+i], j < k.
@@ -156,10 +157,19 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
 // columns); FP32 storage widened into FP64 products and sums
 enum { kStoreF64 = 0, kStoreF32 = 1, kStoreF32Acc64 = 2 };
 // waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (kStoreF32);
-// active (ensembles, may be null): tasks of networks with active[net] == 0 return at once
+// active (ensembles, may be null): tasks of networks with active[net] == 0 return at once;
+// kmax: the largest rank of any task (at most kMaxRank)
 void launch_vtx(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
-                int waves = 4, const float *xf = nullptr, const int *active = nullptr);
+                int waves = 4, const float *xf = nullptr, const int *active = nullptr, int kmax = kMaxRank);
+// the rank ceiling of the phase-1 kernel instance that serves tasks of rank <= kmax
+int vtx_rank_ceiling(int kmax);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);
+// phase 1 over FP32 factors with FP64 sums together with the FP64 near-field panels
+// (pt: the FP64 panel set of a mixed-precision layout) in one launch; rows and
+// active as for launch_panel
+void launch_vtx_near(const VTask *vt, int nvt, const float *vdata, const PTask *pt, int npt, const double *pdata,
+                     const int *widx, double *w, double *zpart, double *ypart, hipStream_t s, int rows = 2,
+                     const int *active = nullptr, int kmax = kMaxRank);
 // rows: panel rows per wave step, 2 or 4; active as for launch_vtx
 void launch_panel(const PTask *t, int nt, const void *pdata, int mode, const int *widx, const float *cscale,
                   const double *w, double *ypart, hipStream_t s, int rows = 2, const int *active = nullptr);

@@ -332,12 +332,7 @@ struct Engine {
     if (!d) return DFN_ERR_STATE;
     double *w = d->w.as<double>();
     launch_gather(d_V, d->perm.as<int>(), w, N, stream, d->xf.as<float>());
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->vmode(), w, d->zpart.as<double>(), stream, d->vtx_waves,
-               d->xf.as<float>());
-    launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
-    if (kt) DFN_HIP(hipEventRecord(kt[0], stream));
-    launch_panels(*d, w, stream, d->panel_rows);
-    if (kt) DFN_HIP(hipEventRecord(kt[1], stream));
+    launch_phases(*d, w, stream, d->vtx_waves, d->panel_rows, nullptr, kt);
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                     d->nops, kernel_scale(), d_sT, d->nops > 1 ? d_sN : d_sT, stream);
     return DFN_OK;
@@ -1005,6 +1000,7 @@ API int dfn_tuning(dfn_handle h, int32_t *choice, double *ms) {
   choice[2] = d->panel_rows;
   choice[3] = d->panel_order;
   choice[4] = d->panel_split;
+  choice[5] = d->fused ? 1 : 0;
   ms[0] = d->tune_ms_default;
   ms[1] = d->tune_ms;
   return DFN_OK;
@@ -1303,10 +1299,7 @@ API int dfn_hgemv(dfn_handle h, int32_t op, double alpha, double beta, const dou
   DFN_HIP(hipMemcpyAsync(dV, xo.data(), N * sizeof(double), hipMemcpyHostToDevice, e.stream));
   double *w = d->w.as<double>();
   dfn::launch_gather(dV, d->perm.as<int>(), w, N, e.stream, d->xf.as<float>());
-  dfn::launch_vtx(d->vtasks.as<dfn::VTask>(), d->nvt, d->vdata.p, d->vmode(), w, d->zpart.as<double>(), e.stream,
-                  d->vtx_waves, d->xf.as<float>());
-  dfn::launch_zreduce(d->zred.as<dfn::ZRed>(), d->nzr, d->zpart.as<double>(), w, e.stream);
-  dfn::launch_panels(*d, w, e.stream, d->panel_rows);
+  dfn::launch_phases(*d, w, e.stream, d->vtx_waves, d->panel_rows, nullptr, nullptr);
   dfn::launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), N,
                   d->nops, 1.0, dT, d->nops > 1 ? dN : dT, e.stream);
   DFN_HIP(hipMemcpyAsync(yT.data(), op == 0 ? dT : dN, N * sizeof(double), hipMemcpyDeviceToHost, e.stream));
@@ -1498,17 +1491,25 @@ API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
   ms[3] = 0.0;
   const dfn::DevOps *d = e.active_ops();
   if (e.hp.kernel == DFN_KERNEL_HMATRIX && d) {
-    // the phase-2 panel kernel alone
+    // the dominant launch alone: the phase-2 panels, or phase 1 with the near-field
+    // panels when they run as one launch
     double *w = d->w.as<double>();
     DFN_HIP(hipEventRecord(e.ev0, e.stream));
-    for (int r = 0; r < reps; r++)
-      dfn::launch_panels(*d, w, e.stream, d->panel_rows);
+    for (int r = 0; r < reps; r++) {
+      if (d->fused)
+        dfn::launch_vtx_near(d->vtasks.as<dfn::VTask>(), d->nvt, static_cast<const float *>(d->vdata.p),
+                             d->ptasks.as<dfn::PTask>(), d->npt, static_cast<const double *>(d->pdata.p),
+                             d->widx.as<int>(), w, d->zpart.as<double>(), d->ypart.as<double>(), e.stream,
+                             d->panel_rows, nullptr, d->kmax);
+      else
+        dfn::launch_panels(*d, w, e.stream, d->panel_rows);
+    }
     DFN_HIP(hipEventRecord(e.ev1, e.stream));
     DFN_HIP(hipEventSynchronize(e.ev1));
     DFN_HIP(hipEventElapsedTime(&t, e.ev0, e.ev1));
     ms[1] = (double)t / reps;
     ms[2] = d->alg_bytes;
-    ms[3] = d->panel_bytes;
+    ms[3] = d->dom_bytes();
   } else {
     // the direct kernels are FP64-arithmetic bound: report algorithmic flops per
     // application, counting sqrt and divide as one each (Mode III: 11 per

@@ -87,12 +87,23 @@ struct DevOps {
   // FP32 storage read into FP64 products and sums (DFN_PREC_FP32_STORE, and the
   // low-rank factors of DFN_PREC_MIXED)
   bool acc64 = false;
+  // DFN_PREC_MIXED: phase 1 and the FP64 near-field panels (set A, which read only
+  // x_h) may run as one launch (launch_vtx_near); near_ok says the layout allows it,
+  // fused that the autotuner chose it
+  bool near_ok = false, fused = false;
   int vmode() const { return fp32 ? (acc64 ? kStoreF32Acc64 : kStoreF32) : kStoreF64; }
   int pmode_a() const { return pa_fp32 ? (acc64 ? kStoreF32Acc64 : kStoreF32) : kStoreF64; }
   int pmode_b() const { return acc64 ? kStoreF32Acc64 : kStoreF32; }
   int nvt = 0, nzr = 0, npt = 0;
+  int kmax = kMaxRank;  // largest rank of any phase-1 task
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
+  // algorithmic bytes of launch_vtx_near (FP32 V entries, FP64 near-field entries,
+  // x_h, z and near-field row partials written)
+  double fused_bytes = 0;
+  // the dominant launch of an evaluation and its algorithmic bytes: the fused
+  // launch when fused, else the phase-2 panels
+  double dom_bytes() const { return fused ? fused_bytes : panel_bytes; }
   int64_t S_total = 0;   // matrix entries streamed per application
   int64_t S_panel = 0;   // of which in the phase-2 panels (dense and U entries)
   // launch shapes picked by autotune (none changes a sum): phase-1 tasks per
@@ -144,12 +155,36 @@ inline void launch_panels(const DevOps &d, const double *w, hipStream_t s, int r
                  d.ypart.as<double>(), s, rows, active);
 }
 
+// Phase 1, the z reduction and phase 2 of one stress evaluation (x_h already
+// gathered into w); kt (may be null) gets an event pair around the dominant launch
+inline void launch_phases(const DevOps &d, double *w, hipStream_t s, int waves, int rows, const int *active,
+                          hipEvent_t *kt) {
+  if (d.fused) {
+    if (kt) (void)hipEventRecord(kt[0], s);
+    launch_vtx_near(d.vtasks.as<VTask>(), d.nvt, static_cast<const float *>(d.vdata.p), d.ptasks.as<PTask>(), d.npt,
+                    static_cast<const double *>(d.pdata.p), d.widx.as<int>(), w, d.zpart.as<double>(),
+                    d.ypart.as<double>(), s, rows, active, d.kmax);
+    if (kt) (void)hipEventRecord(kt[1], s);
+    launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, s);
+    launch_panel(d.ptasks_b.as<PTask>(), d.npt_b, d.pdata_b.p, d.pmode_b(), d.widx_b.as<int>(),
+                 d.cscale_b.as<float>(), w, d.ypart.as<double>(), s, rows, active);
+    return;
+  }
+  launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.vmode(), w, d.zpart.as<double>(), s, waves, d.xf.as<float>(),
+             active, d.kmax);
+  launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, s);
+  if (kt) (void)hipEventRecord(kt[0], s);
+  launch_panels(d, w, s, rows, active);
+  if (kt) (void)hipEventRecord(kt[1], s);
+}
+
 // Autotuner of the stress-evaluation launch shapes, run once per operator layout on
 // the operator itself: phase-1 tasks (waves) per workgroup {2, 4, 8}; the phase-1
 // task order (as built; grouped by rank bucket with the largest ranks first, so a
 // workgroup's waves share one code path; or the largest tasks, rows x rank, first);
 // 2 or 4 panel rows per wave step; the phase-2 schedule (as built, or largest
-// panels first, whole or cut into tasks of 32 or 16 rows).  Every candidate
+// panels first, whole or cut into tasks of 32 or 16 rows); for a mixed-precision
+// layout, phase 1 and the near-field panels in one launch or in two.  Every candidate
 // computes every sum in the same order, so the choice never changes a result.
 // Each candidate is timed over phase 1, the z reduction and phase 2 (HIP events,
 // 2 warm-up + 8 timed applications); a candidate replaces the incumbent only if it
@@ -166,10 +201,7 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   auto measure = [&](int waves, int rows, double &ms) -> int {
     for (int rep = 0; rep < 10; rep++) {
       if (rep == 2) DFN_HIP(hipEventRecord(e0, nullptr));
-      launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.vmode(), w, d.zpart.as<double>(), nullptr, waves,
-                 d.xf.as<float>());
-      launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, nullptr);
-      launch_panels(d, w, nullptr, rows);
+      launch_phases(d, w, nullptr, waves, rows, nullptr, nullptr);
     }
     DFN_HIP(hipEventRecord(e1, nullptr));
     DFN_HIP(hipEventSynchronize(e1));
@@ -202,6 +234,7 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   double best = 0, t = 0;
   if (alt) {
     waves = 8; order = 2; rows = 4; pord = 1; split = 16;
+    d.fused = d.near_ok;
     DFN_TRY(vorder(order));
     DFN_TRY(porder(pord, split));
   } else {
@@ -225,6 +258,12 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
       if (t < 0.99 * best) { best = t; pord = 1; split = cand; }
     }
     DFN_TRY(porder(pord, split));
+    if (d.near_ok) {
+      d.fused = true;
+      DFN_TRY(measure(waves, rows, t));
+      if (t < 0.99 * best) best = t;
+      else d.fused = false;
+    }
   }
   d.vtx_waves = waves;
   d.vtx_order = order;
@@ -288,8 +327,8 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   d.n = n;
   d.nops = nops;
   d.fp32 = sizeof(T) == 4;
-  // w = [x_h | z of op 0 | z of op 1]
-  std::vector<int64_t> zbase(nops + 1, n);
+  // w = [x_h | kXPad zeros | z of op 0 | z of op 1]
+  std::vector<int64_t> zbase(nops + 1, n + kXPad);
   std::vector<std::vector<int>> zoff(nops);
   for (int o = 0; o < nops; o++) {
     const HostOperator &H = *ops[o];
@@ -525,6 +564,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
     DFN_TRY(d.cscale_b.upload(setB.cscale));
   }
   DFN_TRY(d.w.alloc(sizeof(double) * wsize));
+  DFN_HIP(hipMemset(d.w.p, 0, d.w.bytes));  // the zeros after x_h stay: only x_h and z are written
   if (d.fp32) {
     DFN_TRY(d.xf.alloc(sizeof(float) * ((size_t)n + 4)));
     DFN_HIP(hipMemset(d.xf.p, 0, sizeof(float) * ((size_t)n + 4)));
@@ -534,6 +574,8 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   DFN_TRY(d.fin_idx.upload(fin_idx));
   d.nvt = (int)vt.size();
   d.nzr = (int)zr.size();
+  d.kmax = 1;
+  for (const VTask &t : vt) d.kmax = std::max(d.kmax, (int)t.k);
   d.npt = (int)pt.size();
   d.S_total = S_total;
   d.S_panel = S_panel;
@@ -542,8 +584,28 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   d.alg_bytes = (double)sizeof(T) * (double)S_total + dense_extra + 8.0 * n + 8.0 * n * nops;
   d.panel_bytes = (double)setA.esize * (double)setA.entries + (double)setB.esize * (double)setB.entries +
                   4.0 * (double)(setA.cscale.size() + setB.cscale.size()) + 8.0 * (double)yp;
+  // one launch for phase 1 and the near-field panels where its rank ceiling keeps it
+  // at 128 VGPRs without spills (ranks up to 20)
+  d.near_ok = dense64 && d.acc64 && vtx_rank_ceiling(d.kmax) <= 20;
+  if (d.near_ok) {
+    int64_t S_V = 0, nz = 0;  // nz: z entries written straight to w (single-chunk blocks)
+    for (int o = 0; o < nops; o++) {
+      S_V += (int64_t)ops[o]->l_v.size();
+      for (size_t b = 0; b < ops[o]->l_k.size(); b++)
+        if (ops[o]->l_n[b] <= kVChunkRows) nz += ops[o]->l_k[b];
+    }
+    int64_t rows_a = 0;
+    for (const PTask &t : setA.pt) rows_a += t.m;
+    d.fused_bytes = 4.0 * (double)S_V + 8.0 * (double)setA.entries + 8.0 * n + 8.0 * (double)(zp + nz) +
+                    8.0 * (double)rows_a;
+  }
+  d.fused = false;
   const char *at = std::getenv("DFN_AUTOTUNE");
   if (!(at && std::string(at) == "0")) DFN_TRY(autotune(d, vt, pt, setB.pt));
+  // DFN_FUSE_NEAR=1 / 0 takes / refuses the single launch of phase 1 and the
+  // near-field panels whatever the timing (the results are the same either way)
+  const char *fuse = std::getenv("DFN_FUSE_NEAR");
+  if (fuse && d.near_ok) d.fused = std::string(fuse) == "1";
   return DFN_OK;
 }
 

@@ -586,12 +586,7 @@ struct EnsEngine {
     launch_gather(dV, d->perm.as<int>(), w, Ntot, stream, d->xf.as<float>());
     // the tasks of networks that sit out this attempt (accepted, or waiting for
     // others) return at once: a retry streams only the retrying networks' operators
-    launch_vtx(d->vtasks.as<VTask>(), d->nvt, d->vdata.p, d->vmode(), w, d->zpart.as<double>(), stream, d->vtx_waves,
-               d->xf.as<float>(), d_active.as<int>());
-    launch_zreduce(d->zred.as<ZRed>(), d->nzr, d->zpart.as<double>(), w, stream);
-    if (kt) (void)hipEventRecord(kt[0], stream);
-    launch_panels(*d, w, stream, d->panel_rows, d_active.as<int>());
-    if (kt) (void)hipEventRecord(kt[1], stream);
+    launch_phases(*d, w, stream, d->vtx_waves, d->panel_rows, d_active.as<int>(), kt);
     launch_finalize(d->fin_first.as<int>(), d->fin_idx.as<int>(), d->perm.as<int>(), d->ypart.as<double>(), Ntot,
                     d->nops, kscale, sT.as<double>(), d->nops > 1 ? sN.as<double>() : sT.as<double>(), stream);
   }
@@ -1080,7 +1075,7 @@ API int dfn_ensemble_kernel_timing_read(dfn_ensemble h, double *ms_mean, int64_t
   DFN_TRY(e.kt_collect());
   if (ms_mean) *ms_mean = e.kt_count ? e.kt_ms / (double)e.kt_count : 0.0;
   if (count) *count = e.kt_count;
-  if (bytes) *bytes = e.active_ops() ? e.active_ops()->panel_bytes : 0.0;
+  if (bytes) *bytes = e.active_ops() ? e.active_ops()->dom_bytes() : 0.0;
   return DFN_OK;
   DFN_API_END
 }
@@ -1096,6 +1091,8 @@ API int dfn_ensemble_info(dfn_ensemble h, int64_t *info) {
   info[3] = d ? d->S_total : 0;
   info[4] = d ? (int64_t)d->alg_bytes : 0;
   info[5] = e.use_graph ? 1 : 0;
+  info[6] = d && d->fused ? 1 : 0;
+  info[7] = d ? dfn::vtx_rank_ceiling(d->kmax) : 0;
   return DFN_OK;
   DFN_API_END
 }

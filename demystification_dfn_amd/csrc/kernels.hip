@@ -138,8 +138,13 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
       __builtin_memcpy(&x4, xf + task.xcol + task.row0 + i, sizeof(x4));
       xi[0] = x4.x; xi[1] = x4.y; xi[2] = x4.z; xi[3] = x4.w;
     } else {
+      // kVec consecutive entries of x_h, read as 16-byte vectors (8-byte aligned);
+      // past the chunk's rows they meet the zero rows of V, and x_h is followed by
+      // zero entries in w, so every entry read is a finite value times zero
+      double xv[kVec];
+      __builtin_memcpy(xv, x + i, sizeof(xv));
 #pragma unroll
-      for (int u = 0; u < kVec; u++) xi[u] = i + u < task.len ? (TA)x[i + u] : (TA)0;  // padding rows hold zeros
+      for (int u = 0; u < kVec; u++) xi[u] = (TA)xv[u];
     }
     tvec v[K];
 #pragma unroll
@@ -163,7 +168,30 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
   }
 }
 
-template <typename TV, bool ACC64>
+// one phase-1 task on one wave: a wave-uniform switch on the task's rank.  KMAX is
+// the largest rank of the layout rounded up to 16, 20 or 32: the register budget of
+// the whole kernel is that of its largest case, so a layout whose ranks stay at or
+// below 20 keeps more waves resident (FP64 storage 88 instead of 135 VGPRs)
+template <typename TV, bool ACC64, int KMAX>
+__device__ __forceinline__ void vtx_task(const VTask &task, const TV *__restrict__ vdata, double *__restrict__ w,
+                                         const float *__restrict__ xf, double *__restrict__ zpart, int lane) {
+  switch (task.k) {
+#define DFN_VTX_CASE(K) \
+  case K:                                                                         \
+    if constexpr (K <= KMAX) vtx_body<K, TV, ACC64>(task, vdata, w, xf, zpart, w, lane); \
+    break;
+    DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
+    DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
+    DFN_VTX_CASE(13) DFN_VTX_CASE(14) DFN_VTX_CASE(15) DFN_VTX_CASE(16) DFN_VTX_CASE(17) DFN_VTX_CASE(18)
+    DFN_VTX_CASE(19) DFN_VTX_CASE(20) DFN_VTX_CASE(21) DFN_VTX_CASE(22) DFN_VTX_CASE(23) DFN_VTX_CASE(24)
+    DFN_VTX_CASE(25) DFN_VTX_CASE(26) DFN_VTX_CASE(27) DFN_VTX_CASE(28) DFN_VTX_CASE(29) DFN_VTX_CASE(30)
+    DFN_VTX_CASE(31) DFN_VTX_CASE(32)
+#undef DFN_VTX_CASE
+    default: break;
+  }
+}
+
+template <typename TV, bool ACC64, int KMAX>
 __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
                                                           const TV *__restrict__ vdata, double *__restrict__ w,
                                                           const float *__restrict__ xf, double *__restrict__ zpart,
@@ -175,18 +203,7 @@ __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restric
   if (t >= ntasks) return;
   const VTask task = tasks[t];
   if (active && !active[task.net]) return;  // the block's network sits out this evaluation
-  switch (task.k) {
-#define DFN_VTX_CASE(K) \
-  case K: vtx_body<K, TV, ACC64>(task, vdata, w, xf, zpart, w, lane); break;
-    DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
-    DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
-    DFN_VTX_CASE(13) DFN_VTX_CASE(14) DFN_VTX_CASE(15) DFN_VTX_CASE(16) DFN_VTX_CASE(17) DFN_VTX_CASE(18)
-    DFN_VTX_CASE(19) DFN_VTX_CASE(20) DFN_VTX_CASE(21) DFN_VTX_CASE(22) DFN_VTX_CASE(23) DFN_VTX_CASE(24)
-    DFN_VTX_CASE(25) DFN_VTX_CASE(26) DFN_VTX_CASE(27) DFN_VTX_CASE(28) DFN_VTX_CASE(29) DFN_VTX_CASE(30)
-    DFN_VTX_CASE(31) DFN_VTX_CASE(32)
-#undef DFN_VTX_CASE
-    default: break;
-  }
+  vtx_task<TV, ACC64, KMAX>(task, vdata, w, xf, zpart, lane);
 }
 
 // z of each multi-chunk low-rank block = sum of its chunk partials, in chunk
@@ -220,13 +237,10 @@ typedef double dv2 __attribute__((ext_vector_type(2)));
 // every row starts on a cache line and every lane issues one 16-byte load per row:
 // 1 KiB per wave instruction).
 template <int R>
-__global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict__ tasks, const double *__restrict__ pdata,
-                                                        const int *__restrict__ widx, const double *__restrict__ w,
-                                                        double *__restrict__ ypart, const int *__restrict__ active) {
-  __shared__ double2 wl2[kPanelMaxCols / 2];
+__device__ __forceinline__ void panel_f64_body(const PTask &task, const double *__restrict__ pdata,
+                                               const int *__restrict__ widx, const double *__restrict__ w,
+                                               double *__restrict__ ypart, double2 *wl2) {
   double *wl = reinterpret_cast<double *>(wl2);
-  const PTask task = tasks[blockIdx.x];
-  if (active && !active[task.net]) return;  // the panel's network sits out this evaluation
   for (int j = threadIdx.x; j < task.L; j += 256) wl[j] = w[widx[task.widx + j]];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -267,6 +281,47 @@ __global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict_
     a0 = wave_sum(a0);
     if (lane == 0) ypart[task.yp + r] = a0;
   }
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) panel_kernel_f64(const PTask *__restrict__ tasks, const double *__restrict__ pdata,
+                                                        const int *__restrict__ widx, const double *__restrict__ w,
+                                                        double *__restrict__ ypart, const int *__restrict__ active) {
+  __shared__ double2 wl2[kPanelMaxCols / 2];
+  const PTask task = tasks[blockIdx.x];
+  if (active && !active[task.net]) return;  // the panel's network sits out this evaluation
+  panel_f64_body<R>(task, pdata, widx, w, ypart, wl2);
+}
+
+// Phase 1 over FP32 low-rank factors (FP64 products and sums) and phase 2 of the
+// FP64 near-field panels of a mixed-precision layout, in one launch: the
+// near-field panels read only x_h, so they stream beside the low-rank tasks
+// instead of after them.  Workgroup b is a panel workgroup when
+// floor((b + 1) np / nb) > floor(b np / nb) (nb workgroups in all), which spreads
+// the panels evenly over the grid; every other workgroup runs four phase-1 tasks.
+// Each sum is computed as the two separate kernels compute it.  Four waves per SIMD
+// (128 VGPRs) for either kind of workgroup.
+template <int R, int KMAX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) vtx_near_kernel(const VTask *__restrict__ vt, int nvt,
+                                                       const float *__restrict__ vdata, const PTask *__restrict__ pt,
+                                                       int npt, const double *__restrict__ pdata,
+                                                       const int *__restrict__ widx, double *__restrict__ w,
+                                                       double *__restrict__ zpart, double *__restrict__ ypart,
+                                                       const int *__restrict__ active) {
+  __shared__ double2 wl2[kPanelMaxCols / 2];
+  const int64_t nb = (int64_t)((nvt + 3) >> 2) + npt, b = blockIdx.x;
+  const int p0 = (int)(b * npt / nb), p1 = (int)((b + 1) * npt / nb);
+  if (p1 > p0) {
+    const PTask task = pt[p0];
+    if (active && !active[task.net]) return;
+    panel_f64_body<R>(task, pdata, widx, w, ypart, wl2);
+    return;
+  }
+  const int t = __builtin_amdgcn_readfirstlane((int)(b - p1) * 4 + (int)(threadIdx.x >> 6));
+  if (t >= nvt) return;
+  const VTask task = vt[t];
+  if (active && !active[task.net]) return;
+  vtx_task<float, true, KMAX>(task, vdata, w, nullptr, zpart, threadIdx.x & 63);
 }
 
 // FP32 storage: entries are block / 2^e per block, the column scales 2^e fold into
@@ -1871,19 +1926,57 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
   hipLaunchKernelGGL(gather_perm_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, x, perm, xh, xf, n);
 }
 
-void launch_vtx(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
-                int waves, const float *xf, const int *active) {
-  if (nt <= 0) return;
-  if (waves != 2 && waves != 8) waves = 4;
+template <int KMAX>
+static void launch_vtx_k(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
+                         int waves, const float *xf, const int *active) {
   if (mode == kStoreF32)
-    hipLaunchKernelGGL((lowrank_vtx_kernel<float, false>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+    hipLaunchKernelGGL((lowrank_vtx_kernel<float, false, KMAX>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
                        (const float *)vdata, w, xf, zpart, active);
   else if (mode == kStoreF32Acc64)
-    hipLaunchKernelGGL((lowrank_vtx_kernel<float, true>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+    hipLaunchKernelGGL((lowrank_vtx_kernel<float, true, KMAX>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
                        (const float *)vdata, w, xf, zpart, active);
   else
-    hipLaunchKernelGGL((lowrank_vtx_kernel<double, true>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
-                       (const double *)vdata, w, xf, zpart, active);
+    hipLaunchKernelGGL((lowrank_vtx_kernel<double, true, KMAX>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t,
+                       nt, (const double *)vdata, w, xf, zpart, active);
+}
+
+int vtx_rank_ceiling(int kmax) { return kmax <= 16 ? 16 : kmax <= 20 ? 20 : kMaxRank; }
+
+void launch_vtx(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
+                int waves, const float *xf, const int *active, int kmax) {
+  if (nt <= 0) return;
+  if (waves != 2 && waves != 8) waves = 4;
+  const int kc = vtx_rank_ceiling(kmax);
+  if (kc == 16) launch_vtx_k<16>(t, nt, vdata, mode, w, zpart, s, waves, xf, active);
+  else if (kc == 20) launch_vtx_k<20>(t, nt, vdata, mode, w, zpart, s, waves, xf, active);
+  else launch_vtx_k<kMaxRank>(t, nt, vdata, mode, w, zpart, s, waves, xf, active);
+}
+
+template <int R, int KMAX>
+static void launch_vtx_near_k(int64_t nb, const VTask *vt, int nvt, const float *vdata, const PTask *pt, int npt,
+                              const double *pdata, const int *widx, double *w, double *zpart, double *ypart,
+                              hipStream_t s, const int *active) {
+  hipLaunchKernelGGL((vtx_near_kernel<R, KMAX>), dim3((unsigned)nb), dim3(256), 0, s, vt, nvt, vdata, pt, npt, pdata,
+                     widx, w, zpart, ypart, active);
+}
+
+void launch_vtx_near(const VTask *vt, int nvt, const float *vdata, const PTask *pt, int npt, const double *pdata,
+                     const int *widx, double *w, double *zpart, double *ypart, hipStream_t s, int rows,
+                     const int *active, int kmax) {
+  const int64_t nb = (int64_t)nblk(nvt, 4) + npt;
+  if (nb <= 0) return;
+  const int kc = vtx_rank_ceiling(kmax);
+#define DFN_NEAR(R, K) launch_vtx_near_k<R, K>(nb, vt, nvt, vdata, pt, npt, pdata, widx, w, zpart, ypart, s, active)
+  if (rows == 4) {
+    if (kc == 16) DFN_NEAR(4, 16);
+    else if (kc == 20) DFN_NEAR(4, 20);
+    else DFN_NEAR(4, kMaxRank);
+  } else {
+    if (kc == 16) DFN_NEAR(2, 16);
+    else if (kc == 20) DFN_NEAR(2, 20);
+    else DFN_NEAR(2, kMaxRank);
+  }
+#undef DFN_NEAR
 }
 
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s) {

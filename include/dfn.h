@@ -205,10 +205,13 @@ int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats
  * counterpart): choice = {phase-1 tasks per workgroup (2, 4, 8), phase-1 task order
  * (0 as built, 1 grouped by rank bucket, largest ranks first, 2 largest tasks
  * first), panel rows per wave step (2, 4), panel task order (0 as built, 1 largest
- * panels first), rows per panel task (0 whole panels, 32, 16)} (5 entries);
+ * panels first), rows per panel task (0 whole panels, 32, 16), phase 1 and the
+ * FP64 near-field panels of a DFN_PREC_MIXED layout in one launch (1) or two (0)}
+ * (6 entries);
  * ms = {matvec time of the default shapes, of the chosen ones}.  None of the
  * choices changes a result.  DFN_AUTOTUNE=0 keeps the defaults, DFN_AUTOTUNE=alt
- * takes a set of non-default shapes without timing. */
+ * takes a set of non-default shapes without timing; DFN_FUSE_NEAR=1 or 0 takes or
+ * refuses the single launch of entry 5 whatever its timing. */
 int dfn_tuning(dfn_handle h, int32_t *choice, double *ms);
 
 /* ---- Newton-V form ------------------------------------------------------------
@@ -288,12 +291,14 @@ int dfn_flush_output(dfn_handle h);
 /* Time `reps` applications of the stress-transfer kernel on device-resident input
  * (the handle's current V) with HIP events on the handle's stream.  ms[0] = mean ms
  * per application of the whole kernel (all operators), ms[1] = mean ms of the
- * dominant device kernel alone (H-matrix: the phase-2 panel kernel), ms[2] =
+ * dominant device kernel alone (H-matrix: the phase-2 panel kernel, or phase 1
+ * with the near-field panels when they run as one launch), ms[2] =
  * algorithmic bytes per application, ms[3] = algorithmic bytes of the dominant
  * kernel per launch.  ms must hold 4 doubles. */
 int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms);
 /* Per-launch timing of the dominant kernel of every stress evaluation (the
- * phase-2 panel kernel of the H-matrix path, or the direct kernel) while the
+ * phase-2 panel kernel of the H-matrix path, or phase 1 with the near-field panels
+ * when they run as one launch, or the direct kernel) while the
  * solver runs, with a HIP event pair around each launch on the handle's stream (no
  * reference counterpart).  dfn_kernel_timing(h, 1) clears the sums and starts,
  * 0 stops; dfn_kernel_timing_read gives the mean duration in ms and the number of
@@ -397,13 +402,17 @@ int dfn_ensemble_set_recording(dfn_ensemble e, int32_t on);
 /* Moves up to cap recorded pairs (X: 8 floats each, T: 1) out of the ensemble;
  * with X or T NULL, *count = pairs available. */
 int dfn_ensemble_take_records(dfn_ensemble e, float *X, float *T, int64_t cap, int64_t *count);
-/* info[0..5] = networks, total elements, largest network, matrix entries streamed
- * per stress evaluation, algorithmic bytes per stress evaluation, graphs on. */
+/* info[0..7] = networks, total elements, largest network, matrix entries streamed
+ * per stress evaluation, algorithmic bytes per stress evaluation, graphs on, phase 1
+ * and the FP64 near-field panels in one launch (1) or two (0), the rank ceiling of
+ * the phase-1 kernel instance (16, 20 or 32). */
 int dfn_ensemble_info(dfn_ensemble e, int64_t *info);
-/* Per-launch timing of the phase-2 panel kernel while the ensemble steps (no
- * reference counterpart): with timing on, every stress evaluation in which all
- * networks take part records a HIP event pair around its panel launch on the
- * ensemble's stream (the launches are then issued one by one).  1 clears the
+/* Per-launch timing of the dominant kernel while the ensemble steps (no reference
+ * counterpart): the phase-2 panels, or phase 1 with the near-field panels when
+ * they run as one launch (dfn_ensemble_info entry 6).  With timing on, every
+ * stress evaluation in which all networks take part records a HIP event pair
+ * around that launch on the ensemble's stream (the launches are then issued one
+ * by one).  1 clears the
  * sums and starts, 0 stops; _read gives the mean duration in ms, the launches
  * timed and the kernel's algorithmic bytes per such launch. */
 int dfn_ensemble_kernel_timing(dfn_ensemble e, int32_t on);

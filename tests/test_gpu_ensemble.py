@@ -73,6 +73,17 @@ def test_ensemble_reduced_operators_equal_independent_runs(prec):
     _compare(_nets(2, n=4, seed=9), 5, prec=prec)
 
 
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_ensemble_mixed_fused_phase1_equals_unfused_independent_runs(fuse, monkeypatch):
+    """Mixed precision: the ensemble with phase 1 and the near-field panels in one
+    launch (or two), its networks' own runs the other way; every network is
+    bit-identical (the tasks of networks sitting out a retry return at once in
+    either kind of workgroup)."""
+    monkeypatch.setenv("DFN_FUSE_NEAR", fuse)
+    _compare(_nets(2, n=4, seed=9), 5, prec=_abi.PREC_MIXED, env={"DFN_FUSE_NEAR": "0" if fuse == "1" else "1"},
+             monkeypatch=monkeypatch)
+
+
 @pytest.mark.parametrize("cg_path", ["auto", "plain"])
 def test_ensemble_with_a_large_network(cg_path, monkeypatch):
     """A network above the one-workgroup compute_P size beside two small ones: the

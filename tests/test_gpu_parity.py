@@ -168,8 +168,16 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
     p = _hnet(2, n_faults=40, seed=8)
     keys = ("vtx_waves", "vtx_order", "panel_rows", "panel_order", "panel_split")
     out = []
-    for tune in ("1", "0", "alt"):
+    # the mixed layout also runs phase 1 and its near-field panels as one launch or two
+    runs = [("1", None), ("0", None), ("alt", None)]
+    if prec == _abi.PREC_MIXED:
+        runs += [("1", "1"), ("1", "0"), ("0", "1")]
+    for tune, fuse in runs:
         monkeypatch.setenv("DFN_AUTOTUNE", tune)
+        if fuse is None:
+            monkeypatch.delenv("DFN_FUSE_NEAR", raising=False)
+        else:
+            monkeypatch.setenv("DFN_FUSE_NEAR", fuse)
         d = dev(p)
         d.initiate_hmat()
         if prec != _abi.PREC_FP64:
@@ -182,6 +190,10 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
             assert [t[k] for k in keys] == [4, 0, 2, 0, 0]
         if tune == "alt":
             assert [t[k] for k in keys] == [8, 2, 4, 1, 16]
+        if fuse is not None:
+            assert t["fused_near"] == int(fuse)
+        elif prec != _abi.PREC_MIXED:
+            assert t["fused_near"] == 0
         V = 10.0 ** np.random.default_rng(4).uniform(-12, 0, p.N)
         out.append(d.kernel(V))
     for other in out[1:]:
