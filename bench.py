@@ -38,7 +38,11 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 # (FETCH x 2.000, WRITE x 1.000); summaries and commands in
 # profiles/pmc_traffic_network80k_fp64.md and profiles/pmc_traffic_ensemble64_mixed.md.
 # Keyed by (workload, seed, phase 1 fused with the near-field panels).
-MEASURED_TRAFFIC = {("network800_80k", 2024, False): 2.4947e9}
+MEASURED_TRAFFIC = {
+    ("network800_80k", 2024, False): 2.4846e9,
+    ("ensemble64_1374k_mixed_learned", 2024, False): 21.246e9,
+    ("ensemble64_1374k_mixed_learned", 2024, True): 20.702e9,
+}
 YEAR = 365.25 * 86400.0
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 peak (vector and matrix cores alike)

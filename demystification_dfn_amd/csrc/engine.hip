@@ -1001,6 +1001,7 @@ API int dfn_tuning(dfn_handle h, int32_t *choice, double *ms) {
   choice[3] = d->panel_order;
   choice[4] = d->panel_split;
   choice[5] = d->fused ? 1 : 0;
+  choice[6] = d->kc;
   ms[0] = d->tune_ms_default;
   ms[1] = d->tune_ms;
   return DFN_OK;
@@ -1500,7 +1501,7 @@ API int dfn_bench_kernel(dfn_handle h, int32_t reps, double *ms) {
         dfn::launch_vtx_near(d->vtasks.as<dfn::VTask>(), d->nvt, static_cast<const float *>(d->vdata.p),
                              d->ptasks.as<dfn::PTask>(), d->npt, static_cast<const double *>(d->pdata.p),
                              d->widx.as<int>(), w, d->zpart.as<double>(), d->ypart.as<double>(), e.stream,
-                             d->panel_rows, nullptr, d->kmax);
+                             d->panel_rows, nullptr, d->kc);
       else
         dfn::launch_panels(*d, w, e.stream, d->panel_rows);
     }

@@ -96,6 +96,7 @@ struct DevOps {
   int pmode_b() const { return acc64 ? kStoreF32Acc64 : kStoreF32; }
   int nvt = 0, nzr = 0, npt = 0;
   int kmax = kMaxRank;  // largest rank of any phase-1 task
+  int kc = kMaxRank;    // rank ceiling of the phase-1 kernel instance in use (16, 20 or 32, >= kmax)
   double alg_bytes = 0;    // algorithmic HBM bytes per application (all operators, all phases)
   double panel_bytes = 0;  // algorithmic bytes of the phase-2 panel kernel (dense + U entries, y partials)
   // algorithmic bytes of launch_vtx_near (FP32 V entries, FP64 near-field entries,
@@ -163,7 +164,7 @@ inline void launch_phases(const DevOps &d, double *w, hipStream_t s, int waves, 
     if (kt) (void)hipEventRecord(kt[0], s);
     launch_vtx_near(d.vtasks.as<VTask>(), d.nvt, static_cast<const float *>(d.vdata.p), d.ptasks.as<PTask>(), d.npt,
                     static_cast<const double *>(d.pdata.p), d.widx.as<int>(), w, d.zpart.as<double>(),
-                    d.ypart.as<double>(), s, rows, active, d.kmax);
+                    d.ypart.as<double>(), s, rows, active, d.kc);
     if (kt) (void)hipEventRecord(kt[1], s);
     launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, s);
     launch_panel(d.ptasks_b.as<PTask>(), d.npt_b, d.pdata_b.p, d.pmode_b(), d.widx_b.as<int>(),
@@ -171,7 +172,7 @@ inline void launch_phases(const DevOps &d, double *w, hipStream_t s, int waves, 
     return;
   }
   launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.vmode(), w, d.zpart.as<double>(), s, waves, d.xf.as<float>(),
-             active, d.kmax);
+             active, d.kc);
   launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, s);
   if (kt) (void)hipEventRecord(kt[0], s);
   launch_panels(d, w, s, rows, active);
@@ -183,8 +184,10 @@ inline void launch_phases(const DevOps &d, double *w, hipStream_t s, int waves, 
 // task order (as built; grouped by rank bucket with the largest ranks first, so a
 // workgroup's waves share one code path; or the largest tasks, rows x rank, first);
 // 2 or 4 panel rows per wave step; the phase-2 schedule (as built, or largest
-// panels first, whole or cut into tasks of 32 or 16 rows); for a mixed-precision
-// layout, phase 1 and the near-field panels in one launch or in two.  Every candidate
+// panels first, whole or cut into tasks of 32 or 16 rows); the phase-1 kernel
+// instance (the smallest rank ceiling that covers the layout, or 32); for a
+// mixed-precision layout, phase 1 and the near-field panels in one launch or in
+// two.  Every candidate
 // computes every sum in the same order, so the choice never changes a result.
 // Each candidate is timed over phase 1, the z reduction and phase 2 (HIP events,
 // 2 warm-up + 8 timed applications); a candidate replaces the incumbent only if it
@@ -234,7 +237,7 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   double best = 0, t = 0;
   if (alt) {
     waves = 8; order = 2; rows = 4; pord = 1; split = 16;
-    d.fused = d.near_ok;
+    d.kc = kMaxRank;  // the rank-32 instance (the single launch is exercised by DFN_FUSE_NEAR=1)
     DFN_TRY(vorder(order));
     DFN_TRY(porder(pord, split));
   } else {
@@ -258,7 +261,14 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
       if (t < 0.99 * best) { best = t; pord = 1; split = cand; }
     }
     DFN_TRY(porder(pord, split));
-    if (d.near_ok) {
+    if (d.kc < kMaxRank) {  // the rank-32 instance: fewer waves resident, more loads in flight each
+      const int kc = d.kc;
+      d.kc = kMaxRank;
+      DFN_TRY(measure(waves, rows, t));
+      if (t < 0.99 * best) best = t;
+      else d.kc = kc;
+    }
+    if (d.near_ok && d.kc <= 20) {
       d.fused = true;
       DFN_TRY(measure(waves, rows, t));
       if (t < 0.99 * best) best = t;
@@ -576,6 +586,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   d.nzr = (int)zr.size();
   d.kmax = 1;
   for (const VTask &t : vt) d.kmax = std::max(d.kmax, (int)t.k);
+  d.kc = vtx_rank_ceiling(d.kmax);
   d.npt = (int)pt.size();
   d.S_total = S_total;
   d.S_panel = S_panel;
@@ -605,7 +616,10 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   // DFN_FUSE_NEAR=1 / 0 takes / refuses the single launch of phase 1 and the
   // near-field panels whatever the timing (the results are the same either way)
   const char *fuse = std::getenv("DFN_FUSE_NEAR");
-  if (fuse && d.near_ok) d.fused = std::string(fuse) == "1";
+  if (fuse && d.near_ok) {
+    d.fused = std::string(fuse) == "1";
+    if (d.fused) d.kc = vtx_rank_ceiling(d.kmax);
+  }
   return DFN_OK;
 }
 

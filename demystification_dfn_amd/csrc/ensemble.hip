@@ -1092,7 +1092,7 @@ API int dfn_ensemble_info(dfn_ensemble h, int64_t *info) {
   info[4] = d ? (int64_t)d->alg_bytes : 0;
   info[5] = e.use_graph ? 1 : 0;
   info[6] = d && d->fused ? 1 : 0;
-  info[7] = d ? dfn::vtx_rank_ceiling(d->kmax) : 0;
+  info[7] = d ? d->kc : 0;
   return DFN_OK;
   DFN_API_END
 }

@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 // moves.  The task's V is stored strip by strip (64 vectors of rows, column after
 // column; the last strip's height padded with zero rows to a whole vector), so the
 // K loads of one step read one contiguous run.
-template <int K, typename TV, bool ACC64>
+template <int K, typename TV, bool ACC64, bool GROUPS>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
                                          const float *__restrict__ xf, double *__restrict__ zpart,
                                          double *__restrict__ wz, int lane) {
@@ -146,13 +146,23 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
 #pragma unroll
       for (int u = 0; u < kVec; u++) xi[u] = (TA)xv[u];
     }
-    tvec v[K];
+    // FP32 factors with FP64 sums (GROUPS): columns in groups of 10, so that 10
+    // vectors of V are live at a time (128 instead of 138 VGPRs at ranks up to 20: 4
+    // waves per SIMD instead of 3); each column's sum is the same whatever the grouping
+    constexpr int G = (GROUPS && ACC64 && sizeof(TV) == 4 && K > 10) ? 10 : K;
 #pragma unroll
-    for (int j = 0; j < K; j++) v[j] = *reinterpret_cast<const tvec *>(Vs + j * hp);
+    for (int j0 = 0; j0 < K; j0 += G) {
+      tvec v[G];
 #pragma unroll
-    for (int j = 0; j < K; j++)
+      for (int j = 0; j < G; j++)
+        if (j0 + j < K) v[j] = *reinterpret_cast<const tvec *>(Vs + (j0 + j) * hp);
 #pragma unroll
-      for (int u = 0; u < kVec; u++) acc[j] += (TA)v[j][u] * xi[u];
+      for (int j = 0; j < G; j++)
+        if (j0 + j < K)
+#pragma unroll
+          for (int u = 0; u < kVec; u++) acc[j0 + j] += (TA)v[j][u] * xi[u];
+      if constexpr (G < K) __builtin_amdgcn_sched_barrier(0);  // keep the next group's loads after these sums
+    }
   }
   // all 64 lanes are active here: DPP wave totals, lane j keeps column j's
   TA mine = (TA)0;
@@ -172,13 +182,13 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
 // the largest rank of the layout rounded up to 16, 20 or 32: the register budget of
 // the whole kernel is that of its largest case, so a layout whose ranks stay at or
 // below 20 keeps more waves resident (FP64 storage 88 instead of 135 VGPRs)
-template <typename TV, bool ACC64, int KMAX>
+template <typename TV, bool ACC64, int KMAX, bool GROUPS>
 __device__ __forceinline__ void vtx_task(const VTask &task, const TV *__restrict__ vdata, double *__restrict__ w,
                                          const float *__restrict__ xf, double *__restrict__ zpart, int lane) {
   switch (task.k) {
 #define DFN_VTX_CASE(K) \
   case K:                                                                         \
-    if constexpr (K <= KMAX) vtx_body<K, TV, ACC64>(task, vdata, w, xf, zpart, w, lane); \
+    if constexpr (K <= KMAX) vtx_body<K, TV, ACC64, GROUPS>(task, vdata, w, xf, zpart, w, lane); \
     break;
     DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
     DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
@@ -203,7 +213,7 @@ __global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restric
   if (t >= ntasks) return;
   const VTask task = tasks[t];
   if (active && !active[task.net]) return;  // the block's network sits out this evaluation
-  vtx_task<TV, ACC64, KMAX>(task, vdata, w, xf, zpart, lane);
+  vtx_task<TV, ACC64, KMAX, true>(task, vdata, w, xf, zpart, lane);
 }
 
 // z of each multi-chunk low-rank block = sum of its chunk partials, in chunk
@@ -321,7 +331,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) v
   if (t >= nvt) return;
   const VTask task = vt[t];
   if (active && !active[task.net]) return;
-  vtx_task<float, true, KMAX>(task, vdata, w, nullptr, zpart, threadIdx.x & 63);
+  // held to 128 VGPRs as a whole; the column groups would only constrain its schedule
+  vtx_task<float, true, KMAX, false>(task, vdata, w, nullptr, zpart, threadIdx.x & 63);
 }
 
 // FP32 storage: entries are block / 2^e per block, the column scales 2^e fold into

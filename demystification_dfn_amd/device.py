@@ -47,10 +47,10 @@ class DeviceSimulation(Simulation):
 
     def tuning(self):
         """The autotuner's launch shapes for the active operator layout (dfn_tuning)."""
-        c, ms = np.zeros(6, np.int32), np.zeros(2)
+        c, ms = np.zeros(7, np.int32), np.zeros(2)
         _abi.check(self._fns["tuning"](self._h, _abi.i32ptr(c), _abi.dptr(ms)), "dfn_tuning")
         return dict(vtx_waves=int(c[0]), vtx_order=int(c[1]), panel_rows=int(c[2]), panel_order=int(c[3]),
-                    panel_split=int(c[4]), fused_near=int(c[5]), ms_default=float(ms[0]), ms_tuned=float(ms[1]))
+                    panel_split=int(c[4]), fused_near=int(c[5]), rank_ceiling=int(c[6]), ms_default=float(ms[0]), ms_tuned=float(ms[1]))
 
     def kernel_timing(self, on=True):
         """Start (clearing the sums) or stop timing the dominant kernel of every

@@ -206,8 +206,8 @@ int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats
  * (0 as built, 1 grouped by rank bucket, largest ranks first, 2 largest tasks
  * first), panel rows per wave step (2, 4), panel task order (0 as built, 1 largest
  * panels first), rows per panel task (0 whole panels, 32, 16), phase 1 and the
- * FP64 near-field panels of a DFN_PREC_MIXED layout in one launch (1) or two (0)}
- * (6 entries);
+ * FP64 near-field panels of a DFN_PREC_MIXED layout in one launch (1) or two (0),
+ * the rank ceiling of the phase-1 kernel instance (16, 20 or 32)} (7 entries);
  * ms = {matvec time of the default shapes, of the chosen ones}.  None of the
  * choices changes a result.  DFN_AUTOTUNE=0 keeps the defaults, DFN_AUTOTUNE=alt
  * takes a set of non-default shapes without timing; DFN_FUSE_NEAR=1 or 0 takes or
@@ -405,7 +405,7 @@ int dfn_ensemble_take_records(dfn_ensemble e, float *X, float *T, int64_t cap, i
 /* info[0..7] = networks, total elements, largest network, matrix entries streamed
  * per stress evaluation, algorithmic bytes per stress evaluation, graphs on, phase 1
  * and the FP64 near-field panels in one launch (1) or two (0), the rank ceiling of
- * the phase-1 kernel instance (16, 20 or 32). */
+ * the phase-1 kernel instance in use (16, 20 or 32). */
 int dfn_ensemble_info(dfn_ensemble e, int64_t *info);
 /* Per-launch timing of the dominant kernel while the ensemble steps (no reference
  * counterpart): the phase-2 panels, or phase 1 with the near-field panels when

@@ -39,6 +39,14 @@ class OracleSimulation(Simulation):
     def set_threads(self, n):
         _lib.dfno_handle_set_threads(self._h, int(n))
 
+    def set_cg_variant(self, extra=0, reverse=False):
+        """Test hook: every pressure solve runs `extra` CG iterations past its
+        stopping test (src/fluid_diffusion.f90:97-131) and, with `reverse`, sums
+        its dot products from the last element; (0, False) is the reference."""
+        _lib.dfno_debug_set_cg_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        _lib.dfno_debug_set_cg_variant.restype = C.c_int
+        assert _lib.dfno_debug_set_cg_variant(self._h, int(extra), 1 if reverse else 0) == 0
+
     def set_t_try(self, t_try):
         """Trial step of the solver loop (t_try, src/solver.f90:31)."""
         _lib.dfno_set_t_try.argtypes = [C.c_void_p, C.c_double]

@@ -57,6 +57,8 @@ typedef struct {
   double mu, cs, cp, por, fcomp, rcomp, visc, tol;
   double s11d, s12d, s22d, s31d, s32d;
   int coupling;
+  int cg_extra;             /* test hook: CG iterations past the stopping test (0 = the reference) */
+  int cg_reverse;           /* test hook: CG dot products summed from the last element (0 = the reference) */
   int nsrc;
   int64_t *src_fault, *src_idx;
   double *Q, *tb, *te;

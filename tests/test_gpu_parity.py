@@ -190,6 +190,9 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
             assert [t[k] for k in keys] == [4, 0, 2, 0, 0]
         if tune == "alt":
             assert [t[k] for k in keys] == [8, 2, 4, 1, 16]
+        assert t["rank_ceiling"] in (16, 20, 32)
+        if tune == "alt":
+            assert t["rank_ceiling"] == 32
         if fuse is not None:
             assert t["fused_near"] == int(fuse)
         elif prec != _abi.PREC_MIXED:
