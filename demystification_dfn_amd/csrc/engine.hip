@@ -84,21 +84,28 @@ struct Engine {
   StatusWord *h_status = nullptr;  // pinned
   int *h_cgres = nullptr;          // pinned [kSlots*2]
   // CG systems: slots 0-5 are compute_P(t_try) and the five stage abscissae of a
-  // first attempt; slots 6-11 the same for t_try / 2, solved beside them so that a
-  // failed t_try needs no new launch (the reference halves t_try and solves again)
-  static constexpr int kSlots = 12;
-  int slot_base = 0;  // slot set the current attempt's stages read
-  bool halved_last = false;  // the last step began by halving t_try
-  // twelve systems fit one launch: the one-workgroup kernels always, the cooperative
-  // kernel when all their workgroups can be resident together
+  // first attempt; slots 6g..6g+5 the same for t_try / 2^g (rung g of a ladder of up
+  // to kLevels), solved beside them so that a failed t_try needs no new launch while
+  // the ladder lasts (the reference halves t_try and solves again).  A failing solve
+  // runs its 200 iterations whatever else shares the launch, so the lower rungs cost
+  // no time as long as their workgroups are resident together.
+  static constexpr int kLevels = 4;
+  static constexpr int kSlots = 6 * kLevels;
+  int slot_base = 0;        // slot set the current attempt's stages read
+  int last_halvings = 0;    // halvings of t_try at the start of the last step
   int coop_capacity = -1;
-  bool spec_fits() {
+  // rungs (at most want, at least 1) whose 6 systems each fit one launch together:
+  // the one-workgroup kernels always, the cooperative kernel when all their
+  // workgroups can be resident together
+  int ladder_levels(int want) {
+    want = std::max(1, std::min(want, kLevels));
     const int N = hp.N;
-    if (N <= kCgLdsMax && cg_path <= 1) return true;
-    if (!coop_ok || cg_path == 3) return true;  // plain kernel: one workgroup per system
+    if (N <= kCgLdsMax && cg_path <= 1) return want;
+    if (!coop_ok || cg_path == 3) return want;  // plain kernel: one workgroup per system
     if (coop_capacity < 0) coop_capacity = cg_coop_capacity(span_entries);
     const int wps = (N + kCgSpan - 1) / kCgSpan;
-    return 12 * wps <= coop_capacity;
+    while (want > 1 && 6 * want * wps > coop_capacity) want--;
+    return want;
   }
   int last_pd_slot = -1;
   double time = 0, t_try = 100.0;
@@ -645,19 +652,20 @@ struct Engine {
     // step depends on dt alone, so the five stage solves of a first attempt with
     // h = t_try (x + A_i h, x = 0) run in the same launch.
     const bool batch = hp.coupling != 1;
-    // the solves for t_try / 2 run in the same launch when they fit beside the others
-    // and the last step had to halve (otherwise they would only add work)
-    const bool spec = batch && halved_last && spec_fits();
+    // the solves for t_try / 2, / 4, ... run in the same launch when they fit beside
+    // the others: one rung more than the last step needed halvings (none when it
+    // needed none: they would only add work)
+    const int levels = batch ? ladder_levels(1 + last_halvings) : 1;
     const int64_t halvings0 = st.cg_halvings;
     auto launch_try = [&](hipStream_t cs) -> int {
       if (!batch) return cg_launch_on(&t_try, 1, 0, cs);
-      double dts[12];
-      for (int g = 0; g < 2; g++) {
-        const double h = g == 0 ? t_try : t_try / 2.0;
+      double dts[kSlots];
+      double h = t_try;
+      for (int g = 0; g < levels; g++, h = h / 2.0) {  // the values the halving below gives
         const double row[6] = {h, 0.0 + 0.2 * h, 0.0 + 0.3 * h, 0.0 + 0.6 * h, 0.0 + 1.0 * h, 0.0 + 0.875 * h};
         for (int s = 0; s < 6; s++) dts[6 * g + s] = row[s];
       }
-      return cg_launch_on(dts, spec ? 12 : 6, 0, cs);
+      return cg_launch_on(dts, 6 * levels, 0, cs);
     };
     // The first batch of pressure solves runs on the side stream beside the k1
     // stress evaluation, which needs only Y (the cooperative CG too: its launch
@@ -675,7 +683,7 @@ struct Engine {
     // before compute_P(t_try)'s result is read (one host synchronisation less per
     // step); if that solve failed, the attempt's results are dropped and the step
     // halves as below, as the reference does before its k1 evaluation.
-    const bool optimistic = batch && !spec && !halved_last;
+    const bool optimistic = batch && levels == 1 && last_halvings == 0;
     bool attempted = false;
     if (optimistic) {
       StageArgs s1 = stage_args(Y.as<double>(), 0, ytA.as<double>(), t_try);
@@ -689,21 +697,24 @@ struct Engine {
     } else {
       DFN_TRY(cg_collect(0, 1, &fail));
     }
+    int rung = 0;  // the ladder rung whose solves belong to the current t_try
     while (fail) {  // :86-90: halve t_try until compute_P converges
       t_try = t_try / 2.0;
       st.cg_halvings++;
-      if (spec && slot_base == 0) {  // compute_P(t_try) already ran in slots 6-11
-        slot_base = 6;
-        DFN_TRY(cg_collect(6, 1, &fail));
+      if (rung + 1 < levels) {  // compute_P(t_try) already ran on the next rung
+        rung++;
+        slot_base = 6 * rung;
+        DFN_TRY(cg_collect(slot_base, 1, &fail));
         continue;
       }
+      rung = 0;  // a new ladder from this t_try
       slot_base = 0;
       DFN_TRY(launch_try(stream));
       DFN_TRY(cg_collect(0, 1, &fail));
     }
-    halved_last = st.cg_halvings > halvings0;
+    last_halvings = (int)(st.cg_halvings - halvings0);
     if (!attempted) {
-      if (batch) st.cg_solves -= spec ? 11 : 5;  // the stages' solves are counted when used
+      if (batch) st.cg_solves -= 6 * levels - 1;  // the stages' solves are counted when used
       // k1 = f(t_try, Y) and yscal (:101-102): the stress rates are those evaluated
       // above (again if a dropped attempt overwrote them); compute_P(t_try) of the
       // RHS equals the call above unless the permeability evolves between calls
