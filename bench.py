@@ -161,13 +161,16 @@ def mixed_precision(sim, p, args, kb64):
     """Config 3 on a single network: FP32 operator storage (per-block power-of-two
     scales, FP32 accumulation, compensated near-field sums), the same storage read
     into FP64 products and sums, and FP64 near-field blocks beside FP32 low-rank
-    factors read into FP64 sums (the mixed mode), each against the FP64 operator over
+    factors read into FP64 sums (the mixed mode), and the mixed mode with each
+    near-field entry split into an FP32 value and an FP16 residual (6 bytes), each
+    against the FP64 operator over
     random log-uniform slip rates in [1e-12, 1] m/s (norm-wise and on the rows above
     1e-3 of the largest stress rate); then FP32 solver steps continued from the
     FP64 run's state.  State, RHS, CG and error norm stay FP64 throughout."""
     from demystification_dfn_amd import _abi
     rng = np.random.default_rng(7)
-    errs = {_abi.PREC_FP32: ([], []), _abi.PREC_MIXED: ([], []), _abi.PREC_FP32_STORE: ([], [])}
+    errs = {_abi.PREC_FP32: ([], []), _abi.PREC_MIXED: ([], []), _abi.PREC_FP32_STORE: ([], []),
+            _abi.PREC_MIXED_SPLIT: ([], [])}
     # the FP64 stress rates of every vector in one multi-RHS application
     X = 10.0 ** rng.uniform(-12.0, 0.0, (args.mixed_vectors, p.N))
     w0 = time.perf_counter()
@@ -178,7 +181,7 @@ def mixed_precision(sim, p, args, kb64):
     log(f"[bench] FP64 multi-RHS over {multi['vectors']} vectors: {multi['ms']:.1f} ms, phase 2 at "
         f"{multi['panel_TFLOP_s']:.1f} TFLOP/s")
     # one precision switch each (a switch rebuilds the reduced-precision layout)
-    for prec in (_abi.PREC_FP32, _abi.PREC_FP32_STORE, _abi.PREC_MIXED):
+    for prec in (_abi.PREC_FP32, _abi.PREC_FP32_STORE, _abi.PREC_MIXED, _abi.PREC_MIXED_SPLIT):
         sim.set_precision(prec)
         log(f"[bench] stress-rate error over {args.mixed_vectors} vectors, precision {prec}")
         for r in range(args.mixed_vectors):
@@ -195,6 +198,8 @@ def mixed_precision(sim, p, args, kb64):
                     significant_rows_max=float(np.max(g)), significant_rows_median=float(np.median(g)))
     sim.set_precision(_abi.PREC_MIXED)
     kbm = sim.bench_kernel(reps=20)
+    sim.set_precision(_abi.PREC_MIXED_SPLIT)
+    kbs = sim.bench_kernel(reps=20)
     sim.set_precision(_abi.PREC_FP32)
     tune32 = sim.tuning()
     kb = sim.bench_kernel(reps=20)
@@ -216,6 +221,12 @@ def mixed_precision(sim, p, args, kb64):
                                      hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kbm["ms_kernel"],
                                      note="FP64 near-field blocks beside FP32 low-rank factors read into FP64 "
                                           "products and sums (DFN_PREC_MIXED)"),
+                split_near_field=dict(stress_rate_rel_err=stats(_abi.PREC_MIXED_SPLIT), hmatvec_ms=kbs["ms_kernel"],
+                                      alg_bytes=kbs["alg_bytes"],
+                                      hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kbs["ms_kernel"],
+                                      note="DFN_PREC_MIXED with each near-field entry stored as its FP32 value and "
+                                           "an FP16 residual times a power-of-two block scale "
+                                           "(DFN_PREC_MIXED_SPLIT)"),
                 fp32_storage_fp64_sums=dict(stress_rate_rel_err=stats(_abi.PREC_FP32_STORE),
                                             note="the FP32 storage read into FP64 products and sums "
                                                  "(DFN_PREC_FP32_STORE): the error of the storage alone"),

@@ -35,7 +35,7 @@ ERROR_NAMES = {
 MODE_II, MODE_III = 2, 3
 KERNEL_DIRECT, KERNEL_HMATRIX = 0, 1
 OP_T, OP_N = 0, 1
-PREC_FP64, PREC_FP32, PREC_MIXED, PREC_FP32_STORE = 0, 1, 2, 3
+PREC_FP64, PREC_FP32, PREC_MIXED, PREC_FP32_STORE, PREC_MIXED_SPLIT = 0, 1, 2, 3, 4
 STEP_RKQS, STEP_PI, STEP_LEARNED = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)

@@ -170,6 +170,11 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
 void launch_vtx_near(const VTask *vt, int nvt, const float *vdata, const PTask *pt, int npt, const double *pdata,
                      const int *widx, double *w, double *zpart, double *ypart, hipStream_t s, int rows = 2,
                      const int *active = nullptr, int kmax = kMaxRank);
+// near-field panels stored as FP32 values with FP16 residuals (hi, lo; cscale the
+// residual's power-of-two scale per column); rows and active as for launch_panel
+void launch_panel_split(const PTask *t, int nt, const float *hi, const _Float16 *lo, const int *widx,
+                        const float *cscale, const double *w, double *ypart, hipStream_t s, int rows = 2,
+                        const int *active = nullptr);
 // rows: panel rows per wave step, 2 or 4; active as for launch_vtx
 void launch_panel(const PTask *t, int nt, const void *pdata, int mode, const int *widx, const float *cscale,
                   const double *w, double *ypart, hipStream_t s, int rows = 2, const int *active = nullptr);

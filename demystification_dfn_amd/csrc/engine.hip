@@ -285,8 +285,8 @@ struct Engine {
     const int nops = hp.mode == DFN_MODE_II ? 2 : 1;
     const HostOperator *list[2] = {&hop[0], &hop[1]};
     ops32.reset(new DevOps);
-    DFN_TRY(layout_ops<float>(list, nops, *ops32, nullptr, precision == DFN_PREC_MIXED,
-                                   precision == DFN_PREC_FP32_STORE || precision == DFN_PREC_MIXED));
+    const bool split = precision == DFN_PREC_MIXED_SPLIT, mixed = precision == DFN_PREC_MIXED || split;
+    DFN_TRY(layout_ops<float>(list, nops, *ops32, nullptr, mixed, precision == DFN_PREC_FP32_STORE || mixed, split));
     ops32_kind = precision;
     return DFN_OK;
   }
@@ -994,7 +994,7 @@ API int dfn_build_hmat(dfn_handle h) {
 
 API int dfn_set_precision(dfn_handle h, int32_t prec) {
   DFN_API_BEGIN
-  if (!h || (prec < DFN_PREC_FP64 || prec > DFN_PREC_FP32_STORE)) return DFN_ERR_ARG;
+  if (!h || (prec < DFN_PREC_FP64 || prec > DFN_PREC_MIXED_SPLIT)) return DFN_ERR_ARG;
   h->e.precision = prec;
   if (prec != DFN_PREC_FP64 && h->e.ops32_kind != prec) return h->e.layout_reduced();
   return DFN_OK;

@@ -82,6 +82,10 @@ struct DevOps {
   // set A holds the near-field (dense) columns in FP64 and set B the U columns in FP32
   bool pa_fp32 = false;
   DMem ptasks_b, pdata_b, widx_b, cscale_b;
+  // split: set A holds FP32 values (pdata) and FP16 residuals (pdata_lo) of the
+  // near-field entries, cscale the residual's power-of-two scale per column
+  bool split = false;
+  DMem pdata_lo;
   int npt_b = 0;
   DMem xf;  // FP32 operators: FP32 copy of x_h read by phase 1 (n + 4 floats, the tail zero)
   // FP32 storage read into FP64 products and sums (DFN_PREC_FP32_STORE, and the
@@ -149,8 +153,12 @@ inline void launch_mrhs(const DevOps &d, const double *X, int nrhs, int R, doubl
 // Phase 2 of a stress evaluation: every panel set of the layout (rows: panel rows
 // per wave step; active as for launch_panel)
 inline void launch_panels(const DevOps &d, const double *w, hipStream_t s, int rows, const int *active = nullptr) {
-  launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.pmode_a(), d.widx.as<int>(), d.cscale.as<float>(), w,
-               d.ypart.as<double>(), s, rows, active);
+  if (d.split)
+    launch_panel_split(d.ptasks.as<PTask>(), d.npt, d.pdata.as<float>(), d.pdata_lo.as<_Float16>(), d.widx.as<int>(),
+                       d.cscale.as<float>(), w, d.ypart.as<double>(), s, rows, active);
+  else
+    launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.pmode_a(), d.widx.as<int>(), d.cscale.as<float>(), w,
+                 d.ypart.as<double>(), s, rows, active);
   if (d.npt_b > 0)
     launch_panel(d.ptasks_b.as<PTask>(), d.npt_b, d.pdata_b.p, d.pmode_b(), d.widx_b.as<int>(), d.cscale_b.as<float>(), w,
                  d.ypart.as<double>(), s, rows, active);
@@ -324,13 +332,16 @@ inline double pow2_scale(const double *v, size_t n) {
 // phase-2 task records the network its block or panel belongs to, so that the
 // kernels can skip the tasks of networks that sit out an evaluation.
 // T: storage of the low-rank factors (and of the near-field blocks unless dense64,
-// which keeps them in FP64 in their own panel set).
+// which keeps them in their own panel set: in FP64, or with split as an FP32 value
+// and an FP16 residual per entry).
 template <class T>
 int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::vector<int> *hnet = nullptr,
-               bool dense64 = false, bool acc64 = false) {
+               bool dense64 = false, bool acc64 = false, bool split = false) {
   const bool scaled = sizeof(T) == 4;  // FP32 storage keeps each block divided by its power-of-two scale
   dense64 = dense64 && scaled;
+  split = split && dense64;
   d.acc64 = acc64 && scaled;
+  d.split = split;
   const int n = ops[0]->n;
   for (int o = 1; o < nops; o++)
     if (ops[o]->perm != ops[0]->perm) return DFN_ERR_UNSUPPORTED;  // T and N share the cluster tree
@@ -436,6 +447,16 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
         parallel_for((int64_t)H.d_m.size(), [&](int64_t b) {
           dscale[o][b] = pow2_scale(H.d_data.data() + H.d_off[b], (size_t)H.d_m[b] * H.d_n[b]);
         });
+      if (split)  // the residual's scale: the largest |entry - (float)entry| of the block near 2^14
+        parallel_for((int64_t)H.d_m.size(), [&](int64_t b) {
+          const double *src = H.d_data.data() + H.d_off[b];
+          double rmax = 0.0;
+          for (size_t i = 0, e = (size_t)H.d_m[b] * H.d_n[b]; i < e; i++)
+            rmax = std::fmax(rmax, std::fabs(src[i] - (double)(float)src[i]));
+          int ex = 0;
+          if (rmax > 0.0) (void)std::frexp(rmax, &ex);  // rmax in [2^(ex-1), 2^ex)
+          dscale[o][b] = rmax > 0.0 ? std::ldexp(1.0, ex - 15) : 1.0;  // |residual / scale| < 2^15 < 65504
+        });
       parallel_for((int64_t)H.l_m.size(), [&](int64_t b) {
         uscale[o][b] = pow2_scale(H.l_u.data() + H.l_uoff[b], (size_t)H.l_m[b] * H.l_k[b]);
       });
@@ -447,14 +468,17 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   struct PFill { int64_t poff, col0; int L, Lp, m; };
   struct PanelSet {
     int esize;    // bytes per stored entry
-    bool scales;  // FP32 storage: per-column block scales
+    bool scales;  // per-column scales: FP32 storage's block scales, or the split form's residual scales
+    int pad;      // rows padded to a multiple of this many entries (128 bytes of every stream)
     std::vector<PTask> pt;
     std::vector<int> widx;
     std::vector<float> cscale;
     std::vector<PFill> pfill;
     int64_t psize = 0, entries = 0;
   };
-  PanelSet setA{dense64 ? 8 : (int)sizeof(T), scaled && !dense64}, setB{(int)sizeof(T), scaled};
+  PanelSet setA{split ? 6 : dense64 ? 8 : (int)sizeof(T), scaled && (!dense64 || split),
+                split ? 64 : dense64 ? 16 : 128 / (int)sizeof(T)},
+      setB{(int)sizeof(T), scaled, 128 / (int)sizeof(T)};
   std::vector<Col> cols;
   std::vector<std::vector<int>> fin((size_t)n * nops);
   int yp = 0;
@@ -465,7 +489,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
     // every row starts on a 128-byte line: a line shared by two rows is fetched
     // twice by the streaming loads (9 % more bytes fetched and 5-10 % more time
     // with 16-byte rows); a zero entry adds an exact zero to its lane's sum
-    const int pad = 128 / S.esize;
+    const int pad = S.pad;
     S.entries += ncols * m;
     for (int64_t q0 = 0; q0 < ncols; q0 += kPanelMaxCols) {
       const int L = (int)std::min<int64_t>(kPanelMaxCols, ncols - q0);
@@ -538,7 +562,29 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
       }
     });
   };
-  if (dense64) {
+  if (split) {  // near-field entries as hi = (float)entry and lo = (entry - hi) / scale in FP16
+    std::unique_ptr<float[]> ph(new float[std::max<int64_t>(setA.psize, 1)]);
+    std::unique_ptr<_Float16[]> pl(new _Float16[std::max<int64_t>(setA.psize, 1)]);
+    parallel_for((int64_t)setA.pfill.size(), [&](int64_t q) {
+      const PFill &f = setA.pfill[q];
+      const Col *c = cols.data() + f.col0;
+      for (int r = 0; r < f.m; r++) {
+        float *hr = ph.get() + f.poff + (int64_t)r * f.Lp;
+        _Float16 *lr = pl.get() + f.poff + (int64_t)r * f.Lp;
+        for (int j = 0; j < f.Lp; j++) {
+          const double v = j < f.L ? c[j].src[r] : 0.0;
+          const float h = (float)v;
+          hr[j] = h;
+          lr[j] = (_Float16)((v - (double)h) / (j < f.L ? c[j].scale : 1.0));
+        }
+      }
+    });
+    std::unique_ptr<T[]> pb(new T[std::max<int64_t>(setB.psize, 1)]);
+    fill(setB, pb.get());
+    DFN_TRY(d.pdata.upload(ph.get(), (size_t)setA.psize));
+    DFN_TRY(d.pdata_lo.upload(pl.get(), (size_t)setA.psize));
+    DFN_TRY(d.pdata_b.upload(pb.get(), (size_t)setB.psize));
+  } else if (dense64) {
     std::unique_ptr<double[]> pa(new double[std::max<int64_t>(setA.psize, 1)]);
     std::unique_ptr<T[]> pb(new T[std::max<int64_t>(setB.psize, 1)]);
     fill(setA, pa.get());
@@ -591,13 +637,13 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   d.S_total = S_total;
   d.S_panel = S_panel;
   // bytes: factors + x read + y written, per operator (x shared in Mode II)
-  const double dense_extra = dense64 ? (8.0 - (double)sizeof(T)) * (double)S_dense : 0.0;
+  const double dense_extra = dense64 ? ((split ? 6.0 : 8.0) - (double)sizeof(T)) * (double)S_dense : 0.0;
   d.alg_bytes = (double)sizeof(T) * (double)S_total + dense_extra + 8.0 * n + 8.0 * n * nops;
   d.panel_bytes = (double)setA.esize * (double)setA.entries + (double)setB.esize * (double)setB.entries +
                   4.0 * (double)(setA.cscale.size() + setB.cscale.size()) + 8.0 * (double)yp;
   // one launch for phase 1 and the near-field panels where its rank ceiling keeps it
   // at 128 VGPRs without spills (ranks up to 20)
-  d.near_ok = dense64 && d.acc64 && vtx_rank_ceiling(d.kmax) <= 20;
+  d.near_ok = dense64 && d.acc64 && !split && vtx_rank_ceiling(d.kmax) <= 20;
   if (d.near_ok) {
     int64_t S_V = 0, nz = 0;  // nz: z entries written straight to w (single-chunk blocks)
     for (int o = 0; o < nops; o++) {

@@ -545,8 +545,8 @@ struct EnsEngine {
     std::vector<int> hnet(Ntot);
     for (int i = 0; i < B; i++) std::fill(hnet.begin() + base[i], hnet.begin() + base[i] + hp[i].N, i);
     ops32.reset(new DevOps);
-    DFN_TRY(layout_ops<float>(list, nops, *ops32, &hnet, precision == DFN_PREC_MIXED,
-                                   precision == DFN_PREC_FP32_STORE || precision == DFN_PREC_MIXED));
+    const bool split = precision == DFN_PREC_MIXED_SPLIT, mixed = precision == DFN_PREC_MIXED || split;
+    DFN_TRY(layout_ops<float>(list, nops, *ops32, &hnet, mixed, precision == DFN_PREC_FP32_STORE || mixed, split));
     ops32_kind = precision;
     return DFN_OK;
   }
@@ -933,7 +933,7 @@ API int dfn_ensemble_build_hmat(dfn_ensemble h) {
 
 API int dfn_ensemble_set_precision(dfn_ensemble h, int32_t prec) {
   DFN_API_BEGIN
-  if (!h || (prec < DFN_PREC_FP64 || prec > DFN_PREC_FP32_STORE)) return DFN_ERR_ARG;
+  if (!h || (prec < DFN_PREC_FP64 || prec > DFN_PREC_MIXED_SPLIT)) return DFN_ERR_ARG;
   h->e.precision = prec;
   if (prec != DFN_PREC_FP64 && h->e.ops32_kind != prec) return h->e.layout_reduced();
   h->e.drop_graphs();

@@ -456,6 +456,72 @@ __global__ void __launch_bounds__(256) panel_kernel_f32w(const PTask *__restrict
   }
 }
 
+// Phase 2 over near-field entries stored as an FP32 value and an FP16 residual:
+// entry = hi + lo * s, with hi = (float)entry, lo the FP16 rounding of
+// (entry - hi) / s and s a power of two per block (folded into a second gathered
+// input, w * s, exact since s is a power of two).  Every product and sum is FP64;
+// an entry keeps about 35 significant bits in 6 bytes.
+typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
+
+template <int R>
+__global__ void __launch_bounds__(256) panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi,
+                                                          const _Float16 *__restrict__ lo,
+                                                          const int *__restrict__ widx,
+                                                          const float *__restrict__ cscale,
+                                                          const double *__restrict__ w, double *__restrict__ ypart,
+                                                          const int *__restrict__ active) {
+  __shared__ double2 wl2[kPanelMaxCols / 2];
+  __shared__ double2 sl2[kPanelMaxCols / 2];
+  double *wl = reinterpret_cast<double *>(wl2), *sl = reinterpret_cast<double *>(sl2);
+  const PTask task = tasks[blockIdx.x];
+  if (active && !active[task.net]) return;  // the panel's network sits out this evaluation
+  for (int j = threadIdx.x; j < task.L; j += 256) {
+    const double v = w[widx[task.widx + j]];
+    wl[j] = v;
+    sl[j] = v * (double)cscale[task.widx + j];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int L4 = task.L >> 2;
+  const fv4 *H = reinterpret_cast<const fv4 *>(hi + task.poff);
+  const hv4 *Lo = reinterpret_cast<const hv4 *>(lo + task.poff);
+  for (int r = wave * R; r < task.m; r += 4 * R) {
+    const fv4 *hrow[R];
+    const hv4 *lrow[R];
+    double a[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int rr = r + q < task.m ? r + q : r;  // rows past the panel re-read its row r and are not stored
+      hrow[q] = H + (int64_t)rr * L4;
+      lrow[q] = Lo + (int64_t)rr * L4;
+      a[q] = 0.0;
+    }
+#pragma unroll 2
+    for (int j = lane; j < L4; j += 64) {
+      const double2 w01 = wl2[2 * j], w23 = wl2[2 * j + 1], s01 = sl2[2 * j], s23 = sl2[2 * j + 1];
+      fv4 h[R];
+      hv4 l[R];
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        h[q] = __builtin_nontemporal_load(hrow[q] + j);
+        l[q] = __builtin_nontemporal_load(lrow[q] + j);
+      }
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        a[q] = fma((double)h[q].w, w23.y, fma((double)h[q].z, w23.x, fma((double)h[q].y, w01.y,
+                   fma((double)h[q].x, w01.x, a[q]))));
+        a[q] = fma((double)l[q].w, s23.y, fma((double)l[q].z, s23.x, fma((double)l[q].y, s01.y,
+                   fma((double)l[q].x, s01.x, a[q]))));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const double t = wave_sum(a[q]);
+      if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
+    }
+  }
+}
+
 // y(perm(i)) = scale * sum of the partials of H row i (one thread per row and operator).
 __global__ void __launch_bounds__(256) finalize_kernel(const int *__restrict__ fin_first, const int *__restrict__ fin_idx,
                                                        const int *__restrict__ perm, const double *__restrict__ ypart,
@@ -2017,6 +2083,16 @@ void launch_panel(const PTask *t, int nt, const void *pdata, int mode, const int
   else
     hipLaunchKernelGGL(panel_kernel_f64<2>, dim3(nt), dim3(256), 0, s, t, (const double *)pdata, widx, w, ypart,
                        active);
+}
+
+void launch_panel_split(const PTask *t, int nt, const float *hi, const _Float16 *lo, const int *widx,
+                        const float *cscale, const double *w, double *ypart, hipStream_t s, int rows,
+                        const int *active) {
+  if (nt <= 0) return;
+  if (rows == 4)
+    hipLaunchKernelGGL(panel_kernel_split<4>, dim3(nt), dim3(256), 0, s, t, hi, lo, widx, cscale, w, ypart, active);
+  else
+    hipLaunchKernelGGL(panel_kernel_split<2>, dim3(nt), dim3(256), 0, s, t, hi, lo, widx, cscale, w, ypart, active);
 }
 
 void launch_finalize(const int *fin_first, const int *fin_idx, const int *perm, const double *ypart, int n, int nops,

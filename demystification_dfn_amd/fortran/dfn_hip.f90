@@ -23,6 +23,7 @@ integer(c_int), parameter, public :: DFN_OK = 0, DFN_MODE_II = 2, DFN_MODE_III =
 integer(c_int), parameter, public :: DFN_KERNEL_DIRECT = 0, DFN_KERNEL_HMATRIX = 1
 integer(c_int), parameter, public :: DFN_ERR_CG_TIMEOUT = 9
 integer(c_int), parameter, public :: DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1, DFN_PREC_MIXED = 2, DFN_PREC_FP32_STORE = 3
+integer(c_int), parameter, public :: DFN_PREC_MIXED_SPLIT = 4
 integer(c_int), parameter, public :: DFN_STEP_RKQS = 0, DFN_STEP_PI = 1, DFN_STEP_LEARNED = 2
 integer(c_int), parameter, public :: DFN_MLP_INPUTS = 8, DFN_MLP_PARAMS = 4801
 

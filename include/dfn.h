@@ -48,7 +48,7 @@ enum { DFN_OP_T = 0, DFN_OP_N = 1 };                      /* shear (Mode II/III)
 /* storage of the operator blocks: all FP64; all FP32 (power-of-two block scales,
  * FP32 accumulation, compensated near-field sums); FP32 low-rank factors beside
  * FP64 near-field (dense) blocks */
-enum { DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1, DFN_PREC_MIXED = 2, DFN_PREC_FP32_STORE = 3 };
+enum { DFN_PREC_FP64 = 0, DFN_PREC_FP32 = 1, DFN_PREC_MIXED = 2, DFN_PREC_FP32_STORE = 3, DFN_PREC_MIXED_SPLIT = 4 };
 
 /* Problem description: the content of problems/<name>/config
  * (src/configuration.f90:70-379) as flat arrays.  Per-element arrays have
@@ -114,7 +114,10 @@ int dfn_build_hmat(dfn_handle h);
  * DFN_PREC_MIXED: FP64 near-field blocks beside FP32 low-rank factors (stored as
  *   for DFN_PREC_FP32) read into FP64 products and sums;
  * DFN_PREC_FP32_STORE: the FP32 storage of DFN_PREC_FP32 (same bytes) read into
- *   FP64 products and sums, so that only the storage rounds.
+ *   FP64 products and sums, so that only the storage rounds;
+ * DFN_PREC_MIXED_SPLIT: as DFN_PREC_MIXED, with each near-field entry stored in
+ *   6 bytes instead of 8: its FP32 rounding and the FP16 residual times a
+ *   power-of-two scale per block (about 35 significant bits).
  * No reference counterpart (the reference is FP64 throughout). */
 int dfn_set_precision(dfn_handle h, int32_t precision);
 /* stats[0..9] = {nb_dense, nb_lowrank, S_dense, S_lowrank_U, S_lowrank_V, max_rank,

@@ -68,7 +68,7 @@ def test_ensemble_mode3_equals_independent_runs():
     _compare(_nets(3, n=4, seed=5), 6)
 
 
-@pytest.mark.parametrize("prec", [_abi.PREC_FP32, _abi.PREC_MIXED, _abi.PREC_FP32_STORE])
+@pytest.mark.parametrize("prec", [_abi.PREC_FP32, _abi.PREC_MIXED, _abi.PREC_FP32_STORE, _abi.PREC_MIXED_SPLIT])
 def test_ensemble_reduced_operators_equal_independent_runs(prec):
     _compare(_nets(2, n=4, seed=9), 5, prec=prec)
 

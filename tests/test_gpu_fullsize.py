@@ -90,14 +90,16 @@ def test_full_size_mixed_operator_error(pair):
     """The 80k network with FP64 near-field blocks and FP32 low-rank factors read
     into FP64 sums: the stress-rate error over 20 log-uniform slip-rate vectors,
     norm-wise and on the rows above 1e-3 of the largest stress rate (bar 1e-5),
-    beside all-FP32 storage.  The FP64 references come from one multi-RHS call."""
+    beside all-FP32 storage, and with the near-field entries split into an FP32
+    value and an FP16 residual (DFN_PREC_MIXED_SPLIT).  The FP64 references come
+    from one multi-RHS call."""
     p, d, _ = pair
     rng = np.random.default_rng(13)
     X = 10.0 ** rng.uniform(-12.0, 0.0, (20, p.N))
     Y64 = d.kernel_multi(X)
-    worst = {_abi.PREC_FP32: [0.0, 0.0], _abi.PREC_MIXED: [0.0, 0.0]}
+    worst = {_abi.PREC_FP32: [0.0, 0.0], _abi.PREC_MIXED: [0.0, 0.0], _abi.PREC_MIXED_SPLIT: [0.0, 0.0]}
     try:
-        for prec in (_abi.PREC_FP32, _abi.PREC_MIXED):
+        for prec in (_abi.PREC_FP32, _abi.PREC_MIXED, _abi.PREC_MIXED_SPLIT):
             d.set_precision(prec)
             for r in range(len(X)):
                 for y, y64 in zip(d.kernel(X[r]), (Y64[0][r], Y64[1][r])):
@@ -108,9 +110,10 @@ def test_full_size_mixed_operator_error(pair):
     finally:
         d.set_precision(_abi.PREC_FP64)
     print(f"[gpu] full-size norm-wise / significant-row errors: FP32 {worst[_abi.PREC_FP32]}, "
-          f"mixed {worst[_abi.PREC_MIXED]}", flush=True)
-    assert worst[_abi.PREC_MIXED][0] <= 1e-6
-    assert worst[_abi.PREC_MIXED][1] <= 1e-5
+          f"mixed {worst[_abi.PREC_MIXED]}, split near-field {worst[_abi.PREC_MIXED_SPLIT]}", flush=True)
+    for prec in (_abi.PREC_MIXED, _abi.PREC_MIXED_SPLIT):
+        assert worst[prec][0] <= 1e-6
+        assert worst[prec][1] <= 1e-5
 
 
 @pytest.fixture(scope="module")
