@@ -8,6 +8,7 @@ namespace dfn {
 
 constexpr int kMaxRank = 32;          // largest low-rank block rank the phase-1 kernel keeps in registers
 constexpr int kPanelMaxCols = 2048;   // columns per phase-2 panel task (w staged in 16 KB of LDS)
+constexpr int kSplitMaxCols = 1024;   // columns per task of split near-field panels (w in FP64 and FP32: 12 KB)
 constexpr int kVChunkRows = 512;      // rows of V per phase-1 wave task
 constexpr int kXPad = 4;              // zero entries after x_h in w: phase 1 reads whole 16-byte vectors
 constexpr int kCgMaxIter = 200;       // compute_P gives up after 200 iterations (fluid_diffusion.f90:120)

@@ -83,7 +83,8 @@ struct DevOps {
   bool pa_fp32 = false;
   DMem ptasks_b, pdata_b, widx_b, cscale_b;
   // split: set A holds FP32 values (pdata) and FP16 residuals (pdata_lo) of the
-  // near-field entries, cscale the residual's power-of-two scale per column
+  // near-field entries, cscale the residual's power-of-two scale of each task
+  // (repeated over the task's columns)
   bool split = false;
   DMem pdata_lo;
   int npt_b = 0;
@@ -447,16 +448,6 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
         parallel_for((int64_t)H.d_m.size(), [&](int64_t b) {
           dscale[o][b] = pow2_scale(H.d_data.data() + H.d_off[b], (size_t)H.d_m[b] * H.d_n[b]);
         });
-      if (split)  // the residual's scale: the largest |entry - (float)entry| of the block near 2^14
-        parallel_for((int64_t)H.d_m.size(), [&](int64_t b) {
-          const double *src = H.d_data.data() + H.d_off[b];
-          double rmax = 0.0;
-          for (size_t i = 0, e = (size_t)H.d_m[b] * H.d_n[b]; i < e; i++)
-            rmax = std::fmax(rmax, std::fabs(src[i] - (double)(float)src[i]));
-          int ex = 0;
-          if (rmax > 0.0) (void)std::frexp(rmax, &ex);  // rmax in [2^(ex-1), 2^ex)
-          dscale[o][b] = rmax > 0.0 ? std::ldexp(1.0, ex - 15) : 1.0;  // |residual / scale| < 2^15 < 65504
-        });
       parallel_for((int64_t)H.l_m.size(), [&](int64_t b) {
         uscale[o][b] = pow2_scale(H.l_u.data() + H.l_uoff[b], (size_t)H.l_m[b] * H.l_k[b]);
       });
@@ -468,8 +459,9 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   struct PFill { int64_t poff, col0; int L, Lp, m; };
   struct PanelSet {
     int esize;    // bytes per stored entry
-    bool scales;  // per-column scales: FP32 storage's block scales, or the split form's residual scales
+    bool scales;  // per-column scales: FP32 storage's block scales, or the split form's task residual scale
     int pad;      // rows padded to a multiple of this many entries (128 bytes of every stream)
+    int maxc;     // columns per task at most (the w entries its kernel stages in LDS)
     std::vector<PTask> pt;
     std::vector<int> widx;
     std::vector<float> cscale;
@@ -477,8 +469,8 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
     int64_t psize = 0, entries = 0;
   };
   PanelSet setA{split ? 6 : dense64 ? 8 : (int)sizeof(T), scaled && (!dense64 || split),
-                split ? 64 : dense64 ? 16 : 128 / (int)sizeof(T)},
-      setB{(int)sizeof(T), scaled, 128 / (int)sizeof(T)};
+                split ? 64 : dense64 ? 16 : 128 / (int)sizeof(T), split ? kSplitMaxCols : kPanelMaxCols},
+      setB{(int)sizeof(T), scaled, 128 / (int)sizeof(T), kPanelMaxCols};
   std::vector<Col> cols;
   std::vector<std::vector<int>> fin((size_t)n * nops);
   int yp = 0;
@@ -491,8 +483,8 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
     // with 16-byte rows); a zero entry adds an exact zero to its lane's sum
     const int pad = S.pad;
     S.entries += ncols * m;
-    for (int64_t q0 = 0; q0 < ncols; q0 += kPanelMaxCols) {
-      const int L = (int)std::min<int64_t>(kPanelMaxCols, ncols - q0);
+    for (int64_t q0 = 0; q0 < ncols; q0 += S.maxc) {
+      const int L = (int)std::min<int64_t>(S.maxc, ncols - q0);
       const int Lp = (L + pad - 1) / pad * pad;
       PTask t{};
       t.poff = S.psize;
@@ -562,12 +554,21 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
       }
     });
   };
-  if (split) {  // near-field entries as hi = (float)entry and lo = (entry - hi) / scale in FP16
+  if (split) {  // near-field entries as hi = (float)entry and lo = (entry - hi) / s in FP16
     std::unique_ptr<float[]> ph(new float[std::max<int64_t>(setA.psize, 1)]);
     std::unique_ptr<_Float16[]> pl(new _Float16[std::max<int64_t>(setA.psize, 1)]);
     parallel_for((int64_t)setA.pfill.size(), [&](int64_t q) {
-      const PFill &f = setA.pfill[q];
+      const PFill &f = setA.pfill[q];  // built beside setA.pt[q]
       const Col *c = cols.data() + f.col0;
+      // s: one power of two per task, the largest |entry - (float)entry| of the task
+      // times 2^-15 rounded up, so that |residual / s| < 2^15 (FP16 holds 65504)
+      double rmax = 0.0;
+      for (int j = 0; j < f.L; j++)
+        for (int r = 0; r < f.m; r++) rmax = std::fmax(rmax, std::fabs(c[j].src[r] - (double)(float)c[j].src[r]));
+      int ex = 0;
+      if (rmax > 0.0) (void)std::frexp(rmax, &ex);  // rmax in [2^(ex-1), 2^ex)
+      const double s = rmax > 0.0 ? std::ldexp(1.0, ex - 15) : 1.0;
+      std::fill_n(setA.cscale.begin() + setA.pt[q].widx, f.Lp, (float)s);
       for (int r = 0; r < f.m; r++) {
         float *hr = ph.get() + f.poff + (int64_t)r * f.Lp;
         _Float16 *lr = pl.get() + f.poff + (int64_t)r * f.Lp;
@@ -575,7 +576,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
           const double v = j < f.L ? c[j].src[r] : 0.0;
           const float h = (float)v;
           hr[j] = h;
-          lr[j] = (_Float16)((v - (double)h) / (j < f.L ? c[j].scale : 1.0));
+          lr[j] = (_Float16)((v - (double)h) / s);
         }
       }
     });
@@ -640,7 +641,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   const double dense_extra = dense64 ? ((split ? 6.0 : 8.0) - (double)sizeof(T)) * (double)S_dense : 0.0;
   d.alg_bytes = (double)sizeof(T) * (double)S_total + dense_extra + 8.0 * n + 8.0 * n * nops;
   d.panel_bytes = (double)setA.esize * (double)setA.entries + (double)setB.esize * (double)setB.entries +
-                  4.0 * (double)(setA.cscale.size() + setB.cscale.size()) + 8.0 * (double)yp;
+                  4.0 * (double)((split ? setA.pt.size() : setA.cscale.size()) + setB.cscale.size()) + 8.0 * (double)yp;
   // one launch for phase 1 and the near-field panels where its rank ceiling keeps it
   // at 128 VGPRs without spills (ranks up to 20)
   d.near_ok = dense64 && d.acc64 && !split && vtx_rank_ceiling(d.kmax) <= 20;

@@ -458,28 +458,32 @@ __global__ void __launch_bounds__(256) panel_kernel_f32w(const PTask *__restrict
 
 // Phase 2 over near-field entries stored as an FP32 value and an FP16 residual:
 // entry = hi + lo * s, with hi = (float)entry, lo the FP16 rounding of
-// (entry - hi) / s and s a power of two per block (folded into a second gathered
-// input, w * s, exact since s is a power of two).  Every product and sum is FP64;
-// an entry keeps about 35 significant bits in 6 bytes.
+// (entry - hi) / s and s a power of two per task (cscale at the task's first
+// column).  The hi products and sums are FP64; the residual terms, each below
+// 2^-24 of its entry, are summed per lane in FP32 against (float)w (their rounding
+// stays below 2^-44 of the row's absolute sum), scaled by s and added to the FP64
+// sum before the wave's reduction.  An entry keeps about 35 significant bits in 6
+// bytes.  Tasks of at most kSplitMaxCols columns keep w (FP64 and FP32) in 12 KB
+// of LDS, so that LDS leaves room for eight workgroups per CU.
 typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
 
 template <int R>
-__global__ void __launch_bounds__(256) panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi,
-                                                          const _Float16 *__restrict__ lo,
-                                                          const int *__restrict__ widx,
-                                                          const float *__restrict__ cscale,
-                                                          const double *__restrict__ w, double *__restrict__ ypart,
-                                                          const int *__restrict__ active) {
-  __shared__ double2 wl2[kPanelMaxCols / 2];
-  __shared__ double2 sl2[kPanelMaxCols / 2];
-  double *wl = reinterpret_cast<double *>(wl2), *sl = reinterpret_cast<double *>(sl2);
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 2 ? 6 : 4)))
+panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi, const _Float16 *__restrict__ lo,
+                   const int *__restrict__ widx, const float *__restrict__ cscale, const double *__restrict__ w,
+                   double *__restrict__ ypart, const int *__restrict__ active) {
+  __shared__ double2 wl2[kSplitMaxCols / 2];
+  __shared__ fv4 wf4[kSplitMaxCols / 4];
+  double *wl = reinterpret_cast<double *>(wl2);
+  float *wf = reinterpret_cast<float *>(wf4);
   const PTask task = tasks[blockIdx.x];
   if (active && !active[task.net]) return;  // the panel's network sits out this evaluation
   for (int j = threadIdx.x; j < task.L; j += 256) {
     const double v = w[widx[task.widx + j]];
     wl[j] = v;
-    sl[j] = v * (double)cscale[task.widx + j];
+    wf[j] = (float)v;
   }
+  const double s = (double)cscale[task.widx];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int L4 = task.L >> 2;
@@ -489,16 +493,19 @@ __global__ void __launch_bounds__(256) panel_kernel_split(const PTask *__restric
     const fv4 *hrow[R];
     const hv4 *lrow[R];
     double a[R];
+    float b[R];
 #pragma unroll
     for (int q = 0; q < R; q++) {
       const int rr = r + q < task.m ? r + q : r;  // rows past the panel re-read its row r and are not stored
       hrow[q] = H + (int64_t)rr * L4;
       lrow[q] = Lo + (int64_t)rr * L4;
       a[q] = 0.0;
+      b[q] = 0.0f;
     }
-#pragma unroll 2
+#pragma unroll 1
     for (int j = lane; j < L4; j += 64) {
-      const double2 w01 = wl2[2 * j], w23 = wl2[2 * j + 1], s01 = sl2[2 * j], s23 = sl2[2 * j + 1];
+      const double2 w01 = wl2[2 * j], w23 = wl2[2 * j + 1];
+      const fv4 f = wf4[j];
       fv4 h[R];
       hv4 l[R];
 #pragma unroll
@@ -510,13 +517,12 @@ __global__ void __launch_bounds__(256) panel_kernel_split(const PTask *__restric
       for (int q = 0; q < R; q++) {
         a[q] = fma((double)h[q].w, w23.y, fma((double)h[q].z, w23.x, fma((double)h[q].y, w01.y,
                    fma((double)h[q].x, w01.x, a[q]))));
-        a[q] = fma((double)l[q].w, s23.y, fma((double)l[q].z, s23.x, fma((double)l[q].y, s01.y,
-                   fma((double)l[q].x, s01.x, a[q]))));
+        b[q] = fmaf((float)l[q].w, f.w, fmaf((float)l[q].z, f.z, fmaf((float)l[q].y, f.y, fmaf((float)l[q].x, f.x, b[q]))));
       }
     }
 #pragma unroll
     for (int q = 0; q < R; q++) {
-      const double t = wave_sum(a[q]);
+      const double t = wave_sum(fma((double)b[q], s, a[q]));
       if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
     }
   }
