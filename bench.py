@@ -7,9 +7,10 @@ on the device.
 
 The default workload is the headline one: an ensemble of 64 seeded fault
 networks of 50-400 faults each (~1.37M elements, H-matrix eta 2 / ACA 1e-6 /
-leaf 64) advanced together, with mixed-precision operators (FP64 near-field
-blocks, FP32 low-rank factors read into FP64 products and sums; FP64 state, RHS,
-CG and error norm) and the learned step-size proposals (data/step_network.npz), against
+leaf 64) advanced together, with reduced-precision operator storage read into
+FP64 products and sums (near-field entries as an FP32 value plus an FP16
+residual, FP32 low-rank factors; --headline-precision mixed keeps the near-field
+in FP64; FP64 state, RHS, CG and error norm) and the learned step-size proposals (data/step_network.npz), against
 the plain FP64 path (one dfn_handle per network, FP64 operators, rkqs) over the
 same simulated interval of every network.
 
@@ -36,10 +37,12 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 # HBM bytes per launch of the dominant kernel, from two rocprofv3 PMC runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a 1 GiB copy kernel
 # (FETCH x 2.000, WRITE x 1.000); summaries and commands in
-# profiles/pmc_traffic_network80k_fp64.md and profiles/pmc_traffic_ensemble64_mixed.md.
+# profiles/pmc_traffic_network80k_fp64.md, profiles/pmc_traffic_ensemble64_split.md and
+# profiles/pmc_traffic_ensemble64_mixed.md.
 # Keyed by (workload, seed, phase 1 fused with the near-field panels).
 MEASURED_TRAFFIC = {
     ("network800_80k", 2024, False): 2.4846e9,
+    ("ensemble64_1374k_split_learned", 2024, False): 18.762e9,
     ("ensemble64_1374k_mixed_learned", 2024, False): 21.246e9,
     ("ensemble64_1374k_mixed_learned", 2024, True): 20.702e9,
 }
@@ -471,12 +474,13 @@ def run_network(workload, args, mixed=True, cpu_seconds=None):
 
 
 def run_headline(args):
-    """Configs 4 and 5: the 64-network ensemble with mixed-precision operators
-    (FP64 near-field blocks, FP32 low-rank factors read into FP64 sums) and the
-    learned step-size proposals, against the plain FP64 path (one dfn_handle per
-    network, FP64 operators, rkqs) over the same simulated interval of every
-    network; then, continuing, the ensemble with rkqs in mixed precision and in
-    FP64."""
+    """Configs 4 and 5: the 64-network ensemble with reduced-precision operator
+    storage read into FP64 products and sums (FP32 low-rank factors; near-field
+    entries as an FP32 value and an FP16 residual, DFN_PREC_MIXED_SPLIT, or in FP64,
+    DFN_PREC_MIXED) and the learned step-size proposals, against the plain FP64 path
+    (one dfn_handle per network, FP64 operators, rkqs) over the same simulated
+    interval of every network; then, continuing, the ensemble with rkqs in the same
+    precision and in FP64."""
     from demystification_dfn_amd import _abi, problems
     from demystification_dfn_amd import train_step_network as tsn
     from demystification_dfn_amd.device import DeviceEnsemble, DeviceMlp, DeviceSimulation
@@ -484,15 +488,20 @@ def run_headline(args):
     probs = problems.network_ensemble(args.networks, faults=(lo, hi), box=args.ensemble_box, seed=args.seed)
     B = len(probs)
     Ns = np.array([p.N for p in probs])
-    desc = dict(workload=f"ensemble{B}_{Ns.sum() // 1000}k_mixed_learned", networks=B,
+    split = args.headline_precision == "split"
+    prec = _abi.PREC_MIXED_SPLIT if split else _abi.PREC_MIXED
+    desc = dict(workload=f"ensemble{B}_{Ns.sum() // 1000}k_{args.headline_precision}_learned", networks=B,
                 faults_per_network=f"{lo}-{hi}", box_m=args.ensemble_box,
                 length_law="p(L)~L^-0.1 on [50 m, 2 km]", element_m=10, fracture_mode="II",
                 kernel="hmatrix", eta=2.0, eps_aca=1e-6, leaf=64, elements=int(Ns.sum()), min_elements=int(Ns.min()),
                 max_elements=int(Ns.max()), seed=args.seed,
                 friction="per network: a~U[0.008,0.012] per fault, b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa",
                 pulse="10 MPa on one seeded element per network (fluid diffusion on)",
-                precision="DFN_PREC_MIXED: FP64 near-field blocks; FP32 low-rank factors (power-of-two block "
-                          "scales) read into FP64 products and sums; FP64 state, RHS, CG and error norm",
+                precision=("DFN_PREC_MIXED_SPLIT: near-field entries as their FP32 value plus an FP16 residual "
+                           "times a power-of-two scale per panel task (about 35 bits); " if split else
+                           "DFN_PREC_MIXED: FP64 near-field blocks; ") +
+                          "FP32 low-rank factors (power-of-two block scales); all read into FP64 products and sums; "
+                          "FP64 state, RHS, CG and error norm",
                 controller="learned proposals (data/step_network.npz) with the PI controller as fallback; the "
                            "Cash-Karp error test accepts or rejects every step")
     log(f"[bench] headline: ensemble of {B} networks, {Ns.sum()} elements (max {Ns.max()})")
@@ -504,9 +513,9 @@ def run_headline(args):
     ens.initiate_hmat()
     setup_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ens.set_precision(_abi.PREC_MIXED)
+    ens.set_precision(prec)
     layout32_s = time.perf_counter() - t0
-    log(f"[bench] ensemble setup {setup_s:.1f} s, mixed layout {layout32_s:.1f} s {ens.info()}")
+    log(f"[bench] ensemble setup {setup_s:.1f} s, {args.headline_precision} layout {layout32_s:.1f} s {ens.info()}")
     ens.set_controller(_abi.STEP_LEARNED, mlp)
 
     def window(steps):
@@ -525,10 +534,13 @@ def run_headline(args):
                                   accepted=acc, rejected=rej, rejected_fraction=rej / max(acc + rej, 1),
                                   window_start_s_median=float(np.median(ta)))
 
+    head_dtype = ("f32+f16-residual near-field / f32 low-rank storage, f64 arithmetic and state" if split else
+                  "f64 near-field / f32 low-rank storage, f64 arithmetic and state")
     ens.solve(args.warmup)
     tA, tB, wall, head = window(args.steps)
     value = head["value"]
-    log(f"[bench] mixed+learned: {args.steps} steps in {wall:.2f} s ({head['ms_per_step']:.2f} ms/step)")
+    log(f"[bench] {args.headline_precision}+learned: {args.steps} steps in {wall:.2f} s "
+        f"({head['ms_per_step']:.2f} ms/step)")
     # the dominant kernel, HIP events on the ensemble's stream over further steps in
     # which every network takes part
     n_extra = min(args.steps, 10)
@@ -540,6 +552,8 @@ def run_headline(args):
     fused = bool(ens.info()["fused_near"])
     dom_name = ("vtx_near_kernel: phase 1 over the FP32 low-rank V factors (FP64 sums) and the FP64 near-field "
                 "panels in one launch; the whole ensemble's tasks" if fused else
+                "phase-2 panels (panel_kernel_split on the near-field set stored as FP32 + FP16 residual, then "
+                "panel_kernel_f32w on the FP32 low-rank set; the whole ensemble's panels)" if split else
                 "phase-2 panels (panel_kernel_f64 on the FP64 near-field set, then panel_kernel_f32w on the FP32 "
                 "low-rank set; the whole ensemble's panels)")
     log(f"[bench] dominant kernel ({'phase 1 + near-field' if fused else 'phase-2 panels'}): {ms_dom:.3f} ms mean "
@@ -551,14 +565,15 @@ def run_headline(args):
                 timing=f"mean over the dominant launches of {n_extra} ensemble steps after the timed ones in which "
                        f"every network took part, HIP events on the ensemble's stream")
     info32 = ens.info()
-    # the same ensemble, continuing: rkqs with mixed-precision operators, then with FP64 operators
+    # the same ensemble, continuing: rkqs with the same operators, then with FP64 operators
     ens.set_controller(_abi.STEP_RKQS)
     _, _, _, rkqs32 = window(args.steps)
     ens.set_precision(_abi.PREC_FP64)
     info64 = ens.info()
     _, _, _, rkqs64 = window(args.steps)
     ens.close()
-    log(f"[bench] continuing: mixed rkqs {rkqs32['ms_per_step']:.2f} ms/step, FP64 rkqs {rkqs64['ms_per_step']:.2f}")
+    log(f"[bench] continuing: {args.headline_precision} rkqs {rkqs32['ms_per_step']:.2f} ms/step, FP64 rkqs "
+        f"{rkqs64['ms_per_step']:.2f}")
     # the plain FP64 path: every network on its own dfn_handle, from t = 0 to the start
     # of the ensemble's timed window (untimed), then timed over the same interval
     mlp.close()
@@ -567,7 +582,7 @@ def run_headline(args):
     if args.no_plain:  # profiling runs: the ensemble's kernels only
         return dict(metric=METRIC, value=value, unit="simulated network-yr / wall hr", n_gpus=1, steps=args.steps,
                     warmup=args.warmup, ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak",
-                    vs_baseline=None, dtype="f64 near-field / f32 low-rank storage, f64 arithmetic and state",
+                    vs_baseline=None, dtype=head_dtype,
                     data="synthetic (seeded generator)",
                     config=desc, roofline=roof, window=head, cpu_baseline=None,
                     note="--no-plain: the plain FP64 path and the CPU baseline were not run")
@@ -596,11 +611,12 @@ def run_headline(args):
     sim_total = float((tB - tA).sum())
     plain_value = (sim_total / YEAR) / (plain_wall / 3600.0)
     ratio = value / plain_value
-    log(f"[bench] plain FP64 path {plain_value:.1f} network-yr/hr; mixed+learned ensemble {ratio:.2f}x")
+    log(f"[bench] plain FP64 path {plain_value:.1f} network-yr/hr; {args.headline_precision}+learned ensemble "
+        f"{ratio:.2f}x")
     out = dict(
         metric=METRIC, value=value, unit="simulated network-yr / wall hr", n_gpus=1, steps=args.steps,
         warmup=args.warmup, ms_per_step=head["ms_per_step"], higher_is_better=True, scaling="weak", vs_baseline=None,
-        dtype="f64 near-field / f32 low-rank storage, f64 arithmetic and state", data="synthetic (seeded generator)",
+        dtype=head_dtype, data="synthetic (seeded generator)",
         config=desc,
         roofline=roof,
         vs_plain_fp64=dict(plain_value=plain_value, unit="simulated network-yr / wall hr", ratio=ratio,
@@ -610,10 +626,10 @@ def run_headline(args):
                                 "from t=0 to the start of the ensemble's timed window (untimed) and then timed to "
                                 "the window's end: the same simulated interval as the ensemble's, per network"),
         window=head,
-        ensemble=dict(setup_s=setup_s, mixed_layout_s=layout32_s, S_entries=info64["S_entries"],
+        ensemble=dict(setup_s=setup_s, reduced_layout_s=layout32_s, S_entries=info64["S_entries"],
                       alg_bytes_per_stress_eval_fp64=info64["alg_bytes"],
-                      alg_bytes_per_stress_eval_mixed=info32["alg_bytes"],
-                      continuing_mixed_rkqs=rkqs32, continuing_fp64_rkqs=rkqs64,
+                      alg_bytes_per_stress_eval_reduced=info32["alg_bytes"],
+                      continuing_reduced_rkqs=rkqs32, continuing_fp64_rkqs=rkqs64,
                       note="continuing windows follow the timed one in simulated time; rates over different "
                            "windows compare per step (ms_per_step) rather than per simulated year"),
         proposal_network=dict(pairs=meta["recording"]["pairs"], loss_initial=meta["training"]["loss_initial"],
@@ -676,6 +692,9 @@ def main():
     ap.add_argument("--networks", type=int, default=64, help="networks of the ensemble workloads")
     ap.add_argument("--ensemble-faults", default="50,400", help="fault-count range per network of the ensembles")
     ap.add_argument("--ensemble-box", type=float, default=7000.0, help="box size (m) of the ensemble's networks")
+    ap.add_argument("--headline-precision", choices=("split", "mixed"), default="split",
+                    help="operator storage of the headline ensemble: split (DFN_PREC_MIXED_SPLIT, near-field entries "
+                         "as FP32 + FP16 residual) or mixed (DFN_PREC_MIXED, FP64 near-field)")
     ap.add_argument("--ensemble-single-sample", type=int, default=16,
                     help="--workload ensemble: networks also run one handle at a time")
     ap.add_argument("--no-graph", action="store_true",

@@ -86,6 +86,99 @@ __device__ __forceinline__ float wave_sum_full(float sum) {
   return (at(sum, 0) + at(sum, 16)) + (at(sum, 32) + at(sum, 48));
 }
 
+// Reduce-scatter of N per-lane values over the 64 lanes of a wave (all lanes
+// active), one step per lane bit: each step sums every value over a lane pair
+// and leaves the pair's lanes half of the values each (the lane with the bit clear
+// keeps the first half), until a lane holds one value; later steps are plain pair
+// sums.  Bits 5 and 4 go through v_permlane32_swap / v_permlane16_swap, which
+// hand each lane its partner's copy of the value the lane keeps (no select);
+// bits 3..0 through DPP row-mirror, half-row-mirror, xor-2 and xor-1 moves.  A
+// K-column reduction costs about 2K value moves in all, against 6K to 8K for K
+// whole-wave reductions.  Every pair sum is a + b on both lanes, so lanes that
+// hold the same value hold it bit for bit.
+__device__ __forceinline__ double swap32_sum(double a, double b) {
+  // lanes 0-31: a(l) + a(l + 32); lanes 32-63: b(l - 32) + b(l)
+  const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(a), __double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(a), __double2hiint(b), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+__device__ __forceinline__ float swap32_sum(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  const unsigned r0 = r[0], r1 = r[1];
+  return __uint_as_float(r0) + __uint_as_float(r1);
+}
+__device__ __forceinline__ double swap16_sum(double a, double b) {
+  // even 16-lane rows: a(l) + a(l + 16); odd rows: b(l - 16) + b(l)
+  const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(a), __double2loint(b), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(a), __double2hiint(b), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+__device__ __forceinline__ float swap16_sum(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+  const unsigned r0 = r[0], r1 = r[1];
+  return __uint_as_float(r0) + __uint_as_float(r1);
+}
+template <int CTRL>
+__device__ __forceinline__ double lane_move(double v) { return dpp_move<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ float lane_move(float v) { return dpp_move_f<CTRL>(v); }
+
+template <int STEP, int N, typename T>
+__device__ __forceinline__ void rs_step(const T (&in)[N], T (&out)[(N + 1) / 2], int lane) {
+  constexpr int H = (N + 1) / 2;
+  if constexpr (STEP >= 2 && N == 1) {  // one value left: a plain pair sum
+    constexpr int C = STEP == 2 ? kDppMirror : STEP == 3 ? kDppHalfMirror : STEP == 4 ? kDppXor2 : kDppXor1;
+    out[0] = in[0] + lane_move<C>(in[0]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+      const T a = in[j], b = j + H < N ? in[j + H < N ? j + H : 0] : (T)0;
+      if constexpr (STEP == 0) {
+        out[j] = swap32_sum(a, b);
+      } else if constexpr (STEP == 1) {
+        out[j] = swap16_sum(a, b);
+      } else {
+        constexpr int C = STEP == 2 ? kDppMirror : STEP == 3 ? kDppHalfMirror : STEP == 4 ? kDppXor2 : kDppXor1;
+        const bool up = (lane >> (5 - STEP)) & 1;
+        out[j] = (up ? b : a) + lane_move<C>(up ? a : b);
+      }
+    }
+  }
+}
+
+// the reduce-scatter of acc[0..K) (K <= 64); returns the lane's value and sets col
+// to the column it totals, or -1 (a lane of a pair whose other lane holds it)
+template <int K, typename T>
+__device__ __forceinline__ T wave_reduce_scatter(const T (&acc)[K], int lane, int &col) {
+  constexpr int N1 = (K + 1) / 2, N2 = (N1 + 1) / 2, N3 = (N2 + 1) / 2, N4 = (N3 + 1) / 2, N5 = (N4 + 1) / 2;
+  constexpr int N6 = (N5 + 1) / 2;
+  T a1[N1], a2[N2], a3[N3], a4[N4], a5[N5], a6[N6];
+  rs_step<0>(acc, a1, lane);
+  rs_step<1>(a1, a2, lane);
+  rs_step<2>(a2, a3, lane);
+  rs_step<3>(a3, a4, lane);
+  rs_step<4>(a4, a5, lane);
+  rs_step<5>(a5, a6, lane);
+  // the column, from the last step back: a splitting step adds bit * (its output
+  // count) to the index, which must stay below its input count; a plain step keeps
+  // the lane with the bit clear
+  const int nin[6] = {K, N1, N2, N3, N4, N5}, nout[6] = {N1, N2, N3, N4, N5, N6};
+  int idx = 0;
+  bool ok = true;
+#pragma unroll
+  for (int s = 5; s >= 0; s--) {
+    const bool bit = (lane >> (5 - s)) & 1;
+    if (s < 2 || nin[s] > 1) {
+      idx += bit ? nout[s] : 0;
+      ok = ok && idx < nin[s];
+    } else {
+      ok = ok && !bit;
+    }
+  }
+  col = ok ? idx : -1;
+  return a6[0];
+}
+
 // ---------------------------------------------------------------------------
 // H-matvec
 // ---------------------------------------------------------------------------
@@ -164,17 +257,13 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
       if constexpr (G < K) __builtin_amdgcn_sched_barrier(0);  // keep the next group's loads after these sums
     }
   }
-  // all 64 lanes are active here: DPP wave totals, lane j keeps column j's
-  TA mine = (TA)0;
-#pragma unroll
-  for (int j = 0; j < K; j++) {
-    const TA t = wave_sum_full(acc[j]);
-    if (lane == j) mine = t;
-  }
-  if (lane < K) {
+  // all 64 lanes are active here: one lane per column ends with its total
+  int col;
+  const TA mine = wave_reduce_scatter<K>(acc, lane, col);
+  if (col >= 0) {
     const double v = (double)mine * task.scale;
-    if (task.wout >= 0) wz[task.wout + lane] = v;
-    else zpart[task.zp + lane] = v;
+    if (task.wout >= 0) wz[task.wout + col] = v;
+    else zpart[task.zp + col] = v;
   }
 }
 
