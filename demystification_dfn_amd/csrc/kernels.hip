@@ -179,6 +179,16 @@ __device__ __forceinline__ T wave_reduce_scatter(const T (&acc)[K], int lane, in
   return a6[0];
 }
 
+// the R row sums of a panel's wave step, reduced across the wave and stored by the
+// lanes that hold them (rows from nrows on are not stored).  The pairing of lanes
+// is the same for every R, so a row's sum does not depend on R.
+template <int R>
+__device__ __forceinline__ void store_row_sums(const double (&a)[R], int lane, double *__restrict__ out, int nrows) {
+  int col;
+  const double t = wave_reduce_scatter<R>(a, lane, col);
+  if (col >= 0 && col < nrows) out[col] = t;
+}
+
 // ---------------------------------------------------------------------------
 // H-matvec
 // ---------------------------------------------------------------------------
@@ -363,22 +373,17 @@ __device__ __forceinline__ void panel_f64_body(const PTask &task, const double *
 #pragma unroll
       for (int q = 0; q < R; q++) a[q] = fma(v[q].y, wv.y, fma(v[q].x, wv.x, a[q]));
     }
-#pragma unroll
-    for (int q = 0; q < R; q++) a[q] = wave_sum(a[q]);
-    if (lane == 0)
-#pragma unroll
-      for (int q = 0; q < R; q++) ypart[task.yp + r + q] = a[q];
+    store_row_sums<R>(a, lane, ypart + task.yp + r, R);
   }
   for (; r < task.m; r++) {  // fewer than R rows left for this wave
     const dv2 *row0 = P + (int64_t)r * L2;
-    double a0 = 0.0;
+    double a0[1] = {0.0};
     for (int j = lane; j < L2; j += 64) {
       const double2 wv = wl2[j];
       const dv2 v0 = __builtin_nontemporal_load(row0 + j);
-      a0 = fma(v0.y, wv.y, fma(v0.x, wv.x, a0));  // as the R-row loop
+      a0[0] = fma(v0.y, wv.y, fma(v0.x, wv.x, a0[0]));  // as the R-row loop
     }
-    a0 = wave_sum(a0);
-    if (lane == 0) ypart[task.yp + r] = a0;
+    store_row_sums<1>(a0, lane, ypart + task.yp + r, 1);
   }
 }
 
@@ -492,12 +497,10 @@ __global__ void __launch_bounds__(256) panel_kernel_f32(const PTask *__restrict_
 #pragma unroll
       for (int q = 0; q < R; q++) acc4(v[q], wv, 4 * j, nd, s[q], c[q], a[q]);
     }
+    double t[R];
 #pragma unroll
-    for (int q = 0; q < R; q++) {
-      double t = ((double)s[q] - (double)c[q]) + (double)a[q];
-      t = wave_sum(t);
-      if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
-    }
+    for (int q = 0; q < R; q++) t[q] = ((double)s[q] - (double)c[q]) + (double)a[q];
+    store_row_sums<R>(t, lane, ypart + task.yp + r, task.m - r);
   }
 }
 
@@ -537,11 +540,7 @@ __global__ void __launch_bounds__(256) panel_kernel_f32w(const PTask *__restrict
         a[q] = fma((double)v[q].w, w23.y, fma((double)v[q].z, w23.x, fma((double)v[q].y, w01.y,
                    fma((double)v[q].x, w01.x, a[q]))));
     }
-#pragma unroll
-    for (int q = 0; q < R; q++) {
-      const double t = wave_sum(a[q]);
-      if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
-    }
+    store_row_sums<R>(a, lane, ypart + task.yp + r, task.m - r);
   }
 }
 
@@ -609,11 +608,10 @@ panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi
         b[q] = fmaf((float)l[q].w, f.w, fmaf((float)l[q].z, f.z, fmaf((float)l[q].y, f.y, fmaf((float)l[q].x, f.x, b[q]))));
       }
     }
+    double t[R];
 #pragma unroll
-    for (int q = 0; q < R; q++) {
-      const double t = wave_sum(fma((double)b[q], s, a[q]));
-      if (lane == 0 && r + q < task.m) ypart[task.yp + r + q] = t;
-    }
+    for (int q = 0; q < R; q++) t[q] = fma((double)b[q], s, a[q]);
+    store_row_sums<R>(t, lane, ypart + task.yp + r, task.m - r);
   }
 }
 
