@@ -301,7 +301,8 @@ __device__ __forceinline__ void vtx_task(const VTask &task, const TV *__restrict
 }
 
 template <typename TV, bool ACC64, int KMAX>
-__global__ void __launch_bounds__(512) lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
+lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
                                                           const TV *__restrict__ vdata, double *__restrict__ w,
                                                           const float *__restrict__ xf, double *__restrict__ zpart,
                                                           const int *__restrict__ active) {
