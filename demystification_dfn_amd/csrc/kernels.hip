@@ -250,8 +250,8 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
       for (int u = 0; u < kVec; u++) xi[u] = (TA)xv[u];
     }
     // FP32 factors with FP64 sums (GROUPS): columns in groups of 10, so that 10
-    // vectors of V are live at a time (128 instead of 138 VGPRs at ranks up to 20: 4
-    // waves per SIMD instead of 3); each column's sum is the same whatever the grouping
+    // vectors of V are live at a time (96 VGPRs at ranks up to 20: 5 waves per SIMD);
+    // each column's sum is the same whatever the grouping
     constexpr int G = (GROUPS && ACC64 && sizeof(TV) == 4 && K > 10) ? 10 : K;
 #pragma unroll
     for (int j0 = 0; j0 < K; j0 += G) {
