@@ -1288,9 +1288,10 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   return s;
 }
 
-// Per network (one workgroup each, in a fixed order): over its real elements, max
-// |V|, the mean of log10 |V| and the fraction with |V| > 1e-3 m/s; out[3 b + 0..2].
-__global__ void __launch_bounds__(256) ens_vstats_kernel(const double *__restrict__ V,
+// Per network (one 1024-thread workgroup each, in a fixed order): over its real
+// elements, max |V|, the mean of log10 |V| and the fraction with |V| > 1e-3 m/s;
+// out[3 b + 0..2].
+__global__ void __launch_bounds__(1024) ens_vstats_kernel(const double *__restrict__ V,
                                                          const uint8_t *__restrict__ fake,
                                                          const CgNet *__restrict__ nets, double *__restrict__ out) {
   __shared__ double red[16];
@@ -2265,7 +2266,7 @@ void launch_ens_restart_stage2(const EnsStageArgs &a, hipStream_t s) {
 }
 
 void launch_ens_vstats(const double *V, const uint8_t *fake, const CgNet *nets, int nnet, double *out, hipStream_t s) {
-  hipLaunchKernelGGL(ens_vstats_kernel, dim3(nnet), dim3(256), 0, s, V, fake, nets, out);
+  hipLaunchKernelGGL(ens_vstats_kernel, dim3(nnet), dim3(1024), 0, s, V, fake, nets, out);
 }
 
 // One save of the solver output (save_data, src/solver.f90:211-222): column col
