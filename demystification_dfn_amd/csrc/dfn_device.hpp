@@ -27,6 +27,12 @@ struct VTask {
   int32_t net;    // network of the block (ensembles; 0 for one network)
 };
 
+// One wave of phase 1: task a alone (b < 0), or tasks a and b, of the same rank and
+// network and each a single strip of at most 32 row vectors, on the wave's two halves.
+struct VItem {
+  int32_t a, b;
+};
+
 // z of one low-rank block: w[zout + j] = sum_t zpart[zp_first + t*k + j]
 struct ZRed {
   int32_t zp_first, ntask, k, zout;
@@ -160,8 +166,10 @@ enum { kStoreF64 = 0, kStoreF32 = 1, kStoreF32Acc64 = 2 };
 // waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (kStoreF32);
 // active (ensembles, may be null): tasks of networks with active[net] == 0 return at once;
 // kmax: the largest rank of any task (at most kMaxRank)
+// items (may be null): the waves' task lists (nitems of them) in place of one task per wave
 void launch_vtx(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
-                int waves = 4, const float *xf = nullptr, const int *active = nullptr, int kmax = kMaxRank);
+                int waves = 4, const float *xf = nullptr, const int *active = nullptr, int kmax = kMaxRank,
+                const VItem *items = nullptr, int nitems = 0);
 // the rank ceiling of the phase-1 kernel instance that serves tasks of rank <= kmax
 int vtx_rank_ceiling(int kmax);
 void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipStream_t s);

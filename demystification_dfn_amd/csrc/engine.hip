@@ -1013,6 +1013,7 @@ API int dfn_tuning(dfn_handle h, int32_t *choice, double *ms) {
   choice[4] = d->panel_split;
   choice[5] = d->fused ? 1 : 0;
   choice[6] = d->kc;
+  choice[7] = d->vpair ? 1 : 0;
   ms[0] = d->tune_ms_default;
   ms[1] = d->tune_ms;
   return DFN_OK;

@@ -119,7 +119,48 @@ struct DevOps {
   // the default and of the chosen shapes
   int vtx_waves = 4, vtx_order = 0, panel_rows = 2, panel_order = 0, panel_split = 0;
   double tune_ms_default = 0, tune_ms = 0;
+  // phase-1 wave items (vpair): short tasks of one rank and network two to a wave
+  bool vpair = false;
+  DMem vitems;
+  int nvi = 0;
 };
+
+// Phase-1 wave items over the task list in its launch order: each task of at most 32
+// row vectors (kvec rows each) waits for the next such task of its rank in the same
+// network and shares a wave with it; the other tasks keep a wave each.
+inline std::vector<VItem> vtx_items(const std::vector<VTask> &v, int kvec) {
+  std::vector<VItem> items;
+  items.reserve(v.size());
+  int pending[kMaxRank + 1];
+  std::fill_n(pending, kMaxRank + 1, -1);
+  int net = -1;
+  auto flush = [&]() {
+    for (int k = 0; k <= kMaxRank; k++)
+      if (pending[k] >= 0) {
+        items.push_back({pending[k], -1});
+        pending[k] = -1;
+      }
+  };
+  for (int i = 0; i < (int)v.size(); i++) {
+    const VTask &t = v[i];
+    if (t.len > 32 * kvec || t.k < 1 || t.k > kMaxRank) {
+      items.push_back({i, -1});
+      continue;
+    }
+    if (t.net != net) {
+      flush();
+      net = t.net;
+    }
+    if (pending[t.k] >= 0) {
+      items.push_back({pending[t.k], i});
+      pending[t.k] = -1;
+    } else {
+      pending[t.k] = i;
+    }
+  }
+  flush();
+  return items;
+}
 
 // Phase-2 schedule: each panel task cut into tasks of at most `split` rows (0: not
 // cut; a row's sum is computed the same way whichever task holds it), the largest
@@ -181,7 +222,7 @@ inline void launch_phases(const DevOps &d, double *w, hipStream_t s, int waves, 
     return;
   }
   launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.vmode(), w, d.zpart.as<double>(), s, waves, d.xf.as<float>(),
-             active, d.kc);
+             active, d.kc, d.vpair ? d.vitems.as<VItem>() : nullptr, d.nvi);
   launch_zreduce(d.zred.as<ZRed>(), d.nzr, d.zpart.as<double>(), w, s);
   if (kt) (void)hipEventRecord(kt[0], s);
   launch_panels(d, w, s, rows, active);
@@ -192,6 +233,7 @@ inline void launch_phases(const DevOps &d, double *w, hipStream_t s, int waves, 
 // the operator itself: phase-1 tasks (waves) per workgroup {2, 4, 8}; the phase-1
 // task order (as built; grouped by rank bucket with the largest ranks first, so a
 // workgroup's waves share one code path; or the largest tasks, rows x rank, first);
+// short phase-1 tasks one per wave or two (same rank and network) to a wave;
 // 2 or 4 panel rows per wave step; the phase-2 schedule (as built, or largest
 // panels first, whole or cut into tasks of 32 or 16 rows); the phase-1 kernel
 // instance (the smallest rank ceiling that covers the layout, or 32); for a
@@ -200,8 +242,9 @@ inline void launch_phases(const DevOps &d, double *w, hipStream_t s, int waves, 
 // computes every sum in the same order, so the choice never changes a result.
 // Each candidate is timed over phase 1, the z reduction and phase 2 (HIP events,
 // 2 warm-up + 8 timed applications); a candidate replaces the incumbent only if it
-// is 1 % faster.  DFN_AUTOTUNE=alt takes non-default shapes (8 waves, largest
-// phase-1 tasks first, 4 rows, 16-row panel tasks) without timing, so that a test
+// is 1 % faster (the phase-1 wave items: 1 % faster over phase 1 alone).  DFN_AUTOTUNE=alt takes non-default shapes (8 waves, largest
+// phase-1 tasks first, two short tasks to a wave, 4 rows, 16-row panel tasks)
+// without timing, so that a test
 // can check that the shapes agree.
 inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<PTask> &pt,
                     const std::vector<PTask> &pt_b) {
@@ -222,14 +265,38 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
     ms = t / 8.0;
     return DFN_OK;
   };
+  auto measure_p1 = [&](int waves, double &ms) -> int {  // phase 1 alone
+    for (int rep = 0; rep < 10; rep++) {
+      if (rep == 2) DFN_HIP(hipEventRecord(e0, nullptr));
+      launch_vtx(d.vtasks.as<VTask>(), d.nvt, d.vdata.p, d.vmode(), w, d.zpart.as<double>(), nullptr, waves,
+                 d.xf.as<float>(), nullptr, d.kc, d.vpair ? d.vitems.as<VItem>() : nullptr, d.nvi);
+    }
+    DFN_HIP(hipEventRecord(e1, nullptr));
+    DFN_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    DFN_HIP(hipEventElapsedTime(&t, e0, e1));
+    ms = t / 8.0;
+    return DFN_OK;
+  };
+  std::vector<VTask> vcur;  // the phase-1 task list as uploaded
   auto vorder = [&](int order) -> int {
-    std::vector<VTask> v(vt);
+    vcur = vt;
     if (order == 1)
-      std::stable_sort(v.begin(), v.end(), [](const VTask &x, const VTask &y) { return x.k > y.k; });
+      std::stable_sort(vcur.begin(), vcur.end(), [](const VTask &x, const VTask &y) { return x.k > y.k; });
     else if (order == 2)
-      std::stable_sort(v.begin(), v.end(),
+      std::stable_sort(vcur.begin(), vcur.end(),
                        [](const VTask &x, const VTask &y) { return x.len * x.k > y.len * y.k; });
-    return d.vtasks.upload(v);
+    d.vpair = false;
+    return d.vtasks.upload(vcur);
+  };
+  auto vpairs = [&](bool on) -> int {  // short tasks two to a wave, over the current order
+    d.vpair = false;
+    if (!on) return DFN_OK;
+    const std::vector<VItem> items = vtx_items(vcur, d.fp32 ? 4 : 2);
+    DFN_TRY(d.vitems.upload(items));
+    d.nvi = (int)items.size();
+    d.vpair = true;
+    return DFN_OK;
   };
   auto porder = [&](int order, int split) -> int {  // order 0: as built
     const std::vector<PTask> p = order ? panel_schedule(pt, split) : pt;
@@ -241,13 +308,15 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
     return d.ptasks_b.upload(q);
   };
   int waves = 4, order = 0, rows = 2, pord = 0, split = 0;
+  bool pair = false;
   const char *mode = std::getenv("DFN_AUTOTUNE");
   const bool alt = mode && std::string(mode) == "alt";
   double best = 0, t = 0;
   if (alt) {
-    waves = 8; order = 2; rows = 4; pord = 1; split = 16;
+    waves = 8; order = 2; rows = 4; pord = 1; split = 16; pair = true;
     d.kc = kMaxRank;  // the rank-32 instance (the single launch is exercised by DFN_FUSE_NEAR=1)
     DFN_TRY(vorder(order));
+    DFN_TRY(vpairs(pair));
     DFN_TRY(porder(pord, split));
   } else {
     DFN_TRY(measure(waves, rows, best));
@@ -262,6 +331,15 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
       if (t < 0.99 * best) { best = t; order = cand; }
     }
     DFN_TRY(vorder(order));
+    {  // wave items, judged on phase 1 alone (3-5 % there is under 1 % of the whole)
+      double t1 = 0;
+      DFN_TRY(measure_p1(waves, t));
+      DFN_TRY(vpairs(true));
+      DFN_TRY(measure_p1(waves, t1));
+      pair = t1 < 0.99 * t;
+      DFN_TRY(vpairs(pair));
+      if (pair) DFN_TRY(measure(waves, rows, best));
+    }
     DFN_TRY(measure(waves, 4, t));
     if (t < 0.99 * best) { best = t; rows = 4; }
     for (int cand : {0, 32, 16}) {
@@ -284,6 +362,9 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
       else d.fused = false;
     }
   }
+  // DFN_VTX_PAIRS=1 / 0 takes / refuses the phase-1 wave items whatever their timing
+  const char *pe = std::getenv("DFN_VTX_PAIRS");
+  if (pe && (pe[0] == '0' || pe[0] == '1')) DFN_TRY(vpairs(pe[0] == '1'));
   d.vtx_waves = waves;
   d.vtx_order = order;
   d.panel_rows = rows;
@@ -659,7 +740,14 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   }
   d.fused = false;
   const char *at = std::getenv("DFN_AUTOTUNE");
-  if (!(at && std::string(at) == "0")) DFN_TRY(autotune(d, vt, pt, setB.pt));
+  if (!(at && std::string(at) == "0")) {
+    DFN_TRY(autotune(d, vt, pt, setB.pt));
+  } else if (const char *pe = std::getenv("DFN_VTX_PAIRS"); pe && pe[0] == '1') {
+    const std::vector<VItem> items = vtx_items(vt, d.fp32 ? 4 : 2);  // over the tasks as built
+    DFN_TRY(d.vitems.upload(items));
+    d.nvi = (int)items.size();
+    d.vpair = true;
+  }
   // DFN_FUSE_NEAR=1 / 0 takes / refuses the single launch of phase 1 and the
   // near-field panels whatever the timing (the results are the same either way)
   const char *fuse = std::getenv("DFN_FUSE_NEAR");

@@ -1093,6 +1093,7 @@ API int dfn_ensemble_info(dfn_ensemble h, int64_t *info) {
   info[5] = e.use_graph ? 1 : 0;
   info[6] = d && d->fused ? 1 : 0;
   info[7] = d ? d->kc : 0;
+  info[8] = d && d->vpair ? 1 : 0;
   return DFN_OK;
   DFN_API_END
 }

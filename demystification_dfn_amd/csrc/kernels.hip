@@ -147,13 +147,20 @@ __device__ __forceinline__ void rs_step(const T (&in)[N], T (&out)[(N + 1) / 2],
 }
 
 // the reduce-scatter of acc[0..K) (K <= 64); returns the lane's value and sets col
-// to the column it totals, or -1 (a lane of a pair whose other lane holds it)
-template <int K, typename T>
+// to the column it totals, or -1 (a lane of a pair whose other lane holds it).
+// HALVES: the two 32-lane halves hold separate sums (lane bit 5 takes no part), and
+// each half ends with its own totals.
+template <int K, bool HALVES = false, typename T>
 __device__ __forceinline__ T wave_reduce_scatter(const T (&acc)[K], int lane, int &col) {
-  constexpr int N1 = (K + 1) / 2, N2 = (N1 + 1) / 2, N3 = (N2 + 1) / 2, N4 = (N3 + 1) / 2, N5 = (N4 + 1) / 2;
-  constexpr int N6 = (N5 + 1) / 2;
+  constexpr int N1 = HALVES ? K : (K + 1) / 2, N2 = (N1 + 1) / 2, N3 = (N2 + 1) / 2, N4 = (N3 + 1) / 2;
+  constexpr int N5 = (N4 + 1) / 2, N6 = (N5 + 1) / 2;
   T a1[N1], a2[N2], a3[N3], a4[N4], a5[N5], a6[N6];
-  rs_step<0>(acc, a1, lane);
+  if constexpr (HALVES) {
+#pragma unroll
+    for (int j = 0; j < K; j++) a1[j] = acc[j];
+  } else {
+    rs_step<0>(acc, a1, lane);
+  }
   rs_step<1>(a1, a2, lane);
   rs_step<2>(a2, a3, lane);
   rs_step<3>(a3, a4, lane);
@@ -166,7 +173,7 @@ __device__ __forceinline__ T wave_reduce_scatter(const T (&acc)[K], int lane, in
   int idx = 0;
   bool ok = true;
 #pragma unroll
-  for (int s = 5; s >= 0; s--) {
+  for (int s = 5; s >= (HALVES ? 1 : 0); s--) {
     const bool bit = (lane >> (5 - s)) & 1;
     if (s < 2 || nin[s] > 1) {
       idx += bit ? nout[s] : 0;
@@ -297,6 +304,105 @@ __device__ __forceinline__ void vtx_task(const VTask &task, const TV *__restrict
     DFN_VTX_CASE(31) DFN_VTX_CASE(32)
 #undef DFN_VTX_CASE
     default: break;
+  }
+}
+
+// Two single-strip tasks of rank K on one wave: lanes 0-31 take task a, lanes 32-63
+// task b (each lane its task's fields), every lane the same kVec rows of its task as
+// lane (lane & 31) of a wave of its own would; the halves are reduced apart.  A
+// task's sums are those of vtx_body: there the lanes from 32 on hold zeros for a
+// task of at most 32 row vectors, and the reduction's first step adds them.
+template <int K, typename TV, bool ACC64, bool GROUPS>
+__device__ __forceinline__ void vtx_pair_body(const VTask &ta, const VTask &tb, const TV *__restrict__ vdata,
+                                              const double *__restrict__ w, const float *__restrict__ xf,
+                                              double *__restrict__ zpart, double *__restrict__ wz, int lane) {
+  constexpr int kVec = 16 / (int)sizeof(TV);
+  typedef TV tvec __attribute__((ext_vector_type(kVec)));
+  typedef typename std::conditional<ACC64, double, TV>::type TA;
+  TA acc[K];
+#pragma unroll
+  for (int j = 0; j < K; j++) acc[j] = (TA)0;
+  const bool hb = lane >= 32;
+  const int len = hb ? tb.len : ta.len;
+  const int i = (lane & 31) * kVec;
+  if (i < len) {
+    const int hp = (len + kVec - 1) / kVec * kVec;  // the single strip's column stride
+    const int xo = hb ? tb.xcol + tb.row0 : ta.xcol + ta.row0;
+    const TV *Vs = vdata + (hb ? tb.voff : ta.voff) + i;
+    TA xi[kVec];
+    if constexpr (sizeof(TV) == 4 && !ACC64) {
+      float4 x4;
+      __builtin_memcpy(&x4, xf + xo + i, sizeof(x4));
+      xi[0] = x4.x; xi[1] = x4.y; xi[2] = x4.z; xi[3] = x4.w;
+    } else {
+      double xv[kVec];
+      __builtin_memcpy(xv, w + xo + i, sizeof(xv));
+#pragma unroll
+      for (int u = 0; u < kVec; u++) xi[u] = (TA)xv[u];
+    }
+    constexpr int G = (GROUPS && ACC64 && sizeof(TV) == 4 && K > 10) ? 10 : K;
+#pragma unroll
+    for (int j0 = 0; j0 < K; j0 += G) {
+      tvec v[G];
+#pragma unroll
+      for (int j = 0; j < G; j++)
+        if (j0 + j < K) v[j] = *reinterpret_cast<const tvec *>(Vs + (j0 + j) * hp);
+#pragma unroll
+      for (int j = 0; j < G; j++)
+        if (j0 + j < K)
+#pragma unroll
+          for (int u = 0; u < kVec; u++) acc[j0 + j] += (TA)v[j][u] * xi[u];
+      if constexpr (G < K) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  int col;
+  const TA mine = wave_reduce_scatter<K, true>(acc, lane, col);
+  if (col >= 0) {
+    const double v = (double)mine * (hb ? tb.scale : ta.scale);
+    const int wout = hb ? tb.wout : ta.wout;
+    if (wout >= 0) wz[wout + col] = v;
+    else zpart[(hb ? tb.zp : ta.zp) + col] = v;
+  }
+}
+
+template <typename TV, bool ACC64, int KMAX, bool GROUPS>
+__device__ __forceinline__ void vtx_pair_task(const VTask &ta, const VTask &tb, const TV *__restrict__ vdata,
+                                              double *__restrict__ w, const float *__restrict__ xf,
+                                              double *__restrict__ zpart, int lane) {
+  switch (ta.k) {
+#define DFN_VTX_CASE(K) \
+  case K:                                                                                       \
+    if constexpr (K <= KMAX) vtx_pair_body<K, TV, ACC64, GROUPS>(ta, tb, vdata, w, xf, zpart, w, lane); \
+    break;
+    DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
+    DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
+    DFN_VTX_CASE(13) DFN_VTX_CASE(14) DFN_VTX_CASE(15) DFN_VTX_CASE(16) DFN_VTX_CASE(17) DFN_VTX_CASE(18)
+    DFN_VTX_CASE(19) DFN_VTX_CASE(20) DFN_VTX_CASE(21) DFN_VTX_CASE(22) DFN_VTX_CASE(23) DFN_VTX_CASE(24)
+    DFN_VTX_CASE(25) DFN_VTX_CASE(26) DFN_VTX_CASE(27) DFN_VTX_CASE(28) DFN_VTX_CASE(29) DFN_VTX_CASE(30)
+    DFN_VTX_CASE(31) DFN_VTX_CASE(32)
+#undef DFN_VTX_CASE
+    default: break;
+  }
+}
+
+// Phase 1 by wave items: one task, or two short tasks of the same rank and network
+// (VItem), per wave.
+template <typename TV, bool ACC64, int KMAX>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
+lowrank_vtx_items_kernel(const VTask *__restrict__ tasks, const VItem *__restrict__ items, int nitems,
+                         const TV *__restrict__ vdata, double *__restrict__ w, const float *__restrict__ xf,
+                         double *__restrict__ zpart, const int *__restrict__ active) {
+  const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  if (t >= nitems) return;
+  const VItem it = items[t];
+  const VTask ta = tasks[it.a];
+  if (active && !active[ta.net]) return;  // the items' network sits out this evaluation
+  if (it.b < 0) {
+    vtx_task<TV, ACC64, KMAX, true>(ta, vdata, w, xf, zpart, lane);
+  } else {
+    const VTask tb = tasks[it.b];
+    vtx_pair_task<TV, ACC64, KMAX, true>(ta, tb, vdata, w, xf, zpart, lane);
   }
 }
 
@@ -2100,7 +2206,20 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
 
 template <int KMAX>
 static void launch_vtx_k(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
-                         int waves, const float *xf, const int *active) {
+                         int waves, const float *xf, const int *active, const VItem *items, int nitems) {
+  if (items) {
+    if (nitems <= 0) return;
+    if (mode == kStoreF32)
+      hipLaunchKernelGGL((lowrank_vtx_items_kernel<float, false, KMAX>), dim3(nblk(nitems, waves)), dim3(64 * waves),
+                         0, s, t, items, nitems, (const float *)vdata, w, xf, zpart, active);
+    else if (mode == kStoreF32Acc64)
+      hipLaunchKernelGGL((lowrank_vtx_items_kernel<float, true, KMAX>), dim3(nblk(nitems, waves)), dim3(64 * waves),
+                         0, s, t, items, nitems, (const float *)vdata, w, xf, zpart, active);
+    else
+      hipLaunchKernelGGL((lowrank_vtx_items_kernel<double, true, KMAX>), dim3(nblk(nitems, waves)), dim3(64 * waves),
+                         0, s, t, items, nitems, (const double *)vdata, w, xf, zpart, active);
+    return;
+  }
   if (mode == kStoreF32)
     hipLaunchKernelGGL((lowrank_vtx_kernel<float, false, KMAX>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
                        (const float *)vdata, w, xf, zpart, active);
@@ -2115,13 +2234,13 @@ static void launch_vtx_k(const VTask *t, int nt, const void *vdata, int mode, do
 int vtx_rank_ceiling(int kmax) { return kmax <= 16 ? 16 : kmax <= 20 ? 20 : kMaxRank; }
 
 void launch_vtx(const VTask *t, int nt, const void *vdata, int mode, double *w, double *zpart, hipStream_t s,
-                int waves, const float *xf, const int *active, int kmax) {
+                int waves, const float *xf, const int *active, int kmax, const VItem *items, int nitems) {
   if (nt <= 0) return;
   if (waves != 2 && waves != 8) waves = 4;
   const int kc = vtx_rank_ceiling(kmax);
-  if (kc == 16) launch_vtx_k<16>(t, nt, vdata, mode, w, zpart, s, waves, xf, active);
-  else if (kc == 20) launch_vtx_k<20>(t, nt, vdata, mode, w, zpart, s, waves, xf, active);
-  else launch_vtx_k<kMaxRank>(t, nt, vdata, mode, w, zpart, s, waves, xf, active);
+  if (kc == 16) launch_vtx_k<16>(t, nt, vdata, mode, w, zpart, s, waves, xf, active, items, nitems);
+  else if (kc == 20) launch_vtx_k<20>(t, nt, vdata, mode, w, zpart, s, waves, xf, active, items, nitems);
+  else launch_vtx_k<kMaxRank>(t, nt, vdata, mode, w, zpart, s, waves, xf, active, items, nitems);
 }
 
 template <int R, int KMAX>

@@ -47,10 +47,11 @@ class DeviceSimulation(Simulation):
 
     def tuning(self):
         """The autotuner's launch shapes for the active operator layout (dfn_tuning)."""
-        c, ms = np.zeros(7, np.int32), np.zeros(2)
+        c, ms = np.zeros(8, np.int32), np.zeros(2)
         _abi.check(self._fns["tuning"](self._h, _abi.i32ptr(c), _abi.dptr(ms)), "dfn_tuning")
         return dict(vtx_waves=int(c[0]), vtx_order=int(c[1]), panel_rows=int(c[2]), panel_order=int(c[3]),
-                    panel_split=int(c[4]), fused_near=int(c[5]), rank_ceiling=int(c[6]), ms_default=float(ms[0]), ms_tuned=float(ms[1]))
+                    panel_split=int(c[4]), fused_near=int(c[5]), rank_ceiling=int(c[6]), vtx_pairs=int(c[7]),
+                    ms_default=float(ms[0]), ms_tuned=float(ms[1]))
 
     def kernel_timing(self, on=True):
         """Start (clearing the sums) or stop timing the dominant kernel of every
@@ -206,10 +207,10 @@ class DeviceEnsemble:
         return X[:n.value], T[:n.value]
 
     def info(self):
-        v = np.zeros(8, dtype=np.int64)
+        v = np.zeros(9, dtype=np.int64)
         _abi.check(self._fns["ensemble_info"](self._h, _abi.i64ptr(v)), "dfn_ensemble_info")
         return dict(zip(("networks", "elements", "max_elements", "S_entries", "alg_bytes", "graph", "fused_near",
-                         "rank_ceiling"),
+                         "rank_ceiling", "vtx_pairs"),
                         (int(x) for x in v)))
 
     def kernel_timing(self, on=True):

@@ -222,7 +222,7 @@ interface
   integer(c_int) function dfn_ensemble_info(e, info) bind(C, name='dfn_ensemble_info')
     import :: c_ptr, c_int, c_int64_t
     type(c_ptr), value :: e
-    integer(c_int64_t), intent(out) :: info(8)
+    integer(c_int64_t), intent(out) :: info(9)
   end function
   integer(c_int) function dfn_ensemble_kernel_timing(e, on) bind(C, name='dfn_ensemble_kernel_timing')
     import :: c_ptr, c_int, c_int32_t

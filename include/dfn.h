@@ -210,11 +210,13 @@ int dfn_solve(dfn_handle h, int64_t nsteps, double t_final, dfn_run_stats *stats
  * first), panel rows per wave step (2, 4), panel task order (0 as built, 1 largest
  * panels first), rows per panel task (0 whole panels, 32, 16), phase 1 and the
  * FP64 near-field panels of a DFN_PREC_MIXED layout in one launch (1) or two (0),
- * the rank ceiling of the phase-1 kernel instance (16, 20 or 32)} (7 entries);
+ * the rank ceiling of the phase-1 kernel instance (16, 20 or 32), phase-1 tasks of
+ * at most 32 row vectors two to a wave (1) or one (0)} (8 entries);
  * ms = {matvec time of the default shapes, of the chosen ones}.  None of the
  * choices changes a result.  DFN_AUTOTUNE=0 keeps the defaults, DFN_AUTOTUNE=alt
  * takes a set of non-default shapes without timing; DFN_FUSE_NEAR=1 or 0 takes or
- * refuses the single launch of entry 5 whatever its timing. */
+ * refuses the single launch of entry 5 whatever its timing, DFN_VTX_PAIRS=1 or 0
+ * the wave items of entry 7. */
 int dfn_tuning(dfn_handle h, int32_t *choice, double *ms);
 
 /* ---- Newton-V form ------------------------------------------------------------
@@ -405,10 +407,11 @@ int dfn_ensemble_set_recording(dfn_ensemble e, int32_t on);
 /* Moves up to cap recorded pairs (X: 8 floats each, T: 1) out of the ensemble;
  * with X or T NULL, *count = pairs available. */
 int dfn_ensemble_take_records(dfn_ensemble e, float *X, float *T, int64_t cap, int64_t *count);
-/* info[0..7] = networks, total elements, largest network, matrix entries streamed
+/* info[0..8] = networks, total elements, largest network, matrix entries streamed
  * per stress evaluation, algorithmic bytes per stress evaluation, graphs on, phase 1
  * and the FP64 near-field panels in one launch (1) or two (0), the rank ceiling of
- * the phase-1 kernel instance in use (16, 20 or 32). */
+ * the phase-1 kernel instance in use (16, 20 or 32), short phase-1 tasks two to a
+ * wave (1) or one (0). */
 int dfn_ensemble_info(dfn_ensemble e, int64_t *info);
 /* Per-launch timing of the dominant kernel while the ensemble steps (no reference
  * counterpart): the phase-2 panels, or phase 1 with the near-field panels when

@@ -165,26 +165,32 @@ def test_hmatvec_on_identical_block_lists(mode):
 def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
     """The autotuner only picks launch shapes that keep every sum in the same order:
     the stress rates of the tuned layout, of the default shapes and of every
-    alternative shape (DFN_AUTOTUNE=alt) are equal bit for bit."""
+    alternative shape (DFN_AUTOTUNE=alt) are equal bit for bit, and so are those with
+    short phase-1 tasks two to a wave or one (DFN_VTX_PAIRS)."""
     p = _hnet(2, n_faults=40, seed=8)
     keys = ("vtx_waves", "vtx_order", "panel_rows", "panel_order", "panel_split")
     out = []
-    # the mixed layout also runs phase 1 and its near-field panels as one launch or two
-    runs = [("1", None), ("0", None), ("alt", None)]
+    # (autotune, single launch of phase 1 and the near-field panels, phase-1 wave items)
+    runs = [("1", None, None), ("0", None, None), ("alt", None, None), ("0", None, "1"), ("1", None, "0"),
+            ("alt", None, "0")]
     if prec == _abi.PREC_MIXED:
-        runs += [("1", "1"), ("1", "0"), ("0", "1")]
-    for tune, fuse in runs:
+        runs += [("1", "1", None), ("1", "0", None), ("0", "1", None)]
+    for tune, fuse, pairs in runs:
         monkeypatch.setenv("DFN_AUTOTUNE", tune)
         if fuse is None:
             monkeypatch.delenv("DFN_FUSE_NEAR", raising=False)
         else:
             monkeypatch.setenv("DFN_FUSE_NEAR", fuse)
+        if pairs is None:
+            monkeypatch.delenv("DFN_VTX_PAIRS", raising=False)
+        else:
+            monkeypatch.setenv("DFN_VTX_PAIRS", pairs)
         d = dev(p)
         d.initiate_hmat()
         if prec != _abi.PREC_FP64:
             d.set_precision(prec)
         t = d.tuning()
-        print(f"[gpu] autotune={tune} {t}", flush=True)
+        print(f"[gpu] autotune={tune} fuse={fuse} pairs={pairs} {t}", flush=True)
         assert t["vtx_waves"] in (2, 4, 8) and t["vtx_order"] in (0, 1, 2) and t["panel_rows"] in (2, 4)
         assert t["panel_order"] in (0, 1) and t["panel_split"] in (0, 32, 16)
         if tune == "0":
@@ -194,6 +200,12 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
         assert t["rank_ceiling"] in (16, 20, 32)
         if tune == "alt":
             assert t["rank_ceiling"] == 32
+        if pairs is not None:
+            assert t["vtx_pairs"] == int(pairs)
+        elif tune == "alt":
+            assert t["vtx_pairs"] == 1
+        elif tune == "0":
+            assert t["vtx_pairs"] == 0
         if fuse is not None:
             assert t["fused_near"] == int(fuse)
         elif prec != _abi.PREC_MIXED:
