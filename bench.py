@@ -8,7 +8,7 @@ on the device.
 The default workload is the headline one: an ensemble of 64 seeded fault
 networks of 50-400 faults each (~1.37M elements, H-matrix eta 2 / ACA 1e-6 /
 leaf 64) advanced together, with reduced-precision operator storage read into
-FP64 products and sums (near-field entries as an FP32 value plus an FP16
+FP64 products and sums (near-field entries as an FP32 value plus an 8-bit
 residual, FP32 low-rank factors; --headline-precision mixed keeps the near-field
 in FP64; FP64 state, RHS, CG and error norm) and the learned step-size proposals (data/step_network.npz), against
 the plain FP64 path (one dfn_handle per network, FP64 operators, rkqs) over the
@@ -165,7 +165,7 @@ def mixed_precision(sim, p, args, kb64):
     scales, FP32 accumulation, compensated near-field sums), the same storage read
     into FP64 products and sums, and FP64 near-field blocks beside FP32 low-rank
     factors read into FP64 sums (the mixed mode), and the mixed mode with each
-    near-field entry split into an FP32 value and an FP16 residual (6 bytes), each
+    near-field entry split into an FP32 value and an 8-bit residual (5 bytes), each
     against the FP64 operator over
     random log-uniform slip rates in [1e-12, 1] m/s (norm-wise and on the rows above
     1e-3 of the largest stress rate); then FP32 solver steps continued from the
@@ -228,7 +228,7 @@ def mixed_precision(sim, p, args, kb64):
                                       alg_bytes=kbs["alg_bytes"],
                                       hmatvec_speedup_vs_fp64=kb64["ms_kernel"] / kbs["ms_kernel"],
                                       note="DFN_PREC_MIXED with each near-field entry stored as its FP32 value and "
-                                           "an FP16 residual times a power-of-two block scale "
+                                           "an 8-bit residual times a scale per panel task "
                                            "(DFN_PREC_MIXED_SPLIT)"),
                 fp32_storage_fp64_sums=dict(stress_rate_rel_err=stats(_abi.PREC_FP32_STORE),
                                             note="the FP32 storage read into FP64 products and sums "
@@ -476,7 +476,7 @@ def run_network(workload, args, mixed=True, cpu_seconds=None):
 def run_headline(args):
     """Configs 4 and 5: the 64-network ensemble with reduced-precision operator
     storage read into FP64 products and sums (FP32 low-rank factors; near-field
-    entries as an FP32 value and an FP16 residual, DFN_PREC_MIXED_SPLIT, or in FP64,
+    entries as an FP32 value and an 8-bit residual, DFN_PREC_MIXED_SPLIT, or in FP64,
     DFN_PREC_MIXED) and the learned step-size proposals, against the plain FP64 path
     (one dfn_handle per network, FP64 operators, rkqs) over the same simulated
     interval of every network; then, continuing, the ensemble with rkqs in the same
@@ -497,8 +497,9 @@ def run_headline(args):
                 max_elements=int(Ns.max()), seed=args.seed,
                 friction="per network: a~U[0.008,0.012] per fault, b=0.01 (70%) / 0.006, Dc=1 mm, sigma_n=-30 MPa",
                 pulse="10 MPa on one seeded element per network (fluid diffusion on)",
-                precision=("DFN_PREC_MIXED_SPLIT: near-field entries as their FP32 value plus an FP16 residual "
-                           "times a power-of-two scale per panel task (about 35 bits); " if split else
+                precision=("DFN_PREC_MIXED_SPLIT: near-field entries as their FP32 value plus an 8-bit residual "
+                           "times a scale per panel task (5 bytes; at least 24, for a task's largest entries "
+                           "about 31 significant bits); " if split else
                            "DFN_PREC_MIXED: FP64 near-field blocks; ") +
                           "FP32 low-rank factors (power-of-two block scales); all read into FP64 products and sums; "
                           "FP64 state, RHS, CG and error norm",
@@ -552,7 +553,7 @@ def run_headline(args):
     fused = bool(ens.info()["fused_near"])
     dom_name = ("vtx_near_kernel: phase 1 over the FP32 low-rank V factors (FP64 sums) and the FP64 near-field "
                 "panels in one launch; the whole ensemble's tasks" if fused else
-                "phase-2 panels (panel_kernel_split on the near-field set stored as FP32 + FP16 residual, then "
+                "phase-2 panels (panel_kernel_split on the near-field set stored as FP32 + 8-bit residual, then "
                 "panel_kernel_f32w on the FP32 low-rank set; the whole ensemble's panels)" if split else
                 "phase-2 panels (panel_kernel_f64 on the FP64 near-field set, then panel_kernel_f32w on the FP32 "
                 "low-rank set; the whole ensemble's panels)")
@@ -694,7 +695,7 @@ def main():
     ap.add_argument("--ensemble-box", type=float, default=7000.0, help="box size (m) of the ensemble's networks")
     ap.add_argument("--headline-precision", choices=("split", "mixed"), default="split",
                     help="operator storage of the headline ensemble: split (DFN_PREC_MIXED_SPLIT, near-field entries "
-                         "as FP32 + FP16 residual) or mixed (DFN_PREC_MIXED, FP64 near-field)")
+                         "as FP32 + 8-bit residual) or mixed (DFN_PREC_MIXED, FP64 near-field)")
     ap.add_argument("--ensemble-single-sample", type=int, default=16,
                     help="--workload ensemble: networks also run one handle at a time")
     ap.add_argument("--no-graph", action="store_true",

@@ -9,6 +9,7 @@ namespace dfn {
 constexpr int kMaxRank = 32;          // largest low-rank block rank the phase-1 kernel keeps in registers
 constexpr int kPanelMaxCols = 2048;   // columns per phase-2 panel task (w staged in 16 KB of LDS)
 constexpr int kSplitMaxCols = 1024;   // columns per task of split near-field panels (w in FP64 and FP32: 12 KB)
+constexpr int kSplitBytes = 5;        // bytes per split near-field entry (FP32 value, 8-bit residual)
 constexpr int kVChunkRows = 512;      // rows of V per phase-1 wave task
 constexpr int kXPad = 4;              // zero entries after x_h in w: phase 1 reads whole 16-byte vectors
 constexpr int kCgMaxIter = 200;       // compute_P gives up after 200 iterations (fluid_diffusion.f90:120)
@@ -179,9 +180,10 @@ void launch_zreduce(const ZRed *zr, int nb, const double *zpart, double *w, hipS
 void launch_vtx_near(const VTask *vt, int nvt, const float *vdata, const PTask *pt, int npt, const double *pdata,
                      const int *widx, double *w, double *zpart, double *ypart, hipStream_t s, int rows = 2,
                      const int *active = nullptr, int kmax = kMaxRank);
-// near-field panels stored as FP32 values with FP16 residuals (hi, lo; cscale the
-// residual's power-of-two scale per column); rows and active as for launch_panel
-void launch_panel_split(const PTask *t, int nt, const float *hi, const _Float16 *lo, const int *widx,
+// near-field panels stored as FP32 values (hi) and 8-bit residuals (lo, same entry
+// offsets; cscale at a task's first column the residual's scale); rows and active
+// as for launch_panel
+void launch_panel_split(const PTask *t, int nt, const float *hi, const signed char *lo, const int *widx,
                         const float *cscale, const double *w, double *ypart, hipStream_t s, int rows = 2,
                         const int *active = nullptr);
 // rows: panel rows per wave step, 2 or 4; active as for launch_vtx

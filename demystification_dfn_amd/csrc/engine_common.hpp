@@ -82,9 +82,9 @@ struct DevOps {
   // set A holds the near-field (dense) columns in FP64 and set B the U columns in FP32
   bool pa_fp32 = false;
   DMem ptasks_b, pdata_b, widx_b, cscale_b;
-  // split: set A holds FP32 values (pdata) and FP16 residuals (pdata_lo) of the
-  // near-field entries, cscale the residual's power-of-two scale of each task
-  // (repeated over the task's columns)
+  // split: set A holds the near-field entries as FP32 values (pdata) and 8-bit
+  // residuals (pdata_lo, the same entry offsets), cscale the residual's scale of
+  // each task (repeated over the task's columns)
   bool split = false;
   DMem pdata_lo;
   int npt_b = 0;
@@ -196,7 +196,7 @@ inline void launch_mrhs(const DevOps &d, const double *X, int nrhs, int R, doubl
 // per wave step; active as for launch_panel)
 inline void launch_panels(const DevOps &d, const double *w, hipStream_t s, int rows, const int *active = nullptr) {
   if (d.split)
-    launch_panel_split(d.ptasks.as<PTask>(), d.npt, d.pdata.as<float>(), d.pdata_lo.as<_Float16>(), d.widx.as<int>(),
+    launch_panel_split(d.ptasks.as<PTask>(), d.npt, d.pdata.as<float>(), d.pdata_lo.as<signed char>(), d.widx.as<int>(),
                        d.cscale.as<float>(), w, d.ypart.as<double>(), s, rows, active);
   else
     launch_panel(d.ptasks.as<PTask>(), d.npt, d.pdata.p, d.pmode_a(), d.widx.as<int>(), d.cscale.as<float>(), w,
@@ -415,7 +415,7 @@ inline double pow2_scale(const double *v, size_t n) {
 // kernels can skip the tasks of networks that sit out an evaluation.
 // T: storage of the low-rank factors (and of the near-field blocks unless dense64,
 // which keeps them in their own panel set: in FP64, or with split as an FP32 value
-// and an FP16 residual per entry).
+// and an 8-bit residual per entry).
 template <class T>
 int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::vector<int> *hnet = nullptr,
                bool dense64 = false, bool acc64 = false, bool split = false) {
@@ -541,7 +541,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   struct PanelSet {
     int esize;    // bytes per stored entry
     bool scales;  // per-column scales: FP32 storage's block scales, or the split form's task residual scale
-    int pad;      // rows padded to a multiple of this many entries (128 bytes of every stream)
+    int pad;      // rows padded to a multiple of this many entries
     int maxc;     // columns per task at most (the w entries its kernel stages in LDS)
     std::vector<PTask> pt;
     std::vector<int> widx;
@@ -549,7 +549,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
     std::vector<PFill> pfill;
     int64_t psize = 0, entries = 0;
   };
-  PanelSet setA{split ? 6 : dense64 ? 8 : (int)sizeof(T), scaled && (!dense64 || split),
+  PanelSet setA{split ? kSplitBytes : dense64 ? 8 : (int)sizeof(T), scaled && (!dense64 || split),
                 split ? 64 : dense64 ? 16 : 128 / (int)sizeof(T), split ? kSplitMaxCols : kPanelMaxCols},
       setB{(int)sizeof(T), scaled, 128 / (int)sizeof(T), kPanelMaxCols};
   std::vector<Col> cols;
@@ -561,7 +561,9 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
     // rows padded with zero entries (on column 0 of w) to a multiple of 128 bytes, so
     // every row starts on a 128-byte line: a line shared by two rows is fetched
     // twice by the streaming loads (9 % more bytes fetched and 5-10 % more time
-    // with 16-byte rows); a zero entry adds an exact zero to its lane's sum
+    // with 16-byte rows; the split set's rows are 64 entries: 256 bytes of FP32
+    // values, and residual rows that fill a line per two rows); a zero entry adds an
+    // exact zero to its lane's sum
     const int pad = S.pad;
     S.entries += ncols * m;
     for (int64_t q0 = 0; q0 < ncols; q0 += S.maxc) {
@@ -635,36 +637,35 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
       }
     });
   };
-  if (split) {  // near-field entries as hi = (float)entry and lo = (entry - hi) / s in FP16
+  if (split) {  // near-field entries as hi = (float)entry and an 8-bit residual, two streams
     std::unique_ptr<float[]> ph(new float[std::max<int64_t>(setA.psize, 1)]);
-    std::unique_ptr<_Float16[]> pl(new _Float16[std::max<int64_t>(setA.psize, 1)]);
+    std::unique_ptr<signed char[]> pq(new signed char[std::max<int64_t>(setA.psize, 1)]);
     parallel_for((int64_t)setA.pfill.size(), [&](int64_t q) {
       const PFill &f = setA.pfill[q];  // built beside setA.pt[q]
       const Col *c = cols.data() + f.col0;
-      // s: one power of two per task, the largest |entry - (float)entry| of the task
-      // times 2^-15 rounded up, so that |residual / s| < 2^15 (FP16 holds 65504)
+      // s: the task's largest |entry - (float)entry| over 127, so that every residual
+      // rounds to an integer in [-127, 127]
       double rmax = 0.0;
       for (int j = 0; j < f.L; j++)
         for (int r = 0; r < f.m; r++) rmax = std::fmax(rmax, std::fabs(c[j].src[r] - (double)(float)c[j].src[r]));
-      int ex = 0;
-      if (rmax > 0.0) (void)std::frexp(rmax, &ex);  // rmax in [2^(ex-1), 2^ex)
-      const double s = rmax > 0.0 ? std::ldexp(1.0, ex - 15) : 1.0;
-      std::fill_n(setA.cscale.begin() + setA.pt[q].widx, f.Lp, (float)s);
+      const float s = rmax > 0.0 ? (float)(rmax / 127.0) : 1.0f;
+      std::fill_n(setA.cscale.begin() + setA.pt[q].widx, f.Lp, s);
       for (int r = 0; r < f.m; r++) {
         float *hr = ph.get() + f.poff + (int64_t)r * f.Lp;
-        _Float16 *lr = pl.get() + f.poff + (int64_t)r * f.Lp;
+        signed char *qr = pq.get() + f.poff + (int64_t)r * f.Lp;
         for (int j = 0; j < f.Lp; j++) {
           const double v = j < f.L ? c[j].src[r] : 0.0;
           const float h = (float)v;
           hr[j] = h;
-          lr[j] = (_Float16)((v - (double)h) / s);
+          const double qd = std::nearbyint((v - (double)h) / (double)s);
+          qr[j] = (signed char)std::max(-127.0, std::min(127.0, qd));
         }
       }
     });
     std::unique_ptr<T[]> pb(new T[std::max<int64_t>(setB.psize, 1)]);
     fill(setB, pb.get());
     DFN_TRY(d.pdata.upload(ph.get(), (size_t)setA.psize));
-    DFN_TRY(d.pdata_lo.upload(pl.get(), (size_t)setA.psize));
+    DFN_TRY(d.pdata_lo.upload(pq.get(), (size_t)setA.psize));
     DFN_TRY(d.pdata_b.upload(pb.get(), (size_t)setB.psize));
   } else if (dense64) {
     std::unique_ptr<double[]> pa(new double[std::max<int64_t>(setA.psize, 1)]);
@@ -719,7 +720,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
   d.S_total = S_total;
   d.S_panel = S_panel;
   // bytes: factors + x read + y written, per operator (x shared in Mode II)
-  const double dense_extra = dense64 ? ((split ? 6.0 : 8.0) - (double)sizeof(T)) * (double)S_dense : 0.0;
+  const double dense_extra = dense64 ? ((split ? (double)kSplitBytes : 8.0) - (double)sizeof(T)) * (double)S_dense : 0.0;
   d.alg_bytes = (double)sizeof(T) * (double)S_total + dense_extra + 8.0 * n + 8.0 * n * nops;
   d.panel_bytes = (double)setA.esize * (double)setA.entries + (double)setB.esize * (double)setB.entries +
                   4.0 * (double)((split ? setA.pt.size() : setA.cscale.size()) + setB.cscale.size()) + 8.0 * (double)yp;

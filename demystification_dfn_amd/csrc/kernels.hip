@@ -651,20 +651,20 @@ __global__ void __launch_bounds__(256) panel_kernel_f32w(const PTask *__restrict
   }
 }
 
-// Phase 2 over near-field entries stored as an FP32 value and an FP16 residual:
-// entry = hi + lo * s, with hi = (float)entry, lo the FP16 rounding of
-// (entry - hi) / s and s a power of two per task (cscale at the task's first
-// column).  The hi products and sums are FP64; the residual terms, each below
-// 2^-24 of its entry, are summed per lane in FP32 against (float)w (their rounding
-// stays below 2^-44 of the row's absolute sum), scaled by s and added to the FP64
-// sum before the wave's reduction.  An entry keeps about 35 significant bits in 6
-// bytes.  Tasks of at most kSplitMaxCols columns keep w (FP64 and FP32) in 12 KB
-// of LDS, so that LDS leaves room for eight workgroups per CU.
-typedef _Float16 hv4 __attribute__((ext_vector_type(4)));
-
+// Phase 2 over near-field entries stored as an FP32 value and an 8-bit residual:
+// entry = hi + q * s, with hi = (float)entry, q the rounding of (entry - hi) / s to an
+// integer in [-127, 127] and s = (the task's largest |entry - hi|) / 127 (cscale at the
+// task's first column).  Two streams: the FP32 values, rows of L entries (256-byte
+// multiples), and the residual bytes, rows of L bytes (so rows 2i and 2i + 1, which
+// the same wave step reads, fill whole 128-byte lines).  The hi products and sums are
+// FP64; the residual terms, each below 2^-24 of its entry, are summed per lane in
+// FP32 against (float)w, scaled by s and added to the FP64 sum before the wave's
+// reduction.  The largest entries of a task keep about 31 significant bits, every
+// entry at least FP32's 24, in 5 bytes.  Tasks of at most kSplitMaxCols columns keep
+// w (FP64 and FP32) in 12 KB of LDS.
 template <int R>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 2 ? 6 : 4)))
-panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi, const _Float16 *__restrict__ lo,
+panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi, const signed char *__restrict__ lo,
                    const int *__restrict__ widx, const float *__restrict__ cscale, const double *__restrict__ w,
                    double *__restrict__ ypart, const int *__restrict__ active) {
   __shared__ double2 wl2[kSplitMaxCols / 2];
@@ -683,17 +683,17 @@ panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int L4 = task.L >> 2;
   const fv4 *H = reinterpret_cast<const fv4 *>(hi + task.poff);
-  const hv4 *Lo = reinterpret_cast<const hv4 *>(lo + task.poff);
+  const unsigned *Q = reinterpret_cast<const unsigned *>(lo + task.poff);
   for (int r = wave * R; r < task.m; r += 4 * R) {
     const fv4 *hrow[R];
-    const hv4 *lrow[R];
+    const unsigned *qrow[R];
     double a[R];
     float b[R];
 #pragma unroll
     for (int q = 0; q < R; q++) {
       const int rr = r + q < task.m ? r + q : r;  // rows past the panel re-read its row r and are not stored
       hrow[q] = H + (int64_t)rr * L4;
-      lrow[q] = Lo + (int64_t)rr * L4;
+      qrow[q] = Q + (int64_t)rr * L4;
       a[q] = 0.0;
       b[q] = 0.0f;
     }
@@ -702,17 +702,19 @@ panel_kernel_split(const PTask *__restrict__ tasks, const float *__restrict__ hi
       const double2 w01 = wl2[2 * j], w23 = wl2[2 * j + 1];
       const fv4 f = wf4[j];
       fv4 h[R];
-      hv4 l[R];
+      unsigned l[R];
 #pragma unroll
       for (int q = 0; q < R; q++) {
         h[q] = __builtin_nontemporal_load(hrow[q] + j);
-        l[q] = __builtin_nontemporal_load(lrow[q] + j);
+        l[q] = __builtin_nontemporal_load(qrow[q] + j);
       }
 #pragma unroll
       for (int q = 0; q < R; q++) {
         a[q] = fma((double)h[q].w, w23.y, fma((double)h[q].z, w23.x, fma((double)h[q].y, w01.y,
                    fma((double)h[q].x, w01.x, a[q]))));
-        b[q] = fmaf((float)l[q].w, f.w, fmaf((float)l[q].z, f.z, fmaf((float)l[q].y, f.y, fmaf((float)l[q].x, f.x, b[q]))));
+        const float q0 = (float)(signed char)l[q], q1 = (float)(signed char)(l[q] >> 8);
+        const float q2 = (float)(signed char)(l[q] >> 16), q3 = (float)(signed char)(l[q] >> 24);
+        b[q] = fmaf(q3, f.w, fmaf(q2, f.z, fmaf(q1, f.y, fmaf(q0, f.x, b[q]))));
       }
     }
     double t[R];
@@ -2299,7 +2301,7 @@ void launch_panel(const PTask *t, int nt, const void *pdata, int mode, const int
                        active);
 }
 
-void launch_panel_split(const PTask *t, int nt, const float *hi, const _Float16 *lo, const int *widx,
+void launch_panel_split(const PTask *t, int nt, const float *hi, const signed char *lo, const int *widx,
                         const float *cscale, const double *w, double *ypart, hipStream_t s, int rows,
                         const int *active) {
   if (nt <= 0) return;

@@ -116,8 +116,9 @@ int dfn_build_hmat(dfn_handle h);
  * DFN_PREC_FP32_STORE: the FP32 storage of DFN_PREC_FP32 (same bytes) read into
  *   FP64 products and sums, so that only the storage rounds;
  * DFN_PREC_MIXED_SPLIT: as DFN_PREC_MIXED, with each near-field entry stored in
- *   6 bytes instead of 8: its FP32 rounding and the FP16 residual times a
- *   power-of-two scale per block (about 35 significant bits).
+ *   5 bytes instead of 8: its FP32 rounding and an 8-bit residual times a scale
+ *   per panel task (about 31 significant bits for a task's largest entries, at
+ *   least FP32's 24 for every entry).
  * No reference counterpart (the reference is FP64 throughout). */
 int dfn_set_precision(dfn_handle h, int32_t precision);
 /* stats[0..9] = {nb_dense, nb_lowrank, S_dense, S_lowrank_U, S_lowrank_V, max_rank,

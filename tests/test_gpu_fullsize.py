@@ -91,7 +91,7 @@ def test_full_size_mixed_operator_error(pair):
     into FP64 sums: the stress-rate error over 20 log-uniform slip-rate vectors,
     norm-wise and on the rows above 1e-3 of the largest stress rate (bar 1e-5),
     beside all-FP32 storage, and with the near-field entries split into an FP32
-    value and an FP16 residual (DFN_PREC_MIXED_SPLIT).  The FP64 references come
+    value and an 8-bit residual (DFN_PREC_MIXED_SPLIT).  The FP64 references come
     from one multi-RHS call."""
     p, d, _ = pair
     rng = np.random.default_rng(13)

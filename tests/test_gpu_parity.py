@@ -334,9 +334,10 @@ def test_mixed_operator_storage_error(mode):
     largest stress rate within 1e-5 relative -- the rows where all-FP32 storage
     loses the most, because an entry stored in FP32 keeps its rounding times the
     row's cancellation ratio.  The split near-field storage (an FP32 value and an
-    FP16 residual per entry, DFN_PREC_MIXED_SPLIT) keeps about 35 bits of each
-    near-field entry: its stress rates sit within 1e-8 of the FP64 near-field
-    layout's, far closer than the 1e-6 an FP32 near-field alone would allow."""
+    8-bit residual per entry, DFN_PREC_MIXED_SPLIT) keeps about 31 bits of a task's
+    largest near-field entries: its stress rates sit within 1e-8 of the FP64
+    near-field layout's, far closer than the 1e-6 an FP32 near-field alone would
+    allow."""
     p = _hnet(mode, n_faults=20, seed=9)
     d = dev(p)
     d.initiate_hmat()
