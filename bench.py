@@ -42,7 +42,7 @@ METRIC = "simulated yrs/wall-hr + H-matvec GB/s (%HBM peak), mixed+learned vs pl
 # Keyed by (workload, seed, phase 1 fused with the near-field panels).
 MEASURED_TRAFFIC = {
     ("network800_80k", 2024, False): 2.4846e9,
-    ("ensemble64_1374k_split_learned", 2024, False): 18.762e9,
+    ("ensemble64_1374k_split_learned", 2024, False): 17.550e9,
     ("ensemble64_1374k_mixed_learned", 2024, False): 21.246e9,
     ("ensemble64_1374k_mixed_learned", 2024, True): 20.702e9,
 }
