@@ -103,6 +103,8 @@ __device__ __forceinline__ double swap32_sum(double a, double b) {
   return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
 }
 __device__ __forceinline__ float swap32_sum(float a, float b) {
+  // the two results are copied to scalars first: __builtin_bit_cast applied to an
+  // element of the returned vector read element 0 for both
   const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
   const unsigned r0 = r[0], r1 = r[1];
   return __uint_as_float(r0) + __uint_as_float(r1);
