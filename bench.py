@@ -535,7 +535,7 @@ def run_headline(args):
                                   accepted=acc, rejected=rej, rejected_fraction=rej / max(acc + rej, 1),
                                   window_start_s_median=float(np.median(ta)))
 
-    head_dtype = ("f32+f16-residual near-field / f32 low-rank storage, f64 arithmetic and state" if split else
+    head_dtype = ("f32 + 8-bit residual near-field / f32 low-rank storage, f64 arithmetic and state" if split else
                   "f64 near-field / f32 low-rank storage, f64 arithmetic and state")
     ens.solve(args.warmup)
     tA, tB, wall, head = window(args.steps)
