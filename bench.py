@@ -161,7 +161,7 @@ def device_state(sim):
 
 
 def mixed_precision(sim, p, args, kb64):
-    """Config 3 on a single network: FP32 operator storage (per-block power-of-two
+    """Reduced operator storage on a single network: FP32 (per-block power-of-two
     scales, FP32 accumulation, compensated near-field sums), the same storage read
     into FP64 products and sums, and FP64 near-field blocks beside FP32 low-rank
     factors read into FP64 sums (the mixed mode), and the mixed mode with each
@@ -382,7 +382,7 @@ def run_ensemble(args):
 def run_network(workload, args, mixed=True, cpu_seconds=None):
     """One network on one dfn_handle: warmup, `steps` timed solver steps, the
     dominant kernel timed live over further steps, the isolated matvec, FP32
-    operators (config 3) and the CPU baseline from the state at the start of the
+    operator storage and the CPU baseline from the state at the start of the
     timed window.  Returns the result dict."""
     from demystification_dfn_amd.device import DeviceSimulation
     p, desc = make_workload(workload, args.seed)
@@ -721,7 +721,7 @@ def main():
         return
     out = run_headline(args)
     if not args.no_sub:
-        # config 2 (FP64, with the FP32-operator run of config 3) and its connected variant
+        # the 800-fault 80k-element network (FP64, with its reduced-storage runs) and its connected variant
         out["network80k"] = run_network("network80k", args, mixed=True, cpu_seconds=args.cpu_seconds / 2)
         out["network80k_connected"] = run_network("network80k_connected", args, mixed=False, cpu_seconds=0)
     out["copy_kernel_GB_s"] = bench_stream(int(args.copy_probe_gb * (1 << 30)), 20) if args.copy_probe_gb else None
