@@ -163,7 +163,8 @@ void launch_gather(const double *x, const int *perm, double *xh, int n, hipStrea
 // storage and arithmetic of an operator's factors and panels (mode of launch_vtx and
 // launch_panel): FP64; FP32 with FP32 products and sums (compensated on near-field
 // columns); FP32 storage widened into FP64 products and sums
-enum { kStoreF64 = 0, kStoreF32 = 1, kStoreF32Acc64 = 2 };
+// kStoreF64NT: FP64 factors read with non-temporal loads (phase 1 only)
+enum { kStoreF64 = 0, kStoreF32 = 1, kStoreF32Acc64 = 2, kStoreF64NT = 3 };
 // waves: tasks (waves) per workgroup, 2, 4 or 8; xf: the FP32 copy (kStoreF32);
 // active (ensembles, may be null): tasks of networks with active[net] == 0 return at once;
 // kmax: the largest rank of any task (at most kMaxRank)

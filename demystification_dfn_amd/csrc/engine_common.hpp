@@ -96,7 +96,7 @@ struct DevOps {
   // x_h) may run as one launch (launch_vtx_near); near_ok says the layout allows it,
   // fused that the autotuner chose it
   bool near_ok = false, fused = false;
-  int vmode() const { return fp32 ? (acc64 ? kStoreF32Acc64 : kStoreF32) : kStoreF64; }
+  int vmode() const { return fp32 ? (acc64 ? kStoreF32Acc64 : kStoreF32) : vnt ? kStoreF64NT : kStoreF64; }
   int pmode_a() const { return pa_fp32 ? (acc64 ? kStoreF32Acc64 : kStoreF32) : kStoreF64; }
   int pmode_b() const { return acc64 ? kStoreF32Acc64 : kStoreF32; }
   int nvt = 0, nzr = 0, npt = 0;
@@ -123,6 +123,8 @@ struct DevOps {
   bool vpair = false;
   DMem vitems;
   int nvi = 0;
+  // FP64 phase-1 factors read with non-temporal loads (the same values)
+  bool vnt = false;
 };
 
 // Phase-1 wave items over the task list in its launch order: each task of at most 32
@@ -314,6 +316,7 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
   double best = 0, t = 0;
   if (alt) {
     waves = 8; order = 2; rows = 4; pord = 1; split = 16; pair = true;
+    d.vnt = !d.fp32;
     d.kc = kMaxRank;  // the rank-32 instance (the single launch is exercised by DFN_FUSE_NEAR=1)
     DFN_TRY(vorder(order));
     DFN_TRY(vpairs(pair));
@@ -339,6 +342,12 @@ inline int autotune(DevOps &d, const std::vector<VTask> &vt, const std::vector<P
       pair = t1 < 0.995 * t;
       DFN_TRY(vpairs(pair));
       if (pair) DFN_TRY(measure(waves, rows, best));
+    }
+    if (!d.fp32) {  // non-temporal FP64 factor loads, judged on the whole product
+      d.vnt = true;
+      DFN_TRY(measure(waves, rows, t));
+      if (t < 0.99 * best) best = t;
+      else d.vnt = false;
     }
     DFN_TRY(measure(waves, 4, t));
     if (t < 0.99 * best) { best = t; rows = 4; }
@@ -740,6 +749,7 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
                     8.0 * (double)rows_a;
   }
   d.fused = false;
+  d.vnt = false;
   const char *at = std::getenv("DFN_AUTOTUNE");
   if (!(at && std::string(at) == "0")) {
     DFN_TRY(autotune(d, vt, pt, setB.pt));
@@ -749,6 +759,9 @@ int layout_ops(const HostOperator *const *ops, int nops, DevOps &d, const std::v
     d.nvi = (int)items.size();
     d.vpair = true;
   }
+  // DFN_VTX_NT=1 / 0 takes / refuses the non-temporal FP64 factor loads
+  const char *nt = std::getenv("DFN_VTX_NT");
+  if (nt && (nt[0] == '0' || nt[0] == '1')) d.vnt = !d.fp32 && nt[0] == '1';
   // DFN_FUSE_NEAR=1 / 0 takes / refuses the single launch of phase 1 and the
   // near-field panels whatever the timing (the results are the same either way)
   const char *fuse = std::getenv("DFN_FUSE_NEAR");

@@ -214,6 +214,14 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
   }
 }
 
+// A vector of V factors; NT: a non-temporal load (an autotuner candidate for FP64
+// factors: the values read are the same either way)
+template <bool NT, typename VEC>
+__device__ __forceinline__ VEC load_v(const VEC *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // Phase 1: one wave per task.  The body is specialised on the rank K (a
 // wave-uniform switch), so the K column loads of a row strip are all in flight
 // together; each lane takes a 16-byte vector of consecutive rows (2 FP64 / 4 FP32),
@@ -221,7 +229,7 @@ __global__ void __launch_bounds__(256) gather_perm_kernel(const double *__restri
 // moves.  The task's V is stored strip by strip (64 vectors of rows, column after
 // column; the last strip's height padded with zero rows to a whole vector), so the
 // K loads of one step read one contiguous run.
-template <int K, typename TV, bool ACC64, bool GROUPS>
+template <int K, typename TV, bool ACC64, bool GROUPS, bool NT = false>
 __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict__ vdata, const double *__restrict__ w,
                                          const float *__restrict__ xf, double *__restrict__ zpart,
                                          double *__restrict__ wz, int lane) {
@@ -267,7 +275,7 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
       tvec v[G];
 #pragma unroll
       for (int j = 0; j < G; j++)
-        if (j0 + j < K) v[j] = *reinterpret_cast<const tvec *>(Vs + (j0 + j) * hp);
+        if (j0 + j < K) v[j] = load_v<NT>(reinterpret_cast<const tvec *>(Vs + (j0 + j) * hp));
 #pragma unroll
       for (int j = 0; j < G; j++)
         if (j0 + j < K)
@@ -290,13 +298,13 @@ __device__ __forceinline__ void vtx_body(const VTask &task, const TV *__restrict
 // the largest rank of the layout rounded up to 16, 20 or 32: the register budget of
 // the whole kernel is that of its largest case, so a layout whose ranks stay at or
 // below 20 keeps more waves resident (FP64 storage 88 instead of 135 VGPRs)
-template <typename TV, bool ACC64, int KMAX, bool GROUPS>
+template <typename TV, bool ACC64, int KMAX, bool GROUPS, bool NT = false>
 __device__ __forceinline__ void vtx_task(const VTask &task, const TV *__restrict__ vdata, double *__restrict__ w,
                                          const float *__restrict__ xf, double *__restrict__ zpart, int lane) {
   switch (task.k) {
 #define DFN_VTX_CASE(K) \
   case K:                                                                         \
-    if constexpr (K <= KMAX) vtx_body<K, TV, ACC64, GROUPS>(task, vdata, w, xf, zpart, w, lane); \
+    if constexpr (K <= KMAX) vtx_body<K, TV, ACC64, GROUPS, NT>(task, vdata, w, xf, zpart, w, lane); \
     break;
     DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
     DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
@@ -314,7 +322,7 @@ __device__ __forceinline__ void vtx_task(const VTask &task, const TV *__restrict
 // lane (lane & 31) of a wave of its own would; the halves are reduced apart.  A
 // task's sums are those of vtx_body: there the lanes from 32 on hold zeros for a
 // task of at most 32 row vectors, and the reduction's first step adds them.
-template <int K, typename TV, bool ACC64, bool GROUPS>
+template <int K, typename TV, bool ACC64, bool GROUPS, bool NT = false>
 __device__ __forceinline__ void vtx_pair_body(const VTask &ta, const VTask &tb, const TV *__restrict__ vdata,
                                               const double *__restrict__ w, const float *__restrict__ xf,
                                               double *__restrict__ zpart, double *__restrict__ wz, int lane) {
@@ -348,7 +356,7 @@ __device__ __forceinline__ void vtx_pair_body(const VTask &ta, const VTask &tb, 
       tvec v[G];
 #pragma unroll
       for (int j = 0; j < G; j++)
-        if (j0 + j < K) v[j] = *reinterpret_cast<const tvec *>(Vs + (j0 + j) * hp);
+        if (j0 + j < K) v[j] = load_v<NT>(reinterpret_cast<const tvec *>(Vs + (j0 + j) * hp));
 #pragma unroll
       for (int j = 0; j < G; j++)
         if (j0 + j < K)
@@ -367,14 +375,14 @@ __device__ __forceinline__ void vtx_pair_body(const VTask &ta, const VTask &tb, 
   }
 }
 
-template <typename TV, bool ACC64, int KMAX, bool GROUPS>
+template <typename TV, bool ACC64, int KMAX, bool GROUPS, bool NT = false>
 __device__ __forceinline__ void vtx_pair_task(const VTask &ta, const VTask &tb, const TV *__restrict__ vdata,
                                               double *__restrict__ w, const float *__restrict__ xf,
                                               double *__restrict__ zpart, int lane) {
   switch (ta.k) {
 #define DFN_VTX_CASE(K) \
   case K:                                                                                       \
-    if constexpr (K <= KMAX) vtx_pair_body<K, TV, ACC64, GROUPS>(ta, tb, vdata, w, xf, zpart, w, lane); \
+    if constexpr (K <= KMAX) vtx_pair_body<K, TV, ACC64, GROUPS, NT>(ta, tb, vdata, w, xf, zpart, w, lane); \
     break;
     DFN_VTX_CASE(1) DFN_VTX_CASE(2) DFN_VTX_CASE(3) DFN_VTX_CASE(4) DFN_VTX_CASE(5) DFN_VTX_CASE(6)
     DFN_VTX_CASE(7) DFN_VTX_CASE(8) DFN_VTX_CASE(9) DFN_VTX_CASE(10) DFN_VTX_CASE(11) DFN_VTX_CASE(12)
@@ -389,11 +397,11 @@ __device__ __forceinline__ void vtx_pair_task(const VTask &ta, const VTask &tb, 
 
 // Phase 1 by wave items: one task, or two short tasks of the same rank and network
 // (VItem), per wave.
-template <typename TV, bool ACC64, int KMAX>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
-lowrank_vtx_items_kernel(const VTask *__restrict__ tasks, const VItem *__restrict__ items, int nitems,
-                         const TV *__restrict__ vdata, double *__restrict__ w, const float *__restrict__ xf,
-                         double *__restrict__ zpart, const int *__restrict__ active) {
+template <typename TV, bool ACC64, int KMAX, bool NT>
+__device__ __forceinline__ void vtx_items_wave(const VTask *__restrict__ tasks, const VItem *__restrict__ items,
+                                               int nitems, const TV *__restrict__ vdata, double *__restrict__ w,
+                                               const float *__restrict__ xf, double *__restrict__ zpart,
+                                               const int *__restrict__ active) {
   const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (t >= nitems) return;
@@ -401,27 +409,58 @@ lowrank_vtx_items_kernel(const VTask *__restrict__ tasks, const VItem *__restric
   const VTask ta = tasks[it.a];
   if (active && !active[ta.net]) return;  // the items' network sits out this evaluation
   if (it.b < 0) {
-    vtx_task<TV, ACC64, KMAX, true>(ta, vdata, w, xf, zpart, lane);
+    vtx_task<TV, ACC64, KMAX, true, NT>(ta, vdata, w, xf, zpart, lane);
   } else {
     const VTask tb = tasks[it.b];
-    vtx_pair_task<TV, ACC64, KMAX, true>(ta, tb, vdata, w, xf, zpart, lane);
+    vtx_pair_task<TV, ACC64, KMAX, true, NT>(ta, tb, vdata, w, xf, zpart, lane);
   }
 }
 
-template <typename TV, bool ACC64, int KMAX>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
-lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks,
-                                                          const TV *__restrict__ vdata, double *__restrict__ w,
-                                                          const float *__restrict__ xf, double *__restrict__ zpart,
-                                                          const int *__restrict__ active) {
-  // one task per wave, blockDim.x / 64 waves per workgroup; the task index is
-  // uniform across the wave: scalar loads of the task and a scalar branch on its rank
+// one task per wave, blockDim.x / 64 waves per workgroup; the task index is
+// uniform across the wave: scalar loads of the task and a scalar branch on its rank
+template <typename TV, bool ACC64, int KMAX, bool NT>
+__device__ __forceinline__ void vtx_tasks_wave(const VTask *__restrict__ tasks, int ntasks,
+                                               const TV *__restrict__ vdata, double *__restrict__ w,
+                                               const float *__restrict__ xf, double *__restrict__ zpart,
+                                               const int *__restrict__ active) {
   const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   if (t >= ntasks) return;
   const VTask task = tasks[t];
   if (active && !active[task.net]) return;  // the block's network sits out this evaluation
-  vtx_task<TV, ACC64, KMAX, true>(task, vdata, w, xf, zpart, lane);
+  vtx_task<TV, ACC64, KMAX, true, NT>(task, vdata, w, xf, zpart, lane);
+}
+
+template <typename TV, bool ACC64, int KMAX>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
+lowrank_vtx_items_kernel(const VTask *__restrict__ tasks, const VItem *__restrict__ items, int nitems,
+                         const TV *__restrict__ vdata, double *__restrict__ w, const float *__restrict__ xf,
+                         double *__restrict__ zpart, const int *__restrict__ active) {
+  vtx_items_wave<TV, ACC64, KMAX, false>(tasks, items, nitems, vdata, w, xf, zpart, active);
+}
+
+template <typename TV, bool ACC64, int KMAX>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
+lowrank_vtx_kernel(const VTask *__restrict__ tasks, int ntasks, const TV *__restrict__ vdata, double *__restrict__ w,
+                   const float *__restrict__ xf, double *__restrict__ zpart, const int *__restrict__ active) {
+  vtx_tasks_wave<TV, ACC64, KMAX, false>(tasks, ntasks, vdata, w, xf, zpart, active);
+}
+
+// the same two kernels for FP64 factors read with non-temporal loads
+template <int KMAX>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
+lowrank_vtx_items_nt_kernel(const VTask *__restrict__ tasks, const VItem *__restrict__ items, int nitems,
+                            const double *__restrict__ vdata, double *__restrict__ w, const float *__restrict__ xf,
+                            double *__restrict__ zpart, const int *__restrict__ active) {
+  vtx_items_wave<double, true, KMAX, true>(tasks, items, nitems, vdata, w, xf, zpart, active);
+}
+
+template <int KMAX>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(KMAX <= 20 ? 5 : 2)))
+lowrank_vtx_nt_kernel(const VTask *__restrict__ tasks, int ntasks, const double *__restrict__ vdata,
+                      double *__restrict__ w, const float *__restrict__ xf, double *__restrict__ zpart,
+                      const int *__restrict__ active) {
+  vtx_tasks_wave<double, true, KMAX, true>(tasks, ntasks, vdata, w, xf, zpart, active);
 }
 
 // z of each multi-chunk low-rank block = sum of its chunk partials, in chunk
@@ -2219,6 +2258,9 @@ static void launch_vtx_k(const VTask *t, int nt, const void *vdata, int mode, do
     else if (mode == kStoreF32Acc64)
       hipLaunchKernelGGL((lowrank_vtx_items_kernel<float, true, KMAX>), dim3(nblk(nitems, waves)), dim3(64 * waves),
                          0, s, t, items, nitems, (const float *)vdata, w, xf, zpart, active);
+    else if (mode == kStoreF64NT)
+      hipLaunchKernelGGL((lowrank_vtx_items_nt_kernel<KMAX>), dim3(nblk(nitems, waves)), dim3(64 * waves), 0, s, t,
+                         items, nitems, (const double *)vdata, w, xf, zpart, active);
     else
       hipLaunchKernelGGL((lowrank_vtx_items_kernel<double, true, KMAX>), dim3(nblk(nitems, waves)), dim3(64 * waves),
                          0, s, t, items, nitems, (const double *)vdata, w, xf, zpart, active);
@@ -2230,6 +2272,9 @@ static void launch_vtx_k(const VTask *t, int nt, const void *vdata, int mode, do
   else if (mode == kStoreF32Acc64)
     hipLaunchKernelGGL((lowrank_vtx_kernel<float, true, KMAX>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
                        (const float *)vdata, w, xf, zpart, active);
+  else if (mode == kStoreF64NT)
+    hipLaunchKernelGGL((lowrank_vtx_nt_kernel<KMAX>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t, nt,
+                       (const double *)vdata, w, xf, zpart, active);
   else
     hipLaunchKernelGGL((lowrank_vtx_kernel<double, true, KMAX>), dim3(nblk(nt, waves)), dim3(64 * waves), 0, s, t,
                        nt, (const double *)vdata, w, xf, zpart, active);

@@ -166,17 +166,25 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
     """The autotuner only picks launch shapes that keep every sum in the same order:
     the stress rates of the tuned layout, of the default shapes and of every
     alternative shape (DFN_AUTOTUNE=alt) are equal bit for bit, and so are those with
-    short phase-1 tasks two to a wave or one (DFN_VTX_PAIRS)."""
+    short phase-1 tasks two to a wave or one (DFN_VTX_PAIRS) and, with FP64 factors,
+    those read with non-temporal loads or ordinary ones (DFN_VTX_NT)."""
     p = _hnet(2, n_faults=40, seed=8)
     keys = ("vtx_waves", "vtx_order", "panel_rows", "panel_order", "panel_split")
     out = []
-    # (autotune, single launch of phase 1 and the near-field panels, phase-1 wave items)
-    runs = [("1", None, None), ("0", None, None), ("alt", None, None), ("0", None, "1"), ("1", None, "0"),
-            ("alt", None, "0")]
+    # (autotune, single launch of phase 1 and the near-field panels, phase-1 wave items,
+    # non-temporal FP64 factor loads)
+    runs = [("1", None, None, None), ("0", None, None, None), ("alt", None, None, None), ("0", None, "1", None),
+            ("1", None, "0", None), ("alt", None, "0", None)]
     if prec == _abi.PREC_MIXED:
-        runs += [("1", "1", None), ("1", "0", None), ("0", "1", None)]
-    for tune, fuse, pairs in runs:
+        runs += [("1", "1", None, None), ("1", "0", None, None), ("0", "1", None, None)]
+    if prec == _abi.PREC_FP64:
+        runs += [("0", None, None, "1"), ("1", None, "1", "1"), ("alt", None, None, "0"), ("1", None, None, "0")]
+    for tune, fuse, pairs, nt in runs:
         monkeypatch.setenv("DFN_AUTOTUNE", tune)
+        if nt is None:
+            monkeypatch.delenv("DFN_VTX_NT", raising=False)
+        else:
+            monkeypatch.setenv("DFN_VTX_NT", nt)
         if fuse is None:
             monkeypatch.delenv("DFN_FUSE_NEAR", raising=False)
         else:
@@ -190,7 +198,7 @@ def test_autotuned_launch_shapes_leave_results_unchanged(prec, monkeypatch):
         if prec != _abi.PREC_FP64:
             d.set_precision(prec)
         t = d.tuning()
-        print(f"[gpu] autotune={tune} fuse={fuse} pairs={pairs} {t}", flush=True)
+        print(f"[gpu] autotune={tune} fuse={fuse} pairs={pairs} nt={nt} {t}", flush=True)
         assert t["vtx_waves"] in (2, 4, 8) and t["vtx_order"] in (0, 1, 2) and t["panel_rows"] in (2, 4)
         assert t["panel_order"] in (0, 1) and t["panel_split"] in (0, 32, 16)
         if tune == "0":
